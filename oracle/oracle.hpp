@@ -1,0 +1,343 @@
+// ORACLE — test infrastructure only.  FP64 CPU restatement of the reference's
+// Metropolis energy path (Hamiltonian::energy(Change) with its Nonbonded and Ewald
+// terms).  Only the tests, the smoke check and the CPU-baseline leg of the bench
+// may load it; the product library never links or calls it.
+//
+// Each function cites the reference file:line it follows (paths relative to the
+// mlund/faunus checkout).  Loop orders, summation orders and special cases follow
+// the reference so that ΔU agrees to rounding with what the reference computes.
+//
+// Parity status of the pieces (what pins each one):
+//  * units, mixing, LJ/WCA/HS/plain Coulomb: reference doctests (potentials.cpp)
+//    and the `examples/minimal` golden initial energy.
+//  * Andrea spline: the reference's own doctest (tabulate.h:313-365).
+//  * Ewald k-vectors/Q/reciprocal/surface/self: doctests energy.cpp:74-99,
+//    249-317, 665-762, plus a Madelung-constant check and partial==full update.
+//  * Nonbonded pair enumeration: brute force O(N^2) differences, the
+//    `examples/bulk` and `examples/minimal` golden energies.
+//  * Ewald real-space S(q) spline inside CoulombGalore (absent third-party
+//    dependency, pinned commit 4055f58538d781acccb2937ab4580855fcba31f8):
+//    PARITY UNPINNED.  Two modes are offered: exact erfc and an Andrea spline of
+//    S(q) on q in [0,1] with utol = 0.005/lB (potentials.cpp:1632).
+#pragma once
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../faunus_amd/csrc/json_lite.hpp"
+
+namespace oracle {
+
+constexpr double pi = 3.141592653589793238462643383279502884;
+constexpr double infty = std::numeric_limits<double>::infinity();
+constexpr double max_exp_argument = 709.782712893384;
+
+// units.h:14-41 — physical constants and derived units.
+struct Units {
+    static constexpr double vacuum_permittivity = 8.85419e-12;
+    static constexpr double elementary_charge = 1.602177e-19;
+    static constexpr double boltzmann_constant = 1.380658e-23;
+    static constexpr double avogadro = 6.022137e23;
+    double temperature = 298.15;
+    double kT() const { return temperature * boltzmann_constant; }
+    double bjerrum_length(double epsr) const {
+        return elementary_charge * elementary_charge /
+               (4 * pi * vacuum_permittivity * epsr * 1e-10 * kT());
+    }
+    double kJmol(double v) const { return v / kT() / avogadro * 1e3; } // units.h:230-233
+};
+
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+    Vec3() = default;
+    Vec3(double a, double b, double c) : x(a), y(b), z(c) {}
+    Vec3 operator+(const Vec3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vec3 operator-(const Vec3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vec3 operator*(double s) const { return {x * s, y * s, z * s}; }
+    double dot(const Vec3& o) const { return x * o.x + y * o.y + z * o.z; }
+    double squared_norm() const { return x * x + y * y + z * z; }
+};
+
+// geometry.h:407-485 (Chameleon orthogonal fast paths), geometry.cpp:91-129.
+struct Cuboid {
+    Vec3 len{1, 1, 1}, half{0.5, 0.5, 0.5}, inv{1, 1, 1};
+    void set(const Vec3& L) {
+        len = L;
+        half = L * 0.5;
+        inv = {1.0 / L.x, 1.0 / L.y, 1.0 / L.z};
+    }
+    double volume() const { return len.x * len.y * len.z; }
+    static int anint(double x) { return int(x > 0.0 ? x + 0.5 : x - 0.5); } // geometry.h:132-135
+    Vec3 vdist(const Vec3& a, const Vec3& b) const {
+        Vec3 d = a - b;
+        if (d.x > half.x) d.x -= len.x; else if (d.x < -half.x) d.x += len.x;
+        if (d.y > half.y) d.y -= len.y; else if (d.y < -half.y) d.y += len.y;
+        if (d.z > half.z) d.z -= len.z; else if (d.z < -half.z) d.z += len.z;
+        return d;
+    }
+    double sqdist(const Vec3& a, const Vec3& b) const {
+        Vec3 d{std::fabs(a.x - b.x), std::fabs(a.y - b.y), std::fabs(a.z - b.z)};
+        d.x -= len.x * static_cast<double>(d.x > half.x);
+        d.y -= len.y * static_cast<double>(d.y > half.y);
+        d.z -= len.z * static_cast<double>(d.z > half.z);
+        return d.x * d.x + d.y * d.y + d.z * d.z;
+    }
+    void boundary(Vec3& a) const {
+        if (std::fabs(a.x) > half.x) a.x -= len.x * anint(a.x * inv.x);
+        if (std::fabs(a.y) > half.y) a.y -= len.y * anint(a.y * inv.y);
+        if (std::fabs(a.z) > half.z) a.z -= len.z * anint(a.z * inv.z);
+    }
+};
+
+// tabulate.h:95-308 — "Andrea" quintic spline with adaptive knots, restated.
+struct SplineTable {
+    std::vector<double> knots; // ascending abscissae (the reference calls them r2)
+    std::vector<double> coeff; // 6 per interval
+    double xmin = 0, xmax = 0;
+    size_t num_knots() const { return knots.size(); }
+};
+
+class AndreaSpline {
+  public:
+    double utol = 1e-5, ftol = -1, umaxtol = -1, fmaxtol = -1;
+    double numdr = 0.0001;
+    int max_grid = 1200, max_shrink = 100;
+    double shrink = 0.9;
+    void set_tolerance(double u, double f = -1, double umax = -1, double fmax = -1) {
+        utol = u; ftol = f; umaxtol = umax; fmaxtol = fmax;
+    }
+    SplineTable generate(const std::function<double(double)>& f, double xmin, double xmax) const;
+    static double eval(const SplineTable& t, double x);
+
+  private:
+    double d1(const std::function<double(double)>& f, double x) const {
+        return (f(x + numdr * 0.5) - f(x - numdr * 0.5)) / numdr;
+    }
+    double d2(const std::function<double(double)>& f, double x) const {
+        return (d1(f, x + numdr * 0.5) - d1(f, x - numdr * 0.5)) / numdr;
+    }
+};
+
+struct AtomType {
+    std::string name;
+    double charge = 0, sigma = 0, eps = 0, dp = 0, dprot = 0, mw = 1;
+    bool implicit = false;
+    bool has_dp = false;
+};
+
+struct MoleculeType {
+    std::string name;
+    bool atomic = false, rigid = false, compressible = false;
+    std::vector<int> atoms;
+    std::vector<std::pair<int, int>> excluded; // excluded intra-molecular pairs (i<j)
+    bool is_pair_excluded(int i, int j) const {
+        if (i > j) std::swap(i, j);
+        for (auto& e : excluded)
+            if (e.first == i && e.second == j) return true;
+        return false;
+    }
+};
+
+struct Particle {
+    Vec3 pos;
+    double charge = 0;
+    int id = 0;
+};
+
+struct Group {
+    int begin = 0;   // index of first particle in Space::particles
+    int size = 0;    // active size
+    int molid = 0;
+    bool atomic = true, rigid = false, compressible = false;
+    Vec3 mass_center;
+};
+
+// space.h:29-75 — description of what changed in a move.
+struct GroupChange {
+    int group_index = 0;
+    bool all = false;
+    bool internal = false;
+    std::vector<int> relative_atom_indices; // sorted
+};
+struct Change {
+    bool everything = false, volume_change = false, matter_change = false;
+    std::vector<GroupChange> groups;
+    bool empty() const { return !everything && !volume_change && !matter_change && groups.empty(); }
+    explicit operator bool() const { return !empty(); }
+};
+
+struct Space {
+    Cuboid geometry;
+    std::vector<Particle> particles;
+    std::vector<Group> groups;
+};
+
+// One additive pair term (potentials.h): LJ, WCA, HS, plain Coulomb or a
+// CoulombGalore-style "coulomb" with a short-range splitting function S(q).
+enum class PotKind { LennardJones, WCA, HardSphere, CoulombPlainPM, CoulombGalore };
+enum class Scheme { Plain, Ewald, Fanourgakis };
+
+struct PairTerm {
+    PotKind kind = PotKind::LennardJones;
+    int ntypes = 0;
+    std::vector<double> sigma2, eps4; // per type pair (row-major ntypes x ntypes)
+    double lj_cutoff2 = infty;        // truncation extension (no reference counterpart)
+    double lB = 0;
+    Scheme scheme = Scheme::Plain;
+    double rc = infty, rc_inv = 0, alpha = 0;
+    bool splined = true;
+    SplineTable S;
+    double self_prefactor = 0; // self energy per q^2 (kT), CoulombGalore selfEnergyFunctor
+    double short_range(double q) const;
+    double operator()(const Particle& a, const Particle& b, double r2) const;
+};
+
+// Per type pair list of PairTerms summed in order: FunctorPotential
+// (potentials.cpp:1202-1330) and CombinedPairPotential (potentials_base.h:226-240).
+struct PairPotential {
+    int ntypes = 0;
+    std::vector<PairTerm> terms;
+    std::vector<std::vector<int>> umatrix; // ntypes*ntypes -> term indices
+    bool anisotropic_form = true;           // PairEnergy<.., true> uses vdist (energy.h:428-438)
+    bool has_self_energy = false;
+    double self_prefactor = 0;
+    double operator()(const Particle& a, const Particle& b, double r2) const {
+        double u = 0.0;
+        for (int t : umatrix[a.id * ntypes + b.id]) u = u + terms[t](a, b, r2);
+        return u;
+    }
+};
+
+// energy.h:92-122 and energy.cpp:28-59.
+struct EwaldData {
+    double alpha = 0, r_cutoff = 0, n_cutoff = 0, lB = 0;
+    double surface_dielectric_constant = 0, const_inf = 0, kappa = 0, kappa2 = 0;
+    bool spherical_sum = true;
+    bool ipbc = false;
+    Vec3 box;
+    double check_k2_zero = 0;
+    std::vector<Vec3> kvec;
+    std::vector<double> Ak;
+    std::vector<double> Qre, Qim;
+    int num_kvectors() const { return static_cast<int>(kvec.size()); }
+};
+
+struct MCStats {
+    long moves = 0, accepted = 0;
+    double sum_du = 0;
+};
+
+class Hamiltonian;
+
+// Energy terms (externalpotential.h:21-43 interface, restated).
+struct Term {
+    std::string name;
+    virtual ~Term() = default;
+    virtual double energy(const Space& spc, const Space* old_spc, const Change& c) = 0;
+    virtual void update_state(const Space& spc, const Space* old_spc, const Change& c) { (void)spc; (void)old_spc; (void)c; }
+    virtual void sync_from(const Term& other, const Change& c) { (void)other; (void)c; }
+    virtual void init(const Space& spc) { (void)spc; }
+};
+
+class Hamiltonian {
+  public:
+    Units units;
+    std::vector<AtomType> atoms;
+    std::vector<MoleculeType> molecules;
+    std::vector<std::unique_ptr<Term>> terms;
+    double maximum_allowed_energy = infty;
+    std::vector<double> latest;
+
+    double energy(const Space& spc, const Space* old_spc, const Change& c);
+    void update_state(const Space& spc, const Space* old_spc, const Change& c) {
+        for (auto& t : terms) t->update_state(spc, old_spc, c);
+    }
+    void init(const Space& spc) {
+        for (auto& t : terms) t->init(spc);
+    }
+    void sync_from(const Hamiltonian& other, const Change& c) {
+        for (size_t i = 0; i < terms.size(); ++i) terms[i]->sync_from(*other.terms[i], c);
+    }
+};
+
+// The system: description parsed from the reference's input JSON plus both MC
+// states (montecarlo.h:52-65).
+class System {
+  public:
+    Units units;
+    std::vector<AtomType> atoms;
+    std::vector<MoleculeType> molecules;
+    jsonlite::Value energy_json;
+    Space accepted_space, trial_space;
+    std::unique_ptr<Hamiltonian> accepted_pot, trial_pot;
+    double initial_energy = 0;
+    MCStats stats;
+
+    explicit System(const std::string& config_json);
+    void set_particles(int n, const double* xyz, const double* charge, const int* atom_id);
+    void set_groups(int ngroups, const int* begin, const int* size, const int* molid, const double* mass_center);
+    void init();                                   // montecarlo.cpp:43-71
+    double total_energy(bool trial, std::vector<double>* terms_out);
+    // Energies of the affected pairs in both states for a Change whose new
+    // coordinates are already applied to the trial space.
+    void energy_both(const Change& c, std::vector<double>& e_new, std::vector<double>& e_old);
+    double metropolis_move(const Change& c, double uniform, bool* accepted_out, double* du_out,
+                           std::vector<double>* e_new = nullptr, std::vector<double>* e_old = nullptr);
+    int num_terms() const { return static_cast<int>(accepted_pot->terms.size()); }
+    std::vector<std::string> term_names() const;
+    Hamiltonian& pot(bool trial) { return trial ? *trial_pot : *accepted_pot; }
+
+  private:
+    std::unique_ptr<Hamiltonian> build_hamiltonian();
+};
+
+// Terms exposed for the known-answer tests.
+struct EwaldTerm : Term {
+    EwaldData data;
+    bool have_old = false;
+    EwaldTerm(const jsonlite::Value& j, const Units& u);
+    static void update_box(EwaldData& d, const Vec3& box);            // energy.cpp:133-186, 356-410
+    static void full_update(EwaldData& d, const Space& spc);          // energy.cpp:191-206, 412-425
+    static void partial_update(EwaldData& d, const Change& c, const Space& spc, const Space& old); // 219-247
+    static double reciprocal_energy(const EwaldData& d);              // energy.cpp:524-531
+    static double surface_energy(const EwaldData& d, const Change& c, const Space& spc); // 466-482
+    static double self_energy(const EwaldData& d, const Change& c, const Space& spc);    // 484-518
+    double energy(const Space& spc, const Space* old_spc, const Change& c) override;
+    void update_state(const Space& spc, const Space* old_spc, const Change& c) override;
+    void sync_from(const Term& other, const Change& c) override;
+    void init(const Space& spc) override;
+};
+
+struct NonbondedTerm : Term {
+    PairPotential pot;
+    double default_g2g_cutoff2 = std::numeric_limits<double>::max();
+    const std::vector<MoleculeType>* molecules = nullptr;
+    double energy(const Space& spc, const Space* old_spc, const Change& c) override;
+    double pair(const Space& spc, const Particle& a, const Particle& b) const {
+        if (pot.anisotropic_form) {
+            Vec3 r = spc.geometry.vdist(a.pos, b.pos);
+            return pot(a, b, r.squared_norm());
+        }
+        return pot(a, b, spc.geometry.sqdist(a.pos, b.pos));
+    }
+    bool cut(const Space& spc, const Group& g1, const Group& g2) const;
+};
+
+struct SelfEnergyTerm : Term {
+    double prefactor = 0; // energy = prefactor * q^2 per particle
+    double energy(const Space& spc, const Space* old_spc, const Change& c) override;
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used by the device-resident
+// MC proposal scheme; identical arithmetic on the host and on the GPU.
+std::array<uint32_t, 4> philox4x32_10(std::array<uint32_t, 4> ctr, std::array<uint32_t, 2> key);
+
+// Parse the `coulomb` block into a PairTerm (potentials.cpp:1628-1699).
+PairTerm make_coulomb_term(const jsonlite::Value& j, const Units& u);
+
+} // namespace oracle

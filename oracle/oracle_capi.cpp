@@ -1,0 +1,338 @@
+// ORACLE — test infrastructure only.  C entry points over the FP64 restatement
+// so that the pytest suite (ctypes) can drive it with the same inputs it gives
+// the HIP library.
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+
+template <class F> int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+Change make_change(int everything, int ngroups, const int* group_index, const int* all, const int* internal,
+                   const int* offsets, const int* indices) {
+    Change c;
+    c.everything = everything != 0;
+    for (int g = 0; g < ngroups; ++g) {
+        GroupChange gc;
+        gc.group_index = group_index[g];
+        gc.all = all[g] != 0;
+        gc.internal = internal[g] != 0;
+        for (int k = offsets[g]; k < offsets[g + 1]; ++k) gc.relative_atom_indices.push_back(indices[k]);
+        c.groups.push_back(gc);
+    }
+    return c;
+}
+
+// Apply new coordinates for the listed absolute particle indices to the trial space.
+void stage(System& s, int m, const int* pidx, const double* xyz) {
+    for (int k = 0; k < m; ++k) {
+        Particle& p = s.trial_space.particles.at(static_cast<size_t>(pidx[k]));
+        p.pos = {xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]};
+    }
+}
+} // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_create(const char* json) {
+    try {
+        return new System(json);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_destroy(void* h) { delete static_cast<System*>(h); }
+
+int orc_set_particles(void* h, int n, const double* xyz, const double* q, const int* id) {
+    return guard([&] { static_cast<System*>(h)->set_particles(n, xyz, q, id); });
+}
+
+int orc_set_groups(void* h, int ng, const int* begin, const int* size, const int* molid, const double* cm) {
+    return guard([&] { static_cast<System*>(h)->set_groups(ng, begin, size, molid, cm); });
+}
+
+int orc_init(void* h, double* initial_energy) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        s->init();
+        if (initial_energy) *initial_energy = s->initial_energy;
+    });
+}
+
+int orc_num_terms(void* h) { return static_cast<System*>(h)->num_terms(); }
+
+int orc_term_name(void* h, int i, char* buf, int len) {
+    return guard([&] {
+        auto names = static_cast<System*>(h)->term_names();
+        std::strncpy(buf, names.at(static_cast<size_t>(i)).c_str(), static_cast<size_t>(len - 1));
+        buf[len - 1] = 0;
+    });
+}
+
+int orc_total_energy(void* h, int trial, double* terms, double* total) {
+    return guard([&] {
+        std::vector<double> t;
+        const double e = static_cast<System*>(h)->total_energy(trial != 0, &t);
+        for (size_t i = 0; i < t.size(); ++i) terms[i] = t[i];
+        if (total) *total = e;
+    });
+}
+
+// Energies of the affected pairs of a Change in the trial state (new
+// coordinates given for the moved particles) and in the accepted state, per term.
+// Leaves the trial state as it was (the staged coordinates are reverted).
+int orc_energy_change(void* h, int everything, int ngroups, const int* gi, const int* all, const int* internal,
+                      const int* off, const int* idx, int m, const int* pidx, const double* xyz, double* e_new,
+                      double* e_old) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        Change c = make_change(everything, ngroups, gi, all, internal, off, idx);
+        stage(*s, m, pidx, xyz);
+        std::vector<double> en, eo;
+        s->energy_both(c, en, eo);
+        for (size_t i = 0; i < en.size(); ++i) {
+            e_new[i] = en[i];
+            e_old[i] = eo[i];
+        }
+        // revert trial state (as a rejected move would)
+        for (int k = 0; k < m; ++k) s->trial_space.particles[static_cast<size_t>(pidx[k])] =
+                                        s->accepted_space.particles[static_cast<size_t>(pidx[k])];
+        s->trial_pot->sync_from(*s->accepted_pot, c);
+    });
+}
+
+// Metropolis driver for single-particle translations (montecarlo.cpp:139-187,
+// move.cpp:225-293, core.cpp:249-261) with its own std::mt19937 proposal source.
+// Records each proposal so that another implementation can replay it.
+int orc_run_translate(void* h, long nmoves, unsigned seed, int molid, double default_dp, int* rec_group,
+                      int* rec_index, double* rec_xyz, double* rec_u, double* rec_du, unsigned char* rec_acc,
+                      double* rec_enew, double* rec_eold) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        std::mt19937 engine(seed);
+        std::uniform_real_distribution<double> unif(0.0, 1.0);
+        auto rnd = [&] { return unif(engine); };
+        std::vector<int> candidates;
+        for (size_t g = 0; g < s->accepted_space.groups.size(); ++g)
+            if (s->accepted_space.groups[g].molid == molid) candidates.push_back(static_cast<int>(g));
+        if (candidates.empty()) throw std::runtime_error("no groups of the requested molecule");
+        const int nterm = s->num_terms();
+        std::vector<double> en, eo;
+        for (long n = 0; n < nmoves; ++n) {
+            const int gidx = candidates[std::uniform_int_distribution<size_t>(0, candidates.size() - 1)(engine)];
+            Group& g = s->trial_space.groups[static_cast<size_t>(gidx)];
+            if (g.size == 0) throw std::runtime_error("empty group");
+            const int rel = static_cast<int>(std::uniform_int_distribution<int>(0, g.size - 1)(engine));
+            Particle& p = s->trial_space.particles[static_cast<size_t>(g.begin + rel)];
+            const AtomType& at = s->atoms.at(static_cast<size_t>(p.id));
+            const double dp = at.has_dp ? at.dp : default_dp;
+            Vec3 dir;
+            double r2;
+            do {
+                dir = {rnd() - 0.5, rnd() - 0.5, rnd() - 0.5};
+                r2 = dir.squared_norm();
+            } while (r2 > 0.25);
+            dir = dir * (1.0 / std::sqrt(r2));
+            p.pos = p.pos + dir * dp * rnd();
+            s->trial_space.geometry.boundary(p.pos);
+            const double u = rnd();
+            Change c;
+            GroupChange gc;
+            gc.group_index = gidx;
+            gc.internal = true;
+            gc.relative_atom_indices = {rel};
+            c.groups.push_back(gc);
+            bool acc = false;
+            double du = 0;
+            if (rec_xyz) {
+                rec_xyz[3 * n] = p.pos.x;
+                rec_xyz[3 * n + 1] = p.pos.y;
+                rec_xyz[3 * n + 2] = p.pos.z;
+            }
+            s->metropolis_move(c, u, &acc, &du, &en, &eo);
+            if (rec_group) rec_group[n] = gidx;
+            if (rec_index) rec_index[n] = rel;
+            if (rec_u) rec_u[n] = u;
+            if (rec_du) rec_du[n] = du;
+            if (rec_acc) rec_acc[n] = acc ? 1 : 0;
+            for (int t = 0; t < nterm; ++t) {
+                if (rec_enew) rec_enew[n * nterm + t] = en[static_cast<size_t>(t)];
+                if (rec_eold) rec_eold[n * nterm + t] = eo[static_cast<size_t>(t)];
+            }
+        }
+    });
+}
+
+// Replay a recorded single-particle proposal stream.
+int orc_replay_translate(void* h, long nmoves, const int* group, const int* index, const double* xyz,
+                         const double* u, double* out_du, unsigned char* out_acc, double* out_enew,
+                         double* out_eold) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        const int nterm = s->num_terms();
+        std::vector<double> en, eo;
+        for (long n = 0; n < nmoves; ++n) {
+            Group& g = s->trial_space.groups.at(static_cast<size_t>(group[n]));
+            Particle& p = s->trial_space.particles.at(static_cast<size_t>(g.begin + index[n]));
+            p.pos = {xyz[3 * n], xyz[3 * n + 1], xyz[3 * n + 2]};
+            Change c;
+            GroupChange gc;
+            gc.group_index = group[n];
+            gc.internal = true;
+            gc.relative_atom_indices = {index[n]};
+            c.groups.push_back(gc);
+            bool acc = false;
+            double du = 0;
+            s->metropolis_move(c, u[n], &acc, &du, &en, &eo);
+            if (out_du) out_du[n] = du;
+            if (out_acc) out_acc[n] = acc ? 1 : 0;
+            for (int t = 0; t < nterm; ++t) {
+                if (out_enew) out_enew[n * nterm + t] = en[static_cast<size_t>(t)];
+                if (out_eold) out_eold[n * nterm + t] = eo[static_cast<size_t>(t)];
+            }
+        }
+    });
+}
+
+int orc_get_positions(void* h, int trial, double* xyz) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        const Space& sp = trial ? s->trial_space : s->accepted_space;
+        for (size_t i = 0; i < sp.particles.size(); ++i) {
+            xyz[3 * i] = sp.particles[i].pos.x;
+            xyz[3 * i + 1] = sp.particles[i].pos.y;
+            xyz[3 * i + 2] = sp.particles[i].pos.z;
+        }
+    });
+}
+
+int orc_mc_stats(void* h, long* moves, long* accepted, double* sum_du) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        *moves = s->stats.moves;
+        *accepted = s->stats.accepted;
+        *sum_du = s->stats.sum_du;
+    });
+}
+
+// ---- known-answer-test helpers
+
+// Ewald policy pieces on an explicit particle set (energy.cpp:249-317).
+int orc_ewald_parts(const char* coulomb_json, double temperature, double box, int n, const double* xyz,
+                    const double* q, int* num_k, double* self, double* surface, double* reciprocal) {
+    return guard([&] {
+        Units u;
+        u.temperature = temperature;
+        EwaldTerm t(jsonlite::parse(coulomb_json), u);
+        Space spc;
+        spc.geometry.set({box, box, box});
+        for (int i = 0; i < n; ++i) {
+            Particle p;
+            p.pos = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+            p.charge = q[i];
+            spc.particles.push_back(p);
+        }
+        Group g;
+        g.begin = 0;
+        g.size = n;
+        spc.groups.push_back(g);
+        EwaldTerm::update_box(t.data, spc.geometry.len);
+        EwaldTerm::full_update(t.data, spc);
+        Change c;
+        c.everything = true;
+        *num_k = t.data.num_kvectors();
+        *self = EwaldTerm::self_energy(t.data, c, spc);
+        *surface = EwaldTerm::surface_energy(t.data, c, spc);
+        *reciprocal = EwaldTerm::reciprocal_energy(t.data);
+    });
+}
+
+// k-vector table (n as integers would lose nothing; kx,ky,kz,Ak are returned).
+int orc_ewald_kvectors(const char* coulomb_json, double temperature, double box, int cap, double* out, int* num_k) {
+    return guard([&] {
+        Units u;
+        u.temperature = temperature;
+        EwaldTerm t(jsonlite::parse(coulomb_json), u);
+        EwaldTerm::update_box(t.data, {box, box, box});
+        *num_k = t.data.num_kvectors();
+        for (int k = 0; k < std::min(cap, *num_k); ++k) {
+            out[4 * k] = t.data.kvec[k].x;
+            out[4 * k + 1] = t.data.kvec[k].y;
+            out[4 * k + 2] = t.data.kvec[k].z;
+            out[4 * k + 3] = t.data.Ak[k];
+        }
+    });
+}
+
+// Andrea spline of the tabulate.h doctest function f(x) = x sin(x)/2 + 2, or of the
+// short-range function S(q) of a `coulomb` block (which = 1).
+int orc_spline_test(int which, const char* coulomb_json, double utol, double ftol, double xmin, double xmax,
+                    int cap, double* knots, double* coeff, int* nknots, int neval, const double* xs, double* ys) {
+    return guard([&] {
+        AndreaSpline sp;
+        sp.set_tolerance(utol, ftol);
+        SplineTable t;
+        if (which == 0) {
+            t = sp.generate([](double x) { return 0.5 * x * std::sin(x) + 2; }, xmin, xmax);
+        } else {
+            Units u;
+            PairTerm pt = make_coulomb_term(jsonlite::parse(coulomb_json), u);
+            t = pt.S;
+        }
+        *nknots = static_cast<int>(t.knots.size());
+        for (int i = 0; i < std::min(cap, *nknots); ++i) knots[i] = t.knots[i];
+        for (int i = 0; i < std::min(cap * 6, static_cast<int>(t.coeff.size())); ++i) coeff[i] = t.coeff[i];
+        for (int i = 0; i < neval; ++i) ys[i] = AndreaSpline::eval(t, xs[i]);
+    });
+}
+
+// One pair energy through the configured nonbonded term (types and charges given).
+int orc_pair_energy(void* h, int type_a, double qa, int type_b, double qb, double r2, double* out) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        for (auto& t : s->accepted_pot->terms) {
+            if (auto* nb = dynamic_cast<NonbondedTerm*>(t.get())) {
+                Particle a, b;
+                a.id = type_a;
+                a.charge = qa;
+                b.id = type_b;
+                b.charge = qb;
+                *out = nb->pot(a, b, r2);
+                return;
+            }
+        }
+        throw std::runtime_error("no nonbonded term");
+    });
+}
+
+double orc_bjerrum_length(double temperature, double epsr) {
+    Units u;
+    u.temperature = temperature;
+    return u.bjerrum_length(epsr);
+}
+
+void orc_philox(const unsigned* ctr, const unsigned* key, unsigned* out) {
+    auto r = philox4x32_10({ctr[0], ctr[1], ctr[2], ctr[3]}, {key[0], key[1]});
+    for (int i = 0; i < 4; ++i) out[i] = r[static_cast<size_t>(i)];
+}
+
+} // extern "C"
