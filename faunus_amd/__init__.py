@@ -1,0 +1,252 @@
+"""faunus_amd — MI355X energy engine for Faunus's Metropolis Monte Carlo.
+
+Python front-end over the C ABI declared in ``include/faunus_amd.h`` (the library
+``faunus_amd/libfaunus_amd.so`` built from ``faunus_amd/csrc``).  There is no CPU
+fallback: every energy is computed by the HIP kernels, and loading fails loudly
+when the library is missing.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfaunus_amd.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+U8 = C.POINTER(C.c_ubyte)
+
+STATE_ACCEPTED = 0
+STATE_TRIAL = 1
+
+# every entry point of include/faunus_amd.h with its ctypes signature
+SIGNATURES = {
+    "fmc_last_error": (C.c_char_p, []),
+    "fmc_device_count": (C.c_int, [I]),
+    "fmc_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "fmc_destroy": (C.c_int, [C.c_void_p]),
+    "fmc_set_particles": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
+    "fmc_set_groups": (C.c_int, [C.c_void_p, C.c_int, I, I, I, D]),
+    "fmc_init": (C.c_int, [C.c_void_p, D]),
+    "fmc_num_terms": (C.c_int, [C.c_void_p, I]),
+    "fmc_term_name": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    "fmc_stage_trial": (C.c_int, [C.c_void_p, C.c_int, I, D]),
+    "fmc_delta_u": (C.c_int, [C.c_void_p, C.c_void_p, D, D]),
+    "fmc_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, D, D]),
+    "fmc_sync": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_total_energy": (C.c_int, [C.c_void_p, D, D]),
+    "fmc_check_q": (C.c_int, [C.c_void_p, D, D]),
+    "fmc_get_positions": (C.c_int, [C.c_void_p, D]),
+    "fmc_replay_translate": (C.c_int, [C.c_void_p, C.c_long, I, I, D, D, D, U8, D, D]),
+    "fmc_run_translate": (C.c_int, [C.c_void_p, C.c_long, I, I, D, D, D, U8]),
+    "fmc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_get_timing": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_long)]),
+    "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
+    "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
+}
+
+
+class Change(C.Structure):
+    """ctypes mirror of ``fmc_change`` (Faunus::Change)."""
+    _fields_ = [("everything", C.c_int), ("volume_change", C.c_int), ("matter_change", C.c_int),
+                ("num_groups", C.c_int), ("group_index", I), ("all", U8), ("internal", U8),
+                ("offsets", I), ("indices", I)]
+
+
+def lib():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().fmc_last_error().decode())
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def device_count():
+    n = C.c_int()
+    _check(lib().fmc_device_count(C.byref(n)))
+    return n.value
+
+
+def make_change(change):
+    """Build an ``fmc_change`` from a dict {"everything": bool, "groups": [{group_index,
+    all, internal, relative_atom_indices}]}; returns (struct, keep-alive arrays)."""
+    groups = change.get("groups", [])
+    gi = np.array([g["group_index"] for g in groups], np.int32)
+    al = np.array([int(g.get("all", False)) for g in groups], np.uint8)
+    it = np.array([int(g.get("internal", False)) for g in groups], np.uint8)
+    off = np.zeros(len(groups) + 1, np.int32)
+    idx = []
+    for k, g in enumerate(groups):
+        rel = list(g.get("relative_atom_indices", []))
+        idx += rel
+        off[k + 1] = off[k] + len(rel)
+    idx = np.array(idx if idx else [0], np.int32)
+    c = Change(int(change.get("everything", False)), int(change.get("volume_change", False)),
+               int(change.get("matter_change", False)), len(groups), _p(gi, C.c_int), _p(al, C.c_ubyte),
+               _p(it, C.c_ubyte), _p(off, C.c_int), _p(idx, C.c_int))
+    return c, (gi, al, it, off, idx)
+
+
+class Engine:
+    """One Metropolis system on the GPU (both MC states)."""
+
+    def __init__(self, config):
+        text = config if isinstance(config, str) else json.dumps(config)
+        h = C.c_void_p()
+        _check(lib().fmc_create(text.encode(), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_particles(self, xyz, charge, atom_id):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        charge = np.ascontiguousarray(charge, np.float64)
+        atom_id = np.ascontiguousarray(atom_id, np.int32)
+        self.n = len(charge)
+        _check(lib().fmc_set_particles(self.h, self.n, _p(xyz), _p(charge), _p(atom_id, C.c_int)))
+
+    def set_groups(self, begin, size, molid, mass_center=None):
+        begin = np.ascontiguousarray(begin, np.int32)
+        size = np.ascontiguousarray(size, np.int32)
+        molid = np.ascontiguousarray(molid, np.int32)
+        cm = None if mass_center is None else np.ascontiguousarray(mass_center, np.float64)
+        _check(lib().fmc_set_groups(self.h, len(begin), _p(begin, C.c_int), _p(size, C.c_int),
+                                    _p(molid, C.c_int), _p(cm)))
+
+    def init(self):
+        e = C.c_double()
+        _check(lib().fmc_init(self.h, C.byref(e)))
+        return e.value
+
+    @property
+    def num_terms(self):
+        n = C.c_int()
+        _check(lib().fmc_num_terms(self.h, C.byref(n)))
+        return n.value
+
+    def term_names(self):
+        out = []
+        for i in range(self.num_terms):
+            buf = C.create_string_buffer(64)
+            _check(lib().fmc_term_name(self.h, i, buf, 64))
+            out.append(buf.value.decode())
+        return out
+
+    def stage_trial(self, index, xyz):
+        index = np.ascontiguousarray(np.atleast_1d(index), np.int32)
+        xyz = np.ascontiguousarray(np.atleast_2d(xyz), np.float64)
+        _check(lib().fmc_stage_trial(self.h, len(index), _p(index, C.c_int), _p(xyz)))
+
+    def delta_u(self, change):
+        c, keep = make_change(change)
+        en = np.zeros(self.num_terms)
+        eo = np.zeros(self.num_terms)
+        _check(lib().fmc_delta_u(self.h, C.byref(c), _p(en), _p(eo)))
+        return en, eo
+
+    def energy(self, change, state):
+        c, keep = make_change(change)
+        t = np.zeros(self.num_terms)
+        tot = C.c_double()
+        _check(lib().fmc_energy(self.h, C.byref(c), state, _p(t), C.byref(tot)))
+        return tot.value, t
+
+    def sync(self, accept):
+        _check(lib().fmc_sync(self.h, int(bool(accept))))
+
+    def total_energy(self):
+        t = np.zeros(self.num_terms)
+        tot = C.c_double()
+        _check(lib().fmc_total_energy(self.h, _p(t), C.byref(tot)))
+        return tot.value, t
+
+    def check_q(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fmc_check_q(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def positions(self):
+        xyz = np.zeros((self.n, 3))
+        _check(lib().fmc_get_positions(self.h, _p(xyz)))
+        return xyz
+
+    def replay_translate(self, group, index, xyz, uniform, terms=True):
+        n = len(uniform)
+        nt = self.num_terms
+        group = np.ascontiguousarray(group, np.int32)
+        index = np.ascontiguousarray(index, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        uniform = np.ascontiguousarray(uniform, np.float64)
+        out = dict(du=np.zeros(n), accepted=np.zeros(n, np.uint8))
+        if terms:
+            out["e_new"] = np.zeros((n, nt))
+            out["e_old"] = np.zeros((n, nt))
+        _check(lib().fmc_replay_translate(self.h, n, _p(group, C.c_int), _p(index, C.c_int), _p(xyz), _p(uniform),
+                                          _p(out["du"]), _p(out["accepted"], C.c_ubyte), _p(out.get("e_new")),
+                                          _p(out.get("e_old"))))
+        return out
+
+    def run_translate(self, group, index, displacement, uniform):
+        """Plain Metropolis run (one GPU evaluation per move) from displacements."""
+        n = len(uniform)
+        group = np.ascontiguousarray(group, np.int32)
+        index = np.ascontiguousarray(index, np.int32)
+        displacement = np.ascontiguousarray(displacement, np.float64)
+        uniform = np.ascontiguousarray(uniform, np.float64)
+        du = np.zeros(n)
+        acc = np.zeros(n, np.uint8)
+        _check(lib().fmc_run_translate(self.h, n, _p(group, C.c_int), _p(index, C.c_int), _p(displacement),
+                                       _p(uniform), _p(du), _p(acc, C.c_ubyte)))
+        return du, acc
+
+    def set_timing(self, on):
+        _check(lib().fmc_set_timing(self.h, int(bool(on))))
+
+    def timing(self):
+        ms, n = C.c_double(), C.c_long()
+        _check(lib().fmc_get_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def coulomb_spline(self):
+        cap = 256
+        k = np.zeros(cap)
+        c = np.zeros(6 * cap)
+        n = C.c_int()
+        _check(lib().fmc_coulomb_spline(self.h, cap, _p(k), _p(c), C.byref(n)))
+        return k[: n.value], c[: 6 * max(n.value - 1, 0)]
+
+    @property
+    def num_kvectors(self):
+        n = C.c_int()
+        _check(lib().fmc_num_kvectors(self.h, C.byref(n)))
+        return n.value

@@ -1,0 +1,293 @@
+// C ABI of the engine (declared in include/faunus_amd.h).
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/faunus_amd.h"
+#include "engine.hpp"
+
+struct fmc_engine {
+    std::unique_ptr<fmc::Engine> impl;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return 1;
+    } catch (...) {
+        g_error = "unknown error";
+        return 1;
+    }
+}
+
+fmc::Engine& eng(fmc_engine* e) {
+    if (!e || !e->impl) throw std::runtime_error("null engine handle");
+    return *e->impl;
+}
+
+struct ChangeLists {
+    std::vector<int> groups;
+    std::vector<char> all, internal;
+    std::vector<std::vector<int>> rel;
+};
+
+ChangeLists unpack(const fmc_change* c) {
+    if (!c) throw std::runtime_error("null change");
+    if (c->everything || c->volume_change || c->matter_change)
+        throw std::runtime_error("everything/volume/matter changes go through fmc_total_energy or are unsupported");
+    ChangeLists L;
+    for (int g = 0; g < c->num_groups; ++g) {
+        L.groups.push_back(c->group_index[g]);
+        L.all.push_back(c->all ? static_cast<char>(c->all[g] != 0) : 0);
+        L.internal.push_back(c->internal ? static_cast<char>(c->internal[g] != 0) : 0);
+        std::vector<int> r;
+        if (c->offsets)
+            for (int k = c->offsets[g]; k < c->offsets[g + 1]; ++k) r.push_back(c->indices[k]);
+        L.rel.push_back(r);
+    }
+    return L;
+}
+
+double sum_terms(const double* t, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += t[i];
+    return s;
+}
+
+// MetropolisMonteCarlo::getEnergyChange (montecarlo.cpp:193-209)
+double energy_change(double new_energy, double old_energy) {
+    if (std::isnan(old_energy) && !std::isnan(new_energy)) return -INFINITY;
+    if (std::isnan(new_energy)) return INFINITY;
+    if (new_energy > 0.0 && std::isinf(new_energy)) return INFINITY;
+    const double du = new_energy - old_energy;
+    if (std::isnan(du)) return 0.0;
+    return du;
+}
+
+// MetropolisMonteCarlo::metropolisCriterion (montecarlo.cpp:17-34); the uniform
+// deviate is supplied by the caller.
+bool metropolis(double du, double u) {
+    if (std::isnan(du)) throw std::runtime_error("Metropolis error: energy cannot be NaN");
+    if (std::isinf(du) && du < 0.0) return true;
+    if (-du > 709.782712893384) return true;
+    return u <= std::exp(-du);
+}
+} // namespace
+
+extern "C" {
+
+const char* fmc_last_error(void) { return g_error.c_str(); }
+
+int fmc_device_count(int* count) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int fmc_create(const char* config_json, fmc_engine** out) {
+    return guarded([&] {
+        if (!config_json || !out) throw std::runtime_error("null argument");
+        auto* h = new fmc_engine;
+        try {
+            h->impl = std::make_unique<fmc::Engine>(config_json);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fmc_destroy(fmc_engine* engine) {
+    return guarded([&] { delete engine; });
+}
+
+int fmc_set_particles(fmc_engine* e, int n, const double* xyz, const double* charge, const int* atom_id) {
+    return guarded([&] { eng(e).set_particles(n, xyz, charge, atom_id); });
+}
+
+int fmc_set_groups(fmc_engine* e, int ng, const int* begin, const int* size, const int* molid, const double* cm) {
+    return guarded([&] { eng(e).set_groups(ng, begin, size, molid, cm); });
+}
+
+int fmc_init(fmc_engine* e, double* initial_energy) {
+    return guarded([&] {
+        eng(e).init();
+        if (initial_energy) {
+            std::vector<double> t(static_cast<size_t>(eng(e).num_terms()));
+            eng(e).total_energy(t.data());
+            *initial_energy = sum_terms(t.data(), static_cast<int>(t.size()));
+        }
+    });
+}
+
+int fmc_num_terms(fmc_engine* e, int* count) {
+    return guarded([&] { *count = eng(e).num_terms(); });
+}
+
+int fmc_term_name(fmc_engine* e, int index, char* buffer, int length) {
+    return guarded([&] {
+        const auto& names = eng(e).model().term_names;
+        if (index < 0 || index >= static_cast<int>(names.size())) throw std::runtime_error("term index out of range");
+        if (length <= 0) throw std::runtime_error("buffer too small");
+        std::strncpy(buffer, names[static_cast<size_t>(index)].c_str(), static_cast<size_t>(length - 1));
+        buffer[length - 1] = 0;
+    });
+}
+
+int fmc_stage_trial(fmc_engine* e, int count, const int* particle_index, const double* xyz) {
+    return guarded([&] { eng(e).stage_trial(count, particle_index, xyz); });
+}
+
+int fmc_delta_u(fmc_engine* e, const fmc_change* change, double* new_terms, double* old_terms) {
+    return guarded([&] {
+        ChangeLists L = unpack(change);
+        eng(e).delta(L.groups, L.all, L.internal, L.rel, new_terms, old_terms);
+    });
+}
+
+int fmc_energy(fmc_engine* e, const fmc_change* change, int state, double* terms, double* total) {
+    return guarded([&] {
+        fmc::Engine& E = eng(e);
+        const int nt = E.num_terms();
+        if (change && change->everything) {
+            if (state != FMC_STATE_ACCEPTED) throw std::runtime_error("Change::everything is evaluated on the accepted state");
+            E.total_energy(terms);
+        } else {
+            ChangeLists L = unpack(change);
+            std::vector<double> en(static_cast<size_t>(nt)), eo(static_cast<size_t>(nt));
+            E.delta(L.groups, L.all, L.internal, L.rel, en.data(), eo.data());
+            const std::vector<double>& src = (state == FMC_STATE_TRIAL) ? en : eo;
+            std::copy(src.begin(), src.end(), terms);
+        }
+        if (total) *total = sum_terms(terms, nt);
+    });
+}
+
+int fmc_sync(fmc_engine* e, int accept) {
+    return guarded([&] { eng(e).sync(accept != 0); });
+}
+
+int fmc_total_energy(fmc_engine* e, double* terms, double* total) {
+    return guarded([&] {
+        eng(e).total_energy(terms);
+        if (total) *total = sum_terms(terms, eng(e).num_terms());
+    });
+}
+
+int fmc_check_q(fmc_engine* e, double* max_abs_dq, double* rel_reciprocal) {
+    return guarded([&] { eng(e).check_q(max_abs_dq, rel_reciprocal); });
+}
+
+int fmc_get_positions(fmc_engine* e, double* xyz) {
+    return guarded([&] { eng(e).get_positions(xyz); });
+}
+
+int fmc_replay_translate(fmc_engine* e, long count, const int* group, const int* index, const double* xyz,
+                         const double* uniform, double* du_out, unsigned char* accepted, double* new_terms,
+                         double* old_terms) {
+    return guarded([&] {
+        fmc::Engine& E = eng(e);
+        const int nt = E.num_terms();
+        std::vector<double> en(static_cast<size_t>(nt)), eo(static_cast<size_t>(nt));
+        const auto& gb = E.group_begin();
+        std::vector<int> groups(1);
+        std::vector<char> all(1, 0), internal(1, 1);
+        std::vector<std::vector<int>> rel(1, std::vector<int>(1));
+        for (long n = 0; n < count; ++n) {
+            const int g = group[n];
+            if (g < 0 || g >= static_cast<int>(gb.size())) throw std::runtime_error("replay: group out of range");
+            const int abs_index = gb[static_cast<size_t>(g)] + index[n];
+            E.stage_trial(1, &abs_index, xyz + 3 * n);
+            groups[0] = g;
+            rel[0][0] = index[n];
+            E.delta(groups, all, internal, rel, en.data(), eo.data());
+            const double du = energy_change(sum_terms(en.data(), nt), sum_terms(eo.data(), nt));
+            const bool acc = metropolis(du, uniform[n]);
+            E.sync(acc);
+            if (du_out) du_out[n] = du;
+            if (accepted) accepted[n] = acc ? 1 : 0;
+            for (int t = 0; t < nt; ++t) {
+                if (new_terms) new_terms[n * nt + t] = en[static_cast<size_t>(t)];
+                if (old_terms) old_terms[n * nt + t] = eo[static_cast<size_t>(t)];
+            }
+        }
+    });
+}
+
+int fmc_run_translate(fmc_engine* e, long count, const int* group, const int* index, const double* displacement,
+                      const double* uniform, double* du_out, unsigned char* accepted) {
+    return guarded([&] {
+        fmc::Engine& E = eng(e);
+        const int nt = E.num_terms();
+        std::vector<double> en(static_cast<size_t>(nt)), eo(static_cast<size_t>(nt));
+        const auto& gb = E.group_begin();
+        const auto& box = E.model().box;
+        std::vector<int> groups(1);
+        std::vector<char> all(1, 0), internal(1, 1);
+        std::vector<std::vector<int>> rel(1, std::vector<int>(1));
+        std::vector<double> xyz(3 * static_cast<size_t>(E.num_particles()));
+        E.get_positions(xyz.data());
+        for (long n = 0; n < count; ++n) {
+            const int g = group[n];
+            if (g < 0 || g >= static_cast<int>(gb.size())) throw std::runtime_error("run: group out of range");
+            const int i = gb[static_cast<size_t>(g)] + index[n];
+            // AtomicTranslateRotate::translateParticle (move.cpp:225-240) and
+            // Cuboid::boundary (geometry.cpp:91-104)
+            double p[3];
+            for (int d = 0; d < 3; ++d) {
+                double x = xyz[3 * static_cast<size_t>(i) + d] + displacement[3 * n + d];
+                if (std::fabs(x) > 0.5 * box[d]) {
+                    const double s = x * (1.0 / box[d]);
+                    x -= box[d] * static_cast<int>(s > 0.0 ? s + 0.5 : s - 0.5);
+                }
+                p[d] = x;
+            }
+            E.stage_trial(1, &i, p);
+            groups[0] = g;
+            rel[0][0] = index[n];
+            E.delta(groups, all, internal, rel, en.data(), eo.data());
+            const double du = energy_change(sum_terms(en.data(), nt), sum_terms(eo.data(), nt));
+            const bool acc = metropolis(du, uniform[n]);
+            E.sync(acc);
+            if (acc)
+                for (int d = 0; d < 3; ++d) xyz[3 * static_cast<size_t>(i) + d] = p[d];
+            if (du_out) du_out[n] = du;
+            if (accepted) accepted[n] = acc ? 1 : 0;
+        }
+    });
+}
+
+int fmc_set_timing(fmc_engine* e, int enable) {
+    return guarded([&] { eng(e).set_timing(enable != 0); });
+}
+
+int fmc_get_timing(fmc_engine* e, double* total_ms, long* launches) {
+    return guarded([&] { eng(e).timing(total_ms, launches); });
+}
+
+int fmc_coulomb_spline(fmc_engine* e, int capacity, double* knots, double* coeff, int* num_knots) {
+    return guarded([&] {
+        const auto& S = eng(e).model().coulomb.S;
+        *num_knots = static_cast<int>(S.knots.size());
+        for (int i = 0; i < capacity && i < *num_knots; ++i) knots[i] = S.knots[static_cast<size_t>(i)];
+        for (int i = 0; i < 6 * capacity && i < static_cast<int>(S.coeff.size()); ++i) coeff[i] = S.coeff[static_cast<size_t>(i)];
+    });
+}
+
+int fmc_num_kvectors(fmc_engine* e, int* count) {
+    return guarded([&] { *count = eng(e).num_kvectors(); });
+}
+
+} // extern "C"

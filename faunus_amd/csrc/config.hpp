@@ -1,0 +1,103 @@
+// Host-side model of the reference's input for the nonbonded + Ewald path:
+// atom types, molecule types, the `energy:` list and the cuboid geometry, parsed
+// from the reference's JSON (the YAML input after yason.py conversion).
+//
+// Supported energy terms (energy.cpp:1280-1327): nonbonded / nonbonded_exact
+// (FunctorPotential with "default" and "A B" lists), nonbonded_coulomblj,
+// nonbonded_coulombwca, nonbonded_pm / nonbonded_coulombhs, nonbonded_pmwca, plus
+// `maxenergy`.  Pair potentials: lennardjones, wca, hardsphere, coulomb
+// {type: plain|ewald|fanourgakis}.  Ewald reciprocal space is added when the
+// coulomb type is "ewald" (addEwald, energy.cpp:1134-1160).
+#pragma once
+#include <cstdint>
+#include <functional>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "json_lite.hpp"
+
+namespace fmc {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// units.h:14-41 (same constants as the reference).
+struct UnitSystem {
+    double temperature = 298.15;
+    double kT() const { return temperature * 1.380658e-23; }
+    double bjerrum(double epsr) const {
+        return 1.602177e-19 * 1.602177e-19 / (4 * kPi * 8.85419e-12 * epsr * 1e-10 * kT());
+    }
+    double kJmol(double v) const { return v / kT() / 6.022137e23 * 1e3; }
+};
+
+// Quintic spline with adaptive knots (the reference's Andrea tabulator,
+// tabulate.h:95-308), used for the Coulomb short-range function S(q).
+struct Spline {
+    std::vector<double> knots; // ascending
+    std::vector<double> coeff; // 6 per interval
+    double eval(double x) const;
+    static Spline build(const std::function<double(double)>& f, double xmin, double xmax, double utol,
+                        double ftol = -1.0);
+};
+
+enum CoulombScheme : int { kCoulNone = 0, kCoulPlain = 1, kCoulPM = 2, kCoulEwald = 3, kCoulFanourgakis = 4 };
+
+struct CoulombModel {
+    int scheme = kCoulNone;
+    double lB = 0.0;
+    double rc = kInf, rc_inv = 0.0, alpha = 0.0;
+    bool splined = true;
+    Spline S;
+    double self_prefactor = 0.0; // per-particle self energy = self_prefactor * q^2 (kT)
+    double short_range_exact(double q) const;
+};
+
+// Per type pair parameters of the additive pair form u = LJ + WCA + HS + Coulomb.
+// Terms absent for a pair have zero coefficients.
+struct PairParam {
+    double lj_s2 = 0, lj_e4 = 0, lj_cut2 = kInf;
+    double wca_s2 = 0, wca_e4 = 0;
+    double hs_s2 = 0;
+    double coulomb = 0; // 1 if the pair carries the Coulomb term
+};
+
+struct EwaldModel {
+    bool present = false;
+    double alpha = 0, rc = 0, n_cutoff = 0, lB = 0, epss = 0, const_inf = 0;
+    bool spherical_sum = true;
+};
+
+struct AtomTypeCfg {
+    std::string name;
+    double charge = 0, sigma = 0, eps = 0, dp = 0, dprot = 0, mw = 1;
+    bool has_dp = false;
+};
+
+struct MoleculeCfg {
+    std::string name;
+    bool atomic = false, rigid = false, compressible = false;
+    std::vector<std::pair<int, int>> excluded;
+};
+
+struct Model {
+    UnitSystem units;
+    double box[3] = {0, 0, 0};
+    std::vector<AtomTypeCfg> atoms;
+    std::vector<MoleculeCfg> molecules;
+    int ntypes = 0;
+    std::vector<PairParam> pairs; // ntypes*ntypes
+    CoulombModel coulomb;
+    EwaldModel ewald;
+    bool has_self_term = false;
+    double g2g_cutoff2 = std::numeric_limits<double>::max();
+    double maximum_allowed_energy = kInf;
+    bool anisotropic_form = true;
+    std::vector<std::string> term_names; // Hamiltonian order: [selfenergy], nonbonded, [ewald]
+    double pair_range = kInf; // distance beyond which every pair term vanishes
+
+    static Model from_json(const std::string& text);
+};
+
+} // namespace fmc

@@ -1,0 +1,108 @@
+// Device-side pair arithmetic shared by the per-move kernels and the
+// device-resident Metropolis kernels.  Formulas follow the reference's pair
+// potentials (potentials.h:42-49 LJ, 151-160 WCA, 203-207 hard sphere, 472-476
+// plain Coulomb, 591-598 CoulombGalore wrapper) and the orthogonal minimum image
+// (geometry.h:429-485).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+
+namespace fmc {
+
+struct Box {
+    double L[3], half[3];
+};
+
+__device__ __forceinline__ double min_image(double d, double L, double half) {
+    if (d > half) d -= L;
+    else if (d < -half) d += L;
+    return d;
+}
+
+__device__ __forceinline__ double dist2(const Box& b, double ax, double ay, double az, double bx, double by,
+                                        double bz) {
+    const double dx = min_image(ax - bx, b.L[0], b.half[0]);
+    const double dy = min_image(ay - by, b.L[1], b.half[1]);
+    const double dz = min_image(az - bz, b.L[2], b.half[2]);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// tabulate.h:184-202: interval by lower_bound, Horner from the top coefficient.
+template <class R>
+__device__ __forceinline__ R spline_eval(const R* __restrict__ knots, const R* __restrict__ coeff, int n, R x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (knots[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    int pos = lo - 1;
+    pos = pos < 0 ? 0 : (pos > n - 2 ? n - 2 : pos);
+    const R* c = coeff + 6 * pos;
+    const R z = x - knots[pos];
+    R s = R(0);
+    s = z * (s + c[5]);
+    s = z * (s + c[4]);
+    s = z * (s + c[3]);
+    s = z * (s + c[2]);
+    s = z * (s + c[1]);
+    return s + c[0];
+}
+
+// Coulomb short-range part of one pair (kT).
+__device__ __forceinline__ double coulomb_pair(const DevCoulomb& c, double qa, double qb, double r2) {
+    if (c.scheme == kCoulPM) return c.lB * qa * qb / sqrt(r2);
+    const double r = sqrt(r2) + 2.220446049250313e-16;
+    if (c.scheme == kCoulPlain) return c.lB * (qa * qb / r);
+    if (!(r < c.rc)) return 0.0;
+    const double q = r * c.rc_inv;
+    double s;
+    if (c.splined) s = spline_eval<double>(c.knots, c.coeff, c.nknots, q);
+    else if (c.scheme == kCoulEwald) s = erfc(c.alpha * c.rc * q);
+    else {
+        const double q5 = q * q * q * q * q;
+        s = 1.0 - 1.75 * q + 5.25 * q5 - 7.0 * q5 * q + 2.5 * q5 * q * q;
+    }
+    return c.lB * (qa * qb / r * s);
+}
+
+// Additive pair energy u = LJ + WCA + HS + Coulomb (absent terms have zero parameters).
+__device__ __forceinline__ double pair_energy(const DevPair& p, const DevCoulomb& c, double qa, double qb,
+                                              double r2) {
+    double u = 0.0;
+    if (p.lj_e4 != 0.0 && r2 < p.lj_cut2) {
+        double x = p.lj_s2 / r2;
+        x = x * x * x;
+        u = u + p.lj_e4 * (x * x - x);
+    }
+    if (p.wca_e4 != 0.0 && !(r2 > p.wca_s2 * 1.2599210498948732)) {
+        double x = p.wca_s2 / r2;
+        x = x * x * x;
+        u = u + p.wca_e4 * (x * x - x + 0.25);
+    }
+    if (r2 < p.hs_s2) u = u + __builtin_huge_val();
+    if (p.coul != 0.0) u = u + coulomb_pair(c, qa, qb, r2);
+    return u;
+}
+
+// Wavefront (64 lanes) sum with cross-lane shuffles, then across the block's waves.
+template <int kThreads>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kThreads / 64; ++w) s += lds[w];
+    }
+    return s; // valid in thread 0
+}
+
+__device__ __forceinline__ int unpack_n(int packed, int which) {
+    return static_cast<int>(static_cast<signed char>((packed >> (8 * which)) & 0xff));
+}
+
+} // namespace fmc

@@ -1,0 +1,188 @@
+// GPU energy engine for one Metropolis system: both MC states (accepted + trial
+// delta) resident in HBM, the nonbonded pair ΔU and the Ewald reciprocal ΔU
+// computed by hand-written gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+
+namespace fmc {
+
+constexpr int kMaxInlineMoved = 4;     // moved particles passed in the kernel arguments
+constexpr int kMaxMoved = 4096;        // moved particles per Change
+constexpr int kMaxKnots = 64;          // Coulomb S(q) spline knots
+constexpr int kMaxTypes = 16;          // atom types handled by the device kernels
+constexpr int kPhaseTableMoved = 8;    // moved particles with LDS phase tables (Ewald)
+constexpr int kBlock = 256;
+
+// Device copy of one type pair's parameters (64 bytes).
+struct DevPair {
+    double lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
+};
+
+// Device copy of the Coulomb model (scheme, constants, spline of S(q)).
+struct DevCoulomb {
+    int scheme;   // CoulombScheme
+    int splined;
+    int nknots;
+    int pad;
+    double lB, rc, rc_inv, alpha;
+    double knots[kMaxKnots];
+    double coeff[6 * kMaxKnots];
+};
+
+// One moved particle: absolute index, type/group, new and old coordinates (x,y,z,q).
+struct MovedParticle {
+    int idx, type, gid, pad;
+    double4 newp;
+    double4 oldp;
+};
+
+// Per-move kernel arguments (fits in the dispatch packet's kernarg segment).
+struct MoveArgs {
+    int m;                // moved particles
+    int change_group;     // group index of a single-group change, -1 for several groups
+    int internal;         // same-group pairs are part of the change (Change::GroupChange::internal)
+    int pad;
+    const MovedParticle* ext; // used when m > kMaxInlineMoved (device memory)
+    MovedParticle inl[kMaxInlineMoved];
+};
+
+// Results of one evaluation, written by the last block into pinned host memory.
+struct DeltaResult {
+    double pair_new, pair_old, rec_new, rec_old;
+    unsigned long long seq;
+    unsigned long long pad[3];
+};
+
+struct CellList {
+    bool enabled = false;
+    int nc[3] = {0, 0, 0};
+    int cap = 0;
+    double inv_cell[3] = {0, 0, 0};
+    int* count = nullptr;   // [ncell]
+    int* members = nullptr; // [ncell * cap]
+    int* cell_of = nullptr; // [N]
+    int* overflow = nullptr;
+};
+
+class Engine {
+  public:
+    explicit Engine(const std::string& config_json);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    const Model& model() const { return model_; }
+    int num_particles() const { return n_; }
+    int num_kvectors() const { return nk_; }
+    int num_terms() const { return static_cast<int>(model_.term_names.size()); }
+
+    void set_particles(int n, const double* xyz, const double* charge, const int* atom_id);
+    void set_groups(int ng, const int* begin, const int* size, const int* molid, const double* mass_center);
+    void init(); // k-vectors, A_k, full Q(k), cell list (Ewald::init + MetropolisMonteCarlo::init)
+
+    // Trial coordinates of the particles a move displaced (absolute indices).
+    void stage_trial(int m, const int* idx, const double* xyz);
+    // Energies of the affected pairs/terms of `change` in both states, per term in
+    // Hamiltonian order.  Writes the trial Q(k) (Ewald::updateState).
+    void delta(const std::vector<int>& groups, const std::vector<char>& all, const std::vector<char>& internal,
+               const std::vector<std::vector<int>>& rel, double* e_new, double* e_old);
+    // accepted.sync(trial) (accept) or trial.sync(accepted) (reject)
+    void sync(bool accept);
+    // Change::everything energies of the accepted state, per term.
+    void total_energy(double* terms);
+    // Full recompute of Q(k) from positions compared with the running Q(k):
+    // max |Q_run - Q_full| and relative difference of the reciprocal energy.
+    void check_q(double* max_abs_dq, double* rel_rec);
+    void get_positions(double* xyz) const;
+    hipStream_t stream() const { return stream_; }
+    // timing of the fused per-move kernel (HIP events on the engine's stream)
+    void set_timing(bool on);
+    void timing(double* total_ms, long* launches);
+    long launches() const { return launches_; }
+    bool cell_list_enabled() const { return cells_.enabled; }
+
+    // Pair-parameter and spline tables (as uploaded).
+    const std::vector<DevPair>& dev_pairs() const { return h_pairs_; }
+    const DevCoulomb& dev_coulomb() const { return h_coul_; }
+
+    // Raw device state for the device-resident MC drivers.
+    double4* d_pos() { return d_pos_; }
+    const uint32_t* d_meta() const { return d_meta_; }
+    double2* d_q_acc() { return d_q_acc_; }
+    const int* d_kn() const { return d_kn_; }
+    const double* d_ak() const { return d_ak_; }
+    const DevPair* d_pairs() const { return d_pairs_; }
+    const DevCoulomb* d_coul() const { return d_coul_; }
+    CellList& cells() { return cells_; }
+    double self_energy_of(int idx) const;
+    const std::vector<int>& group_of() const { return h_gid_; }
+    const std::vector<int>& group_begin() const { return g_begin_; }
+    const std::vector<int>& group_size() const { return g_size_; }
+    const std::vector<double>& host_charges() const { return h_q_; }
+    const std::vector<int>& host_types() const { return h_type_; }
+    void refresh_host_positions();
+
+  private:
+    Model model_;
+    int n_ = 0, nk_ = 0;
+    hipStream_t stream_ = nullptr;
+    int device_ = 0;
+    int num_cu_ = 0;
+    // particles (accepted state)
+    double4* d_pos_ = nullptr;
+    uint32_t* d_meta_ = nullptr; // type | group << 8
+    std::vector<double> h_xyz_, h_q_;
+    std::vector<int> h_type_, h_gid_;
+    // groups
+    std::vector<int> g_begin_, g_size_, g_molid_;
+    std::vector<double> g_cm_;
+    bool any_molecular_ = false;
+    // potentials
+    std::vector<DevPair> h_pairs_;
+    DevCoulomb h_coul_{};
+    DevPair* d_pairs_ = nullptr;
+    DevCoulomb* d_coul_ = nullptr;
+    // Ewald
+    int* d_kn_ = nullptr;      // packed (nx, ny, nz) as signed bytes
+    double* d_ak_ = nullptr;
+    double2* d_q_acc_ = nullptr;
+    double2* d_q_trial_ = nullptr;
+    double dipole_[3] = {0, 0, 0};
+    double dipole_trial_[3] = {0, 0, 0};
+    // reduction scratch
+    double2* d_partials_ = nullptr;
+    unsigned* d_ticket_ = nullptr;
+    int max_blocks_ = 0;
+    DeltaResult* h_result_ = nullptr; // pinned
+    MovedParticle* d_moved_ = nullptr;
+    std::vector<MovedParticle> staged_;
+    // cell list
+    CellList cells_;
+    // pending change (between delta and sync)
+    bool pending_ = false;
+    std::vector<MovedParticle> pending_moved_;
+    unsigned long long seq_ = 0;
+    // timing
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    std::vector<std::pair<int, int>> ev_pairs_;
+    long launches_ = 0;
+
+    void upload_potentials();
+    void build_kvectors();
+    void full_q(double2* out);
+    void build_cell_list();
+    void launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks);
+    double reciprocal_energy_host(const std::vector<double2>& q) const;
+    double surface_energy(const double* dipole) const;
+};
+
+void check_hip(hipError_t e, const char* what);
+
+} // namespace fmc

@@ -1,0 +1,148 @@
+/*
+ * faunus_amd — C ABI of the MI355X energy engine for Faunus's Metropolis MC.
+ *
+ * The engine replaces the reference's nonbonded and Ewald energy terms behind
+ * the Energy::EnergyTerm / Hamiltonian surface (reference: src/externalpotential.h:21-43,
+ * src/energy.h:1954-1977).  Plain pointers and sizes only; every call returns 0 on
+ * success and a non-zero status on error, with the message in fmc_last_error()
+ * (thread-local).  No exception crosses this boundary.  +inf / NaN energies are
+ * returned as values, exactly as the reference's terms return them.
+ *
+ * Energies are in kT, lengths in angstrom, charges in e.  Term energies are
+ * reported in Hamiltonian order: [selfenergy], nonbonded, [ewald]
+ * (src/energy.cpp:1171-1211; self energy is added before the nonbonded term by
+ * PairEnergy::addPairPotentialSelfEnergy, src/energy.h:462-468).
+ */
+#ifndef FAUNUS_AMD_H
+#define FAUNUS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fmc_engine fmc_engine;
+
+#define FMC_STATE_ACCEPTED 0
+#define FMC_STATE_TRIAL 1
+
+/*
+ * Flat form of Faunus::Change (src/space.h:29-75): flags plus a list of group
+ * changes; relative_atom_indices of group g are indices[offsets[g] .. offsets[g+1]).
+ */
+typedef struct fmc_change {
+    int everything;
+    int volume_change;
+    int matter_change;
+    int num_groups;
+    const int* group_index;         /* Change::GroupChange::group_index */
+    const unsigned char* all;       /* Change::GroupChange::all */
+    const unsigned char* internal;  /* Change::GroupChange::internal */
+    const int* offsets;             /* num_groups + 1 */
+    const int* indices;             /* relative_atom_indices, sorted per group */
+} fmc_change;
+
+/* Thread-local message of the last failed call. */
+const char* fmc_last_error(void);
+
+/* Number of visible HIP devices (0 means the engine cannot run). */
+int fmc_device_count(int* count);
+
+/*
+ * Build an engine from the reference's input JSON: {"temperature", "geometry",
+ * "atomlist", "moleculelist", "energy": [...]} — the same keys the reference parses in
+ * Hamiltonian::Hamiltonian (src/energy.cpp:1171-1211), Hamiltonian::createEnergy
+ * (1280-1327), EwaldData(json) (28-59) and NewCoulombGalore::from_json
+ * (src/potentials.cpp:1628-1699).  Replaces constructing the Nonbonded and Ewald
+ * EnergyTerms.  Unsupported keys or terms are an error.
+ */
+int fmc_create(const char* config_json, fmc_engine** out);
+int fmc_destroy(fmc_engine* engine);
+
+/* Space::particles (src/particle.h:220-227): positions (3n), charges, atom ids. */
+int fmc_set_particles(fmc_engine* engine, int n, const double* xyz, const double* charge, const int* atom_id);
+
+/* Space::groups (src/group.h:177-194): contiguous ranges in particle order. */
+int fmc_set_groups(fmc_engine* engine, int num_groups, const int* begin, const int* size, const int* molid,
+                   const double* mass_center);
+
+/*
+ * MetropolisMonteCarlo::init (src/montecarlo.cpp:43-71) / Ewald::init
+ * (src/energy.cpp:554-558): k-vectors, A_k, full Q(k); both states aligned.
+ * Writes the initial total energy (Change::everything) if initial_energy != NULL.
+ */
+int fmc_init(fmc_engine* engine, double* initial_energy);
+
+int fmc_num_terms(fmc_engine* engine, int* count);
+int fmc_term_name(fmc_engine* engine, int index, char* buffer, int length);
+
+/*
+ * Trial coordinates of the particles a move displaced (absolute indices, 3 doubles
+ * each) — the part of the trial Space that Move::move (src/move.cpp:65-76) mutated.
+ */
+int fmc_stage_trial(fmc_engine* engine, int count, const int* particle_index, const double* xyz);
+
+/*
+ * Ewald::updateState (src/energy.cpp:564-575) followed by Hamiltonian::energy(Change)
+ * (src/energy.cpp:1227-1241) on both states in one launch: per-term energies of the
+ * affected pairs/terms in the trial state (new_terms) and accepted state (old_terms).
+ * This is the fused form of the two energy() calls in
+ * MetropolisMonteCarlo::performMove (src/montecarlo.cpp:153-157).
+ */
+int fmc_delta_u(fmc_engine* engine, const fmc_change* change, double* new_terms, double* old_terms);
+
+/* Hamiltonian::energy(Change) of one state (FMC_STATE_TRIAL or FMC_STATE_ACCEPTED). */
+int fmc_energy(fmc_engine* engine, const fmc_change* change, int state, double* terms, double* total);
+
+/*
+ * State::sync after the Metropolis decision (src/montecarlo.cpp:170-176,
+ * src/energy.cpp:640-658, src/space.cpp:199-239): accept != 0 copies the trial state
+ * into the accepted state, accept == 0 restores the trial state.
+ */
+int fmc_sync(fmc_engine* engine, int accept);
+
+/* Hamiltonian::energy with Change::everything on the accepted state, per term. */
+int fmc_total_energy(fmc_engine* engine, double* terms, double* total);
+
+/*
+ * Drift check: rebuild Q(k) from positions (Ewald full update, src/energy.cpp:191-206)
+ * and compare it with the running Q(k): max |dQ| and relative reciprocal-energy difference.
+ */
+int fmc_check_q(fmc_engine* engine, double* max_abs_dq, double* rel_reciprocal);
+
+/* Accepted-state positions (3n doubles). */
+int fmc_get_positions(fmc_engine* engine, double* xyz);
+
+/*
+ * The reference's Metropolis step (MetropolisMonteCarlo::performMove,
+ * src/montecarlo.cpp:139-187 with getEnergyChange 193-209 and metropolisCriterion
+ * 17-34) for a stream of single-particle translations, one GPU evaluation per move.
+ * Inputs per move: group index, index within the group, new position (3), uniform
+ * deviate of the acceptance test.  Outputs per move: du (new-old, before the
+ * reject reset), accepted flag and per-term energies of both states (may be NULL).
+ */
+int fmc_replay_translate(fmc_engine* engine, long count, const int* group, const int* index, const double* xyz,
+                         const double* uniform, double* du, unsigned char* accepted, double* new_terms,
+                         double* old_terms);
+
+/*
+ * Same Metropolis step for translations given as displacements from the current
+ * accepted position: new = boundary(old + displacement) as in
+ * AtomicTranslateRotate::translateParticle (src/move.cpp:225-240) and
+ * Cuboid::boundary (src/geometry.cpp:91-104).  One GPU evaluation per move.
+ */
+int fmc_run_translate(fmc_engine* engine, long count, const int* group, const int* index, const double* displacement,
+                      const double* uniform, double* du, unsigned char* accepted);
+
+/* Kernel timing of the per-move launch (HIP events on the engine's stream). */
+int fmc_set_timing(fmc_engine* engine, int enable);
+int fmc_get_timing(fmc_engine* engine, double* total_ms, long* launches);
+
+/* Andrea spline of the Coulomb short-range function built by the engine (for tests). */
+int fmc_coulomb_spline(fmc_engine* engine, int capacity, double* knots, double* coeff, int* num_knots);
+int fmc_num_kvectors(fmc_engine* engine, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FAUNUS_AMD_H */
