@@ -1,16 +1,19 @@
 """Benchmark of the Metropolis energy path on one MI355X.
 
-Workload (default): N = 20 000 Na/Cl ions (q=+-1, LJ sigma 3.3/4.4 A, eps 0.5/0.4
-kJ/mol, Lorentz-Berthelot, LJ truncated at 12 A), cubic L = 255 A, Ewald alpha=0.25,
-Rc=12 A, kmax=20 (K=17 328), single-particle translations dp = 2 A.  A step is
-STEP_MOVES trial moves; the metric is trial moves per second.
+Headline workload (N=1 line): N = 20 000 Na/Cl ions (q=+-1, LJ sigma 3.3/4.4 A,
+eps 0.5/0.4 kJ/mol, Lorentz-Berthelot, LJ truncated at 12 A), cubic L = 255 A,
+Ewald alpha=0.25 A^-1, Rc=12 A, kmax=20 (K=17 328), single-particle translations
+dp = 2 A, FP64, device-resident Metropolis (one persistent launch per step).
+A step is STEP_MOVES trial moves; the metric is trial moves per second.
 
-Prints one JSON line (rank 0) with the roofline of the dominant kernel (HIP
-events on the engine's stream) and a CPU baseline (the FP64 oracle, one core, on a
-bounded sample of the same workload).
+Also measured and reported beside it: the plain FP64 one-move-per-launch path on
+the same system, the FP32-pair form, the 256-replica 2:1-salt mix (8 compositions
+x 32-rung temperature ladder, FP32 pairs / FP64 Q(k), and FP64) with the plain path
+on the same mix, and a CPU baseline (FP64 oracle, one core, bounded sample).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -21,7 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STEP_MOVES = 1000
+FP64_PEAK_TFLOPS = 78.6
+STEP_MOVES = 20000
+PLAIN_MOVES = 2000
 
 
 def log(*a):
@@ -33,63 +38,132 @@ def nacl_system(seed=1):
     return systems.nacl_lj(n_pairs=10000, box=255.0, seed=seed)
 
 
-def translate_proposals(n_moves, n_particles, dp, rng):
-    """Random particle, random unit vector (uniform on the sphere) times dp*U(0,1)."""
-    idx = rng.integers(0, n_particles, size=n_moves).astype(np.int32)
-    v = rng.normal(size=(n_moves, 3))
-    v /= np.linalg.norm(v, axis=1)[:, None]
-    disp = v * (dp * rng.random(n_moves))[:, None]
-    u = rng.random(n_moves)
-    return idx, disp, u
+def ewald_flops_per_move(K, m=1):
+    """Algorithmic flops of the Ewald dU of one move: 2mK(5+2) + 8K plus 2mK sincos
+    counted as 20 flops each."""
+    return 2 * m * K * 7 + 8 * K + 2 * m * K * 20
 
 
-def cell_candidates(eng_box, n, cutoff):
-    nc = int(eng_box // cutoff)
-    return 27.0 * n / nc ** 3
+def pair_bytes_cells(n, box, cutoff):
+    nc = int(box // cutoff)
+    return 27.0 * n / nc ** 3 * 36 + 2 * 27 * 8
 
 
-def bench_plain(args):
+def bench_plain_single(sysd, moves, seed=7):
     import faunus_amd as F
-    cfg, xyz, q, ids, g = nacl_system()
+    cfg, xyz, q, ids, g = sysd
     eng = F.Engine(cfg)
     eng.set_particles(xyz, q, ids)
     eng.set_groups(g["begin"], g["size"], g["molid"])
     eng.init()
-    K = eng.num_kvectors
+    rng = np.random.default_rng(seed)
     n = len(q)
-    rng = np.random.default_rng(7)
-    total = (args.warmup + args.steps) * STEP_MOVES
-    idx, disp, u = translate_proposals(total, n, 2.0, rng)
-    grp = np.zeros(total, np.int32)
-    # warmup
-    w = args.warmup * STEP_MOVES
-    if w:
-        eng.run_translate(grp[:w], idx[:w], disp[:w], u[:w])
+    dp = cfg["atomlist"][0][next(iter(cfg["atomlist"][0]))]["dp"]
+    idx = rng.integers(0, n, size=moves).astype(np.int32)
+    v = rng.normal(size=(moves, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    disp = v * (dp * rng.random(moves))[:, None]
+    u = rng.random(moves)
+    grp = np.zeros(moves, np.int32)
+    w = min(200, moves // 5)
+    eng.run_translate(grp[:w], idx[:w], disp[:w], u[:w])
     eng.set_timing(True)
     t0 = time.perf_counter()
     du, acc = eng.run_translate(grp[w:], idx[w:], disp[w:], u[w:])
-    t1 = time.perf_counter()
+    dt = time.perf_counter() - t0
     kern_ms, launches = eng.timing()
-    eng.set_timing(False)
+    eng.close()
+    return dict(rate=(moves - w) / dt, acceptance=float(acc.mean()), kernel_us=kern_ms * 1e3 / max(1, launches))
+
+
+def bench_device_single(sysd, args, fp32):
+    import faunus_amd as F
+    b = F.Batch(fp32_pairs=fp32)
+    b.add(sysd, seed=2024)
+    b.set_timing(True)
+    for _ in range(args.warmup):
+        b.run([STEP_MOVES])
+    b.sync()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run([STEP_MOVES])
+        kernel_ms += b.sync()
+    dt = time.perf_counter() - t0
+    info = b.info(0)
+    tot, _ = b.total_energy(0)
+    dq, drec = b.check_q(0)
+    b.close()
     moves = args.steps * STEP_MOVES
-    rate = moves / (t1 - t0)
-    avg_kernel_s = kern_ms / 1e3 / max(launches, 1)
-    n_cand = cell_candidates(255.0, n, 12.0)
-    bytes_pair = n_cand * 36 + 2 * 27 * 8
-    bytes_ewald = 64.0 * K
-    algo_bytes = bytes_pair + bytes_ewald
-    achieved = algo_bytes / avg_kernel_s / 1e9
-    dq, drec = eng.check_q()
-    log(f"plain: {rate:.0f} moves/s, acceptance {acc.mean():.3f}, kernel {avg_kernel_s*1e6:.2f} us/launch, "
-        f"{achieved:.1f} GB/s algorithmic, Q drift {dq:.2e}")
-    return dict(rate=rate, ms_per_step=(t1 - t0) * 1e3 / args.steps, acceptance=float(acc.mean()),
-                kernel_us=avg_kernel_s * 1e6, achieved=achieved, algo_bytes=algo_bytes, K=K, n=n,
-                q_drift=dq)
+    return dict(rate=moves / dt, ms_per_step=dt * 1e3 / args.steps, kernel_ms=kernel_ms, moves=moves,
+                acceptance=info["accepted"] / max(1, info["moves"]),
+                drift=(info["energy"] - tot) / abs(tot), q_drift=dq)
+
+
+def salt_mix(n_comp=8, rungs=32, seed=1):
+    """8 compositions x 32 rungs: N ~ U[600, 9000] (multiple of 3), c ~ logU[0.01, 1] M."""
+    from tests import systems
+    rng = np.random.default_rng(seed)
+    comps = []
+    for c in range(n_comp):
+        n = int(rng.integers(200, 3001)) * 3
+        conc = float(10 ** rng.uniform(-2, 0))
+        comps.append((n, conc, systems.salt21(n=n, conc=conc, seed=100 + c)))
+    temps = np.linspace(280.0, 360.0, rungs)
+    return comps, temps
+
+
+def bench_batch(comps, temps, fp32, args, budget_moves):
+    import faunus_amd as F
+    b = F.Batch(fp32_pairs=fp32)
+    tref = 298.15
+    cost = []
+    for ci, (n, conc, sysd) in enumerate(comps):
+        for ti, T in enumerate(temps):
+            b.add(sysd, beta=tref / T, seed=10_000 + ci * 100 + ti, ladder=ci)
+            cost.append(n + 1152.0)
+    cost = np.array(cost)
+    # equal wall time per replica per launch: moves inversely proportional to the per-move cost
+    moves = np.maximum(1, np.round(budget_moves * cost.mean() / cost)).astype(np.int64)
+    b.set_timing(True)
+    b.run(moves)
+    b.sync()
+    sweeps = np.zeros(len(comps))
+    kernel_ms = 0.0
+    swaps = (0, 0)
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        b.run(moves)
+        kernel_ms += b.sync()
+        swaps = b.swap(seed=4242, rnd=step)
+    dt = time.perf_counter() - t0
+    total_moves = int(moves.sum()) * args.steps
+    # per-move drift check on a few replicas
+    drifts = []
+    for r in (0, len(cost) // 2, len(cost) - 1):
+        tot, _ = b.total_energy(r)
+        drifts.append(abs(b.info(r)["energy"] - tot) / abs(tot))
+    for ci, (n, _, _) in enumerate(comps):
+        sweeps[ci] = moves[ci * len(temps)] * args.steps / n
+    b.close()
+    return dict(rate=total_moves / dt, kernel_ms=kernel_ms, seconds=dt, moves=total_moves, swaps=swaps,
+                max_drift=float(max(drifts)), sweeps_per_composition=[round(float(s), 1) for s in sweeps])
+
+
+def bench_plain_mix(comps, per_replica_moves=300):
+    """Plain FP64 one-move-per-launch path on replicas of the same mix (one engine per
+    composition at T_ref; the per-move cost does not depend on T)."""
+    total, t = 0, 0.0
+    for n, conc, sysd in comps:
+        r = bench_plain_single(sysd, per_replica_moves + 60)
+        total += per_replica_moves
+        t += per_replica_moves / r["rate"]
+    return total / t
 
 
 def cpu_baseline(seconds=15.0):
-    """FP64 oracle (reference Release flags), 1 core, single-particle translations on
-    the same N = 20 000 system; bounded sample of moves."""
+    """FP64 oracle (the reference's Release flags), one core, single-particle
+    translations on the N = 20 000 system; bounded sample of moves."""
     from oracle import oracle as O
     cfg, xyz, q, ids, g = nacl_system()
     s = O.System(cfg, fast=True)
@@ -104,49 +178,98 @@ def cpu_baseline(seconds=15.0):
     s.run_translate(nm, seed=5)
     dt = time.perf_counter() - t0
     return dict(value=nm / dt, unit="trial moves/s", cores=1, kind="port",
-                sample=f"{nm} single-particle translations of the N=20000 system "
-                       f"(oracle FP64 restatement, all-pairs as in the reference, 1 thread, {dt:.1f} s)")
+                sample=f"{nm} single-particle translations of the N=20000 system (FP64 oracle, all pairs as in "
+                       f"the reference, no cell list, 1 thread, {dt:.1f} s)")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
+    ap.add_argument("--only", choices=["device", "plain", "batch"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
-              "in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
+              "GPUs is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     import faunus_amd as F
     if F.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    res = bench_plain(args)
+    sysd = nacl_system()
+    K = 17328
+    n = 20000
+    out = {}
+    if args.only in (None, "plain"):
+        out["plain"] = bench_plain_single(sysd, PLAIN_MOVES)
+        log("plain FP64 one move per launch: %.0f moves/s, kernel %.1f us" % (out["plain"]["rate"], out["plain"]["kernel_us"]))
+    if args.only in (None, "device"):
+        out["device64"] = bench_device_single(sysd, args, fp32=False)
+        log("device-resident FP64: %.0f moves/s (drift %.2e)" % (out["device64"]["rate"], out["device64"]["drift"]))
+        out["device32"] = bench_device_single(sysd, args, fp32=True)
+        log("device-resident FP32-pair: %.0f moves/s" % out["device32"]["rate"])
+    if args.only in (None, "batch") and not args.no_batch:
+        comps, temps = salt_mix()
+        budget = 4000
+        out["batch32"] = bench_batch(comps, temps, True, args, budget)
+        log("batched FP32-pair/FP64-Q: %.3e replica-moves/s" % out["batch32"]["rate"])
+        out["batch64"] = bench_batch(comps, temps, False, args, budget)
+        log("batched FP64: %.3e replica-moves/s" % out["batch64"]["rate"])
+        out["plain_mix"] = bench_plain_mix(comps)
+        log("plain FP64 one move per launch on the mix: %.0f replica-moves/s" % out["plain_mix"])
+    if args.only is not None:
+        print(json.dumps({k: v for k, v in out.items()}, default=float))
+        return
     cpu = None if args.no_cpu else cpu_baseline()
+    d = out["device64"]
+    kernel_s_per_move = d["kernel_ms"] / 1e3 / d["moves"]
+    flops = ewald_flops_per_move(K)
+    algo_bytes = 64.0 * K + pair_bytes_cells(n, 255.0, 12.0)
+    achieved_gbs = algo_bytes / kernel_s_per_move / 1e9
+    achieved_tf = flops / kernel_s_per_move / 1e12
     line = {
         "metric": "trial moves/s at N=20k Ewald",
-        "value": res["rate"],
+        "value": d["rate"],
         "unit": "trial moves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["ms_per_step"],
+        "ms_per_step": d["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (random sequential addition, seed 1)",
-        "config": {"workload": "N=20000 Na/Cl LJ(cut 12)+Ewald(alpha 0.25, Rc 12, kmax 20, K=%d), L=255, "
-                               "translate dp=2, plain FP64 one move per launch" % res["K"],
-                   "step_moves": STEP_MOVES, "acceptance": res["acceptance"]},
-        "roofline": {"bound": "hbm", "achieved": res["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": res["achieved"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "delta_kernel", "kernel_us": res["kernel_us"],
-                     "algorithmic_bytes_per_launch": res["algo_bytes"]},
+        "data": "synthetic (random sequential addition, seed 1; device Philox proposals)",
+        "config": {"workload": "N=20000 Na/Cl LJ(LB, cut 12 A)+Ewald(alpha 0.25, Rc 12, kmax 20, K=17328), "
+                               "L=255 A, translate dp=2 A, device-resident Metropolis, one workgroup",
+                   "step_moves": STEP_MOVES, "acceptance": d["acceptance"], "energy_drift": d["drift"]},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "mc_advance_kernel",
+                     "algorithmic_bytes_per_move": algo_bytes, "kernel_us_per_move": kernel_s_per_move * 1e6,
+                     "fp64_tflops": achieved_tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                     "algorithmic_flops_per_move": flops,
+                     "note": "one replica = one workgroup on one of 256 CUs; Q(k) and k-vectors stay in "
+                             "registers/LDS, so HBM traffic is far below the algorithmic bytes"},
         "cpu_baseline": cpu,
+        "forms": {
+            "plain_fp64_one_move_per_launch": {"moves_per_s": out["plain"]["rate"],
+                                               "kernel_us": out["plain"]["kernel_us"]},
+            "device_fp64": {"moves_per_s": d["rate"]},
+            "device_fp32_pairs": {"moves_per_s": out["device32"]["rate"]},
+        },
     }
+    if "batch32" in out:
+        line["forms"]["batched_256_replicas"] = {
+            "fp32_pairs_fp64_q_replica_moves_per_s": out["batch32"]["rate"],
+            "fp64_replica_moves_per_s": out["batch64"]["rate"],
+            "plain_fp64_replica_moves_per_s": out["plain_mix"],
+            "speedup_fp32_batched_vs_plain": out["batch32"]["rate"] / out["plain_mix"],
+            "swaps_attempted_accepted": out["batch32"]["swaps"],
+            "sweeps_per_composition": out["batch32"]["sweeps_per_composition"],
+            "max_energy_drift": out["batch32"]["max_drift"]}
     print(json.dumps(line), flush=True)
 
 

@@ -44,6 +44,21 @@ SIGNATURES = {
     "fmc_run_translate": (C.c_int, [C.c_void_p, C.c_long, I, I, D, D, D, U8]),
     "fmc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmc_get_timing": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_long)]),
+    "fmc_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "fmc_batch_destroy": (C.c_int, [C.c_void_p]),
+    "fmc_batch_add_replica": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_double,
+                                        C.c_ulonglong, C.c_int, I]),
+    "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
+    "fmc_batch_sync": (C.c_int, [C.c_void_p, D]),
+    "fmc_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_batch_swap": (C.c_int, [C.c_void_p, C.c_ulonglong, C.c_ulonglong, C.POINTER(C.c_longlong),
+                                 C.POINTER(C.c_longlong)]),
+    "fmc_batch_replica_info": (C.c_int, [C.c_void_p, C.c_int, D, D, C.POINTER(C.c_ulonglong),
+                                         C.POINTER(C.c_ulonglong), I]),
+    "fmc_batch_positions": (C.c_int, [C.c_void_p, C.c_int, D]),
+    "fmc_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, I, U8, D, D]),
+    "fmc_batch_total_energy": (C.c_int, [C.c_void_p, C.c_int, D, D]),
+    "fmc_batch_check_q": (C.c_int, [C.c_void_p, C.c_int, D, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
 }
@@ -250,3 +265,89 @@ class Engine:
         n = C.c_int()
         _check(lib().fmc_num_kvectors(self.h, C.byref(n)))
         return n.value
+
+
+class Batch:
+    """Replicas advanced by the device-resident Metropolis kernel (one workgroup each)."""
+
+    def __init__(self, fp32_pairs=False):
+        h = C.c_void_p()
+        _check(lib().fmc_batch_create(int(bool(fp32_pairs)), C.byref(h)))
+        self.h = h
+        self.sizes = []
+        self.nterms = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, system, beta=1.0, seed=1, ladder=-1):
+        cfg, xyz, q, ids, g = system
+        text = cfg if isinstance(cfg, str) else json.dumps(cfg)
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        q = np.ascontiguousarray(q, np.float64)
+        ids = np.ascontiguousarray(ids, np.int32)
+        gb = np.ascontiguousarray(g["begin"], np.int32)
+        gs = np.ascontiguousarray(g["size"], np.int32)
+        gm = np.ascontiguousarray(g["molid"], np.int32)
+        r = C.c_int()
+        _check(lib().fmc_batch_add_replica(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
+                                           _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), float(beta),
+                                           int(seed), int(ladder), C.byref(r)))
+        self.sizes.append(len(q))
+        return r.value
+
+    def run(self, moves, trace=0):
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(moves, np.int64), (len(self.sizes),)), np.int64)
+        _check(lib().fmc_batch_run(self.h, m.ctypes.data_as(C.POINTER(C.c_longlong)), int(trace)))
+
+    def sync(self):
+        ms = C.c_double()
+        _check(lib().fmc_batch_sync(self.h, C.byref(ms)))
+        return ms.value
+
+    def set_timing(self, on):
+        _check(lib().fmc_batch_set_timing(self.h, int(bool(on))))
+
+    def swap(self, seed, rnd):
+        a, b = C.c_longlong(), C.c_longlong()
+        _check(lib().fmc_batch_swap(self.h, int(seed), int(rnd), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def info(self, r):
+        e, b = C.c_double(), C.c_double()
+        m, a = C.c_ulonglong(), C.c_ulonglong()
+        err = C.c_int()
+        _check(lib().fmc_batch_replica_info(self.h, r, C.byref(e), C.byref(b), C.byref(m), C.byref(a), C.byref(err)))
+        return dict(energy=e.value, beta=b.value, moves=m.value, accepted=a.value, error=err.value)
+
+    def positions(self, r):
+        xyz = np.zeros((self.sizes[r], 3))
+        _check(lib().fmc_batch_positions(self.h, r, _p(xyz)))
+        return xyz
+
+    def trace(self, r, count):
+        idx = np.zeros(count, np.int32)
+        acc = np.zeros(count, np.uint8)
+        du = np.zeros(count)
+        xyz = np.zeros((count, 3))
+        _check(lib().fmc_batch_trace(self.h, r, count, _p(idx, C.c_int), _p(acc, C.c_ubyte), _p(du), _p(xyz)))
+        return dict(index=idx, accepted=acc, du=du, xyz=xyz)
+
+    def total_energy(self, r, nterms=3):
+        t = np.zeros(8)
+        tot = C.c_double()
+        _check(lib().fmc_batch_total_energy(self.h, r, _p(t), C.byref(tot)))
+        return tot.value, t
+
+    def check_q(self, r):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fmc_batch_check_q(self.h, r, C.byref(a), C.byref(b)))
+        return a.value, b.value

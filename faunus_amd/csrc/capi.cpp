@@ -291,3 +291,130 @@ int fmc_num_kvectors(fmc_engine* e, int* count) {
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- batches
+#include "mc_batch.hpp"
+
+struct fmc_batch {
+    std::unique_ptr<fmc::Batch> impl;
+};
+
+namespace {
+fmc::Batch& bat(fmc_batch* b) {
+    if (!b || !b->impl) throw std::runtime_error("null batch handle");
+    return *b->impl;
+}
+} // namespace
+
+extern "C" {
+
+int fmc_batch_create(int fp32_pairs, fmc_batch** out) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw std::runtime_error("no HIP device visible");
+        auto* b = new fmc_batch;
+        b->impl = std::make_unique<fmc::Batch>(fp32_pairs);
+        *out = b;
+    });
+}
+
+int fmc_batch_destroy(fmc_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+int fmc_batch_add_replica(fmc_batch* b, const char* config_json, int n, const double* xyz, const double* charge,
+                          const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                          double beta, unsigned long long seed, int ladder, int* replica) {
+    return guarded([&] {
+        const int r = bat(b).add_replica(config_json, n, xyz, charge, atom_id, num_groups, begin, size, molid, beta,
+                                         seed, ladder);
+        if (replica) *replica = r;
+    });
+}
+
+int fmc_batch_run(fmc_batch* b, const long long* moves, int trace_capacity) {
+    return guarded([&] {
+        std::vector<long long> m(moves, moves + bat(b).size());
+        bat(b).run(m, trace_capacity > 0, trace_capacity);
+    });
+}
+
+int fmc_batch_sync(fmc_batch* b, double* kernel_ms) {
+    return guarded([&] {
+        bat(b).sync();
+        if (kernel_ms) *kernel_ms = bat(b).last_kernel_ms();
+    });
+}
+
+int fmc_batch_set_timing(fmc_batch* b, int enable) {
+    return guarded([&] { bat(b).set_timing(enable != 0); });
+}
+
+int fmc_batch_swap(fmc_batch* b, unsigned long long seed, unsigned long long round, long long* attempted,
+                   long long* accepted) {
+    return guarded([&] {
+        bat(b).swap_ladders(seed, round);
+        if (attempted) *attempted = bat(b).swaps_attempted();
+        if (accepted) *accepted = bat(b).swaps_accepted();
+    });
+}
+
+int fmc_batch_replica_info(fmc_batch* b, int replica, double* energy, double* beta, unsigned long long* moves,
+                           unsigned long long* accepted, int* error) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        const fmc::ReplicaState s = bat(b).state(replica);
+        if (energy) *energy = s.energy;
+        if (beta) *beta = s.beta;
+        if (moves) *moves = s.moves;
+        if (accepted) *accepted = s.accepted;
+        if (error) *error = s.error;
+    });
+}
+
+int fmc_batch_positions(fmc_batch* b, int replica, double* xyz) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        bat(b).positions(replica, xyz);
+    });
+}
+
+int fmc_batch_trace(fmc_batch* b, int replica, int count, int* index, unsigned char* accepted, double* du,
+                    double* xyz) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        auto t = bat(b).trace(replica, count);
+        for (int k = 0; k < count; ++k) {
+            const auto& e = t[static_cast<size_t>(k)];
+            if (index) index[k] = e.index;
+            if (accepted) accepted[k] = static_cast<unsigned char>(e.accepted);
+            if (du) du[k] = e.du;
+            if (xyz) {
+                xyz[3 * k] = e.x;
+                xyz[3 * k + 1] = e.y;
+                xyz[3 * k + 2] = e.z;
+            }
+        }
+    });
+}
+
+int fmc_batch_total_energy(fmc_batch* b, int replica, double* terms, double* total) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        bat(b).sync();
+        fmc::Engine& e = bat(b).engine(replica);
+        e.refresh_host_positions();
+        e.total_energy(terms);
+        if (total) *total = sum_terms(terms, e.num_terms());
+    });
+}
+
+int fmc_batch_check_q(fmc_batch* b, int replica, double* max_abs_dq, double* rel_reciprocal) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        bat(b).sync();
+        bat(b).engine(replica).check_q(max_abs_dq, rel_reciprocal);
+    });
+}
+
+} // extern "C"

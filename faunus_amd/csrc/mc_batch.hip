@@ -1,0 +1,703 @@
+// Device-resident Metropolis kernels: one workgroup advances one replica through
+// many trial moves (the reference's MetropolisMonteCarlo::sweep/performMove loop,
+// montecarlo.cpp:139-227, with AtomicTranslateRotate proposals, move.cpp:225-293),
+// computing the nonbonded pair dU and the Ewald reciprocal dU in the same pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "device_math.hpp"
+#include "mc_batch.hpp"
+#include "philox.hpp"
+
+namespace fmc {
+namespace {
+
+constexpr int kMaxGk = 1025; // |n|^2 table (ncut <= 32)
+
+struct DevPairF {
+    float lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
+};
+
+__device__ __forceinline__ float dist2f(float L, float half, float ax, float ay, float az, float bx, float by,
+                                        float bz) {
+    float dx = ax - bx, dy = ay - by, dz = az - bz;
+    dx = dx > half ? dx - L : (dx < -half ? dx + L : dx);
+    dy = dy > half ? dy - L : (dy < -half ? dy + L : dy);
+    dz = dz > half ? dz - L : (dz < -half ? dz + L : dz);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// FP32 pair energy (same algebra as pair_energy, single precision).
+__device__ __forceinline__ float pair_energy_f(const DevPairF& p, const DevCoulomb& c, const float* knots,
+                                               const float* coeff, float qa, float qb, float r2) {
+    float u = 0.0f;
+    if (p.lj_e4 != 0.0f && r2 < p.lj_cut2) {
+        float x = p.lj_s2 / r2;
+        x = x * x * x;
+        u = u + p.lj_e4 * (x * x - x);
+    }
+    if (p.wca_e4 != 0.0f && !(r2 > p.wca_s2 * 1.2599210498948732f)) {
+        float x = p.wca_s2 / r2;
+        x = x * x * x;
+        u = u + p.wca_e4 * (x * x - x + 0.25f);
+    }
+    if (r2 < p.hs_s2) u = u + __builtin_huge_valf();
+    if (p.coul != 0.0f) {
+        const float lB = static_cast<float>(c.lB);
+        if (c.scheme == kCoulPM) {
+            u = u + lB * qa * qb / sqrtf(r2);
+        } else {
+            const float r = sqrtf(r2);
+            if (c.scheme == kCoulPlain) u = u + lB * (qa * qb / r);
+            else if (r < static_cast<float>(c.rc)) {
+                const float qq = r * static_cast<float>(c.rc_inv);
+                float s;
+                if (c.splined) s = spline_eval<float>(knots, coeff, c.nknots, qq);
+                else if (c.scheme == kCoulEwald) s = erfcf(static_cast<float>(c.alpha * c.rc) * qq);
+                else {
+                    const float q5 = qq * qq * qq * qq * qq;
+                    s = 1.0f - 1.75f * qq + 5.25f * q5 - 7.0f * q5 * qq + 2.5f * q5 * qq * qq;
+                }
+                u = u + lB * (qa * qb / r * s);
+            }
+        }
+    }
+    return u;
+}
+
+__device__ __forceinline__ int cell_of_coord(double x, double half, double inv_cell, int nc) {
+    int c = static_cast<int>(floor((x + half) * inv_cell));
+    c %= nc;
+    if (c < 0) c += nc;
+    return c;
+}
+
+__device__ __forceinline__ double wrap(double x, double L, double half, double inv) {
+    // Cuboid::boundary (geometry.cpp:91-104): x -= L * anint(x / L) when |x| > L/2
+    if (fabs(x) > half) {
+        const double s = x * inv;
+        x -= L * static_cast<double>(static_cast<int>(s > 0.0 ? s + 0.5 : s - 0.5));
+    }
+    return x;
+}
+
+// getEnergyChange (montecarlo.cpp:193-209) on the split totals
+__device__ __forceinline__ double energy_change(double pair_new, double pair_old, double drec) {
+    const double e_new = pair_new + drec; // the reciprocal term enters as a difference
+    if (isnan(pair_old) && !isnan(e_new)) return -__builtin_huge_val();
+    if (isnan(e_new)) return __builtin_huge_val();
+    if (e_new > 0.0 && isinf(e_new)) return __builtin_huge_val();
+    const double du = (pair_new - pair_old) + drec;
+    if (isnan(du)) return 0.0;
+    return du;
+}
+
+// metropolisCriterion (montecarlo.cpp:17-34)
+__device__ __forceinline__ bool metropolis(double du, double u) {
+    if (isnan(du)) return false;
+    if (isinf(du) && du < 0.0) return true;
+    if (-du > 709.782712893384) return true;
+    return u <= exp(-du);
+}
+
+template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
+__global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __restrict__ states,
+                                                               const long long* __restrict__ moves,
+                                                               TraceEntry* __restrict__ trace, int trace_cap) {
+    ReplicaState& S = states[blockIdx.x];
+    const long long M = moves[blockIdx.x];
+    if (M <= 0 || S.error) return;
+    __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
+    __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
+    __shared__ float s_knots_f[kF32 ? kMaxKnots : 1];
+    __shared__ float s_coeff_f[kF32 ? 6 * kMaxKnots : 1];
+    __shared__ DevCoulomb s_coul;
+    __shared__ double s_dp[kMaxTypes];
+    __shared__ double s_gk[kMaxGk];
+    __shared__ double s_red[3][kThreads / 64];
+    __shared__ int s_accept;
+    extern __shared__ int s_dyn[]; // kn[nk] followed by the phase tables (double2)
+
+    const int tid = threadIdx.x;
+    const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
+    const int width = 2 * ncut + 1;
+    const int kpt = S.kpt;
+    const double L = S.L, half = S.half, inv = S.inv;
+    int* s_kn = s_dyn;
+    double2* s_phase = reinterpret_cast<double2*>(s_dyn + ((nk + 3) & ~3));
+
+    for (int i = tid; i < nt * nt; i += kThreads) {
+        s_pairs[i] = S.pairs[i];
+        if (kF32) {
+            const DevPair& p = S.pairs[i];
+            s_pairs_f[i] = DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4),
+                                    static_cast<float>(p.lj_cut2), static_cast<float>(p.wca_s2),
+                                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2),
+                                    static_cast<float>(p.coul), 0.0f};
+        }
+    }
+    if (tid == 0) s_coul = *S.coul;
+    for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
+    for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
+    for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
+    __syncthreads();
+    if (kF32) {
+        for (int i = tid; i < s_coul.nknots; i += kThreads) s_knots_f[i] = static_cast<float>(s_coul.knots[i]);
+        for (int i = tid; i < 6 * s_coul.nknots; i += kThreads) s_coeff_f[i] = static_cast<float>(s_coul.coeff[i]);
+    }
+
+    // this thread's chunk of k-vectors and their running structure factors
+    const int k0 = tid * kpt;
+    double2 q[kMaxKPT];
+#pragma unroll
+    for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < nk) ? S.q[k0 + s] : make_double2(0.0, 0.0);
+
+    double4* __restrict__ pos = S.pos;
+    float4* __restrict__ pos32 = S.pos32;
+    const uint32_t* __restrict__ meta = S.meta;
+    const unsigned long long move0 = S.moves;
+    const double beta = S.beta;
+    const double pref = S.rec_prefactor;
+    double energy = S.energy, sum_du = S.sum_du;
+    unsigned long long accepted = S.accepted;
+    const int replica = S.replica_id;
+    const unsigned long long seed = S.seed;
+    __syncthreads();
+
+    for (long long t = 0; t < M; ++t) {
+        const Proposal P = make_proposal(seed, static_cast<uint32_t>(replica), move0 + static_cast<unsigned long long>(t), n);
+        const int i = P.index;
+        const double4 po = pos[i];
+        const int ti = static_cast<int>(meta[i] & 0xffu);
+        const double dp = s_dp[ti];
+        const double scale = dp * P.length_u;
+        const double nx_ = wrap(po.x + P.dir[0] * scale, L, half, inv);
+        const double ny_ = wrap(po.y + P.dir[1] * scale, L, half, inv);
+        const double nz_ = wrap(po.z + P.dir[2] * scale, L, half, inv);
+
+        // Ewald phase tables e^{i k_a(n) r_a}, n in [-ncut, ncut], new (st=0) and old (st=1)
+        for (int e = tid; e < 6 * width; e += kThreads) {
+            const int nn = e % width - ncut;
+            const int axis = (e / width) % 3;
+            const int st = e / (3 * width);
+            const double x = st == 0 ? (axis == 0 ? nx_ : (axis == 1 ? ny_ : nz_))
+                                     : (axis == 0 ? po.x : (axis == 1 ? po.y : po.z));
+            const double kk = 6.283185307179586 * (static_cast<double>(nn) / L);
+            double sn, cs;
+            sincos(kk * x, &sn, &cs);
+            s_phase[e] = make_double2(cs, sn);
+        }
+
+        // nonbonded pair sums of the moved particle in the new and old position
+        double e_new = 0.0, e_old = 0.0;
+        if (!kCells) {
+            for (int j = tid; j < n; j += kThreads) {
+                if (j == i) continue;
+                const int tj = static_cast<int>(meta[j] & 0xffu);
+                if (kF32) {
+                    const float4 pj = pos32[j];
+                    const DevPairF& p = s_pairs_f[ti * nt + tj];
+                    const float fL = static_cast<float>(L), fh = static_cast<float>(half);
+                    e_new += static_cast<double>(pair_energy_f(p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj.w,
+                                                               dist2f(fL, fh, static_cast<float>(nx_), static_cast<float>(ny_),
+                                                                      static_cast<float>(nz_), pj.x, pj.y, pj.z)));
+                    e_old += static_cast<double>(pair_energy_f(p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj.w,
+                                                               dist2f(fL, fh, static_cast<float>(po.x), static_cast<float>(po.y),
+                                                                      static_cast<float>(po.z), pj.x, pj.y, pj.z)));
+                } else {
+                    const double4 pj = pos[j];
+                    const DevPair& p = s_pairs[ti * nt + tj];
+                    Box b{{L, L, L}, {half, half, half}};
+                    e_new += pair_energy(p, s_coul, po.w, pj.w, dist2(b, nx_, ny_, nz_, pj.x, pj.y, pj.z));
+                    e_old += pair_energy(p, s_coul, po.w, pj.w, dist2(b, po.x, po.y, po.z, pj.x, pj.y, pj.z));
+                }
+            }
+        } else {
+            // 27-cell stencils around the new (st=0) and old (st=1) position
+            const int lanes_per = kThreads / 54 > 0 ? kThreads / 54 : 1;
+            const int combo = tid / lanes_per;
+            const int slot0 = tid % lanes_per;
+            if (combo < 54) {
+                const int st = combo / 27, c = combo % 27;
+                const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
+                const int cx = (cell_of_coord(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                const int cy = (cell_of_coord(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                const int cz = (cell_of_coord(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                const int cnt = S.cell_count[cell];
+                for (int k = slot0; k < cnt; k += lanes_per) {
+                    const int j = S.cell_members[cell * S.cap + k];
+                    if (j == i) continue;
+                    const int tj = static_cast<int>(meta[j] & 0xffu);
+                    double u;
+                    if (kF32) {
+                        const float4 pj = pos32[j];
+                        const float fL = static_cast<float>(L), fh = static_cast<float>(half);
+                        u = static_cast<double>(pair_energy_f(s_pairs_f[ti * nt + tj], s_coul, s_knots_f, s_coeff_f,
+                                                              static_cast<float>(po.w), pj.w,
+                                                              dist2f(fL, fh, static_cast<float>(px), static_cast<float>(py),
+                                                                     static_cast<float>(pz), pj.x, pj.y, pj.z)));
+                    } else {
+                        const double4 pj = pos[j];
+                        Box b{{L, L, L}, {half, half, half}};
+                        u = pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w, dist2(b, px, py, pz, pj.x, pj.y, pj.z));
+                    }
+                    if (st == 0) e_new += u;
+                    else e_old += u;
+                }
+            }
+        }
+        __syncthreads(); // phase tables complete
+
+        // Ewald: dE = sum_k A_k (2 Re(conj(Q) dQ) + |dQ|^2) over this thread's k-vectors
+        double de = 0.0;
+        {
+            int prev_col = -1;
+            double2 exy_n = make_double2(1.0, 0.0), exy_o = make_double2(1.0, 0.0);
+            const double qi = po.w;
+#pragma unroll
+            for (int s = 0; s < kMaxKPT; ++s) {
+                const int k = k0 + s;
+                if (s < kpt && k < nk) {
+                    const int packed = s_kn[k];
+                    const int nx = unpack_n(packed, 0), ny = unpack_n(packed, 1), nz = unpack_n(packed, 2);
+                    const int col = packed & 0xffff;
+                    if (col != prev_col) {
+                        const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut];
+                        const double2 xo = s_phase[3 * width + nx + ncut], yo = s_phase[4 * width + ny + ncut];
+                        exy_n = make_double2(xn.x * yn.x - xn.y * yn.y, xn.x * yn.y + xn.y * yn.x);
+                        exy_o = make_double2(xo.x * yo.x - xo.y * yo.y, xo.x * yo.y + xo.y * yo.x);
+                        prev_col = col;
+                    }
+                    const double2 zn = s_phase[2 * width + nz + ncut], zo = s_phase[5 * width + nz + ncut];
+                    const double en_re = exy_n.x * zn.x - exy_n.y * zn.y, en_im = exy_n.x * zn.y + exy_n.y * zn.x;
+                    const double eo_re = exy_o.x * zo.x - exy_o.y * zo.y, eo_im = exy_o.x * zo.y + exy_o.y * zo.x;
+                    const double dre = qi * (en_re - eo_re), dim = qi * (en_im - eo_im);
+                    const int n2 = nx * nx + ny * ny + nz * nz;
+                    const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[n2];
+                    de += A * (2.0 * (q[s].x * dre + q[s].y * dim) + (dre * dre + dim * dim));
+                }
+            }
+        }
+
+        // block reduction (fixed order) and the acceptance test on thread 0
+        {
+            double a = e_new, b = e_old, c = de;
+            for (int off = 32; off > 0; off >>= 1) {
+                a += __shfl_xor(a, off, 64);
+                b += __shfl_xor(b, off, 64);
+                c += __shfl_xor(c, off, 64);
+            }
+            if ((tid & 63) == 0) {
+                s_red[0][tid >> 6] = a;
+                s_red[1][tid >> 6] = b;
+                s_red[2][tid >> 6] = c;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) {
+                a += s_red[0][w];
+                b += s_red[1][w];
+                c += s_red[2][w];
+            }
+            const double du = energy_change(a, b, pref * c);
+            const bool acc = metropolis(beta * du, P.accept_u);
+            s_accept = acc ? 1 : 0;
+            if (trace && t < trace_cap) {
+                TraceEntry& T = trace[static_cast<size_t>(blockIdx.x) * trace_cap + t];
+                T.index = i;
+                T.accepted = acc ? 1 : 0;
+                T.du = du;
+                T.x = nx_;
+                T.y = ny_;
+                T.z = nz_;
+            }
+            if (acc) {
+                energy += du;
+                sum_du += du;
+                ++accepted;
+                pos[i] = make_double4(nx_, ny_, nz_, po.w);
+                if (kF32) pos32[i] = make_float4(static_cast<float>(nx_), static_cast<float>(ny_), static_cast<float>(nz_),
+                                                 static_cast<float>(po.w));
+                if (kCells) {
+                    const int cxn = cell_of_coord(nx_, half, S.inv_cell, S.nc0);
+                    const int cyn = cell_of_coord(ny_, half, S.inv_cell, S.nc1);
+                    const int czn = cell_of_coord(nz_, half, S.inv_cell, S.nc2);
+                    const int cnew = (czn * S.nc1 + cyn) * S.nc0 + cxn;
+                    const int cold = S.cell_of[i];
+                    if (cnew != cold) {
+                        const int nold = S.cell_count[cold];
+                        for (int kk = 0; kk < nold; ++kk)
+                            if (S.cell_members[cold * S.cap + kk] == i) {
+                                S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                break;
+                            }
+                        S.cell_count[cold] = nold - 1;
+                        const int nn = S.cell_count[cnew];
+                        if (nn >= S.cap) {
+                            S.error = 1; // cell capacity exceeded: the run stops with an error
+                        } else {
+                            S.cell_members[cnew * S.cap + nn] = i;
+                            S.cell_count[cnew] = nn + 1;
+                            S.cell_of[i] = cnew;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (s_accept) {
+            // Q += dQ for this thread's k-vectors (phase tables still hold this move)
+            int prev_col = -1;
+            double2 exy_n = make_double2(1.0, 0.0), exy_o = make_double2(1.0, 0.0);
+            const double qi = po.w;
+#pragma unroll
+            for (int s = 0; s < kMaxKPT; ++s) {
+                const int k = k0 + s;
+                if (s < kpt && k < nk) {
+                    const int packed = s_kn[k];
+                    const int nx = unpack_n(packed, 0), ny = unpack_n(packed, 1), nz = unpack_n(packed, 2);
+                    const int col = packed & 0xffff;
+                    if (col != prev_col) {
+                        const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut];
+                        const double2 xo = s_phase[3 * width + nx + ncut], yo = s_phase[4 * width + ny + ncut];
+                        exy_n = make_double2(xn.x * yn.x - xn.y * yn.y, xn.x * yn.y + xn.y * yn.x);
+                        exy_o = make_double2(xo.x * yo.x - xo.y * yo.y, xo.x * yo.y + xo.y * yo.x);
+                        prev_col = col;
+                    }
+                    const double2 zn = s_phase[2 * width + nz + ncut], zo = s_phase[5 * width + nz + ncut];
+                    const double en_re = exy_n.x * zn.x - exy_n.y * zn.y, en_im = exy_n.x * zn.y + exy_n.y * zn.x;
+                    const double eo_re = exy_o.x * zo.x - exy_o.y * zo.y, eo_im = exy_o.x * zo.y + exy_o.y * zo.x;
+                    q[s].x += qi * (en_re - eo_re);
+                    q[s].y += qi * (en_im - eo_im);
+                }
+            }
+        }
+        if (S.error) break;
+        __syncthreads(); // phase tables and the accepted position are consumed before the next move
+    }
+
+#pragma unroll
+    for (int s = 0; s < kMaxKPT; ++s)
+        if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
+    if (tid == 0) {
+        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.accepted = accepted;
+        S.energy = energy;
+        S.sum_du = sum_du;
+    }
+}
+
+// Replica exchange along each ladder (odd/even neighbour pairs by round parity):
+// accept with min(1, exp((beta_a - beta_b)(U_a - U_b))), energies at T_ref; swaps the
+// beta labels (ParallelTempering semantics, move.cpp:844-965, without moving data).
+__global__ void ladder_swap_kernel(ReplicaState* states, const int* ladder_begin, const int* ladder_size,
+                                   int* rung_slot, int nladders, unsigned long long seed, unsigned long long round,
+                                   unsigned long long* counters) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nladders) return;
+    const int b = ladder_begin[l], m = ladder_size[l];
+    for (int k = static_cast<int>(round & 1); k + 1 < m; k += 2) {
+        const int sa = rung_slot[b + k], sb = rung_slot[b + k + 1];
+        ReplicaState& A = states[sa];
+        ReplicaState& B = states[sb];
+        const double x = (A.beta - B.beta) * (A.energy - B.energy);
+        const Philox4 r = philox4x32_10(static_cast<uint32_t>(round), static_cast<uint32_t>(round >> 32),
+                                        static_cast<uint32_t>(l), 0x5a5a0000u + static_cast<uint32_t>(k),
+                                        static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+        const double u = (static_cast<double>(r.v[0] >> 5) * 67108864.0 + static_cast<double>(r.v[1] >> 6)) *
+                         1.1102230246251565e-16;
+        atomicAdd(&counters[0], 1ull);
+        if (x >= 0.0 || u < exp(x)) {
+            const double tb = A.beta;
+            A.beta = B.beta;
+            B.beta = tb;
+            rung_slot[b + k] = sb;
+            rung_slot[b + k + 1] = sa;
+            atomicAdd(&counters[1], 1ull);
+        }
+    }
+}
+
+template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
+void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap) {
+    auto fn = mc_advance_kernel<kThreads, kMaxKPT, kCells, kF32>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 40 * 1024),
+                  "hipFuncSetAttribute");
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, s, m, tr, cap);
+}
+
+template <int kThreads, int kMaxKPT>
+void dispatch_cells(bool cells, bool f32, int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m,
+                    TraceEntry* tr, int cap) {
+    if (cells) {
+        if (f32) launch_advance<kThreads, kMaxKPT, true, true>(grid, lds, st, s, m, tr, cap);
+        else launch_advance<kThreads, kMaxKPT, true, false>(grid, lds, st, s, m, tr, cap);
+    } else {
+        if (f32) launch_advance<kThreads, kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap);
+        else launch_advance<kThreads, kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap);
+    }
+}
+
+} // namespace
+
+// ==============================================================================
+Batch::Batch(int precision_fp32) : fp32_(precision_fp32) {
+    check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreate(&ev0_), "hipEventCreate");
+    check_hip(hipEventCreate(&ev1_), "hipEventCreate");
+}
+
+Batch::~Batch() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (auto p : gk_bufs_) (void)hipFree(p);
+    for (auto p : pos32_bufs_) (void)hipFree(p);
+    for (auto p : dp_bufs_) (void)hipFree(p);
+    (void)hipFree(d_states_);
+    (void)hipFree(d_moves_);
+    (void)hipFree(d_trace_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Batch::add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
+                       int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
+                       unsigned long long seed, int ladder) {
+    if (finalized_) throw std::runtime_error("batch already finalized");
+    auto e = std::make_unique<Engine>(config_json);
+    e->set_particles(n, xyz, charge, ids);
+    e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
+    e->init();
+    const Model& m = e->model();
+    if (!(m.box[0] == m.box[1] && m.box[1] == m.box[2])) throw std::runtime_error("device MC needs a cubic box");
+    if (m.g2g_cutoff2 < std::numeric_limits<double>::max() && !m.molecules.empty())
+        for (const auto& mol : m.molecules)
+            if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
+    ReplicaState s{};
+    s.n = n;
+    s.nk = e->num_kvectors();
+    s.ncut = m.ewald.present ? static_cast<int>(std::ceil(m.ewald.n_cutoff)) : 0;
+    if (s.ncut * s.ncut >= kMaxGk) throw std::runtime_error("ncutoff too large for the device MC (max 32)");
+    if (m.ewald.present && !m.ewald.spherical_sum) throw std::runtime_error("device MC needs spherical_sum: true");
+    for (const auto& mol : m.molecules)
+        if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
+    s.ntypes = m.ntypes;
+    s.L = m.box[0];
+    s.half = 0.5 * m.box[0];
+    s.inv = 1.0 / m.box[0];
+    s.rec_prefactor = m.ewald.present ? 2 * kPi * m.ewald.lB / (m.box[0] * m.box[1] * m.box[2]) : 0.0;
+    s.pos = e->d_pos();
+    s.meta = e->d_meta();
+    s.q = e->d_q_acc();
+    s.kn = e->d_kn();
+    s.pairs = e->d_pairs();
+    s.coul = e->d_coul();
+    CellList& cl = e->cells();
+    s.cells = cl.enabled ? 1 : 0;
+    if (cl.enabled) {
+        if (!(cl.nc[0] == cl.nc[1] && cl.nc[1] == cl.nc[2])) throw std::runtime_error("cubic cell grid expected");
+        s.nc0 = cl.nc[0];
+        s.nc1 = cl.nc[1];
+        s.nc2 = cl.nc[2];
+        s.cap = cl.cap;
+        s.inv_cell = cl.inv_cell[0];
+        s.cell_count = cl.count;
+        s.cell_members = cl.members;
+        s.cell_of = cl.cell_of;
+    }
+    // A_k as a function of |n|^2 for a cubic box (the n_x>0 factor 2 is applied in the kernel)
+    std::vector<double> gk(static_cast<size_t>(kMaxGk), 0.0);
+    if (m.ewald.present) {
+        const double kb = 2 * kPi / m.box[0];
+        for (int n2 = 1; n2 <= s.ncut * s.ncut; ++n2) {
+            const double k2 = kb * kb * n2;
+            gk[static_cast<size_t>(n2)] = std::exp(-k2 / (4 * m.ewald.alpha * m.ewald.alpha)) / k2;
+        }
+    }
+    double* d_gk = nullptr;
+    check_hip(hipMalloc(&d_gk, sizeof(double) * kMaxGk), "hipMalloc gk");
+    check_hip(hipMemcpy(d_gk, gk.data(), sizeof(double) * kMaxGk, hipMemcpyHostToDevice), "copy gk");
+    gk_bufs_.push_back(d_gk);
+    s.gk = d_gk;
+    std::vector<double> dpt(static_cast<size_t>(m.ntypes));
+    for (int t = 0; t < m.ntypes; ++t) dpt[static_cast<size_t>(t)] = m.atoms[static_cast<size_t>(t)].dp;
+    double* d_dp = nullptr;
+    check_hip(hipMalloc(&d_dp, sizeof(double) * m.ntypes), "hipMalloc dp");
+    check_hip(hipMemcpy(d_dp, dpt.data(), sizeof(double) * m.ntypes, hipMemcpyHostToDevice), "copy dp");
+    dp_bufs_.push_back(d_dp);
+    s.dp_type = d_dp;
+    float4* d_p32 = nullptr;
+    if (fp32_) {
+        std::vector<float4> p32(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            p32[static_cast<size_t>(i)] = make_float4(static_cast<float>(xyz[3 * i]), static_cast<float>(xyz[3 * i + 1]),
+                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(charge[i]));
+        check_hip(hipMalloc(&d_p32, sizeof(float4) * n), "hipMalloc pos32");
+        check_hip(hipMemcpy(d_p32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
+        pos32_bufs_.push_back(d_p32);
+    }
+    s.pos32 = d_p32;
+    s.seed = seed;
+    s.replica_id = static_cast<int>(engines_.size());
+    s.ladder = ladder;
+    s.beta = beta;
+    std::vector<double> terms(static_cast<size_t>(e->num_terms()));
+    e->total_energy(terms.data());
+    double u0 = 0;
+    for (double v : terms) u0 += v;
+    s.energy = u0;
+    h_states_.push_back(s);
+    engines_.push_back(std::move(e));
+    return s.replica_id;
+}
+
+int Batch::threads_for(int r) const {
+    if (threads_override_) return threads_override_;
+    const ReplicaState& s = h_states_[static_cast<size_t>(r)];
+    (void)s;
+    return 256;
+}
+
+void Batch::finalize() {
+    if (engines_.empty()) throw std::runtime_error("empty batch");
+    const int threads = threads_for(0);
+    for (int r = 0; r < size(); ++r) {
+        if (threads_for(r) != threads) throw std::runtime_error("replicas of one batch must share the workgroup size");
+        ReplicaState& s = h_states_[static_cast<size_t>(r)];
+        s.kpt = (s.nk + threads - 1) / threads;
+        if (s.kpt > 68) throw std::runtime_error("too many k-vectors per replica for the device MC (max 68 per thread)");
+        if (s.cells != h_states_[0].cells) throw std::runtime_error("replicas must agree on the cell list");
+    }
+    check_hip(hipMalloc(&d_states_, sizeof(ReplicaState) * size()), "hipMalloc states");
+    check_hip(hipMemcpy(d_states_, h_states_.data(), sizeof(ReplicaState) * size(), hipMemcpyHostToDevice), "copy states");
+    check_hip(hipMalloc(&d_moves_, sizeof(long long) * size()), "hipMalloc moves");
+    finalized_ = true;
+}
+
+void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) {
+    if (!finalized_) finalize();
+    if (static_cast<int>(moves.size()) != size()) throw std::runtime_error("run: one move count per replica");
+    check_hip(hipMemcpyAsync(d_moves_, moves.data(), sizeof(long long) * size(), hipMemcpyHostToDevice, stream_), "copy moves");
+    if (trace) {
+        if (trace_cap > trace_cap_) {
+            (void)hipFree(d_trace_);
+            check_hip(hipMalloc(&d_trace_, sizeof(TraceEntry) * static_cast<size_t>(trace_cap) * size()), "hipMalloc trace");
+            trace_cap_ = trace_cap;
+        }
+    }
+    const int threads = threads_for(0);
+    int maxkpt = 0, maxnk = 0, maxw = 0;
+    for (auto& s : h_states_) {
+        maxkpt = std::max(maxkpt, s.kpt);
+        maxnk = std::max(maxnk, s.nk);
+        maxw = std::max(maxw, 2 * s.ncut + 1);
+    }
+    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
+    if (lds > 120 * 1024) throw std::runtime_error("k-vector table too large for LDS");
+    const bool cells = h_states_[0].cells != 0;
+    const bool f32 = fp32_ != 0;
+    TraceEntry* tr = trace ? d_trace_ : nullptr;
+    const int cap = trace ? trace_cap : 0;
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    if (threads != 256) throw std::runtime_error("unsupported workgroup size");
+    if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
+    else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
+    else dispatch_cells<256, 68>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
+    check_hip(hipGetLastError(), "mc_advance_kernel launch");
+    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
+void Batch::sync() {
+    check_hip(hipStreamSynchronize(stream_), "batch sync");
+    if (timing_) {
+        float ms = 0;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+        last_ms_ = ms;
+    }
+}
+
+void Batch::swap_ladders(unsigned long long seed, unsigned long long round) {
+    // ladders: replicas with equal ladder id, ordered by decreasing beta at creation
+    static thread_local std::vector<int> dummy;
+    (void)dummy;
+    std::vector<int> lb, ls, slots;
+    std::vector<int> ids;
+    for (auto& s : h_states_) ids.push_back(s.ladder);
+    std::vector<int> uniq = ids;
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    sync();
+    std::vector<ReplicaState> cur(static_cast<size_t>(size()));
+    check_hip(hipMemcpy(cur.data(), d_states_, sizeof(ReplicaState) * size(), hipMemcpyDeviceToHost), "copy states");
+    for (int lid : uniq) {
+        if (lid < 0) continue;
+        std::vector<int> members;
+        for (int r = 0; r < size(); ++r)
+            if (cur[static_cast<size_t>(r)].ladder == lid) members.push_back(r);
+        std::sort(members.begin(), members.end(), [&](int a, int b) { return cur[static_cast<size_t>(a)].beta > cur[static_cast<size_t>(b)].beta; });
+        lb.push_back(static_cast<int>(slots.size()));
+        ls.push_back(static_cast<int>(members.size()));
+        slots.insert(slots.end(), members.begin(), members.end());
+    }
+    if (lb.empty()) return;
+    int *d_lb = nullptr, *d_ls = nullptr, *d_slots = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    check_hip(hipMalloc(&d_lb, sizeof(int) * lb.size()), "hipMalloc");
+    check_hip(hipMalloc(&d_ls, sizeof(int) * ls.size()), "hipMalloc");
+    check_hip(hipMalloc(&d_slots, sizeof(int) * slots.size()), "hipMalloc");
+    check_hip(hipMalloc(&d_cnt, sizeof(unsigned long long) * 2), "hipMalloc");
+    check_hip(hipMemcpy(d_lb, lb.data(), sizeof(int) * lb.size(), hipMemcpyHostToDevice), "copy");
+    check_hip(hipMemcpy(d_ls, ls.data(), sizeof(int) * ls.size(), hipMemcpyHostToDevice), "copy");
+    check_hip(hipMemcpy(d_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice), "copy");
+    check_hip(hipMemset(d_cnt, 0, sizeof(unsigned long long) * 2), "memset");
+    const int nl = static_cast<int>(lb.size());
+    hipLaunchKernelGGL(ladder_swap_kernel, dim3((nl + 63) / 64), dim3(64), 0, stream_, d_states_, d_lb, d_ls, d_slots, nl,
+                       seed, round, d_cnt);
+    check_hip(hipGetLastError(), "ladder_swap_kernel");
+    unsigned long long cnt[2] = {0, 0};
+    check_hip(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, stream_), "copy");
+    check_hip(hipStreamSynchronize(stream_), "sync");
+    swaps_att_ += static_cast<long long>(cnt[0]);
+    swaps_acc_ += static_cast<long long>(cnt[1]);
+    (void)hipFree(d_lb);
+    (void)hipFree(d_ls);
+    (void)hipFree(d_slots);
+    (void)hipFree(d_cnt);
+}
+
+ReplicaState Batch::state(int r) {
+    sync();
+    ReplicaState s;
+    check_hip(hipMemcpy(&s, d_states_ + r, sizeof(ReplicaState), hipMemcpyDeviceToHost), "copy state");
+    return s;
+}
+
+void Batch::positions(int r, double* xyz) {
+    sync();
+    Engine& e = *engines_[static_cast<size_t>(r)];
+    e.refresh_host_positions();
+    e.get_positions(xyz);
+}
+
+std::vector<TraceEntry> Batch::trace(int r, int count) {
+    sync();
+    std::vector<TraceEntry> t(static_cast<size_t>(count));
+    if (!d_trace_ || count > trace_cap_) throw std::runtime_error("trace not recorded");
+    check_hip(hipMemcpy(t.data(), d_trace_ + static_cast<size_t>(r) * trace_cap_, sizeof(TraceEntry) * count,
+                        hipMemcpyDeviceToHost), "copy trace");
+    return t;
+}
+
+} // namespace fmc

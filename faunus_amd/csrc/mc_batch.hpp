@@ -1,0 +1,104 @@
+// Device-resident Metropolis: every replica is advanced by one workgroup of a
+// persistent kernel (proposal, pair dU, Ewald dU, acceptance test and commit all
+// on the GPU).  Replicas of a temperature ladder share one parameter set at the
+// reference temperature and differ in beta = T_ref / T; replica exchange permutes
+// the beta labels (configurations never move).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace fmc {
+
+// Per-replica device state (one per workgroup).
+struct ReplicaState {
+    int n, nk, ncut, ntypes;
+    int cells, nc0, nc1, nc2;
+    int cap, kpt, replica_id, ladder;
+    double L, half, inv;
+    double inv_cell;
+    double rec_prefactor; // 2 pi lB / V
+    double4* pos;
+    float4* pos32;
+    const uint32_t* meta;
+    double2* q;
+    const int* kn;
+    const double* gk;     // A_k / (nx>0 ? 2 : 1) as a function of |n|^2 (cubic box)
+    int* cell_count;
+    int* cell_members;
+    int* cell_of;
+    const DevPair* pairs;
+    const DevCoulomb* coul;
+    const double* dp_type;
+    unsigned long long seed;
+    // dynamic
+    double beta;
+    double energy;      // running total energy (kT at T_ref)
+    unsigned long long moves;
+    unsigned long long accepted;
+    double sum_du;
+    int error;
+    int pad;
+};
+
+// One recorded move of the trace (tests).
+struct TraceEntry {
+    int index;
+    int accepted;
+    double du;     // energy change (kT at T_ref) before the reject reset
+    double x, y, z; // proposed position
+};
+
+// A set of replicas advanced together on one stream.
+class Batch {
+  public:
+    explicit Batch(int precision_fp32);
+    ~Batch();
+    int add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
+                    int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
+                    unsigned long long seed, int ladder);
+    void finalize();
+    // advance every replica by `moves[r]` moves (one launch); trace optional
+    void run(const std::vector<long long>& moves, bool trace, int trace_cap);
+    void swap_ladders(unsigned long long seed, unsigned long long round);
+    void sync();
+    int size() const { return static_cast<int>(engines_.size()); }
+    ReplicaState state(int r);
+    void positions(int r, double* xyz);
+    std::vector<TraceEntry> trace(int r, int count);
+    Engine& engine(int r) { return *engines_[static_cast<size_t>(r)]; }
+    hipStream_t stream() const { return stream_; }
+    void set_timing(bool on) { timing_ = on; }
+    double last_kernel_ms() const { return last_ms_; }
+    int threads_for(int r) const;
+    void set_threads(int threads) { threads_override_ = threads; }
+    long long swaps_attempted() const { return swaps_att_; }
+    long long swaps_accepted() const { return swaps_acc_; }
+    hipGraphExec_t graph_ = nullptr;
+
+  private:
+    int fp32_ = 0;
+    std::vector<std::unique_ptr<Engine>> engines_;
+    std::vector<ReplicaState> h_states_;
+    std::vector<double*> gk_bufs_;
+    std::vector<float4*> pos32_bufs_;
+    std::vector<double*> dp_bufs_;
+    ReplicaState* d_states_ = nullptr;
+    long long* d_moves_ = nullptr;
+    TraceEntry* d_trace_ = nullptr;
+    int trace_cap_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool finalized_ = false;
+    bool timing_ = false;
+    double last_ms_ = 0;
+    int threads_override_ = 0;
+    long long swaps_att_ = 0, swaps_acc_ = 0;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+} // namespace fmc

@@ -137,6 +137,42 @@ int fmc_run_translate(fmc_engine* engine, long count, const int* group, const in
 int fmc_set_timing(fmc_engine* engine, int enable);
 int fmc_get_timing(fmc_engine* engine, double* total_ms, long* launches);
 
+/*
+ * Device-resident Metropolis over many replicas (one workgroup per replica, one
+ * persistent launch per call).  Each replica is a system as in fmc_create plus
+ * positions and groups; proposals are single-particle translations drawn on the
+ * GPU from a per-replica Philox4x32-10 stream (the semantics of
+ * AtomicTranslateRotate::_move, src/move.cpp:267-293, and performMove,
+ * src/montecarlo.cpp:139-187).  beta = T_ref/T scales every energy (all terms are
+ * energies in kT at T_ref); replicas with the same ladder id >= 0 form a
+ * parallel-tempering ladder whose temperature labels fmc_batch_swap exchanges
+ * (ParallelTempering, src/move.cpp:844-965).  fp32_pairs != 0 evaluates pair terms in
+ * FP32 with FP64 accumulation; Q(k), dU and the acceptance test stay FP64.
+ * No reference counterpart as one call: it replaces the loop of performMove calls.
+ */
+typedef struct fmc_batch fmc_batch;
+int fmc_batch_create(int fp32_pairs, fmc_batch** out);
+int fmc_batch_destroy(fmc_batch* batch);
+int fmc_batch_add_replica(fmc_batch* batch, const char* config_json, int n, const double* xyz, const double* charge,
+                          const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                          double beta, unsigned long long seed, int ladder, int* replica);
+/* Advance replica r by moves[r] trial moves; trace_capacity > 0 records that many moves per replica. */
+int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
+/* Wait for the batch; kernel time of the last run (ms, HIP events) if timing is on. */
+int fmc_batch_sync(fmc_batch* batch, double* kernel_ms);
+int fmc_batch_set_timing(fmc_batch* batch, int enable);
+/* One odd/even round of neighbour exchanges on every ladder. */
+int fmc_batch_swap(fmc_batch* batch, unsigned long long seed, unsigned long long round, long long* attempted,
+                   long long* accepted);
+int fmc_batch_replica_info(fmc_batch* batch, int replica, double* energy, double* beta, unsigned long long* moves,
+                           unsigned long long* accepted, int* error);
+int fmc_batch_positions(fmc_batch* batch, int replica, double* xyz);
+int fmc_batch_trace(fmc_batch* batch, int replica, int count, int* index, unsigned char* accepted, double* du,
+                    double* xyz);
+/* Change::everything energy of a replica recomputed from its positions, per term. */
+int fmc_batch_total_energy(fmc_batch* batch, int replica, double* terms, double* total);
+int fmc_batch_check_q(fmc_batch* batch, int replica, double* max_abs_dq, double* rel_reciprocal);
+
 /* Andrea spline of the Coulomb short-range function built by the engine (for tests). */
 int fmc_coulomb_spline(fmc_engine* engine, int capacity, double* knots, double* coeff, int* num_knots);
 int fmc_num_kvectors(fmc_engine* engine, int* count);

@@ -38,6 +38,8 @@ def _lib(fast=False):
         lib.orc_bjerrum_length.argtypes = [C.c_double, C.c_double]
         lib.orc_run_translate.argtypes = [C.c_void_p, C.c_long, C.c_uint, C.c_int, C.c_double, I, I, D, D, D, U8, D, D]
         lib.orc_replay_translate.argtypes = [C.c_void_p, C.c_long, I, I, D, D, D, U8, D, D]
+        lib.orc_run_philox_translate.argtypes = [C.c_void_p, C.c_long, C.c_ulonglong, C.c_uint, C.c_ulonglong,
+                                                 C.c_double, I, D, D, U8]
         lib.orc_mc_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), D]
         _LIBS[name] = lib
     return _LIBS[name]
@@ -128,6 +130,15 @@ class System:
             self.h, nmoves, seed, molid, default_dp, _p(rec["group"], C.c_int), _p(rec["index"], C.c_int),
             _p(rec["xyz"]), _p(rec["u"]), _p(rec["du"]), _p(rec["accepted"], C.c_ubyte), _p(rec["e_new"]),
             _p(rec["e_old"])))
+        return rec
+
+    def run_philox_translate(self, nmoves, seed, replica=0, move0=0, beta=1.0):
+        """Metropolis run with the device-resident Philox proposal scheme."""
+        rec = dict(index=np.zeros(nmoves, np.int32), xyz=np.zeros((nmoves, 3)), du=np.zeros(nmoves),
+                   accepted=np.zeros(nmoves, np.uint8))
+        _check(self.lib, self.lib.orc_run_philox_translate(
+            self.h, nmoves, seed, replica, move0, beta, _p(rec["index"], C.c_int), _p(rec["xyz"]), _p(rec["du"]),
+            _p(rec["accepted"], C.c_ubyte)))
         return rec
 
     def replay_translate(self, group, index, xyz, u):

@@ -182,6 +182,91 @@ int orc_run_translate(void* h, long nmoves, unsigned seed, int molid, double def
     });
 }
 
+// Metropolis run with the device-resident proposal scheme: move t of replica r
+// draws Philox4x32-10 blocks at counters (t_lo, t_hi, r, 0) and (t_lo, t_hi, r, 1)
+// with key (seed_lo, seed_hi): particle = (x0 * n) >> 32 over all particles, unit
+// vector from z = 2u-1 and phi = 2 pi u', length dp*U[0,1), a 53-bit acceptance
+// deviate; acceptance u <= exp(-beta du).  Records per move: particle, proposed
+// position, du (kT at the configured temperature) and the decision.
+int orc_run_philox_translate(void* h, long nmoves, unsigned long long seed, unsigned replica,
+                             unsigned long long move0, double beta, int* rec_index, double* rec_xyz, double* rec_du,
+                             unsigned char* rec_acc) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        const int n = static_cast<int>(s->trial_space.particles.size());
+        std::vector<int> group_of(static_cast<size_t>(n)), rel_of(static_cast<size_t>(n));
+        for (size_t g = 0; g < s->trial_space.groups.size(); ++g) {
+            const Group& G = s->trial_space.groups[g];
+            for (int k = 0; k < G.size; ++k) {
+                group_of[static_cast<size_t>(G.begin + k)] = static_cast<int>(g);
+                rel_of[static_cast<size_t>(G.begin + k)] = k;
+            }
+        }
+        auto u01 = [](uint32_t x) { return (static_cast<double>(x) + 0.5) * 2.3283064365386963e-10; };
+        for (long t = 0; t < nmoves; ++t) {
+            const unsigned long long mv = move0 + static_cast<unsigned long long>(t);
+            const auto a = philox4x32_10({static_cast<uint32_t>(mv), static_cast<uint32_t>(mv >> 32), replica, 0u},
+                                         {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)});
+            const auto b = philox4x32_10({static_cast<uint32_t>(mv), static_cast<uint32_t>(mv >> 32), replica, 1u},
+                                         {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)});
+            const int i = static_cast<int>((static_cast<uint64_t>(a[0]) * static_cast<uint64_t>(n)) >> 32);
+            const double z = 2.0 * u01(a[1]) - 1.0;
+            const double phi = 6.283185307179586 * u01(a[2]);
+            const double sz = std::sqrt(1.0 - z * z);
+            const Vec3 dir{sz * std::cos(phi), sz * std::sin(phi), z};
+            const double len_u = static_cast<double>(a[3]) * 2.3283064365386963e-10;
+            const double acc_u = (static_cast<double>(b[0] >> 5) * 67108864.0 + static_cast<double>(b[1] >> 6)) *
+                                 1.1102230246251565e-16;
+            Particle& p = s->trial_space.particles[static_cast<size_t>(i)];
+            const double dp = s->atoms.at(static_cast<size_t>(p.id)).dp;
+            const double scale = dp * len_u;
+            p.pos = {p.pos.x + dir.x * scale, p.pos.y + dir.y * scale, p.pos.z + dir.z * scale};
+            s->trial_space.geometry.boundary(p.pos);
+            if (rec_xyz) {
+                rec_xyz[3 * t] = p.pos.x;
+                rec_xyz[3 * t + 1] = p.pos.y;
+                rec_xyz[3 * t + 2] = p.pos.z;
+            }
+            Change c;
+            GroupChange gc;
+            gc.group_index = group_of[static_cast<size_t>(i)];
+            gc.internal = true;
+            gc.relative_atom_indices = {rel_of[static_cast<size_t>(i)]};
+            c.groups.push_back(gc);
+            // performMove with beta-scaled acceptance
+            s->trial_pot->update_state(s->trial_space, &s->accepted_space, c);
+            const double e_new = s->trial_pot->energy(s->trial_space, &s->accepted_space, c);
+            const double e_old = s->accepted_pot->energy(s->accepted_space, nullptr, c);
+            double du;
+            if (std::isnan(e_old) && !std::isnan(e_new)) du = -infty;
+            else if (std::isnan(e_new)) du = infty;
+            else if (e_new > 0.0 && std::isinf(e_new)) du = infty;
+            else {
+                du = e_new - e_old;
+                if (std::isnan(du)) du = 0.0;
+            }
+            const double bdu = beta * du;
+            bool acc;
+            if (std::isinf(bdu) && bdu < 0) acc = true;
+            else if (-bdu > max_exp_argument) acc = true;
+            else acc = acc_u <= std::exp(-bdu);
+            if (acc) {
+                s->accepted_space.particles[static_cast<size_t>(i)] = p;
+                s->accepted_pot->sync_from(*s->trial_pot, c);
+                s->stats.accepted++;
+                s->stats.sum_du += du;
+            } else {
+                p = s->accepted_space.particles[static_cast<size_t>(i)];
+                s->trial_pot->sync_from(*s->accepted_pot, c);
+            }
+            s->stats.moves++;
+            if (rec_index) rec_index[t] = i;
+            if (rec_du) rec_du[t] = du;
+            if (rec_acc) rec_acc[t] = acc ? 1 : 0;
+        }
+    });
+}
+
 // Replay a recorded single-particle proposal stream.
 int orc_replay_translate(void* h, long nmoves, const int* group, const int* index, const double* xyz,
                          const double* u, double* out_du, unsigned char* out_acc, double* out_enew,

@@ -1,0 +1,150 @@
+"""Device-resident Metropolis (one workgroup per replica) against the oracle running
+the same Philox proposal stream.  FP64: every du within 1e-10 * max(|du|, E_scale)
+and identical decisions (a decision may differ only at a logged near-tie); FP32
+pair terms: du within 1e-6 * max(|du|, E_scale) up to the first near-tie."""
+import numpy as np
+import pytest
+
+import faunus_amd as F
+from oracle import oracle as O
+from tests import systems
+from tests.gpu_helpers import log
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_of(system):
+    cfg, xyz, q, ids, g = system
+    s = O.System(cfg)
+    s.set_particles(xyz, q, ids)
+    s.set_groups(g["begin"], g["size"], g["molid"])
+    e0 = s.init()
+    return s, e0
+
+
+def compare_traces(tr, rec, e_scale, rel, beta=1.0):
+    n = len(rec["du"])
+    mism = np.nonzero(tr["accepted"] != rec["accepted"])[0]
+    first = int(mism[0]) if len(mism) else n
+    sl = slice(0, first)
+    assert np.array_equal(tr["index"][sl], rec["index"][sl])
+    fin = np.isfinite(rec["du"][sl])
+    dg, dc = tr["du"][sl][fin], rec["du"][sl][fin]
+    err = np.abs(dg - dc) / np.maximum(np.abs(dc), e_scale)
+    assert np.array_equal(np.isinf(tr["du"][sl]), np.isinf(rec["du"][sl]))
+    ties = []
+    if len(mism):
+        i = first
+        ties.append(float(abs(np.exp(-beta * rec["du"][i]))))
+    return dict(first_mismatch=first, n=n, max_rel=float(err.max()) if len(err) else 0.0,
+                accepted=int(tr["accepted"][sl].sum()), mismatches=len(mism))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_device_mc_primitive_model(fp32):
+    sysd = systems.primitive_model(n_pairs=100, box=118.0, seed=1)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch(fp32_pairs=fp32)
+    b.add(sysd, beta=1.0, seed=12345)
+    nm = 4000
+    b.run([nm], trace=nm)
+    b.sync()
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=12345, replica=0)
+    e_scale = max(1.0, abs(e0))
+    rel = 1e-6 if fp32 else 1e-10
+    rep = compare_traces(tr, rec, e_scale, rel)
+    log("device MC primitive model fp32=%s" % fp32, rep)
+    if not fp32:
+        assert rep["first_mismatch"] == nm, rep
+        assert rep["max_rel"] <= 1e-10
+        np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+    else:
+        assert rep["max_rel"] <= 1e-6, rep
+        assert rep["first_mismatch"] > nm // 4, rep
+    info = b.info(0)
+    assert info["moves"] == nm and info["error"] == 0
+    # running energy equals a full recompute; Q(k) equals the full update
+    tot, terms = b.total_energy(0)
+    log("running", info["energy"], "recomputed", tot)
+    # FP32 pair terms accumulate ~1e-7-relative errors per move into the running sum
+    drift_tol = 1e-6 if fp32 else 1e-9
+    assert abs(info["energy"] - tot) <= drift_tol * max(1.0, abs(tot))
+    dq, drec = b.check_q(0)
+    assert dq < 1e-9 and abs(drec) < 1e-11
+
+
+def test_device_mc_cell_list_nacl():
+    box = 255.0 * (2000 / 20000) ** (1 / 3)
+    sysd = systems.nacl_lj(n_pairs=1000, box=box, seed=2, kmax=10)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=99)
+    nm = 3000
+    b.run([nm], trace=nm)
+    b.sync()
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=99, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("device MC cell list", rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+
+
+def test_device_mc_full_size_nacl_short():
+    """N=20 000, K=17 328: 68 k-vectors per thread in registers."""
+    sysd = systems.nacl_lj(n_pairs=10000, box=255.0, seed=1)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=7)
+    nm = 200
+    b.run([nm], trace=nm)
+    b.sync()
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=7, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("device MC full size", rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    dq, drec = b.check_q(0)
+    assert dq < 1e-9
+
+
+def test_batch_of_uneven_replicas_matches_oracle_per_replica():
+    """Several 2:1 salt replicas of different N and concentration in one launch."""
+    specs = [(600, 0.5, 11), (900, 0.05, 12), (1203, 0.2, 13), (300, 1.0, 14)]
+    b = F.Batch()
+    sysds = []
+    for k, (n, c, seed) in enumerate(specs):
+        sysd = systems.salt21(n=n, conc=c, seed=seed)
+        sysds.append(sysd)
+        b.add(sysd, seed=1000 + k)
+    nm = 1500
+    b.run([nm] * len(specs), trace=nm)
+    b.sync()
+    for r, sysd in enumerate(sysds):
+        orc, e0 = oracle_of(sysd)
+        rec = orc.run_philox_translate(nm, seed=1000 + r, replica=r)
+        tr = b.trace(r, nm)
+        rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+        log("batch replica", r, rep)
+        assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+
+
+def test_temperature_ladder_swaps_permute_betas():
+    sysd = systems.salt21(n=300, conc=0.1, seed=3)
+    betas = [298.15 / t for t in (280.0, 300.0, 320.0, 340.0)]
+    b = F.Batch()
+    for k, beta in enumerate(betas):
+        b.add(sysd, beta=beta, seed=50 + k, ladder=0)
+    for rnd in range(6):
+        b.run([600] * 4)
+        att, acc = b.swap(seed=77, rnd=rnd)
+    b.sync()
+    got = sorted(b.info(r)["beta"] for r in range(4))
+    assert np.allclose(got, sorted(betas))
+    log("ladder swaps attempted", att, "accepted", acc)
+    assert att == 9
+    for r in range(4):
+        tot, _ = b.total_energy(r)
+        assert abs(b.info(r)["energy"] - tot) <= 1e-9 * max(1.0, abs(tot))
