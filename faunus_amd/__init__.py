@@ -49,6 +49,8 @@ SIGNATURES = {
     "fmc_batch_add_replica": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_double,
                                         C.c_ulonglong, C.c_int, I]),
     "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
+    "fmc_batch_run_split": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
+    "fmc_batch_split_error": (C.c_int, [C.c_void_p, I]),
     "fmc_batch_sync": (C.c_int, [C.c_void_p, D]),
     "fmc_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmc_batch_swap": (C.c_int, [C.c_void_p, C.c_ulonglong, C.c_ulonglong, C.POINTER(C.c_longlong),
@@ -307,6 +309,14 @@ class Batch:
     def run(self, moves, trace=0):
         m = np.ascontiguousarray(np.broadcast_to(np.asarray(moves, np.int64), (len(self.sizes),)), np.int64)
         _check(lib().fmc_batch_run(self.h, m.ctypes.data_as(C.POINTER(C.c_longlong)), int(trace)))
+
+    def run_split(self, moves, groups=32, stride=1, trace=0):
+        _check(lib().fmc_batch_run_split(self.h, int(moves), int(groups), int(stride), int(trace)))
+
+    def split_error(self):
+        e = C.c_int()
+        _check(lib().fmc_batch_split_error(self.h, C.byref(e)))
+        return e.value
 
     def sync(self):
         ms = C.c_double()

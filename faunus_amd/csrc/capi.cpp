@@ -339,6 +339,14 @@ int fmc_batch_run(fmc_batch* b, const long long* moves, int trace_capacity) {
     });
 }
 
+int fmc_batch_run_split(fmc_batch* b, long long moves, int groups, int stride, int trace_capacity) {
+    return guarded([&] { bat(b).run_split(moves, groups, stride, trace_capacity > 0, trace_capacity); });
+}
+
+int fmc_batch_split_error(fmc_batch* b, int* error) {
+    return guarded([&] { *error = bat(b).split_error(); });
+}
+
 int fmc_batch_sync(fmc_batch* b, double* kernel_ms) {
     return guarded([&] {
         bat(b).sync();

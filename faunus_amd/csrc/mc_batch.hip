@@ -467,6 +467,8 @@ Batch::~Batch() {
     (void)hipFree(d_states_);
     (void)hipFree(d_moves_);
     (void)hipFree(d_trace_);
+    (void)hipFree(split_slots_);
+    (void)hipFree(split_err_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);

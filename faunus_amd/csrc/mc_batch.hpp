@@ -65,6 +65,9 @@ class Batch {
     void finalize();
     // advance every replica by `moves[r]` moves (one launch); trace optional
     void run(const std::vector<long long>& moves, bool trace, int trace_cap);
+    // one replica spread over `groups` workgroups (every `stride`-th block of the grid)
+    void run_split(long long moves, int groups, int stride, bool trace, int trace_cap);
+    int split_error();
     void swap_ladders(unsigned long long seed, unsigned long long round);
     void sync();
     int size() const { return static_cast<int>(engines_.size()); }
@@ -99,6 +102,8 @@ class Batch {
     int threads_override_ = 0;
     long long swaps_att_ = 0, swaps_acc_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    void* split_slots_ = nullptr;
+    int* split_err_ = nullptr;
 };
 
 } // namespace fmc

@@ -1,0 +1,381 @@
+// One large replica advanced by many workgroups at once.  Workgroup g owns a
+// slice of the k-vectors (its Q(k) stays in registers for the whole launch);
+// workgroup 0 also evaluates the pair dU through the cell list and commits
+// accepted moves.  After each move every workgroup publishes its partial sums with
+// an agent-scope release store, waits (acquire loads, bounded spin) for all others,
+// sums the partials in the same fixed order and so takes the same decision as every
+// other workgroup.  The launch is cooperative, so all workgroups are resident.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "device_math.hpp"
+#include "mc_batch.hpp"
+#include "philox.hpp"
+
+namespace fmc {
+namespace {
+
+constexpr int kSplitMaxKPT = 8;
+constexpr int kSplitMaxG = 64;
+constexpr int kGkMax = 1025;
+
+struct alignas(32) SplitSlot {
+    double de, pn, po;
+    unsigned long long seq;
+};
+
+__device__ __forceinline__ int cell_coord_s(double x, double half, double inv_cell, int nc) {
+    int c = static_cast<int>(floor((x + half) * inv_cell));
+    c %= nc;
+    if (c < 0) c += nc;
+    return c;
+}
+
+__device__ __forceinline__ double wrap_s(double x, double L, double half, double inv) {
+    if (fabs(x) > half) {
+        const double s = x * inv;
+        x -= L * static_cast<double>(static_cast<int>(s > 0.0 ? s + 0.5 : s - 0.5));
+    }
+    return x;
+}
+
+template <bool kCells>
+__global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
+                                                       int stride, SplitSlot* __restrict__ slots,
+                                                       int* __restrict__ err, TraceEntry* __restrict__ trace,
+                                                       int trace_cap, long long spin_limit) {
+    if (blockIdx.x % stride != 0) return;
+    const int g = blockIdx.x / stride;
+    if (g >= G) return;
+    ReplicaState& S = states[0];
+    __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
+    __shared__ DevCoulomb s_coul;
+    __shared__ double s_dp[kMaxTypes];
+    __shared__ double s_gk[kGkMax];
+    __shared__ double s_red[3][4];
+    __shared__ double s_slot[3][kSplitMaxG];
+    __shared__ int s_accept, s_abort;
+    __shared__ double2 s_phase[6 * 65];
+
+    const int tid = threadIdx.x;
+    const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
+    const int width = 2 * ncut + 1;
+    const double L = S.L, half = S.half, inv = S.inv;
+    for (int i = tid; i < nt * nt; i += 256) s_pairs[i] = S.pairs[i];
+    if (tid == 0) {
+        s_coul = *S.coul;
+        s_abort = 0;
+    }
+    for (int i = tid; i < nt; i += 256) s_dp[i] = S.dp_type[i];
+    for (int i = tid; i <= ncut * ncut && i < kGkMax; i += 256) s_gk[i] = S.gk[i];
+
+    // this workgroup's k-slice, interleaved over the threads
+    const int kw = (nk + G - 1) / G;
+    const int kb = g * kw;
+    const int ke = min(nk, kb + kw);
+    int kn[kSplitMaxKPT];
+    double2 q[kSplitMaxKPT], dq[kSplitMaxKPT];
+#pragma unroll
+    for (int s = 0; s < kSplitMaxKPT; ++s) {
+        const int k = kb + s * 256 + tid;
+        const bool on = k < ke;
+        kn[s] = on ? S.kn[k] : 0;
+        q[s] = on ? S.q[k] : make_double2(0.0, 0.0);
+        dq[s] = make_double2(0.0, 0.0);
+    }
+    double4* __restrict__ pos = S.pos;
+    const uint32_t* __restrict__ meta = S.meta;
+    const unsigned long long move0 = S.moves;
+    const double beta = S.beta, pref = S.rec_prefactor;
+    double energy = S.energy, sum_du = S.sum_du;
+    unsigned long long accepted = S.accepted;
+    const unsigned long long seed = S.seed;
+    const uint32_t replica = static_cast<uint32_t>(S.replica_id);
+    // the last two moves (identical in every workgroup): their writes to pos may not
+    // be visible yet, so reads of those particles take these values
+    int h_idx[2] = {-1, -1};
+    double h_pos[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    __syncthreads();
+
+    for (long long t = 0; t < M; ++t) {
+        const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
+        const Proposal P = make_proposal(seed, replica, mid, n);
+        const int i = P.index;
+        double4 po = pos[i];
+        // most recent accepted write of this particle wins
+        if (h_idx[1] == i) { po.x = h_pos[1][0]; po.y = h_pos[1][1]; po.z = h_pos[1][2]; }
+        else if (h_idx[0] == i) { po.x = h_pos[0][0]; po.y = h_pos[0][1]; po.z = h_pos[0][2]; }
+        const int ti = static_cast<int>(meta[i] & 0xffu);
+        const double scale = s_dp[ti] * P.length_u;
+        const double nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
+        const double ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
+        const double nz_ = wrap_s(po.z + P.dir[2] * scale, L, half, inv);
+        for (int e = tid; e < 6 * width; e += 256) {
+            const int nn = e % width - ncut;
+            const int axis = (e / width) % 3;
+            const int st = e / (3 * width);
+            const double x = st == 0 ? (axis == 0 ? nx_ : (axis == 1 ? ny_ : nz_))
+                                     : (axis == 0 ? po.x : (axis == 1 ? po.y : po.z));
+            double sn, cs;
+            sincos((6.283185307179586 * (static_cast<double>(nn) / L)) * x, &sn, &cs);
+            s_phase[e] = make_double2(cs, sn);
+        }
+        double e_new = 0.0, e_old = 0.0;
+        if (g == 0) {
+            const Box b{{L, L, L}, {half, half, half}};
+            if (kCells) {
+                const int combo = tid >> 2, slot0 = tid & 3; // 54 (state, cell) combos x 4 lanes
+                if (combo < 54) {
+                    const int st = combo / 27, c = combo % 27;
+                    const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
+                    const int cx = (cell_coord_s(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                    const int cy = (cell_coord_s(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                    const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                    const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                    const int cnt = S.cell_count[cell];
+                    for (int k = slot0; k < cnt; k += 4) {
+                        const int j = S.cell_members[cell * S.cap + k];
+                        if (j == i) continue;
+                        const double4 pj = pos[j];
+                        const double u = pair_energy(s_pairs[ti * nt + static_cast<int>(meta[j] & 0xffu)], s_coul, po.w,
+                                                     pj.w, dist2(b, px, py, pz, pj.x, pj.y, pj.z));
+                        if (st == 0) e_new += u;
+                        else e_old += u;
+                    }
+                }
+            } else {
+                for (int j = tid; j < n; j += 256) {
+                    if (j == i) continue;
+                    const double4 pj = pos[j];
+                    const DevPair& p = s_pairs[ti * nt + static_cast<int>(meta[j] & 0xffu)];
+                    e_new += pair_energy(p, s_coul, po.w, pj.w, dist2(b, nx_, ny_, nz_, pj.x, pj.y, pj.z));
+                    e_old += pair_energy(p, s_coul, po.w, pj.w, dist2(b, po.x, po.y, po.z, pj.x, pj.y, pj.z));
+                }
+            }
+        }
+        __syncthreads(); // phase tables
+        double de = 0.0;
+#pragma unroll
+        for (int s = 0; s < kSplitMaxKPT; ++s) {
+            const int k = kb + s * 256 + tid;
+            if (k < ke) {
+                const int nx = unpack_n(kn[s], 0), ny = unpack_n(kn[s], 1), nz = unpack_n(kn[s], 2);
+                const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut], zn = s_phase[2 * width + nz + ncut];
+                const double2 xo = s_phase[3 * width + nx + ncut], yo = s_phase[4 * width + ny + ncut],
+                              zo = s_phase[5 * width + nz + ncut];
+                const double an = xn.x * yn.x - xn.y * yn.y, bn = xn.x * yn.y + xn.y * yn.x;
+                const double ao = xo.x * yo.x - xo.y * yo.y, bo = xo.x * yo.y + xo.y * yo.x;
+                const double dre = po.w * ((an * zn.x - bn * zn.y) - (ao * zo.x - bo * zo.y));
+                const double dim = po.w * ((an * zn.y + bn * zn.x) - (ao * zo.y + bo * zo.x));
+                dq[s] = make_double2(dre, dim);
+                const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[nx * nx + ny * ny + nz * nz];
+                de += A * (2.0 * (q[s].x * dre + q[s].y * dim) + (dre * dre + dim * dim));
+            }
+        }
+        {
+            double a = de, b = e_new, c = e_old;
+            for (int off = 32; off > 0; off >>= 1) {
+                a += __shfl_xor(a, off, 64);
+                b += __shfl_xor(b, off, 64);
+                c += __shfl_xor(c, off, 64);
+            }
+            if ((tid & 63) == 0) {
+                s_red[0][tid >> 6] = a;
+                s_red[1][tid >> 6] = b;
+                s_red[2][tid >> 6] = c;
+            }
+        }
+        __syncthreads();
+        const int parity = static_cast<int>(mid & 1ull);
+        const unsigned long long tag = mid + 1ull;
+        if (tid == 0) {
+            SplitSlot* sl = slots + parity * G + g;
+            sl->de = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
+            sl->pn = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
+            sl->po = ((s_red[2][0] + s_red[2][1]) + s_red[2][2]) + s_red[2][3];
+            __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // gather every workgroup's partial (lane l of wave 0 waits for workgroup l)
+        if (tid < 64) {
+            bool ok = true;
+            if (tid < G) {
+                SplitSlot* sl = slots + parity * G + tid;
+                long long spins = 0;
+                while (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+                    if (++spins > spin_limit) {
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (ok) {
+                    s_slot[0][tid] = sl->de;
+                    s_slot[1][tid] = sl->pn;
+                    s_slot[2][tid] = sl->po;
+                }
+            }
+            if (!ok) {
+                atomicExch(err, 1);
+                s_abort = 1;
+            }
+        }
+        __syncthreads();
+        if (s_abort) break;
+        if (tid == 0) {
+            double sde = 0.0, spn = 0.0, spo = 0.0;
+            for (int w = 0; w < G; ++w) {
+                sde += s_slot[0][w];
+                spn += s_slot[1][w];
+                spo += s_slot[2][w];
+            }
+            const double drec = pref * sde;
+            const double e_tot_new = spn + drec;
+            double du;
+            if (isnan(spo) && !isnan(e_tot_new)) du = -__builtin_huge_val();
+            else if (isnan(e_tot_new)) du = __builtin_huge_val();
+            else if (e_tot_new > 0.0 && isinf(e_tot_new)) du = __builtin_huge_val();
+            else {
+                du = (spn - spo) + drec;
+                if (isnan(du)) du = 0.0;
+            }
+            const double bdu = beta * du;
+            bool acc;
+            if (isinf(bdu) && bdu < 0.0) acc = true;
+            else if (-bdu > 709.782712893384) acc = true;
+            else acc = P.accept_u <= exp(-bdu);
+            s_accept = acc ? 1 : 0;
+            if (g == 0) {
+                if (trace && t < trace_cap) {
+                    TraceEntry& T = trace[t];
+                    T.index = i;
+                    T.accepted = acc ? 1 : 0;
+                    T.du = du;
+                    T.x = nx_;
+                    T.y = ny_;
+                    T.z = nz_;
+                }
+                if (acc) {
+                    energy += du;
+                    sum_du += du;
+                    ++accepted;
+                    pos[i] = make_double4(nx_, ny_, nz_, po.w);
+                    if (kCells) {
+                        const int cnew = (cell_coord_s(nz_, half, S.inv_cell, S.nc2) * S.nc1 +
+                                          cell_coord_s(ny_, half, S.inv_cell, S.nc1)) * S.nc0 +
+                                         cell_coord_s(nx_, half, S.inv_cell, S.nc0);
+                        const int cold = S.cell_of[i];
+                        if (cnew != cold) {
+                            const int nold = S.cell_count[cold];
+                            for (int kk = 0; kk < nold; ++kk)
+                                if (S.cell_members[cold * S.cap + kk] == i) {
+                                    S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                    break;
+                                }
+                            S.cell_count[cold] = nold - 1;
+                            const int nn = S.cell_count[cnew];
+                            if (nn >= S.cap) {
+                                S.error = 1;
+                            } else {
+                                S.cell_members[cnew * S.cap + nn] = i;
+                                S.cell_count[cnew] = nn + 1;
+                                S.cell_of[i] = cnew;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (s_accept) {
+#pragma unroll
+            for (int s = 0; s < kSplitMaxKPT; ++s) {
+                q[s].x += dq[s].x;
+                q[s].y += dq[s].y;
+            }
+            h_idx[0] = h_idx[1];
+            h_pos[0][0] = h_pos[1][0];
+            h_pos[0][1] = h_pos[1][1];
+            h_pos[0][2] = h_pos[1][2];
+            h_idx[1] = i;
+            h_pos[1][0] = nx_;
+            h_pos[1][1] = ny_;
+            h_pos[1][2] = nz_;
+        }
+        __syncthreads(); // phase tables are rewritten by the next move
+    }
+#pragma unroll
+    for (int s = 0; s < kSplitMaxKPT; ++s) {
+        const int k = kb + s * 256 + tid;
+        if (k < ke) S.q[k] = q[s];
+    }
+    if (g == 0 && tid == 0) {
+        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.accepted = accepted;
+        S.energy = energy;
+        S.sum_du = sum_du;
+    }
+}
+
+} // namespace
+
+// Advance replica 0 of the batch by `moves` moves with G workgroups (every
+// `stride`-th block of the grid takes part, so that consecutive participants land on
+// the same XCD when stride is the XCD count).
+void Batch::run_split(long long moves, int groups, int stride, bool trace, int trace_cap) {
+    if (!finalized_) finalize();
+    if (size() != 1) throw std::runtime_error("run_split advances a batch of one replica");
+    const ReplicaState& s = h_states_[0];
+    if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error("run_split: 1..64 workgroups");
+    if (stride < 1) throw std::runtime_error("run_split: stride >= 1");
+    const int kw = (s.nk + groups - 1) / groups;
+    if ((kw + 255) / 256 > kSplitMaxKPT) throw std::runtime_error("run_split: too many k-vectors per workgroup");
+    if (2 * s.ncut + 1 > 65) throw std::runtime_error("run_split: ncutoff too large");
+    if (s.ncut * s.ncut >= kGkMax) throw std::runtime_error("run_split: ncutoff too large");
+    hipDeviceProp_t prop;
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    const int grid = groups * stride;
+    int per_cu = 0;
+    const void* fn = s.cells ? reinterpret_cast<const void*>(mc_split_kernel<true>)
+                             : reinterpret_cast<const void*>(mc_split_kernel<false>);
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0), "occupancy");
+    if (per_cu * prop.multiProcessorCount < grid)
+        throw std::runtime_error("run_split: the grid cannot be co-resident");
+    if (!split_slots_) {
+        check_hip(hipMalloc(&split_slots_, sizeof(SplitSlot) * 2 * kSplitMaxG), "hipMalloc slots");
+        check_hip(hipMemset(split_slots_, 0, sizeof(SplitSlot) * 2 * kSplitMaxG), "memset slots");
+        check_hip(hipMalloc(&split_err_, sizeof(int)), "hipMalloc err");
+        check_hip(hipMemset(split_err_, 0, sizeof(int)), "memset err");
+    }
+    if (trace && trace_cap > trace_cap_) {
+        (void)hipFree(d_trace_);
+        check_hip(hipMalloc(&d_trace_, sizeof(TraceEntry) * static_cast<size_t>(trace_cap)), "hipMalloc trace");
+        trace_cap_ = trace_cap;
+    }
+    ReplicaState* st = d_states_;
+    long long M = moves;
+    int G = groups, str = stride;
+    SplitSlot* slots = static_cast<SplitSlot*>(split_slots_);
+    int* err = split_err_;
+    TraceEntry* tr = trace ? d_trace_ : nullptr;
+    int cap = trace ? trace_cap : 0;
+    long long spin = 1LL << 24;
+    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, stream_), "hipLaunchCooperativeKernel");
+    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
+int Batch::split_error() {
+    if (!split_err_) return 0;
+    int e = 0;
+    check_hip(hipStreamSynchronize(stream_), "sync");
+    check_hip(hipMemcpy(&e, split_err_, sizeof(int), hipMemcpyDeviceToHost), "copy err");
+    return e;
+}
+
+} // namespace fmc

@@ -158,6 +158,16 @@ int fmc_batch_add_replica(fmc_batch* batch, const char* config_json, int n, cons
                           double beta, unsigned long long seed, int ladder, int* replica);
 /* Advance replica r by moves[r] trial moves; trace_capacity > 0 records that many moves per replica. */
 int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
+/*
+ * Advance a batch of ONE replica by `moves` moves with `groups` cooperating
+ * workgroups (each owns a slice of the k-vectors; every `stride`-th block of the grid
+ * takes part, so with stride = number of XCDs the participants share one XCD).
+ * The workgroups exchange per-move partial sums through device memory; a wait that
+ * exceeds its spin bound stops the run and sets the error read by
+ * fmc_batch_split_error instead of hanging.
+ */
+int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int stride, int trace_capacity);
+int fmc_batch_split_error(fmc_batch* batch, int* error);
 /* Wait for the batch; kernel time of the last run (ms, HIP events) if timing is on. */
 int fmc_batch_sync(fmc_batch* batch, double* kernel_ms);
 int fmc_batch_set_timing(fmc_batch* batch, int enable);

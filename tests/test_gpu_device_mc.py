@@ -110,6 +110,52 @@ def test_device_mc_full_size_nacl_short():
     assert dq < 1e-9
 
 
+@pytest.mark.parametrize("groups,stride", [(4, 1), (16, 8), (64, 1)])
+def test_split_replica_matches_oracle(groups, stride):
+    """One replica spread over several cooperating workgroups (k-vector slices)."""
+    box = 255.0 * (2000 / 20000) ** (1 / 3)
+    sysd = systems.nacl_lj(n_pairs=1000, box=box, seed=2, kmax=10)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=31)
+    nm = 1500
+    b.run_split(nm, groups=groups, stride=stride, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=31, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("split", groups, stride, rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    # a second launch continues the same chain
+    b.run_split(500, groups=groups, stride=stride, trace=500)
+    b.sync()
+    tr2 = b.trace(0, 500)
+    rec2 = orc.run_philox_translate(500, seed=31, replica=0, move0=nm)
+    rep2 = compare_traces(tr2, rec2, max(1.0, abs(e0)), 1e-10)
+    assert rep2["first_mismatch"] == 500
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+    dq, _ = b.check_q(0)
+    assert dq < 1e-9
+
+
+def test_split_full_size():
+    sysd = systems.nacl_lj(n_pairs=10000, box=255.0, seed=1)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=5)
+    nm = 300
+    b.run_split(nm, groups=64, stride=1, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=5, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("split full size", rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+
+
 def test_batch_of_uneven_replicas_matches_oracle_per_replica():
     """Several 2:1 salt replicas of different N and concentration in one launch."""
     specs = [(600, 0.5, 11), (900, 0.05, 12), (1203, 0.2, 13), (300, 1.0, 14)]
