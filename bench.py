@@ -127,6 +127,34 @@ def bench_split(sysd, args, groups, stride, steps=None):
                 groups=groups, stride=stride)
 
 
+def bench_pipe(sysd, args, groups, depth, steps=None, fp32=False):
+    """One replica over `groups` workgroups with up to `depth` moves in flight."""
+    import faunus_amd as F
+    steps = steps or args.steps
+    b = F.Batch(fp32_pairs=fp32)
+    b.add(sysd, seed=2024)
+    b.set_timing(True)
+    b.run_pipe(STEP_MOVES, groups=groups, depth=depth)
+    b.sync()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run_pipe(STEP_MOVES, groups=groups, depth=depth)
+        kernel_ms += b.sync()
+    dt = time.perf_counter() - t0
+    err = b.split_error()
+    info = b.info(0)
+    tot, _ = b.total_energy(0)
+    dq, _ = b.check_q(0)
+    b.close()
+    if err:
+        raise RuntimeError("pipelined run reported a synchronisation time-out")
+    moves = steps * STEP_MOVES
+    return dict(rate=moves / dt, ms_per_step=dt * 1e3 / steps, kernel_ms=kernel_ms, moves=moves,
+                acceptance=info["accepted"] / max(1, info["moves"]), drift=(info["energy"] - tot) / abs(tot),
+                q_drift=dq, groups=groups, depth=depth)
+
+
 def salt_mix(n_comp=8, rungs=32, seed=1):
     """8 compositions x 32 rungs: N ~ U[600, 9000] (multiple of 3), c ~ logU[0.01, 1] M."""
     from tests import systems
@@ -216,7 +244,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
-    ap.add_argument("--only", choices=["device", "plain", "batch", "split"], default=None)
+    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
@@ -238,6 +266,12 @@ def main():
         log("device-resident FP64: %.0f moves/s (drift %.2e)" % (out["device64"]["rate"], out["device64"]["drift"]))
         out["device32"] = bench_device_single(sysd, args, fp32=True)
         log("device-resident FP32-pair: %.0f moves/s" % out["device32"]["rate"])
+    if args.only == "pipe":
+        for groups, depth in [(16, 1), (16, 2), (16, 3), (16, 4), (32, 4), (64, 4), (12, 4)]:
+            r = bench_pipe(sysd, args, groups, depth, steps=3)
+            out[f"pipe_{groups}_{depth}"] = r
+            log("pipe G=%d D=%d: %.0f moves/s (kernel %.2f us/move)" % (
+                groups, depth, r["rate"], r["kernel_ms"] * 1e3 / r["moves"]))
     if args.only == "split":
         for groups, stride in [(16, 1), (32, 1), (64, 1), (16, 8), (32, 8), (32, 4), (64, 4)]:
             r = bench_split(sysd, args, groups, stride, steps=3)

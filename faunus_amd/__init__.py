@@ -50,6 +50,7 @@ SIGNATURES = {
                                         C.c_ulonglong, C.c_int, I]),
     "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
     "fmc_batch_run_split": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
+    "fmc_batch_run_pipe": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "fmc_batch_split_error": (C.c_int, [C.c_void_p, I]),
     "fmc_batch_sync": (C.c_int, [C.c_void_p, D]),
     "fmc_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -312,6 +313,9 @@ class Batch:
 
     def run_split(self, moves, groups=32, stride=1, trace=0):
         _check(lib().fmc_batch_run_split(self.h, int(moves), int(groups), int(stride), int(trace)))
+
+    def run_pipe(self, moves, groups=16, depth=2, trace=0):
+        _check(lib().fmc_batch_run_pipe(self.h, int(moves), int(groups), int(depth), int(trace)))
 
     def split_error(self):
         e = C.c_int()

@@ -343,6 +343,10 @@ int fmc_batch_run_split(fmc_batch* b, long long moves, int groups, int stride, i
     return guarded([&] { bat(b).run_split(moves, groups, stride, trace_capacity > 0, trace_capacity); });
 }
 
+int fmc_batch_run_pipe(fmc_batch* b, long long moves, int groups, int depth, int trace_capacity) {
+    return guarded([&] { bat(b).run_pipe(moves, groups, depth, trace_capacity > 0, trace_capacity); });
+}
+
 int fmc_batch_split_error(fmc_batch* b, int* error) {
     return guarded([&] { *error = bat(b).split_error(); });
 }

@@ -468,6 +468,7 @@ Batch::~Batch() {
     (void)hipFree(d_moves_);
     (void)hipFree(d_trace_);
     (void)hipFree(split_slots_);
+    (void)hipFree(pipe_slots_);
     (void)hipFree(split_err_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);

@@ -68,6 +68,8 @@ class Batch {
     // one replica spread over `groups` workgroups (every `stride`-th block of the grid)
     void run_split(long long moves, int groups, int stride, bool trace, int trace_cap);
     int split_error();
+    // one replica over `groups` workgroups with up to `depth` moves in flight
+    void run_pipe(long long moves, int groups, int depth, bool trace, int trace_cap);
     void swap_ladders(unsigned long long seed, unsigned long long round);
     void sync();
     int size() const { return static_cast<int>(engines_.size()); }
@@ -103,6 +105,7 @@ class Batch {
     long long swaps_att_ = 0, swaps_acc_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     void* split_slots_ = nullptr;
+    void* pipe_slots_ = nullptr;
     int* split_err_ = nullptr;
 };
 

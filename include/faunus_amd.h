@@ -167,6 +167,14 @@ int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
  * fmc_batch_split_error instead of hanging.
  */
 int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int stride, int trace_capacity);
+/*
+ * As fmc_batch_run_split, with up to `depth` (1..4) moves evaluated ahead of their
+ * predecessors' decisions: each move's partial sums carry the Ewald cross terms with
+ * the pending moves' dQ(k) and the pair terms with each pending particle at its old
+ * and new position, so every decision is exactly the one-move-at-a-time Metropolis
+ * decision while the exchange between workgroups overlaps the next moves' work.
+ */
+int fmc_batch_run_pipe(fmc_batch* batch, long long moves, int groups, int depth, int trace_capacity);
 int fmc_batch_split_error(fmc_batch* batch, int* error);
 /* Wait for the batch; kernel time of the last run (ms, HIP events) if timing is on. */
 int fmc_batch_sync(fmc_batch* batch, double* kernel_ms);

@@ -140,6 +140,41 @@ def test_split_replica_matches_oracle(groups, stride):
     assert dq < 1e-9
 
 
+@pytest.mark.parametrize("groups,depth,kind", [(4, 1, "cells"), (16, 2, "cells"), (16, 4, "cells"),
+                                               (64, 3, "cells"), (8, 4, "pm200"), (16, 4, "salt")])
+def test_pipelined_replica_matches_oracle(groups, depth, kind):
+    """Moves evaluated ahead of their predecessors' decisions; exact decisions."""
+    if kind == "cells":
+        box = 255.0 * (2000 / 20000) ** (1 / 3)
+        sysd = systems.nacl_lj(n_pairs=1000, box=box, seed=2, kmax=10)
+    elif kind == "pm200":  # N=200: proposals often hit a pending particle (pipeline drains)
+        sysd = systems.primitive_model(n_pairs=100, box=118.0, seed=1)
+    else:  # no cell list (Rc = L/2): all-pairs branch
+        sysd = systems.salt21(n=900, conc=0.05, seed=4)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=77)
+    nm = 2000
+    b.run_pipe(nm, groups=groups, depth=depth, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=77, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("pipe", groups, depth, kind, rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    b.run_pipe(700, groups=groups, depth=depth, trace=700)
+    b.sync()
+    rec2 = orc.run_philox_translate(700, seed=77, replica=0, move0=nm)
+    rep2 = compare_traces(b.trace(0, 700), rec2, max(1.0, abs(e0)), 1e-10)
+    assert rep2["first_mismatch"] == 700
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+    dq, _ = b.check_q(0)
+    assert dq < 1e-9
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+
+
 def test_split_full_size():
     sysd = systems.nacl_lj(n_pairs=10000, box=255.0, seed=1)
     orc, e0 = oracle_of(sysd)
