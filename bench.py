@@ -237,6 +237,9 @@ def cpu_baseline(seconds=15.0):
                        f"the reference, no cell list, 1 thread, {dt:.1f} s)")
 
 
+HEADLINE_GROUPS = 16
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,7 +247,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
-    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe"], default=None)
+    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
@@ -254,26 +257,37 @@ def main():
     if F.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only == "copy":
+        nbytes = 1 << 28
+        ms = F.copy_probe(nbytes, repeats=10)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_launch": ms,
+                          "GBps_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
     sysd = nacl_system()
     K = 17328
     n = 20000
     out = {}
+    if args.only in (None, "headline"):
+        out["split64"] = bench_split(sysd, args, HEADLINE_GROUPS, 1)
+        log("split FP64 (G=%d): %.0f moves/s, drift %.2e" % (HEADLINE_GROUPS, out["split64"]["rate"],
+                                                              out["split64"]["drift"]))
     if args.only in (None, "plain"):
         out["plain"] = bench_plain_single(sysd, PLAIN_MOVES)
         log("plain FP64 one move per launch: %.0f moves/s, kernel %.1f us" % (out["plain"]["rate"], out["plain"]["kernel_us"]))
     if args.only in (None, "device"):
         out["device64"] = bench_device_single(sysd, args, fp32=False)
-        log("device-resident FP64: %.0f moves/s (drift %.2e)" % (out["device64"]["rate"], out["device64"]["drift"]))
+        log("device-resident FP64, one workgroup: %.0f moves/s (drift %.2e)" % (out["device64"]["rate"],
+                                                                                 out["device64"]["drift"]))
         out["device32"] = bench_device_single(sysd, args, fp32=True)
-        log("device-resident FP32-pair: %.0f moves/s" % out["device32"]["rate"])
+        log("device-resident FP32-pair, one workgroup: %.0f moves/s" % out["device32"]["rate"])
     if args.only == "pipe":
-        for groups, depth in [(16, 1), (16, 2), (16, 3), (16, 4), (32, 4), (64, 4), (12, 4)]:
+        for groups, depth in [(16, 1), (16, 2), (16, 4), (32, 2)]:
             r = bench_pipe(sysd, args, groups, depth, steps=3)
             out[f"pipe_{groups}_{depth}"] = r
             log("pipe G=%d D=%d: %.0f moves/s (kernel %.2f us/move)" % (
                 groups, depth, r["rate"], r["kernel_ms"] * 1e3 / r["moves"]))
     if args.only == "split":
-        for groups, stride in [(16, 1), (32, 1), (64, 1), (16, 8), (32, 8), (32, 4), (64, 4)]:
+        for groups, stride in [(12, 1), (16, 1), (24, 1), (32, 1), (16, 8)]:
             r = bench_split(sysd, args, groups, stride, steps=3)
             out[f"split_{groups}_{stride}"] = r
             log("split G=%d stride=%d: %.0f moves/s (kernel %.2f us/move)" % (
@@ -291,7 +305,7 @@ def main():
         print(json.dumps({k: v for k, v in out.items()}, default=float))
         return
     cpu = None if args.no_cpu else cpu_baseline()
-    d = out["device64"]
+    d = out["split64"]
     kernel_s_per_move = d["kernel_ms"] / 1e3 / d["moves"]
     flops = ewald_flops_per_move(K)
     algo_bytes = 64.0 * K + pair_bytes_cells(n, 255.0, 12.0)
@@ -311,21 +325,24 @@ def main():
         "dtype": "f64",
         "data": "synthetic (random sequential addition, seed 1; device Philox proposals)",
         "config": {"workload": "N=20000 Na/Cl LJ(LB, cut 12 A)+Ewald(alpha 0.25, Rc 12, kmax 20, K=17328), "
-                               "L=255 A, translate dp=2 A, device-resident Metropolis, one workgroup",
+                               "L=255 A, translate dp=2 A, device-resident Metropolis, %d cooperating "
+                               "workgroups (k-vector slices)" % HEADLINE_GROUPS,
                    "step_moves": STEP_MOVES, "acceptance": d["acceptance"], "energy_drift": d["drift"]},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "mc_advance_kernel",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "mc_split_kernel",
                      "algorithmic_bytes_per_move": algo_bytes, "kernel_us_per_move": kernel_s_per_move * 1e6,
                      "fp64_tflops": achieved_tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                      "algorithmic_flops_per_move": flops,
-                     "note": "one replica = one workgroup on one of 256 CUs; Q(k) and k-vectors stay in "
-                             "registers/LDS, so HBM traffic is far below the algorithmic bytes"},
+                     "note": "one launch runs STEP_MOVES sequential moves; achieved = algorithmic bytes per move "
+                             "/ kernel time per move (HIP events); Q(k) stays in registers, so HBM traffic is "
+                             "far below the algorithmic bytes; latency-bound (see DESIGN.md)"},
         "cpu_baseline": cpu,
         "forms": {
             "plain_fp64_one_move_per_launch": {"moves_per_s": out["plain"]["rate"],
                                                "kernel_us": out["plain"]["kernel_us"]},
-            "device_fp64": {"moves_per_s": d["rate"]},
-            "device_fp32_pairs": {"moves_per_s": out["device32"]["rate"]},
+            "device_fp64_split_%d_workgroups" % HEADLINE_GROUPS: {"moves_per_s": d["rate"]},
+            "device_fp64_one_workgroup": {"moves_per_s": out["device64"]["rate"]},
+            "device_fp32_pairs_one_workgroup": {"moves_per_s": out["device32"]["rate"]},
         },
     }
     if "batch32" in out:

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fmc_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, I, U8, D, D]),
     "fmc_batch_total_energy": (C.c_int, [C.c_void_p, C.c_int, D, D]),
     "fmc_batch_check_q": (C.c_int, [C.c_void_p, C.c_int, D, D]),
+    "fmc_copy_probe": (C.c_int, [C.c_longlong, C.c_int, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
 }
@@ -103,6 +104,13 @@ def device_count():
     n = C.c_int()
     _check(lib().fmc_device_count(C.byref(n)))
     return n.value
+
+
+def copy_probe(nbytes, repeats=20):
+    """ms per launch of a streaming copy of `nbytes` bytes (HBM counter calibration)."""
+    ms = C.c_double()
+    _check(lib().fmc_copy_probe(int(nbytes), int(repeats), C.byref(ms)))
+    return ms.value
 
 
 def make_change(change):

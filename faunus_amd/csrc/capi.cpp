@@ -269,6 +269,10 @@ int fmc_run_translate(fmc_engine* e, long count, const int* group, const int* in
     });
 }
 
+int fmc_copy_probe(long long bytes, int repeats, double* ms_per_launch) {
+    return guarded([&] { *ms_per_launch = fmc::copy_probe(bytes, repeats); });
+}
+
 int fmc_set_timing(fmc_engine* e, int enable) {
     return guarded([&] { eng(e).set_timing(enable != 0); });
 }

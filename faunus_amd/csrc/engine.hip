@@ -423,7 +423,40 @@ __global__ __launch_bounds__(kBlock) void all_pairs_kernel(const double4* __rest
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// Plain streaming copy of a known byte count; its HBM counter readings calibrate
+// the FETCH_SIZE / WRITE_SIZE scale of the profiles.
+__global__ __launch_bounds__(kBlock) void copy_probe_kernel(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                            long long n) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
+
 } // namespace
+
+double copy_probe(long long bytes, int repeats) {
+    const long long n = bytes / static_cast<long long>(sizeof(double4));
+    if (n <= 0 || repeats <= 0) throw std::runtime_error("copy_probe: bad size");
+    double4 *src = nullptr, *dst = nullptr;
+    check_hip(hipMalloc(&src, sizeof(double4) * n), "hipMalloc probe");
+    check_hip(hipMalloc(&dst, sizeof(double4) * n), "hipMalloc probe");
+    check_hip(hipMemset(src, 0, sizeof(double4) * n), "memset probe");
+    hipEvent_t a, b;
+    check_hip(hipEventCreate(&a), "event");
+    check_hip(hipEventCreate(&b), "event");
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(kBlock), 0, nullptr, src, dst, n);
+    check_hip(hipEventRecord(a, nullptr), "event");
+    for (int r = 0; r < repeats; ++r) hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(kBlock), 0, nullptr, src, dst, n);
+    check_hip(hipEventRecord(b, nullptr), "event");
+    check_hip(hipEventSynchronize(b), "event sync");
+    float ms = 0;
+    check_hip(hipEventElapsedTime(&ms, a, b), "elapsed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return ms / repeats;
+}
 
 // ==============================================================================
 Engine::Engine(const std::string& config_json) : model_(Model::from_json(config_json)) {

@@ -184,5 +184,7 @@ class Engine {
 };
 
 void check_hip(hipError_t e, const char* what);
+// Time (ms per launch) of a streaming copy of `bytes` bytes read + written.
+double copy_probe(long long bytes, int repeats);
 
 } // namespace fmc

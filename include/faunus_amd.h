@@ -191,6 +191,12 @@ int fmc_batch_trace(fmc_batch* batch, int replica, int count, int* index, unsign
 int fmc_batch_total_energy(fmc_batch* batch, int replica, double* terms, double* total);
 int fmc_batch_check_q(fmc_batch* batch, int replica, double* max_abs_dq, double* rel_reciprocal);
 
+/*
+ * Streaming copy of `bytes` bytes (read once, written once), `repeats` launches: the
+ * known-size reference for scaling HBM counter readings.  No reference counterpart.
+ */
+int fmc_copy_probe(long long bytes, int repeats, double* ms_per_launch);
+
 /* Andrea spline of the Coulomb short-range function built by the engine (for tests). */
 int fmc_coulomb_spline(fmc_engine* engine, int capacity, double* knots, double* coeff, int* num_knots);
 int fmc_num_kvectors(fmc_engine* engine, int* count);

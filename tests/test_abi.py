@@ -47,3 +47,11 @@ def test_no_cpu_fallback_without_gpu():
     from tests import systems
     with pytest.raises(RuntimeError, match="device"):
         F.Engine(systems.primitive_model(n_pairs=5, box=40.0)[0])
+
+
+def test_cpp_adapters_compile_standalone():
+    """The EnergyTerm-shaped C++ adapters build against the C header alone."""
+    src = os.path.join(ROOT, "faunus_amd", "host", "gpu_energy.hpp")
+    out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-x", "c++", src],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
