@@ -65,6 +65,16 @@ SIGNATURES = {
     "fmc_copy_probe": (C.c_int, [C.c_longlong, C.c_int, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
+    "fmc_server_create": (C.c_int, [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]),
+    "fmc_server_destroy": (C.c_int, [C.c_void_p]),
+    "fmc_server_submit": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_longlong,
+                                    C.c_ulonglong, I]),
+    "fmc_server_step": (C.c_int, [C.c_void_p, C.c_longlong, I, I]),
+    "fmc_server_job": (C.c_int, [C.c_void_p, C.c_int, I, D, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong),
+                                 D, D, D]),
+    "fmc_server_job_positions": (C.c_int, [C.c_void_p, C.c_int, D]),
+    "fmc_server_job_error": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    "fmc_server_status": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
 }
 
 
@@ -113,6 +123,29 @@ def copy_probe(nbytes, repeats=20):
     return ms.value
 
 
+def _particles(xyz, charge, atom_id):
+    """Contiguous particle arrays with matching lengths (3n positions, n charges, n ids)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    charge = np.ascontiguousarray(charge, np.float64).ravel()
+    atom_id = np.ascontiguousarray(atom_id, np.int32).ravel()
+    n = len(charge)
+    if xyz.size != 3 * n or len(atom_id) != n:
+        raise ValueError(f"particle arrays disagree: {xyz.size // 3} positions, {n} charges, {len(atom_id)} ids")
+    return xyz, charge, atom_id
+
+
+def _groups(begin, size, molid, mass_center=None):
+    begin = np.ascontiguousarray(begin, np.int32).ravel()
+    size = np.ascontiguousarray(size, np.int32).ravel()
+    molid = np.ascontiguousarray(molid, np.int32).ravel()
+    if not (len(begin) == len(size) == len(molid)):
+        raise ValueError("group arrays disagree in length")
+    cm = None if mass_center is None else np.ascontiguousarray(mass_center, np.float64)
+    if cm is not None and cm.size != 3 * len(begin):
+        raise ValueError("mass_center needs 3 values per group")
+    return begin, size, molid, cm
+
+
 def make_change(change):
     """Build an ``fmc_change`` from a dict {"everything": bool, "groups": [{group_index,
     all, internal, relative_atom_indices}]}; returns (struct, keep-alive arrays)."""
@@ -155,17 +188,12 @@ class Engine:
             pass
 
     def set_particles(self, xyz, charge, atom_id):
-        xyz = np.ascontiguousarray(xyz, np.float64)
-        charge = np.ascontiguousarray(charge, np.float64)
-        atom_id = np.ascontiguousarray(atom_id, np.int32)
+        xyz, charge, atom_id = _particles(xyz, charge, atom_id)
         self.n = len(charge)
         _check(lib().fmc_set_particles(self.h, self.n, _p(xyz), _p(charge), _p(atom_id, C.c_int)))
 
     def set_groups(self, begin, size, molid, mass_center=None):
-        begin = np.ascontiguousarray(begin, np.int32)
-        size = np.ascontiguousarray(size, np.int32)
-        molid = np.ascontiguousarray(molid, np.int32)
-        cm = None if mass_center is None else np.ascontiguousarray(mass_center, np.float64)
+        begin, size, molid, cm = _groups(begin, size, molid, mass_center)
         _check(lib().fmc_set_groups(self.h, len(begin), _p(begin, C.c_int), _p(size, C.c_int),
                                     _p(molid, C.c_int), _p(cm)))
 
@@ -302,12 +330,8 @@ class Batch:
     def add(self, system, beta=1.0, seed=1, ladder=-1):
         cfg, xyz, q, ids, g = system
         text = cfg if isinstance(cfg, str) else json.dumps(cfg)
-        xyz = np.ascontiguousarray(xyz, np.float64)
-        q = np.ascontiguousarray(q, np.float64)
-        ids = np.ascontiguousarray(ids, np.int32)
-        gb = np.ascontiguousarray(g["begin"], np.int32)
-        gs = np.ascontiguousarray(g["size"], np.int32)
-        gm = np.ascontiguousarray(g["molid"], np.int32)
+        xyz, q, ids = _particles(xyz, q, ids)
+        gb, gs, gm, _ = _groups(g["begin"], g["size"], g["molid"])
         r = C.c_int()
         _check(lib().fmc_batch_add_replica(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
                                            _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), float(beta),
@@ -373,3 +397,69 @@ class Batch:
         a, b = C.c_double(), C.c_double()
         _check(lib().fmc_batch_check_q(self.h, r, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class Server:
+    """Continuous-batching serving loop: jobs are translational Metropolis runs that
+    share persistent launches while their HBM footprint fits the budget."""
+
+    QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, fp32_pairs=True, hbm_budget_bytes=64 << 30, max_slots=1024):
+        h = C.c_void_p()
+        _check(lib().fmc_server_create(int(bool(fp32_pairs)), int(hbm_budget_bytes), int(max_slots), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fmc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, system, moves, seed):
+        cfg, xyz, q, ids, g = system
+        text = cfg if isinstance(cfg, str) else json.dumps(cfg)
+        xyz, q, ids = _particles(xyz, q, ids)
+        gb, gs, gm, _ = _groups(g["begin"], g["size"], g["molid"])
+        j = C.c_int()
+        _check(lib().fmc_server_submit(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
+                                       _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), int(moves), int(seed),
+                                       C.byref(j)))
+        self.sizes[j.value] = len(q)
+        return j.value
+
+    def step(self, quantum=2000):
+        r, q = C.c_int(), C.c_int()
+        _check(lib().fmc_server_step(self.h, int(quantum), C.byref(r), C.byref(q)))
+        return r.value, q.value
+
+    def job(self, j):
+        st, e = C.c_int(), C.c_double()
+        m, a = C.c_ulonglong(), C.c_ulonglong()
+        t0, t1, t2 = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().fmc_server_job(self.h, int(j), C.byref(st), C.byref(e), C.byref(m), C.byref(a), C.byref(t0),
+                                    C.byref(t1), C.byref(t2)))
+        out = dict(state=st.value, energy=e.value, moves=m.value, accepted=a.value, t_submit=t0.value,
+                   t_admit=t1.value, t_done=t2.value)
+        if st.value == self.FAILED:
+            buf = C.create_string_buffer(512)
+            _check(lib().fmc_server_job_error(self.h, int(j), buf, 512))
+            out["error"] = buf.value.decode()
+        return out
+
+    def positions(self, j):
+        xyz = np.zeros((self.sizes[j], 3))
+        _check(lib().fmc_server_job_positions(self.h, int(j), _p(xyz)))
+        return xyz
+
+    def status(self):
+        t = C.c_double()
+        m, b = C.c_longlong(), C.c_longlong()
+        _check(lib().fmc_server_status(self.h, C.byref(t), C.byref(m), C.byref(b)))
+        return dict(seconds=t.value, moves_done=m.value, bytes_in_use=b.value)

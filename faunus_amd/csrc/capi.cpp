@@ -434,3 +434,91 @@ int fmc_batch_check_q(fmc_batch* b, int replica, double* max_abs_dq, double* rel
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- serving
+#include "serve.hpp"
+
+struct fmc_server {
+    std::unique_ptr<fmc::Server> impl;
+};
+
+namespace {
+fmc::Server& srv(fmc_server* s) {
+    if (!s || !s->impl) throw std::runtime_error("null server handle");
+    return *s->impl;
+}
+} // namespace
+
+extern "C" {
+
+int fmc_server_create(int fp32_pairs, long long hbm_budget_bytes, int max_slots, fmc_server** out) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw std::runtime_error("no HIP device visible");
+        auto* s = new fmc_server;
+        s->impl = std::make_unique<fmc::Server>(fp32_pairs, hbm_budget_bytes, max_slots);
+        *out = s;
+    });
+}
+
+int fmc_server_destroy(fmc_server* s) {
+    return guarded([&] { delete s; });
+}
+
+int fmc_server_submit(fmc_server* s, const char* config_json, int n, const double* xyz, const double* charge,
+                      const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                      long long moves, unsigned long long seed, int* job) {
+    return guarded([&] {
+        const int id = srv(s).submit(config_json, n, xyz, charge, atom_id, num_groups, begin, size, molid, moves, seed);
+        if (job) *job = id;
+    });
+}
+
+int fmc_server_step(fmc_server* s, long long quantum, int* resident, int* queued) {
+    return guarded([&] {
+        srv(s).step(quantum);
+        if (resident) *resident = srv(s).resident();
+        if (queued) *queued = srv(s).queued();
+    });
+}
+
+int fmc_server_job(fmc_server* s, int job, int* state, double* energy, unsigned long long* moves,
+                   unsigned long long* accepted, double* t_submit, double* t_admit, double* t_done) {
+    return guarded([&] {
+        const fmc::JobRecord& j = srv(s).job(job);
+        if (state) *state = j.state;
+        if (energy) *energy = j.energy;
+        if (moves) *moves = j.moves_done;
+        if (accepted) *accepted = j.accepted;
+        if (t_submit) *t_submit = j.t_submit;
+        if (t_admit) *t_admit = j.t_admit;
+        if (t_done) *t_done = j.t_done;
+    });
+}
+
+int fmc_server_job_positions(fmc_server* s, int job, double* xyz) {
+    return guarded([&] {
+        const fmc::JobRecord& j = srv(s).job(job);
+        if (j.state != 2) throw std::runtime_error("job not finished");
+        std::copy(j.final_xyz.begin(), j.final_xyz.end(), xyz);
+    });
+}
+
+int fmc_server_status(fmc_server* s, double* seconds, long long* moves_done, long long* bytes_in_use) {
+    return guarded([&] {
+        if (seconds) *seconds = srv(s).now();
+        if (moves_done) *moves_done = srv(s).moves_done_total();
+        if (bytes_in_use) *bytes_in_use = static_cast<long long>(srv(s).footprint_in_use());
+    });
+}
+
+} // extern "C"
+
+extern "C" int fmc_server_job_error(fmc_server* s, int job, char* buffer, int length) {
+    return guarded([&] {
+        const fmc::JobRecord& j = srv(s).job(job);
+        if (!buffer || length <= 0) throw std::runtime_error("buffer too small");
+        std::strncpy(buffer, j.error.c_str(), static_cast<size_t>(length - 1));
+        buffer[length - 1] = 0;
+    });
+}

@@ -553,6 +553,8 @@ void Engine::set_groups(int ng, const int* begin, const int* size, const int* mo
     any_molecular_ = false;
     for (int g = 0; g < ng; ++g) {
         if (g_begin_[g] != expect) throw std::runtime_error("set_groups: groups must tile the particles in order");
+        if (g_size_[g] < 0 || g_size_[g] > n_ - expect)
+            throw std::runtime_error("set_groups: group extends past the last particle");
         if (g_molid_[g] < 0 || g_molid_[g] >= static_cast<int>(model_.molecules.size()))
             throw std::runtime_error("set_groups: molecule id out of range");
         for (int i = 0; i < g_size_[g]; ++i) h_gid_[static_cast<size_t>(g_begin_[g] + i)] = g;

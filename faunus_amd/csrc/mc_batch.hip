@@ -195,25 +195,57 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         // nonbonded pair sums of the moved particle in the new and old position
         double e_new = 0.0, e_old = 0.0;
         if (!kCells) {
-            for (int j = tid; j < n; j += kThreads) {
-                if (j == i) continue;
-                const int tj = static_cast<int>(meta[j] & 0xffu);
+            // the loads of a batch of neighbours are issued together so that their latency overlaps
+            constexpr int kB = kF32 ? 8 : 4;
+            for (int j0 = tid; j0 < n; j0 += kThreads * kB) {
                 if (kF32) {
-                    const float4 pj = pos32[j];
-                    const DevPairF& p = s_pairs_f[ti * nt + tj];
+                    float4 pj[kB];
+                    uint32_t mj[kB];
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j < n) {
+                            pj[u] = pos32[j];
+                            mj[u] = meta[j];
+                        }
+                    }
                     const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-                    e_new += static_cast<double>(pair_energy_f(p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj.w,
-                                                               dist2f(fL, fh, static_cast<float>(nx_), static_cast<float>(ny_),
-                                                                      static_cast<float>(nz_), pj.x, pj.y, pj.z)));
-                    e_old += static_cast<double>(pair_energy_f(p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj.w,
-                                                               dist2f(fL, fh, static_cast<float>(po.x), static_cast<float>(po.y),
-                                                                      static_cast<float>(po.z), pj.x, pj.y, pj.z)));
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j < n && j != i) {
+                            const DevPairF& p = s_pairs_f[ti * nt + static_cast<int>(mj[u] & 0xffu)];
+                            e_new += static_cast<double>(pair_energy_f(
+                                p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj[u].w,
+                                dist2f(fL, fh, static_cast<float>(nx_), static_cast<float>(ny_), static_cast<float>(nz_),
+                                       pj[u].x, pj[u].y, pj[u].z)));
+                            e_old += static_cast<double>(pair_energy_f(
+                                p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj[u].w,
+                                dist2f(fL, fh, static_cast<float>(po.x), static_cast<float>(po.y), static_cast<float>(po.z),
+                                       pj[u].x, pj[u].y, pj[u].z)));
+                        }
+                    }
                 } else {
-                    const double4 pj = pos[j];
-                    const DevPair& p = s_pairs[ti * nt + tj];
-                    Box b{{L, L, L}, {half, half, half}};
-                    e_new += pair_energy(p, s_coul, po.w, pj.w, dist2(b, nx_, ny_, nz_, pj.x, pj.y, pj.z));
-                    e_old += pair_energy(p, s_coul, po.w, pj.w, dist2(b, po.x, po.y, po.z, pj.x, pj.y, pj.z));
+                    double4 pj[kB];
+                    uint32_t mj[kB];
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j < n) {
+                            pj[u] = pos[j];
+                            mj[u] = meta[j];
+                        }
+                    }
+                    const Box b{{L, L, L}, {half, half, half}};
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j < n && j != i) {
+                            const DevPair& p = s_pairs[ti * nt + static_cast<int>(mj[u] & 0xffu)];
+                            e_new += pair_energy(p, s_coul, po.w, pj[u].w, dist2(b, nx_, ny_, nz_, pj[u].x, pj[u].y, pj[u].z));
+                            e_old += pair_energy(p, s_coul, po.w, pj[u].w, dist2(b, po.x, po.y, po.z, pj[u].x, pj[u].y, pj[u].z));
+                        }
+                    }
                 }
             }
         } else {
@@ -461,9 +493,7 @@ Batch::Batch(int precision_fp32) : fp32_(precision_fp32) {
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_) (void)hipGraphExecDestroy(graph_);
-    for (auto p : gk_bufs_) (void)hipFree(p);
-    for (auto p : pos32_bufs_) (void)hipFree(p);
-    for (auto p : dp_bufs_) (void)hipFree(p);
+    for (auto p : owned_) (void)hipFree(p);
     (void)hipFree(d_states_);
     (void)hipFree(d_moves_);
     (void)hipFree(d_trace_);
@@ -475,22 +505,14 @@ Batch::~Batch() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-int Batch::add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
-                       int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
-                       unsigned long long seed, int ladder) {
-    if (finalized_) throw std::runtime_error("batch already finalized");
-    auto e = std::make_unique<Engine>(config_json);
-    e->set_particles(n, xyz, charge, ids);
-    e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
-    e->init();
-    const Model& m = e->model();
+ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
+                                std::vector<void*>& owned) {
+    const Model& m = e.model();
+    const int n = e.num_particles();
     if (!(m.box[0] == m.box[1] && m.box[1] == m.box[2])) throw std::runtime_error("device MC needs a cubic box");
-    if (m.g2g_cutoff2 < std::numeric_limits<double>::max() && !m.molecules.empty())
-        for (const auto& mol : m.molecules)
-            if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
     ReplicaState s{};
     s.n = n;
-    s.nk = e->num_kvectors();
+    s.nk = e.num_kvectors();
     s.ncut = m.ewald.present ? static_cast<int>(std::ceil(m.ewald.n_cutoff)) : 0;
     if (s.ncut * s.ncut >= kMaxGk) throw std::runtime_error("ncutoff too large for the device MC (max 32)");
     if (m.ewald.present && !m.ewald.spherical_sum) throw std::runtime_error("device MC needs spherical_sum: true");
@@ -501,13 +523,13 @@ int Batch::add_replica(const std::string& config_json, int n, const double* xyz,
     s.half = 0.5 * m.box[0];
     s.inv = 1.0 / m.box[0];
     s.rec_prefactor = m.ewald.present ? 2 * kPi * m.ewald.lB / (m.box[0] * m.box[1] * m.box[2]) : 0.0;
-    s.pos = e->d_pos();
-    s.meta = e->d_meta();
-    s.q = e->d_q_acc();
-    s.kn = e->d_kn();
-    s.pairs = e->d_pairs();
-    s.coul = e->d_coul();
-    CellList& cl = e->cells();
+    s.pos = e.d_pos();
+    s.meta = e.d_meta();
+    s.q = e.d_q_acc();
+    s.kn = e.d_kn();
+    s.pairs = e.d_pairs();
+    s.coul = e.d_coul();
+    CellList& cl = e.cells();
     s.cells = cl.enabled ? 1 : 0;
     if (cl.enabled) {
         if (!(cl.nc[0] == cl.nc[1] && cl.nc[1] == cl.nc[2])) throw std::runtime_error("cubic cell grid expected");
@@ -531,36 +553,69 @@ int Batch::add_replica(const std::string& config_json, int n, const double* xyz,
     }
     double* d_gk = nullptr;
     check_hip(hipMalloc(&d_gk, sizeof(double) * kMaxGk), "hipMalloc gk");
+    owned.push_back(d_gk);
     check_hip(hipMemcpy(d_gk, gk.data(), sizeof(double) * kMaxGk, hipMemcpyHostToDevice), "copy gk");
-    gk_bufs_.push_back(d_gk);
     s.gk = d_gk;
     std::vector<double> dpt(static_cast<size_t>(m.ntypes));
     for (int t = 0; t < m.ntypes; ++t) dpt[static_cast<size_t>(t)] = m.atoms[static_cast<size_t>(t)].dp;
     double* d_dp = nullptr;
     check_hip(hipMalloc(&d_dp, sizeof(double) * m.ntypes), "hipMalloc dp");
+    owned.push_back(d_dp);
     check_hip(hipMemcpy(d_dp, dpt.data(), sizeof(double) * m.ntypes, hipMemcpyHostToDevice), "copy dp");
-    dp_bufs_.push_back(d_dp);
     s.dp_type = d_dp;
     float4* d_p32 = nullptr;
-    if (fp32_) {
+    if (fp32) {
+        std::vector<double> xyz(3 * static_cast<size_t>(n));
+        e.get_positions(xyz.data());
+        const auto& q = e.host_charges();
         std::vector<float4> p32(static_cast<size_t>(n));
         for (int i = 0; i < n; ++i)
             p32[static_cast<size_t>(i)] = make_float4(static_cast<float>(xyz[3 * i]), static_cast<float>(xyz[3 * i + 1]),
-                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(charge[i]));
+                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(q[static_cast<size_t>(i)]));
         check_hip(hipMalloc(&d_p32, sizeof(float4) * n), "hipMalloc pos32");
+        owned.push_back(d_p32);
         check_hip(hipMemcpy(d_p32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
-        pos32_bufs_.push_back(d_p32);
     }
     s.pos32 = d_p32;
     s.seed = seed;
-    s.replica_id = static_cast<int>(engines_.size());
+    s.replica_id = stream_id;
     s.ladder = ladder;
     s.beta = beta;
-    std::vector<double> terms(static_cast<size_t>(e->num_terms()));
-    e->total_energy(terms.data());
+    std::vector<double> terms(static_cast<size_t>(e.num_terms()));
+    e.total_energy(terms.data());
     double u0 = 0;
     for (double v : terms) u0 += v;
     s.energy = u0;
+    return s;
+}
+
+void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
+                    TraceEntry* trace, int trace_cap, hipStream_t stream) {
+    int maxkpt = 0, maxnk = 0, maxw = 0;
+    for (int r = 0; r < count; ++r) {
+        maxkpt = std::max(maxkpt, h_states[r].kpt);
+        maxnk = std::max(maxnk, h_states[r].nk);
+        maxw = std::max(maxw, 2 * h_states[r].ncut + 1);
+        if (h_states[r].cells != h_states[0].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
+    }
+    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
+    if (lds > 120 * 1024) throw std::runtime_error("k-vector table too large for LDS");
+    const bool cells = h_states[0].cells != 0;
+    if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
+    else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
+    else dispatch_cells<256, 68>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
+    check_hip(hipGetLastError(), "mc_advance_kernel launch");
+}
+
+int Batch::add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
+                       int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
+                       unsigned long long seed, int ladder) {
+    if (finalized_) throw std::runtime_error("batch already finalized");
+    auto e = std::make_unique<Engine>(config_json);
+    e->set_particles(n, xyz, charge, ids);
+    e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
+    e->init();
+    ReplicaState s = make_replica_state(*e, fp32_ != 0, seed, static_cast<int>(engines_.size()), beta, ladder, owned_);
     h_states_.push_back(s);
     engines_.push_back(std::move(e));
     return s.replica_id;
@@ -600,25 +655,11 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
             trace_cap_ = trace_cap;
         }
     }
-    const int threads = threads_for(0);
-    int maxkpt = 0, maxnk = 0, maxw = 0;
-    for (auto& s : h_states_) {
-        maxkpt = std::max(maxkpt, s.kpt);
-        maxnk = std::max(maxnk, s.nk);
-        maxw = std::max(maxw, 2 * s.ncut + 1);
-    }
-    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
-    if (lds > 120 * 1024) throw std::runtime_error("k-vector table too large for LDS");
-    const bool cells = h_states_[0].cells != 0;
-    const bool f32 = fp32_ != 0;
+    if (threads_for(0) != 256) throw std::runtime_error("unsupported workgroup size");
     TraceEntry* tr = trace ? d_trace_ : nullptr;
     const int cap = trace ? trace_cap : 0;
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
-    if (threads != 256) throw std::runtime_error("unsupported workgroup size");
-    if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
-    else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
-    else dispatch_cells<256, 68>(cells, f32, size(), lds, stream_, d_states_, d_moves_, tr, cap);
-    check_hip(hipGetLastError(), "mc_advance_kernel launch");
+    launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 
@@ -633,8 +674,7 @@ void Batch::sync() {
 
 void Batch::swap_ladders(unsigned long long seed, unsigned long long round) {
     // ladders: replicas with equal ladder id, ordered by decreasing beta at creation
-    static thread_local std::vector<int> dummy;
-    (void)dummy;
+    if (!finalized_) finalize();
     std::vector<int> lb, ls, slots;
     std::vector<int> ids;
     for (auto& s : h_states_) ids.push_back(s.ladder);
@@ -681,6 +721,8 @@ void Batch::swap_ladders(unsigned long long seed, unsigned long long round) {
 }
 
 ReplicaState Batch::state(int r) {
+    if (r < 0 || r >= size()) throw std::runtime_error("replica index out of range");
+    if (!finalized_) finalize();
     sync();
     ReplicaState s;
     check_hip(hipMemcpy(&s, d_states_ + r, sizeof(ReplicaState), hipMemcpyDeviceToHost), "copy state");
@@ -688,6 +730,7 @@ ReplicaState Batch::state(int r) {
 }
 
 void Batch::positions(int r, double* xyz) {
+    if (r < 0 || r >= size()) throw std::runtime_error("replica index out of range");
     sync();
     Engine& e = *engines_[static_cast<size_t>(r)];
     e.refresh_host_positions();
@@ -695,6 +738,7 @@ void Batch::positions(int r, double* xyz) {
 }
 
 std::vector<TraceEntry> Batch::trace(int r, int count) {
+    if (r < 0 || r >= size() || count < 0) throw std::runtime_error("trace: replica index or count out of range");
     sync();
     std::vector<TraceEntry> t(static_cast<size_t>(count));
     if (!d_trace_ || count > trace_cap_) throw std::runtime_error("trace not recorded");

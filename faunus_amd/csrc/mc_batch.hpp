@@ -54,6 +54,14 @@ struct TraceEntry {
     double x, y, z; // proposed position
 };
 
+// Device state of one replica (engine already initialised); device buffers it makes
+// are appended to `owned`, which the caller frees.
+ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
+                                std::vector<void*>& owned);
+// One persistent launch advancing `count` replicas (states already on the device).
+void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
+                    TraceEntry* trace, int trace_cap, hipStream_t stream);
+
 // A set of replicas advanced together on one stream.
 class Batch {
   public:
@@ -90,9 +98,7 @@ class Batch {
     int fp32_ = 0;
     std::vector<std::unique_ptr<Engine>> engines_;
     std::vector<ReplicaState> h_states_;
-    std::vector<double*> gk_bufs_;
-    std::vector<float4*> pos32_bufs_;
-    std::vector<double*> dp_bufs_;
+    std::vector<void*> owned_; // device buffers made for the replicas
     ReplicaState* d_states_ = nullptr;
     long long* d_moves_ = nullptr;
     TraceEntry* d_trace_ = nullptr;

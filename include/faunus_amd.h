@@ -192,6 +192,33 @@ int fmc_batch_total_energy(fmc_batch* batch, int replica, double* terms, double*
 int fmc_batch_check_q(fmc_batch* batch, int replica, double* max_abs_dq, double* rel_reciprocal);
 
 /*
+ * Serving loop with continuous batching.  Each job is one translational Metropolis run
+ * (the loop of MetropolisMonteCarlo::move, src/montecarlo.cpp:87-106, over a system of
+ * the form fmc_create accepts) of `moves` moves with Philox proposals keyed by `seed`;
+ * the server has no single reference counterpart: it replaces many independent
+ * `faunus` processes.  Jobs are admitted first-in first-out while their estimated HBM
+ * footprint fits `hbm_budget_bytes` and fewer than `max_slots` jobs are resident;
+ * fmc_server_step advances every resident job by up to `quantum`*1000/(n + 2K) moves
+ * (n particles, K k-vectors) in one persistent launch per cell-list kind and retires
+ * finished jobs.  A job's result does not depend on which other jobs share its launches.
+ * Job states: 0 queued, 1 resident, 2 done, 3 failed (reason from fmc_server_job_error).
+ */
+typedef struct fmc_server fmc_server;
+int fmc_server_create(int fp32_pairs, long long hbm_budget_bytes, int max_slots, fmc_server** out);
+int fmc_server_destroy(fmc_server* server);
+int fmc_server_submit(fmc_server* server, const char* config_json, int n, const double* xyz, const double* charge,
+                      const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                      long long moves, unsigned long long seed, int* job);
+int fmc_server_step(fmc_server* server, long long quantum, int* resident, int* queued);
+/* Times are seconds since the server was created (host steady clock). */
+int fmc_server_job(fmc_server* server, int job, int* state, double* energy, unsigned long long* moves,
+                   unsigned long long* accepted, double* t_submit, double* t_admit, double* t_done);
+int fmc_server_job_positions(fmc_server* server, int job, double* xyz);
+/* Reason a job failed (empty string otherwise). */
+int fmc_server_job_error(fmc_server* server, int job, char* buffer, int length);
+int fmc_server_status(fmc_server* server, double* seconds, long long* moves_done, long long* bytes_in_use);
+
+/*
  * Streaming copy of `bytes` bytes (read once, written once), `repeats` launches: the
  * known-size reference for scaling HBM counter readings.  No reference counterpart.
  */

@@ -100,7 +100,10 @@ def salt21_box(n, conc_molar):
 
 
 def salt21(n=600, conc=0.1, seed=1, temperature=298.15, kmax=8, splined=True):
-    """2:1 salt (q=+2 and q=-1, sigma=4 A) hard spheres + Ewald alpha=5/L, Rc=L/2."""
+    """2:1 salt (q=+2 and q=-1, sigma=4 A) hard spheres + Ewald alpha=5/L, Rc=L/2;
+    n is the number of ions and must be a multiple of 3."""
+    if n % 3:
+        raise ValueError("salt21: n must be a multiple of 3 (one M and two X per formula unit)")
     box = salt21_box(n, conc)
     cfg = {
         "temperature": temperature,

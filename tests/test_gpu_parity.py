@@ -154,3 +154,17 @@ def test_unsupported_input_fails_loudly():
     bad["energy"] = [{"nonbonded": {"default": [{"hertz": {}}]}}]
     with pytest.raises(RuntimeError):
         F.Engine(bad)
+
+
+def test_groups_past_the_last_particle_are_rejected():
+    """A group list that claims more particles than were set is an error raised
+    before any per-particle table is written (host-side bounds check)."""
+    cfg, xyz, q, ids, g = systems.primitive_model(n_pairs=10, box=60.0, seed=3)
+    e = F.Engine(cfg)
+    e.set_particles(xyz, q, ids)
+    with pytest.raises(RuntimeError, match="past the last particle"):
+        e.set_groups([0], [len(q) + 2], [0])
+    with pytest.raises(RuntimeError, match="past the last particle"):
+        e.set_groups([0, 10], [10, len(q)], [0, 0])
+    e.set_groups([0], [len(q)], [0])
+    assert np.isfinite(e.init())
