@@ -240,6 +240,68 @@ def cpu_baseline(seconds=15.0):
 HEADLINE_GROUPS = 16
 
 
+def serve_stream(n_jobs, seed=1, rate=50.0):
+    """Poisson arrivals (rate jobs/s) of polyelectrolyte jobs: chain length ~ U[50, 500],
+    total N ~ logU[1k, 50k]; the job id seeds the configuration and the Philox stream."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    out = []
+    for j in range(n_jobs):
+        nb = int(rng.integers(50, 501))
+        nt = max(int(round(10 ** rng.uniform(3.0, math.log10(50000.0)))), 2 * nb + 2)
+        out.append((float(arrivals[j]), nb, nt))
+    return out
+
+
+def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
+    """Continuous batching under a Poisson stream: submit each job at its arrival time,
+    run scheduling rounds while anything is queued or resident, and report the
+    per-job time from arrival to completion of `sweeps` sweeps (N moves per sweep)."""
+    import faunus_amd as F
+    from tests import systems
+    stream = serve_stream(n_jobs, seed)
+    jobs = [systems.polyelectrolyte_job(nb, nt, seed=1000 + j) for j, (_, nb, nt) in enumerate(stream)]
+    srv = F.Server(fp32_pairs=fp32, hbm_budget_bytes=200 << 30, max_slots=512)
+    t0 = srv.status()["seconds"]
+    ids = {}
+    nxt = 0
+    rounds = 0
+    peak = 0
+    while True:
+        now = srv.status()["seconds"] - t0
+        while nxt < n_jobs and stream[nxt][0] <= now:
+            sysd = jobs[nxt]
+            ids[nxt] = srv.submit(sysd, sweeps * len(sysd[2]), seed=1000 + nxt)
+            nxt += 1
+        r, q = srv.step(quantum)
+        rounds += 1
+        peak = max(peak, r)
+        if r == 0 and q == 0:
+            if nxt >= n_jobs:
+                break
+            time.sleep(max(0.0, stream[nxt][0] - (srv.status()["seconds"] - t0)))
+    lat, per_sweep, moves = [], [], 0
+    t_end = 0.0
+    for j, jid in ids.items():
+        info = srv.job(jid)
+        if info["state"] != F.Server.DONE:
+            raise RuntimeError(f"serving job {j} failed: {info}")
+        done = info["t_done"] - t0
+        lat.append(done - stream[j][0])
+        per_sweep.append((info["t_done"] - info["t_admit"]) / sweeps)
+        moves += info["moves"]
+        t_end = max(t_end, done)
+    srv.close()
+    lat = np.array(lat)
+    ns = np.array([len(sysd[2]) for sysd in jobs])
+    return dict(jobs=n_jobs, sweeps_per_job=sweeps, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), moves_per_s=moves / (t_end - stream[0][0]),
+                wall_s=t_end - stream[0][0], rounds=rounds, peak_resident=peak,
+                n_min=int(ns.min()), n_median=int(np.median(ns)), n_max=int(ns.max()),
+                projected_p50_s_to_1000_sweeps=float(np.percentile(per_sweep, 50) * 1000),
+                projected_p99_s_to_1000_sweeps=float(np.percentile(per_sweep, 99) * 1000))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,7 +309,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
-    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy"], default=None)
+    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve"],
+                    default=None)
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-sweeps", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
@@ -301,6 +366,10 @@ def main():
         log("batched FP64: %.3e replica-moves/s" % out["batch64"]["rate"])
         out["plain_mix"] = bench_plain_mix(comps)
         log("plain FP64 one move per launch on the mix: %.0f replica-moves/s" % out["plain_mix"])
+    if args.only in (None, "serve"):
+        out["serve"] = bench_serve(n_jobs=args.serve_jobs, sweeps=args.serve_sweeps)
+        log("serving: p50 %.2f s, p99 %.2f s to %d sweeps, %.3e moves/s" % (
+            out["serve"]["p50_s"], out["serve"]["p99_s"], out["serve"]["sweeps_per_job"], out["serve"]["moves_per_s"]))
     if args.only is not None:
         print(json.dumps({k: v for k, v in out.items()}, default=float))
         return
@@ -354,6 +423,8 @@ def main():
             "swaps_attempted_accepted": out["batch32"]["swaps"],
             "sweeps_per_composition": out["batch32"]["sweeps_per_composition"],
             "max_energy_drift": out["batch32"]["max_drift"]}
+    if "serve" in out:
+        line["forms"]["serving_poisson_50_per_s"] = out["serve"]
     print(json.dumps(line), flush=True)
 
 

@@ -654,11 +654,57 @@ System::System(const std::string& config_json) {
             m.atomic = kv.second.value("atomic", false);
             m.rigid = kv.second.value("rigid", false);
             m.compressible = kv.second.value("compressible", false);
-            if (kv.second.contains("excluded_pairs"))
-                for (auto& p : kv.second.at("excluded_pairs").items()) {
+            if (kv.second.contains("atoms"))
+                for (auto& a : kv.second.at("atoms").items()) m.atoms.push_back(atom_index(atoms, a.as_string()));
+            if (kv.second.contains("bondlist")) // bonds.cpp:23-60
+                for (auto& rec : kv.second.at("bondlist").items()) {
+                    auto& [btype, bp] = jsonlite::single_item(rec);
+                    if (btype != "harmonic") throw std::runtime_error("bond type '" + btype + "' is not supported");
+                    HarmonicBond b;
+                    const Value& idx = bp.at("index");
+                    if (idx.size() != 2) throw std::runtime_error("harmonic bond needs exactly 2 indices");
+                    b.i = static_cast<int>(idx.at(0).as_number());
+                    b.j = static_cast<int>(idx.at(1).as_number());
+                    b.half_k = units.kJmol(bp.at("k").as_number()) / 2.0; // bonds.cpp:102-104
+                    b.req = bp.at("req").as_number();
+                    m.bonds.push_back(b);
+                }
+            // excluded_neighbours: atom pairs joined by a path of at most d bonds
+            // (NeighboursGenerator, molecule.cpp:280-294); exclusionlist: explicit pairs.
+            const int d = static_cast<int>(kv.second.value("excluded_neighbours", 0.0));
+            if (d > 0) {
+                int size = 0;
+                for (auto& b : m.bonds) size = std::max(size, std::max(b.i, b.j) + 1);
+                std::vector<std::vector<int>> adj(static_cast<size_t>(size));
+                for (auto& b : m.bonds) {
+                    adj[b.i].push_back(b.j);
+                    adj[b.j].push_back(b.i);
+                }
+                for (int s0 = 0; s0 < size; ++s0) {
+                    std::vector<int> dist(static_cast<size_t>(size), -1);
+                    std::vector<int> frontier{s0};
+                    dist[s0] = 0;
+                    for (int step = 1; step <= d && !frontier.empty(); ++step) {
+                        std::vector<int> next;
+                        for (int a : frontier)
+                            for (int b : adj[a])
+                                if (dist[b] < 0) {
+                                    dist[b] = step;
+                                    next.push_back(b);
+                                }
+                        frontier = next;
+                    }
+                    for (int t = s0 + 1; t < size; ++t)
+                        if (dist[t] > 0) m.excluded.emplace_back(s0, t);
+                }
+            }
+            if (kv.second.contains("exclusionlist"))
+                for (auto& p : kv.second.at("exclusionlist").items()) {
                     int i = static_cast<int>(p.at(0).as_number()), j = static_cast<int>(p.at(1).as_number());
                     m.excluded.emplace_back(std::min(i, j), std::max(i, j));
                 }
+            std::sort(m.excluded.begin(), m.excluded.end());
+            m.excluded.erase(std::unique(m.excluded.begin(), m.excluded.end()), m.excluded.end());
             molecules.push_back(m);
         }
     } else {
@@ -689,6 +735,15 @@ std::unique_ptr<Hamiltonian> System::build_hamiltonian() {
         auto& [key, value] = jsonlite::single_item(rec);
         if (key == "maxenergy") {
             h->maximum_allowed_energy = value.as_number();
+            continue;
+        }
+        if (key == "bonded") { // Hamiltonian::createEnergy (energy.cpp:1328), intra-molecular bonds only
+            if (value.contains("bondlist") && value.at("bondlist").size() > 0)
+                throw std::runtime_error("inter-molecular bonds are not supported by this path");
+            auto b = std::make_unique<BondedTerm>();
+            b->name = "bonded";
+            b->molecules = &h->molecules;
+            h->terms.push_back(std::move(b));
             continue;
         }
         auto nb = std::make_unique<NonbondedTerm>();
@@ -799,6 +854,8 @@ void System::set_groups(int ngroups, const int* begin, const int* size, const in
             G.rigid = m.rigid;
             G.compressible = m.compressible;
             if (mass_center) G.mass_center = {mass_center[3 * g], mass_center[3 * g + 1], mass_center[3 * g + 2]};
+            else if (!G.atomic && G.size > 0) // PBC-safe centre around the first atom (Group::updateMassCenter)
+                G.mass_center = oracle::mass_center(*s, G, atoms, true, s->particles[static_cast<size_t>(G.begin)].pos * -1.0);
         }
     }
 }
@@ -893,6 +950,64 @@ double System::metropolis_move(const Change& c, double uniform, bool* accepted_o
     if (accepted_out) *accepted_out = acc;
     if (du_out) *du_out = acc ? du : energy_change(new_energy, old_energy);
     return du;
+}
+
+// ---------------------------------------------------------------- Bonded
+double BondedTerm::energy(const Space& spc, const Space*, const Change& change) { // energy.cpp:1047-1088
+    double energy = 0.0;
+    if (!change) return energy;
+    auto group_bonds = [&](const Group& g) -> const std::vector<HarmonicBond>& {
+        return (*molecules)[static_cast<size_t>(g.molid)].bonds;
+    };
+    if (change.everything || change.volume_change) {
+        for (const Group& g : spc.groups) {
+            if (g.size == 0) continue;
+            double e = 0.0;
+            for (const auto& b : group_bonds(g)) e += bond(spc, g, b);
+            energy += e;
+        }
+        return energy;
+    }
+    for (const auto& gc : change.groups) { // internalGroupEnergy, 1068-1088
+        const Group& g = spc.groups.at(static_cast<size_t>(gc.group_index));
+        if (!gc.internal || g.size == 0) continue;
+        double e = 0.0;
+        if (gc.all) {
+            for (const auto& b : group_bonds(g)) e += bond(spc, g, b);
+        } else { // sumEnergy (energy.h:360-376): bonds with any index among the moved atoms
+            const auto& idx = gc.relative_atom_indices;
+            for (const auto& b : group_bonds(g))
+                if (std::binary_search(idx.begin(), idx.end(), b.i) || std::binary_search(idx.begin(), idx.end(), b.j))
+                    e += bond(spc, g, b);
+        }
+        energy += e;
+    }
+    return energy;
+}
+
+Vec3 mass_center(const Space& spc, const Group& g, const std::vector<AtomType>& atoms, bool pbc, const Vec3& shift) {
+    double wsum = 0.0;
+    Vec3 c{0, 0, 0};
+    for (int k = 0; k < g.size; ++k) {
+        const Particle& p = spc.particles[static_cast<size_t>(g.begin + k)];
+        Vec3 t = p.pos + shift;
+        if (pbc) spc.geometry.boundary(t);
+        const double w = atoms.at(static_cast<size_t>(p.id)).mw;
+        c = c + t * w;
+        wsum += w;
+    }
+    if (std::fabs(wsum) <= std::numeric_limits<double>::epsilon()) return {0, 0, 0};
+    c = Vec3{c.x / wsum, c.y / wsum, c.z / wsum} - shift;
+    if (pbc) spc.geometry.boundary(c);
+    return c;
+}
+
+std::array<double, 9> rotation_matrix(const Vec3& u, double angle) {
+    const double s = std::sin(angle), c = std::cos(angle);
+    const Vec3 sa = u * s, ca = u * (1.0 - c);
+    return {ca.x * u.x + c, ca.x * u.y - sa.z, ca.x * u.z + sa.y,
+            ca.x * u.y + sa.z, ca.y * u.y + c, ca.y * u.z - sa.x,
+            ca.x * u.z - sa.y, ca.y * u.z + sa.x, ca.z * u.z + c};
 }
 
 // ---------------------------------------------------------------- Philox

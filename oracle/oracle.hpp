@@ -20,6 +20,7 @@
 //    PARITY UNPINNED.  Two modes are offered: exact erfc and an Andrea spline of
 //    S(q) on q in [0,1] with utol = 0.005/lB (potentials.cpp:1632).
 #pragma once
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -130,16 +131,22 @@ struct AtomType {
     bool has_dp = false;
 };
 
+// HarmonicBond (bonds.cpp:100-105, 130-138): u = k/2 (req - |r_ij|)^2, indices
+// relative to the molecule, k converted from kJ/mol/A^2 to kT/A^2.
+struct HarmonicBond {
+    int i = 0, j = 0;
+    double half_k = 0, req = 0;
+};
+
 struct MoleculeType {
     std::string name;
     bool atomic = false, rigid = false, compressible = false;
     std::vector<int> atoms;
-    std::vector<std::pair<int, int>> excluded; // excluded intra-molecular pairs (i<j)
+    std::vector<HarmonicBond> bonds;            // MoleculeBuilder::readBonds (molecule.cpp:508-517)
+    std::vector<std::pair<int, int>> excluded;  // sorted unique (i<j), MoleculeBuilder::readExclusions (533-547)
     bool is_pair_excluded(int i, int j) const {
         if (i > j) std::swap(i, j);
-        for (auto& e : excluded)
-            if (e.first == i && e.second == j) return true;
-        return false;
+        return std::binary_search(excluded.begin(), excluded.end(), std::make_pair(i, j));
     }
 };
 
@@ -327,6 +334,28 @@ struct NonbondedTerm : Term {
     }
     bool cut(const Space& spc, const Group& g1, const Group& g2) const;
 };
+
+// Bonded (energy.cpp:974-1124): intra-molecular bonds of the groups a Change touches.
+struct BondedTerm : Term {
+    const std::vector<MoleculeType>* molecules = nullptr;
+    double bond(const Space& spc, const Group& g, const HarmonicBond& b) const {
+        const Vec3 d = spc.geometry.vdist(spc.particles[g.begin + b.i].pos, spc.particles[g.begin + b.j].pos);
+        const double x = b.req - std::sqrt(d.squared_norm());
+        return b.half_k * x * x;
+    }
+    double energy(const Space& spc, const Space* old_spc, const Change& c) override;
+};
+
+// Geometry::weightedCenter with masses (geometry.h:505-525, 563-569): shift, wrap,
+// mass-weighted mean, shift back, wrap.
+Vec3 mass_center(const Space& spc, const Group& g, const std::vector<AtomType>& atoms, bool pbc, const Vec3& shift);
+
+// Rotation matrix of an angle about a unit axis (Rodrigues' formula), row-major.
+std::array<double, 9> rotation_matrix(const Vec3& axis, double angle);
+inline Vec3 rotate(const std::array<double, 9>& R, const Vec3& v) {
+    return {R[0] * v.x + R[1] * v.y + R[2] * v.z, R[3] * v.x + R[4] * v.y + R[5] * v.z,
+            R[6] * v.x + R[7] * v.y + R[8] * v.z};
+}
 
 struct SelfEnergyTerm : Term {
     double prefactor = 0; // energy = prefactor * q^2 per particle

@@ -41,6 +41,11 @@ def _lib(fast=False):
         lib.orc_run_philox_translate.argtypes = [C.c_void_p, C.c_long, C.c_ulonglong, C.c_uint, C.c_ulonglong,
                                                  C.c_double, I, D, D, U8]
         lib.orc_mc_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), D]
+        lib.orc_run_philox_moves.argtypes = [C.c_void_p, C.c_long, C.c_ulonglong, C.c_uint, C.c_ulonglong, C.c_double,
+                                             C.c_int, I, I, C.POINTER(C.c_uint), D, D, I, I, C.POINTER(C.c_long),
+                                             C.c_long, I, D, D, D, U8]
+        lib.orc_molecule_excluded.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, I]
+        lib.orc_group_mass_center.argtypes = [C.c_void_p, C.c_int, C.c_int, D]
         _LIBS[name] = lib
     return _LIBS[name]
 
@@ -82,6 +87,7 @@ class System:
         size = np.ascontiguousarray(size, dtype=np.int32)
         molid = np.ascontiguousarray(molid, dtype=np.int32)
         cm = None if mass_center is None else np.ascontiguousarray(mass_center, dtype=np.float64)
+        self.max_group = int(size.max()) if len(size) else 0
         _check(self.lib, self.lib.orc_set_groups(self.h, len(begin), _p(begin, C.c_int), _p(size, C.c_int),
                                                  _p(molid, C.c_int), _p(cm)))
 
@@ -140,6 +146,42 @@ class System:
             self.h, nmoves, seed, replica, move0, beta, _p(rec["index"], C.c_int), _p(rec["xyz"]), _p(rec["du"]),
             _p(rec["accepted"], C.c_ubyte)))
         return rec
+
+    def run_philox_moves(self, nmoves, seed, kinds, replica=0, move0=0, beta=1.0, cap=None):
+        """Metropolis run over a mix of move kinds (0 atom translate, 1 molecule
+        translate+rotate, 2 pivot) with the device-resident Philox proposal scheme.
+        ``kinds``: list of dicts with keys kind, molid, weight, dp, dprot.  Records per
+        move: kind, group, moved atoms (offsets into index/xyz), trial mass centre,
+        du and decision."""
+        nk = len(kinds)
+        kind = np.array([k["kind"] for k in kinds], np.int32)
+        molid = np.array([k["molid"] for k in kinds], np.int32)
+        weight = np.array([k.get("weight", 1) for k in kinds], np.uint32)
+        dp = np.array([k.get("dp", 0.0) for k in kinds])
+        dprot = np.array([k.get("dprot", 0.0) for k in kinds])
+        cap = cap if cap is not None else nmoves * max(1, self.max_group)
+        rec = dict(kind=np.zeros(nmoves, np.int32), group=np.zeros(nmoves, np.int32),
+                   offsets=np.zeros(nmoves + 1, np.int64), index=np.zeros(cap, np.int32), xyz=np.zeros((cap, 3)),
+                   cm=np.zeros((nmoves, 3)), du=np.zeros(nmoves), accepted=np.zeros(nmoves, np.uint8))
+        _check(self.lib, self.lib.orc_run_philox_moves(
+            self.h, nmoves, seed, replica, move0, beta, nk, _p(kind, C.c_int), _p(molid, C.c_int),
+            _p(weight, C.c_uint), _p(dp), _p(dprot), _p(rec["kind"], C.c_int), _p(rec["group"], C.c_int),
+            _p(rec["offsets"], C.c_long), cap, _p(rec["index"], C.c_int), _p(rec["xyz"]), _p(rec["cm"]),
+            _p(rec["du"]), _p(rec["accepted"], C.c_ubyte)))
+        used = int(rec["offsets"][-1])
+        rec["index"] = rec["index"][:used]
+        rec["xyz"] = rec["xyz"][:used]
+        return rec
+
+    def excluded(self, molid, i, j):
+        e = C.c_int()
+        _check(self.lib, self.lib.orc_molecule_excluded(self.h, molid, i, j, C.byref(e)))
+        return bool(e.value)
+
+    def mass_center(self, group, trial=False):
+        cm = np.zeros(3)
+        _check(self.lib, self.lib.orc_group_mass_center(self.h, int(trial), group, _p(cm)))
+        return cm
 
     def replay_translate(self, group, index, xyz, u):
         n = len(u)

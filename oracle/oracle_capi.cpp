@@ -267,6 +267,270 @@ int orc_run_philox_translate(void* h, long nmoves, unsigned long long seed, unsi
     });
 }
 
+// performMove (montecarlo.cpp:139-187) on a Change whose new coordinates are in the
+// trial space, with acceptance u <= exp(-beta du); syncs the changed groups.
+static void philox_metropolis(System& s, const Change& c, double acc_u, double beta, double* du_out, bool* acc_out) {
+    s.trial_pot->update_state(s.trial_space, &s.accepted_space, c);
+    const double e_new = s.trial_pot->energy(s.trial_space, &s.accepted_space, c);
+    const double e_old = s.accepted_pot->energy(s.accepted_space, nullptr, c);
+    double du;
+    if (std::isnan(e_old) && !std::isnan(e_new)) du = -infty;
+    else if (std::isnan(e_new)) du = infty;
+    else if (e_new > 0.0 && std::isinf(e_new)) du = infty;
+    else {
+        du = e_new - e_old;
+        if (std::isnan(du)) du = 0.0;
+    }
+    const double bdu = beta * du;
+    bool acc;
+    if (std::isinf(bdu) && bdu < 0) acc = true;
+    else if (-bdu > max_exp_argument) acc = true;
+    else acc = acc_u <= std::exp(-bdu);
+    auto sync_space = [&](Space& dst, const Space& src) { // space.cpp:199-239
+        for (auto& gc : c.groups) {
+            const Group& gs = src.groups[static_cast<size_t>(gc.group_index)];
+            Group& gd = dst.groups[static_cast<size_t>(gc.group_index)];
+            if (gc.all) {
+                for (int i = 0; i < gs.size; ++i) dst.particles[gs.begin + i] = src.particles[gs.begin + i];
+            } else {
+                for (int i : gc.relative_atom_indices) dst.particles[gs.begin + i] = src.particles[gs.begin + i];
+            }
+            gd.mass_center = gs.mass_center;
+        }
+    };
+    if (acc) {
+        sync_space(s.accepted_space, s.trial_space);
+        s.accepted_pot->sync_from(*s.trial_pot, c);
+        s.stats.accepted++;
+        s.stats.sum_du += du;
+    } else {
+        sync_space(s.trial_space, s.accepted_space);
+        s.trial_pot->sync_from(*s.accepted_pot, c);
+    }
+    s.stats.moves++;
+    *du_out = du;
+    *acc_out = acc;
+}
+
+// Metropolis run over a mix of move kinds with the device-resident proposal scheme.
+// Move t of replica r draws three Philox4x32-10 blocks a, b, c at counters
+// (t_lo, t_hi, r, 0|1|2) with key (seed_lo, seed_hi):
+//   kind        = first k with (b2 * sum(weight)) >> 32 < cumulative weight of k
+//   acceptance  = 53-bit deviate from b0, b1; u <= exp(-beta du)
+//   kind 0, translate one atom of a molecule type (AtomicTranslateRotate,
+//     move.cpp:225-345): group among the groups of that type (c3, only if several),
+//     atom (a0 * size) >> 32, unit vector from z = 2u(a1)-1 and phi = 2 pi u(a2),
+//     length dp * a3/2^32 (the atom's dp if it has one, else the move's dp); a
+//     molecular group's mass centre is recomputed with PBC around the old one.
+//   kind 1, translate and rotate a molecule (TranslateRotate, move.cpp:1618-1689):
+//     group (a0 * count) >> 32; displacement as in kind 0 with the move's dp, then a
+//     rotation by dprot * (u(c2) - 1/2) about the unit vector from c0, c1 through the
+//     mass centre (Geometry::rotate, geometry.h:613-626); Change all=true.
+//   kind 2, pivot (PivotMove, chainmove.cpp:216-268 and 76-104): chain (a0 * count)
+//     >> 32, harmonic bond (c3 * nbonds) >> 32, the atoms after the bond's second
+//     atom if u(b3) > 1/2 (origin = second atom) else the atoms before its first atom
+//     (origin = first atom); rotation by dprot * (u(c2) - 1/2) about the bond axis in
+//     the frame translated by the origin; mass centre recomputed; internal=true.
+//     (The reference also translates the unmoved chain atoms back and forth, which
+//     changes them at rounding level; the restatement leaves them untouched.)
+// A move that changes nothing (empty Change) is counted as accepted with du = 0.
+int orc_run_philox_moves(void* h, long nmoves, unsigned long long seed, unsigned replica, unsigned long long move0,
+                         double beta, int nkinds, const int* kind, const int* molid, const unsigned* weight,
+                         const double* dp, const double* dprot, int* rec_kind, int* rec_group, long* rec_off,
+                         long rec_cap, int* rec_index, double* rec_xyz, double* rec_cm, double* rec_du,
+                         unsigned char* rec_acc) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        if (nkinds < 1) throw std::runtime_error("no move kinds");
+        std::vector<unsigned long long> cum(static_cast<size_t>(nkinds));
+        unsigned long long total = 0;
+        std::vector<std::vector<int>> groups_of(static_cast<size_t>(nkinds));
+        for (int k = 0; k < nkinds; ++k) {
+            total += weight[k];
+            cum[static_cast<size_t>(k)] = total;
+            for (size_t g = 0; g < s->accepted_space.groups.size(); ++g)
+                if (s->accepted_space.groups[g].molid == molid[k]) groups_of[static_cast<size_t>(k)].push_back(static_cast<int>(g));
+            if (groups_of[static_cast<size_t>(k)].empty()) throw std::runtime_error("move kind without groups");
+            if (kind[k] < 0 || kind[k] > 2) throw std::runtime_error("unknown move kind");
+        }
+        if (total == 0 || total > 0xffffffffull) throw std::runtime_error("move weights must sum to 1..2^32-1");
+        auto u01 = [](uint32_t x) { return (static_cast<double>(x) + 0.5) * 2.3283064365386963e-10; };
+        auto pick = [](uint32_t x, size_t n) { return static_cast<size_t>((static_cast<uint64_t>(x) * n) >> 32); };
+        auto unit = [&](uint32_t zw, uint32_t pw) {
+            const double z = 2.0 * u01(zw) - 1.0;
+            const double phi = 6.283185307179586 * u01(pw);
+            const double sz = std::sqrt(1.0 - z * z);
+            return Vec3{sz * std::cos(phi), sz * std::sin(phi), z};
+        };
+        Space& T = s->trial_space;
+        const Cuboid& geo = T.geometry;
+        long off = 0;
+        if (rec_off) rec_off[0] = 0;
+        for (long t = 0; t < nmoves; ++t) {
+            const unsigned long long mv = move0 + static_cast<unsigned long long>(t);
+            const std::array<uint32_t, 2> key{static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+            const auto a = philox4x32_10({static_cast<uint32_t>(mv), static_cast<uint32_t>(mv >> 32), replica, 0u}, key);
+            const auto b = philox4x32_10({static_cast<uint32_t>(mv), static_cast<uint32_t>(mv >> 32), replica, 1u}, key);
+            const auto c = philox4x32_10({static_cast<uint32_t>(mv), static_cast<uint32_t>(mv >> 32), replica, 2u}, key);
+            const uint64_t sel = (static_cast<uint64_t>(b[2]) * total) >> 32;
+            int k = 0;
+            while (sel >= cum[static_cast<size_t>(k)]) ++k;
+            const double acc_u = (static_cast<double>(b[0] >> 5) * 67108864.0 + static_cast<double>(b[1] >> 6)) *
+                                 1.1102230246251565e-16;
+            const auto& G = groups_of[static_cast<size_t>(k)];
+            Change ch;
+            GroupChange gc;
+            int gidx = -1;
+            std::vector<int> moved; // absolute indices, ascending
+            if (kind[k] == 0) {
+                gidx = G.size() == 1 ? G[0] : G[pick(c[3], G.size())];
+                Group& g = T.groups[static_cast<size_t>(gidx)];
+                if (g.size > 0) {
+                    const int rel = static_cast<int>(pick(a[0], static_cast<size_t>(g.size)));
+                    Particle& p = T.particles[static_cast<size_t>(g.begin + rel)];
+                    const AtomType& at = s->atoms.at(static_cast<size_t>(p.id));
+                    const double step = at.has_dp ? at.dp : dp[k];
+                    if (step > 0.0) {
+                        const Vec3 dir = unit(a[1], a[2]);
+                        const double scale = step * (static_cast<double>(a[3]) * 2.3283064365386963e-10);
+                        p.pos = {p.pos.x + dir.x * scale, p.pos.y + dir.y * scale, p.pos.z + dir.z * scale};
+                        geo.boundary(p.pos);
+                        if (!g.atomic) g.mass_center = mass_center(T, g, s->atoms, true, g.mass_center * -1.0);
+                        gc.group_index = gidx;
+                        gc.internal = true;
+                        gc.relative_atom_indices = {rel};
+                        ch.groups.push_back(gc);
+                        moved.push_back(g.begin + rel);
+                    }
+                }
+            } else if (kind[k] == 1) {
+                gidx = G[pick(a[0], G.size())];
+                Group& g = T.groups[static_cast<size_t>(gidx)];
+                bool changed = false;
+                if (dp[k] > 0.0 && g.size > 0) { // Group::translate (group.cpp:115-123)
+                    const Vec3 dir = unit(a[1], a[2]);
+                    const double scale = dp[k] * (static_cast<double>(a[3]) * 2.3283064365386963e-10);
+                    const Vec3 d{dir.x * scale, dir.y * scale, dir.z * scale};
+                    g.mass_center = g.mass_center + d;
+                    geo.boundary(g.mass_center);
+                    for (int i = 0; i < g.size; ++i) {
+                        Particle& p = T.particles[static_cast<size_t>(g.begin + i)];
+                        p.pos = p.pos + d;
+                        geo.boundary(p.pos);
+                    }
+                    changed = true;
+                }
+                if (dprot[k] > std::numeric_limits<double>::epsilon() && g.size > 0) {
+                    const Vec3 axis = unit(c[0], c[1]);
+                    const double angle = dprot[k] * (u01(c[2]) - 0.5);
+                    const auto R = rotation_matrix(axis, angle);
+                    const Vec3 shift = g.mass_center * -1.0;
+                    for (int i = 0; i < g.size; ++i) {
+                        Particle& p = T.particles[static_cast<size_t>(g.begin + i)];
+                        p.pos = p.pos + shift;
+                        geo.boundary(p.pos);
+                        p.pos = rotate(R, p.pos) - shift;
+                        geo.boundary(p.pos);
+                    }
+                    changed = true;
+                }
+                if (changed) {
+                    gc.group_index = gidx;
+                    gc.all = true;
+                    gc.internal = false;
+                    ch.groups.push_back(gc);
+                    for (int i = 0; i < g.size; ++i) moved.push_back(g.begin + i);
+                }
+            } else {
+                gidx = G[pick(a[0], G.size())];
+                Group& g = T.groups[static_cast<size_t>(gidx)];
+                const auto& bonds = s->molecules.at(static_cast<size_t>(g.molid)).bonds;
+                if (g.size > 2 && !bonds.empty() && std::fabs(dprot[k]) > 1e-9) {
+                    const HarmonicBond& bd = bonds[pick(c[3], bonds.size())];
+                    int atom0 = g.begin + bd.i, atom1 = g.begin + bd.j;
+                    std::vector<int> segment;
+                    if (u01(b[3]) > 0.5) {
+                        for (int i = atom1 + 1; i < g.begin + g.size; ++i) segment.push_back(i);
+                        std::swap(atom0, atom1);
+                    } else {
+                        for (int i = g.begin; i < atom0; ++i) segment.push_back(i);
+                    }
+                    if (!segment.empty()) {
+                        const double angle = dprot[k] * (u01(c[2]) - 0.5);
+                        const Vec3 shift = T.particles[static_cast<size_t>(atom0)].pos;
+                        auto shifted = [&](int i) {
+                            Vec3 v = T.particles[static_cast<size_t>(i)].pos - shift;
+                            geo.boundary(v);
+                            return v;
+                        };
+                        const Vec3 origin = shifted(atom0);
+                        const Vec3 v = geo.vdist(origin, shifted(atom1));
+                        const double norm = std::sqrt(v.squared_norm());
+                        const auto R = rotation_matrix(Vec3{v.x / norm, v.y / norm, v.z / norm}, angle);
+                        std::vector<Vec3> frame(static_cast<size_t>(g.size));
+                        for (int i = 0; i < g.size; ++i) frame[static_cast<size_t>(i)] = shifted(g.begin + i);
+                        for (int i : segment)
+                            frame[static_cast<size_t>(i - g.begin)] = rotate(R, frame[static_cast<size_t>(i - g.begin)] - origin) + origin;
+                        double wsum = 0.0;
+                        Vec3 cm{0, 0, 0};
+                        for (int i = 0; i < g.size; ++i) {
+                            const double w = s->atoms.at(static_cast<size_t>(T.particles[static_cast<size_t>(g.begin + i)].id)).mw;
+                            cm = cm + frame[static_cast<size_t>(i)] * w;
+                            wsum += w;
+                        }
+                        cm = Vec3{cm.x / wsum, cm.y / wsum, cm.z / wsum} + shift;
+                        geo.boundary(cm);
+                        g.mass_center = cm;
+                        for (int i : segment) {
+                            Vec3 p = frame[static_cast<size_t>(i - g.begin)] + shift;
+                            geo.boundary(p);
+                            T.particles[static_cast<size_t>(i)].pos = p;
+                        }
+                        gc.group_index = gidx;
+                        gc.all = false;
+                        gc.internal = true;
+                        for (int i : segment) gc.relative_atom_indices.push_back(i - g.begin);
+                        ch.groups.push_back(gc);
+                        moved = segment;
+                    }
+                }
+            }
+            // record the proposal (moved atoms' trial positions, trial mass centre)
+            if (rec_off) {
+                if (off + static_cast<long>(moved.size()) > rec_cap) throw std::runtime_error("record capacity exceeded");
+                for (size_t q = 0; q < moved.size(); ++q) {
+                    const Vec3& p = T.particles[static_cast<size_t>(moved[q])].pos;
+                    if (rec_index) rec_index[off + static_cast<long>(q)] = moved[q];
+                    if (rec_xyz) {
+                        rec_xyz[3 * (off + static_cast<long>(q))] = p.x;
+                        rec_xyz[3 * (off + static_cast<long>(q)) + 1] = p.y;
+                        rec_xyz[3 * (off + static_cast<long>(q)) + 2] = p.z;
+                    }
+                }
+                off += static_cast<long>(moved.size());
+                rec_off[t + 1] = off;
+            }
+            if (rec_cm) {
+                const Vec3 cm = gidx >= 0 ? T.groups[static_cast<size_t>(gidx)].mass_center : Vec3{0, 0, 0};
+                rec_cm[3 * t] = cm.x;
+                rec_cm[3 * t + 1] = cm.y;
+                rec_cm[3 * t + 2] = cm.z;
+            }
+            double du = 0.0;
+            bool acc = true;
+            if (!ch.groups.empty()) philox_metropolis(*s, ch, acc_u, beta, &du, &acc);
+            else {
+                s->stats.moves++;
+                s->stats.accepted++;
+            }
+            if (rec_kind) rec_kind[t] = k;
+            if (rec_group) rec_group[t] = gidx;
+            if (rec_du) rec_du[t] = du;
+            if (rec_acc) rec_acc[t] = acc ? 1 : 0;
+        }
+    });
+}
+
 // Replay a recorded single-particle proposal stream.
 int orc_replay_translate(void* h, long nmoves, const int* group, const int* index, const double* xyz,
                          const double* u, double* out_du, unsigned char* out_acc, double* out_enew,
@@ -295,6 +559,25 @@ int orc_replay_translate(void* h, long nmoves, const int* group, const int* inde
                 if (out_eold) out_eold[n * nterm + t] = eo[static_cast<size_t>(t)];
             }
         }
+    });
+}
+
+// Exclusion query on a parsed molecule type (MoleculeData::isPairExcluded).
+int orc_molecule_excluded(void* h, int molid, int i, int j, int* excluded) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        *excluded = s->molecules.at(static_cast<size_t>(molid)).is_pair_excluded(i, j) ? 1 : 0;
+    });
+}
+
+int orc_group_mass_center(void* h, int trial, int group, double* cm) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        const Space& sp = trial ? s->trial_space : s->accepted_space;
+        const Vec3& c = sp.groups.at(static_cast<size_t>(group)).mass_center;
+        cm[0] = c.x;
+        cm[1] = c.y;
+        cm[2] = c.z;
     });
 }
 

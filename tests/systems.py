@@ -120,3 +120,165 @@ def salt21(n=600, conc=0.1, seed=1, temperature=298.15, kmax=8, splined=True):
     q = np.where(ids == 0, 2.0, -1.0)
     xyz = rsa_positions(n, box, np.full(n, 4.0), rng)
     return cfg, xyz, q, ids, dict(begin=[0], size=[n], molid=[0])
+
+
+def lattice_positions(n, box, sigma, rng):
+    """n non-overlapping spheres of diameter <= sigma: a random subset of the sites of
+    a simple cubic lattice (spacing a = box/m >= sigma) with per-site jitter of at
+    most (a - sigma)/2 per coordinate, so any two centres stay >= sigma apart."""
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    if box / m < sigma:
+        raise RuntimeError("lattice too dense for the requested sigma")
+    a = box / m
+    sites = rng.choice(m ** 3, size=n, replace=False)
+    ijk = np.stack([sites % m, (sites // m) % m, sites // (m * m)], axis=1).astype(np.float64)
+    jit = (a - sigma) / 2 * (2 * rng.random((n, 3)) - 1)
+    return (ijk + 0.5) * a - box / 2 + jit
+
+
+def polyelectrolyte_job(n_beads, n_total, seed, kmax=8, dp=10.0, monomer_molar=0.005):
+    """Polyelectrolyte serving job: n_beads monomers (q=-1), n_beads counterions (q=+1)
+    and 1:1 salt up to n_total particles, hard spheres sigma=4 A + Ewald alpha=5/L,
+    Rc=L/2, kmax=8, cubic box at the given monomer concentration.  Every particle is
+    translated with displacement dp (atomic groups; the chain bonds are not part of
+    this system)."""
+    n_salt = max(0, (n_total - 2 * n_beads) // 2)
+    n = 2 * n_beads + 2 * n_salt
+    box = (n_beads / (monomer_molar * AVOGADRO * 1e-27)) ** (1.0 / 3.0)
+    cfg = {
+        "temperature": 298.15,
+        "geometry": {"type": "cuboid", "length": box},
+        "atomlist": [{"P": {"q": -1.0, "sigma": 4.0, "dp": dp}}, {"C": {"q": 1.0, "sigma": 4.0, "dp": dp}},
+                     {"A": {"q": -1.0, "sigma": 4.0, "dp": dp}}],
+        "moleculelist": [{"ions": {"atoms": ["P", "C", "A"], "atomic": True}}],
+        "energy": [{"nonbonded": {"default": [
+            {"hardsphere": {}},
+            {"coulomb": {"type": "ewald", "epsr": 78.7, "cutoff": box / 2, "alpha": 5.0 / box, "ncutoff": kmax}}]}}],
+    }
+    rng = np.random.default_rng(seed)
+    ids = np.concatenate([np.zeros(n_beads, np.int32), np.ones(n_beads + n_salt, np.int32),
+                          np.full(n_salt, 2, np.int32)])
+    q = np.where(ids == 1, 1.0, -1.0)
+    xyz = lattice_positions(n, box, 4.0, rng)
+    return cfg, xyz, q, ids, dict(begin=[0], size=[n], molid=[0])
+
+
+def water(n_mol=4096, box=None, seed=1, alpha=0.28, cutoff=10.0, kmax=9, lj_cutoff=10.0, dp=0.5, dprot=0.5,
+          splined=True):
+    """Rigid 3-site water (O-H 1.0 A, H-O-H 109.47 deg, q_O=-0.8476, q_H=+0.4238, LJ
+    on O sigma=3.166 A eps=0.650 kJ/mol), molecules on a cubic lattice with random
+    orientations; Ewald alpha=0.28, Rc=10, kmax=9; LJ truncated at lj_cutoff.  The
+    move set is molecular translate+rotate with dp and dprot."""
+    if box is None:
+        box = 49.68 * (n_mol / 4096) ** (1.0 / 3.0)
+    lj = {"mixing": "LB"}
+    if lj_cutoff is not None:
+        lj["cutoff"] = lj_cutoff
+    cfg = {
+        "temperature": 298.15,
+        "geometry": {"type": "cuboid", "length": box},
+        "atomlist": [{"OW": {"q": -0.8476, "sigma": 3.166, "eps": 0.650, "mw": 15.999}},
+                     {"HW": {"q": 0.4238, "sigma": 0.0, "eps": 0.0, "mw": 1.008}}],
+        "moleculelist": [{"water": {"atoms": ["OW", "HW", "HW"], "rigid": True}}],
+        "energy": [{"nonbonded": {"default": [
+            {"lennardjones": lj},
+            {"coulomb": {"type": "ewald", "epsr": 1.0, "cutoff": cutoff, "alpha": alpha, "ncutoff": kmax,
+                         "splined": splined}}]}}],
+    }
+    rng = np.random.default_rng(seed)
+    theta = math.radians(109.47)
+    local = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [math.cos(theta), math.sin(theta), 0.0]])
+    m = int(math.ceil(n_mol ** (1.0 / 3.0)))
+    a = box / m
+    sites = rng.choice(m ** 3, size=n_mol, replace=False)
+    ijk = np.stack([sites % m, (sites // m) % m, sites // (m * m)], axis=1).astype(np.float64)
+    centres = (ijk + 0.5) * a - box / 2
+    xyz = np.zeros((3 * n_mol, 3))
+    for w in range(n_mol):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        pos = centres[w] + local @ q.T
+        pos -= box * np.round(pos / box)
+        xyz[3 * w:3 * w + 3] = pos
+    ids = np.tile(np.array([0, 1, 1], np.int32), n_mol)
+    q = np.where(ids == 0, -0.8476, 0.4238)
+    groups = dict(begin=list(range(0, 3 * n_mol, 3)), size=[3] * n_mol, molid=[0] * n_mol)
+    moves = [dict(kind=1, molid=0, weight=1, dp=dp, dprot=dprot)]
+    return cfg, xyz, q, ids, groups, moves
+
+
+def polyelectrolyte(n_beads, n_total, seed, kmax=8, dp=10.0, bead_dp=2.0, dprot=3.0, monomer_molar=0.005,
+                    bond_k=0.5, bond_req=4.0):
+    """Bead-spring polyelectrolyte (n_beads, q=-1, sigma=4 A, harmonic bonds k=0.5
+    kJ/mol/A^2, req=4 A) with n_beads counterions (q=+1) and 1:1 salt up to n_total
+    particles; hard spheres + Ewald alpha=5/L, Rc=L/2, kmax=8; cubic box at the given
+    monomer concentration.  Groups: the chain (molecular), then one atomic group of
+    ions.  The move set translates ions (dp) and beads (bead_dp) and pivots the chain
+    (dprot), weighted by the number of ions, beads and bonds."""
+    n_salt = max(0, (n_total - 2 * n_beads) // 2)
+    n_ions = n_beads + 2 * n_salt
+    box = (n_beads / (monomer_molar * AVOGADRO * 1e-27)) ** (1.0 / 3.0)
+    bonds = [{"harmonic": {"index": [i, i + 1], "k": bond_k, "req": bond_req}} for i in range(n_beads - 1)]
+    cfg = {
+        "temperature": 298.15,
+        "geometry": {"type": "cuboid", "length": box},
+        "atomlist": [{"P": {"q": -1.0, "sigma": 4.0, "dp": bead_dp}}, {"C": {"q": 1.0, "sigma": 4.0, "dp": dp}},
+                     {"A": {"q": -1.0, "sigma": 4.0, "dp": dp}}],
+        "moleculelist": [{"chain": {"atoms": ["P"] * n_beads, "bondlist": bonds}},
+                         {"ions": {"atoms": ["C", "A"], "atomic": True}}],
+        "energy": [{"bonded": {}},
+                   {"nonbonded": {"default": [
+                       {"hardsphere": {}},
+                       {"coulomb": {"type": "ewald", "epsr": 78.7, "cutoff": box / 2, "alpha": 5.0 / box,
+                                    "ncutoff": kmax}}]}}],
+    }
+    rng = np.random.default_rng(seed)
+    # chain: random walk with step 4.2 A and consecutive bonds at least 75 deg from reversing
+    chain = np.zeros((n_beads, 3))
+    step = bond_req * 1.05
+    prev = None
+    i = 1
+    tries = 0
+    while i < n_beads:
+        tries += 1
+        if tries > 200 * n_beads:
+            raise RuntimeError("chain growth failed")
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        if prev is not None and v @ prev < -0.25:
+            continue
+        p = chain[i - 1] + step * v
+        d = chain[:i - 1] - p
+        if i > 1 and (np.einsum("ij,ij->i", d, d) < 4.4 ** 2).any():
+            continue
+        chain[i] = p
+        prev = v
+        i += 1
+    chain -= chain.mean(axis=0)
+    chain -= box * np.round(chain / box)
+    # ions on a jittered lattice, away from every bead
+    m = int(math.ceil((2 * n_ions) ** (1.0 / 3.0)))
+    a = box / m
+    if a < 4.0:
+        raise RuntimeError("too dense")
+    sites = rng.permutation(m ** 3)
+    ions = []
+    for site in sites:
+        c = (np.array([site % m, (site // m) % m, site // (m * m)], np.float64) + 0.5) * a - box / 2
+        c += (a - 4.0) / 2 * (2 * rng.random(3) - 1)
+        d = chain - c
+        d -= box * np.round(d / box)
+        if (np.einsum("ij,ij->i", d, d) < 4.4 ** 2).any():
+            continue
+        ions.append(c)
+        if len(ions) == n_ions:
+            break
+    if len(ions) < n_ions:
+        raise RuntimeError("ion placement failed")
+    xyz = np.concatenate([chain, np.array(ions)])
+    ids = np.concatenate([np.zeros(n_beads, np.int32), np.ones(n_beads + n_salt, np.int32),
+                          np.full(n_salt, 2, np.int32)])
+    q = np.where(ids == 1, 1.0, -1.0)
+    groups = dict(begin=[0, n_beads], size=[n_beads, n_ions], molid=[0, 1])
+    moves = [dict(kind=0, molid=1, weight=n_ions, dp=dp), dict(kind=0, molid=0, weight=n_beads, dp=bead_dp),
+             dict(kind=2, molid=0, weight=max(1, n_beads - 1), dprot=dprot)]
+    return cfg, xyz, q, ids, groups, moves
