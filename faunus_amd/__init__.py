@@ -33,6 +33,8 @@ SIGNATURES = {
     "fmc_init": (C.c_int, [C.c_void_p, D]),
     "fmc_num_terms": (C.c_int, [C.c_void_p, I]),
     "fmc_term_name": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    "fmc_stage_mass_center": (C.c_int, [C.c_void_p, C.c_int, D]),
+    "fmc_get_mass_center": (C.c_int, [C.c_void_p, C.c_int, D]),
     "fmc_stage_trial": (C.c_int, [C.c_void_p, C.c_int, I, D]),
     "fmc_delta_u": (C.c_int, [C.c_void_p, C.c_void_p, D, D]),
     "fmc_energy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, D, D]),
@@ -220,6 +222,17 @@ class Engine:
         index = np.ascontiguousarray(np.atleast_1d(index), np.int32)
         xyz = np.ascontiguousarray(np.atleast_2d(xyz), np.float64)
         _check(lib().fmc_stage_trial(self.h, len(index), _p(index, C.c_int), _p(xyz)))
+
+    def stage_mass_center(self, group, cm):
+        cm = np.ascontiguousarray(cm, np.float64)
+        if cm.size != 3:
+            raise ValueError("mass centre needs 3 values")
+        _check(lib().fmc_stage_mass_center(self.h, int(group), _p(cm)))
+
+    def mass_center(self, group):
+        cm = np.zeros(3)
+        _check(lib().fmc_get_mass_center(self.h, int(group), _p(cm)))
+        return cm
 
     def delta_u(self, change):
         c, keep = make_change(change)

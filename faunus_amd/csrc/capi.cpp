@@ -150,6 +150,21 @@ int fmc_stage_trial(fmc_engine* e, int count, const int* particle_index, const d
     return guarded([&] { eng(e).stage_trial(count, particle_index, xyz); });
 }
 
+int fmc_stage_mass_center(fmc_engine* e, int group, const double* cm) {
+    return guarded([&] {
+        if (!cm) throw std::runtime_error("null mass centre");
+        eng(e).stage_mass_center(group, cm);
+    });
+}
+
+int fmc_get_mass_center(fmc_engine* e, int group, double* cm) {
+    return guarded([&] {
+        const auto& c = eng(e).mass_centers();
+        if (group < 0 || 3 * static_cast<size_t>(group) + 2 >= c.size()) throw std::runtime_error("group out of range");
+        for (int d = 0; d < 3; ++d) cm[d] = c[3 * static_cast<size_t>(group) + static_cast<size_t>(d)];
+    });
+}
+
 int fmc_delta_u(fmc_engine* e, const fmc_change* change, double* new_terms, double* old_terms) {
     return guarded([&] {
         ChangeLists L = unpack(change);

@@ -219,11 +219,57 @@ Model Model::from_json(const std::string& text) {
             mc.atomic = kv.second.value("atomic", false);
             mc.rigid = kv.second.value("rigid", false);
             mc.compressible = kv.second.value("compressible", false);
-            if (kv.second.contains("excluded_pairs"))
-                for (auto& p : kv.second.at("excluded_pairs").items()) {
+            if (kv.second.contains("atoms"))
+                for (auto& a : kv.second.at("atoms").items()) mc.atoms.push_back(type_index(m, a.as_string()));
+            if (kv.second.contains("bondlist")) // bonds.cpp:23-60, molecule.cpp:508-517
+                for (auto& brec : kv.second.at("bondlist").items()) {
+                    auto& [btype, bp] = jsonlite::single_item(brec);
+                    if (btype != "harmonic") throw std::runtime_error("bond type '" + btype + "' is not supported by the GPU engine");
+                    const Value& idx = bp.at("index");
+                    if (idx.size() != 2) throw std::runtime_error("harmonic bond needs exactly 2 indices");
+                    BondCfg b;
+                    b.i = static_cast<int>(idx.at(0).as_number());
+                    b.j = static_cast<int>(idx.at(1).as_number());
+                    if (b.i < 0 || b.j < 0 || b.i == b.j) throw std::runtime_error("bonded index out of range");
+                    b.half_k = m.units.kJmol(bp.at("k").as_number()) / 2.0;
+                    b.req = bp.at("req").as_number();
+                    mc.bonds.push_back(b);
+                }
+            // exclusions (molecule.cpp:533-547): pairs within excluded_neighbours bonds
+            // of each other (shortest path in the bond graph), plus exclusionlist
+            const int dmax = static_cast<int>(kv.second.value("excluded_neighbours", 0.0));
+            if (dmax > 0) {
+                int size = 0;
+                for (auto& b : mc.bonds) size = std::max(size, std::max(b.i, b.j) + 1);
+                std::vector<std::vector<int>> adj(static_cast<size_t>(size));
+                for (auto& b : mc.bonds) {
+                    adj[static_cast<size_t>(b.i)].push_back(b.j);
+                    adj[static_cast<size_t>(b.j)].push_back(b.i);
+                }
+                for (int s0 = 0; s0 < size; ++s0) {
+                    std::vector<int> dist(static_cast<size_t>(size), -1), frontier{s0};
+                    dist[static_cast<size_t>(s0)] = 0;
+                    for (int d = 1; d <= dmax && !frontier.empty(); ++d) {
+                        std::vector<int> next;
+                        for (int a : frontier)
+                            for (int b : adj[static_cast<size_t>(a)])
+                                if (dist[static_cast<size_t>(b)] < 0) {
+                                    dist[static_cast<size_t>(b)] = d;
+                                    next.push_back(b);
+                                }
+                        frontier.swap(next);
+                    }
+                    for (int t = s0 + 1; t < size; ++t)
+                        if (dist[static_cast<size_t>(t)] > 0) mc.excluded.emplace_back(s0, t);
+                }
+            }
+            if (kv.second.contains("exclusionlist"))
+                for (auto& p : kv.second.at("exclusionlist").items()) {
                     int a = static_cast<int>(p.at(0).as_number()), b = static_cast<int>(p.at(1).as_number());
                     mc.excluded.emplace_back(std::min(a, b), std::max(a, b));
                 }
+            std::sort(mc.excluded.begin(), mc.excluded.end());
+            mc.excluded.erase(std::unique(mc.excluded.begin(), mc.excluded.end()), mc.excluded.end());
             m.molecules.push_back(mc);
         }
     } else {
@@ -245,6 +291,14 @@ Model Model::from_json(const std::string& text) {
         auto& [key, value] = jsonlite::single_item(rec);
         if (key == "maxenergy") {
             m.maximum_allowed_energy = value.as_number();
+            continue;
+        }
+        if (key == "bonded") { // Hamiltonian::createEnergy (energy.cpp:1328): intra-molecular bonds
+            if (value.contains("bondlist") && value.at("bondlist").size() > 0)
+                throw std::runtime_error("inter-molecular bonds are not supported by the GPU engine");
+            if (m.bonded_term >= 0) throw std::runtime_error("only one bonded term is supported");
+            m.bonded_term = static_cast<int>(m.term_names.size());
+            m.term_names.push_back("bonded");
             continue;
         }
         if (++nonbonded_terms > 1) throw std::runtime_error("only one nonbonded energy term is supported");

@@ -5,7 +5,7 @@
 // Supported energy terms (energy.cpp:1280-1327): nonbonded / nonbonded_exact
 // (FunctorPotential with "default" and "A B" lists), nonbonded_coulomblj,
 // nonbonded_coulombwca, nonbonded_pm / nonbonded_coulombhs, nonbonded_pmwca, plus
-// `maxenergy`.  Pair potentials: lennardjones, wca, hardsphere, coulomb
+// `maxenergy` and `bonded` (intra-molecular harmonic bonds, energy.cpp:974-1124).  Pair potentials: lennardjones, wca, hardsphere, coulomb
 // {type: plain|ewald|fanourgakis}.  Ewald reciprocal space is added when the
 // coulomb type is "ewald" (addEwald, energy.cpp:1134-1160).
 #pragma once
@@ -75,10 +75,18 @@ struct AtomTypeCfg {
     bool has_dp = false;
 };
 
+// HarmonicBond (bonds.cpp:100-105): indices relative to the molecule, k in kT/A^2 / 2.
+struct BondCfg {
+    int i = 0, j = 0;
+    double half_k = 0, req = 0;
+};
+
 struct MoleculeCfg {
     std::string name;
     bool atomic = false, rigid = false, compressible = false;
-    std::vector<std::pair<int, int>> excluded;
+    std::vector<int> atoms;                     // atom types listed by "atoms"
+    std::vector<BondCfg> bonds;                 // "bondlist" (harmonic)
+    std::vector<std::pair<int, int>> excluded;  // excluded_neighbours + exclusionlist, sorted (i<j)
 };
 
 struct Model {
@@ -94,7 +102,8 @@ struct Model {
     double g2g_cutoff2 = std::numeric_limits<double>::max();
     double maximum_allowed_energy = kInf;
     bool anisotropic_form = true;
-    std::vector<std::string> term_names; // Hamiltonian order: [selfenergy], nonbonded, [ewald]
+    std::vector<std::string> term_names; // Hamiltonian order: energy list, with [selfenergy] before nonbonded, [ewald] after
+    int bonded_term = -1;                // index of "bonded" in term_names, -1 if absent
     double pair_range = kInf; // distance beyond which every pair term vanishes
 
     static Model from_json(const std::string& text);

@@ -56,7 +56,8 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
     int pair_blocks, const int* __restrict__ kn, const double* __restrict__ ak, const double2* __restrict__ q_acc,
     double2* __restrict__ q_trial, int nk, int ncut, double2* __restrict__ partials, unsigned* __restrict__ ticket,
     DeltaResult* __restrict__ out, unsigned long long seq, double rec_prefactor, double cm_new_x, double cm_new_y,
-    double cm_new_z, double cm_old_x, double cm_old_y, double cm_old_z) {
+    double cm_new_z, double cm_old_x, double cm_old_y, double cm_old_z, int ewald_end,
+    const int* __restrict__ bond_off, const DevBond* __restrict__ bonds) {
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevCoulomb s_coul;
     __shared__ MovedParticle s_moved[kPhaseTableMoved];
@@ -182,11 +183,40 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
                 }
             }
         }
+    } else if (static_cast<int>(blockIdx.x) >= ewald_end) {
+        // Bonded::internalGroupEnergy (energy.cpp:1068-1088): bonds with at least one
+        // moved atom, each counted once, in both states; only for internal changes.
+        __syncthreads();
+        if (mv.internal) {
+            for (int a = tid; a < m; a += kBlock) {
+                const MovedParticle& A = mp[a];
+                for (int e = bond_off[A.idx]; e < bond_off[A.idx + 1]; ++e) {
+                    const DevBond B = bonds[e];
+                    int om = -1;
+                    {
+                        int lo = 0, hi = m;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (mp[mid].idx < B.other) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        if (lo < m && mp[lo].idx == B.other) om = lo;
+                    }
+                    if (om >= 0 && om < a) continue; // the pair is visited from the lower moved atom
+                    const double4 on = om >= 0 ? mp[om].newp : pos[B.other];
+                    const double4 oo = om >= 0 ? mp[om].oldp : on;
+                    const double xn = B.req - sqrt(dist2(box, A.newp.x, A.newp.y, A.newp.z, on.x, on.y, on.z));
+                    const double xo = B.req - sqrt(dist2(box, A.oldp.x, A.oldp.y, A.oldp.z, oo.x, oo.y, oo.z));
+                    acc_new += B.half_k * xn * xn;
+                    acc_old += B.half_k * xo * xo;
+                }
+            }
+        }
     } else if (nk > 0) {
         __syncthreads();
         // Ewald partial update (energy.cpp:219-247): dQ(k) = sum_i q_i (e^{ik.r_new} - e^{ik.r_old}).
         const int eb = blockIdx.x - pair_blocks;
-        const int e_blocks = gridDim.x - pair_blocks;
+        const int e_blocks = ewald_end - pair_blocks;
         const int width = 2 * ncut + 1;
         const double twopi = 6.283185307179586;
         if (m <= kPhaseTableMoved) {
@@ -272,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
     if (!s_last) return;
     __threadfence();
     // last block: fixed-order reduction of the pair partials and of the Ewald partials
-    double pn = 0.0, po = 0.0, en = 0.0, eo = 0.0;
+    double pn = 0.0, po = 0.0, en = 0.0, eo = 0.0, bn_ = 0.0, bo_ = 0.0;
     if (tid == 0) {
         for (int b = 0; b < static_cast<int>(gridDim.x); ++b) {
             const double* pb = reinterpret_cast<const double*>(partials) + 2 * b;
@@ -281,15 +311,20 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
             if (b < pair_blocks) {
                 pn += v.x;
                 po += v.y;
-            } else {
+            } else if (b < ewald_end) {
                 en += v.x;
                 eo += v.y;
+            } else {
+                bn_ += v.x;
+                bo_ += v.y;
             }
         }
         out->pair_new = pn;
         out->pair_old = po;
         out->rec_new = rec_prefactor * en;
         out->rec_old = rec_prefactor * eo;
+        out->bond_new = bn_;
+        out->bond_old = bo_;
         __threadfence_system();
         out->seq = seq;
         __threadfence_system();
@@ -425,6 +460,23 @@ __global__ __launch_bounds__(kBlock) void all_pairs_kernel(const double4* __rest
 
 // Plain streaming copy of a known byte count; its HBM counter readings calibrate
 // the FETCH_SIZE / WRITE_SIZE scale of the profiles.
+// Sum of all intra-molecular bond energies (Bonded::energy on Change::everything,
+// energy.cpp:1047-1066); one partial per block.
+__global__ __launch_bounds__(kBlock) void bond_total_kernel(const double4* __restrict__ pos,
+                                                             const DevBondPair* __restrict__ bonds, int nb, Box box,
+                                                             double* __restrict__ partials) {
+    __shared__ double s_red[kBlock / 64];
+    double acc = 0.0;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < nb; e += gridDim.x * kBlock) {
+        const DevBondPair B = bonds[e];
+        const double4 a = pos[B.i], b = pos[B.j];
+        const double x = B.req - sqrt(dist2(box, a.x, a.y, a.z, b.x, b.y, b.z));
+        acc += B.half_k * x * x;
+    }
+    const double s = block_sum<kBlock>(acc, s_red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void copy_probe_kernel(const double4* __restrict__ src, double4* __restrict__ dst,
                                                             long long n) {
     for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < n;
@@ -499,6 +551,11 @@ Engine::~Engine() {
     (void)hipFree(cells_.members);
     (void)hipFree(cells_.cell_of);
     (void)hipFree(cells_.overflow);
+    (void)hipFree(d_gflags_);
+    (void)hipFree(d_gcm_);
+    (void)hipFree(d_bond_off_);
+    (void)hipFree(d_bond_);
+    (void)hipFree(d_bond_all_);
     if (h_result_) (void)hipHostFree(h_result_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -563,6 +620,96 @@ void Engine::set_groups(int ng, const int* begin, const int* size, const int* mo
         if (!mol.atomic) any_molecular_ = true;
     }
     if (expect != n_) throw std::runtime_error("set_groups: groups must cover all particles");
+    if (!mass_center)
+        for (int g = 0; g < ng; ++g)
+            if (!model_.molecules[static_cast<size_t>(g_molid_[g])].atomic && g_size_[g] > 0)
+                group_mass_center_host(g, &g_cm_[static_cast<size_t>(3 * g)]);
+}
+
+// Geometry::weightedCenter with masses (geometry.h:505-525): shift by minus the first
+// atom, wrap, mass-weighted mean, shift back, wrap.
+void Engine::group_mass_center_host(int g, double* cm) const {
+    const int b = g_begin_[static_cast<size_t>(g)];
+    const double shift[3] = {-h_xyz_[3 * b], -h_xyz_[3 * b + 1], -h_xyz_[3 * b + 2]};
+    auto wrap = [&](double& x, int d) {
+        const double L = model_.box[d];
+        if (std::fabs(x) > 0.5 * L) {
+            const double t = x / L;
+            x -= L * static_cast<double>(static_cast<int>(t > 0.0 ? t + 0.5 : t - 0.5));
+        }
+    };
+    double c[3] = {0, 0, 0}, wsum = 0;
+    for (int k = 0; k < g_size_[static_cast<size_t>(g)]; ++k) {
+        const int i = b + k;
+        const double w = model_.atoms[static_cast<size_t>(h_type_[static_cast<size_t>(i)])].mw;
+        for (int d = 0; d < 3; ++d) {
+            double x = h_xyz_[3 * i + d] + shift[d];
+            wrap(x, d);
+            c[d] += w * x;
+        }
+        wsum += w;
+    }
+    for (int d = 0; d < 3; ++d) {
+        double x = c[d] / wsum - shift[d];
+        wrap(x, d);
+        cm[d] = x;
+    }
+}
+
+void Engine::stage_mass_center(int group, const double* cm) {
+    if (group < 0 || group >= static_cast<int>(g_begin_.size())) throw std::runtime_error("stage_mass_center: group out of range");
+    staged_cm_group_ = group;
+    for (int d = 0; d < 3; ++d) staged_cm_[d] = cm[d];
+}
+
+// Group flags, mass centres and the bond tables on the device.
+void Engine::build_group_tables() {
+    const int ng = static_cast<int>(g_begin_.size());
+    std::vector<int> flags(static_cast<size_t>(ng));
+    for (int g = 0; g < ng; ++g) {
+        const MoleculeCfg& mol = model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(g)])];
+        flags[static_cast<size_t>(g)] = (mol.atomic ? 0 : 1) | (mol.rigid ? 2 : 0);
+    }
+    (void)hipFree(d_gflags_);
+    (void)hipFree(d_gcm_);
+    check_hip(hipMalloc(&d_gflags_, sizeof(int) * std::max(1, ng)), "hipMalloc group flags");
+    check_hip(hipMalloc(&d_gcm_, sizeof(double) * 3 * std::max(1, ng)), "hipMalloc group centres");
+    check_hip(hipMemcpy(d_gflags_, flags.data(), sizeof(int) * ng, hipMemcpyHostToDevice), "copy group flags");
+    check_hip(hipMemcpy(d_gcm_, g_cm_.data(), sizeof(double) * 3 * ng, hipMemcpyHostToDevice), "copy group centres");
+    // bonds (Bonded::updateGroupBonds, energy.cpp:974-984: molecule bonds shifted to absolute indices)
+    std::vector<std::vector<DevBond>> per(static_cast<size_t>(n_));
+    std::vector<DevBondPair> flat;
+    if (model_.bonded_term >= 0) {
+        for (int g = 0; g < ng; ++g) {
+            const MoleculeCfg& mol = model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(g)])];
+            const int b0 = g_begin_[static_cast<size_t>(g)], sz = g_size_[static_cast<size_t>(g)];
+            for (const BondCfg& bd : mol.bonds) {
+                if (bd.i >= sz || bd.j >= sz) throw std::runtime_error("bonded index out of range for group size");
+                per[static_cast<size_t>(b0 + bd.i)].push_back(DevBond{b0 + bd.j, 0, bd.half_k, bd.req});
+                per[static_cast<size_t>(b0 + bd.j)].push_back(DevBond{b0 + bd.i, 0, bd.half_k, bd.req});
+                flat.push_back(DevBondPair{b0 + bd.i, b0 + bd.j, bd.half_k, bd.req});
+            }
+        }
+    }
+    std::vector<int> off(static_cast<size_t>(n_) + 1, 0);
+    std::vector<DevBond> list;
+    for (int i = 0; i < n_; ++i) {
+        off[static_cast<size_t>(i) + 1] = off[static_cast<size_t>(i)] + static_cast<int>(per[static_cast<size_t>(i)].size());
+        list.insert(list.end(), per[static_cast<size_t>(i)].begin(), per[static_cast<size_t>(i)].end());
+    }
+    nbonds_ = static_cast<int>(flat.size());
+    (void)hipFree(d_bond_off_);
+    (void)hipFree(d_bond_);
+    (void)hipFree(d_bond_all_);
+    check_hip(hipMalloc(&d_bond_off_, sizeof(int) * off.size()), "hipMalloc bond offsets");
+    check_hip(hipMalloc(&d_bond_, sizeof(DevBond) * std::max<size_t>(1, list.size())), "hipMalloc bonds");
+    check_hip(hipMalloc(&d_bond_all_, sizeof(DevBondPair) * std::max<size_t>(1, flat.size())), "hipMalloc bond list");
+    check_hip(hipMemcpy(d_bond_off_, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice), "copy bond offsets");
+    if (!list.empty())
+        check_hip(hipMemcpy(d_bond_, list.data(), sizeof(DevBond) * list.size(), hipMemcpyHostToDevice), "copy bonds");
+    if (!flat.empty())
+        check_hip(hipMemcpy(d_bond_all_, flat.data(), sizeof(DevBondPair) * flat.size(), hipMemcpyHostToDevice),
+                  "copy bond list");
 }
 
 void Engine::build_kvectors() {
@@ -691,6 +838,7 @@ void Engine::init() {
         for (int i = 0; i < n_; ++i) dipole_[d] += h_xyz_[3 * i + d] * h_q_[static_cast<size_t>(i)];
     }
     build_cell_list();
+    build_group_tables();
 }
 
 void Engine::stage_trial(int m, const int* idx, const double* xyz) {
@@ -739,14 +887,13 @@ void Engine::timing(double* total_ms, long* launches) {
     *launches = static_cast<long>(ev_pairs_.size());
 }
 
-void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks) {
+void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks, bool bond_block) {
     const Box box = make_box(model_);
-    GroupTable gt{nullptr, nullptr, model_.g2g_cutoff2};
-    static thread_local int* d_flags = nullptr;
-    (void)d_flags;
+    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2};
     CellArgs ca{cells_.count, cells_.members, cells_.nc[0], cells_.nc[1], cells_.nc[2], cells_.cap,
                 cells_.inv_cell[0], cells_.inv_cell[1], cells_.inv_cell[2]};
-    const int grid = pair_blocks + ewald_blocks;
+    const int ewald_end = pair_blocks + ewald_blocks;
+    const int grid = ewald_end + (bond_block ? 1 : 0);
     if (grid > max_blocks_) throw std::runtime_error("too many blocks");
     const int ncut = model_.ewald.present ? static_cast<int>(std::ceil(model_.ewald.n_cutoff)) : 0;
     const size_t lds = (args.m <= kPhaseTableMoved && nk_ > 0)
@@ -768,19 +915,25 @@ void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_block
         e1 = e0 + 1;
         check_hip(hipEventRecord(ev_pool_[static_cast<size_t>(e0)], stream_), "event record");
     }
-    // group table is only needed for molecular groups
-    const bool groups = any_molecular_;
-    if (groups) throw std::runtime_error("molecular groups are not supported by the one-move path yet");
-    if (cells_.enabled) {
-        hipLaunchKernelGGL((delta_kernel<false, true>), dim3(grid), dim3(kBlock), lds, stream_, d_pos_, d_meta_, n_,
-                           d_pairs_, model_.ntypes, d_coul_, box, gt, ca, args, pair_blocks, d_kn_, d_ak_, d_q_acc_,
-                           d_q_trial_, nk_, ncut, d_partials_, d_ticket_, h_result_, seq_, pref, 0.0, 0.0, 0.0, 0.0,
-                           0.0, 0.0);
+    // mass centres of the moved group (group cutoff between molecular groups)
+    double cn[3] = {0, 0, 0}, co[3] = {0, 0, 0};
+    if (args.change_group >= 0) {
+        for (int d = 0; d < 3; ++d) co[d] = cn[d] = g_cm_[static_cast<size_t>(3 * args.change_group + d)];
+        if (staged_cm_group_ == args.change_group)
+            for (int d = 0; d < 3; ++d) cn[d] = staged_cm_[d];
+    }
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, stream_, d_pos_, d_meta_, n_, d_pairs_,
+                           model_.ntypes, d_coul_, box, gt, ca, args, pair_blocks, d_kn_, d_ak_, d_q_acc_, d_q_trial_,
+                           nk_, ncut, d_partials_, d_ticket_, h_result_, seq_, pref, cn[0], cn[1], cn[2], co[0], co[1],
+                           co[2], ewald_end, d_bond_off_, d_bond_);
+    };
+    if (any_molecular_) {
+        if (cells_.enabled) go(delta_kernel<true, true>);
+        else go(delta_kernel<true, false>);
     } else {
-        hipLaunchKernelGGL((delta_kernel<false, false>), dim3(grid), dim3(kBlock), lds, stream_, d_pos_, d_meta_, n_,
-                           d_pairs_, model_.ntypes, d_coul_, box, gt, ca, args, pair_blocks, d_kn_, d_ak_, d_q_acc_,
-                           d_q_trial_, nk_, ncut, d_partials_, d_ticket_, h_result_, seq_, pref, 0.0, 0.0, 0.0, 0.0,
-                           0.0, 0.0);
+        if (cells_.enabled) go(delta_kernel<false, true>);
+        else go(delta_kernel<false, false>);
     }
     check_hip(hipGetLastError(), "delta_kernel launch");
     if (timing_) {
@@ -814,6 +967,10 @@ void Engine::delta(const std::vector<int>& groups, const std::vector<char>& all,
             }
         }
     }
+    if (!single && model_.g2g_cutoff2 < 1e300)
+        for (int gi : groups)
+            if (!model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(gi)])].atomic)
+                throw std::runtime_error("delta: changes of several molecular groups with cutoff_g2g are not supported");
     std::sort(moved.begin(), moved.end());
     if (std::adjacent_find(moved.begin(), moved.end()) != moved.end()) throw std::runtime_error("delta: duplicate moved particle");
     if (static_cast<int>(moved.size()) > kMaxMoved) throw std::runtime_error("delta: too many moved particles");
@@ -849,7 +1006,11 @@ void Engine::delta(const std::vector<int>& groups, const std::vector<char>& all,
     else pair_blocks = std::max(1, std::min((n_ + kBlock * 4 - 1) / (kBlock * 4), 2 * num_cu_));
     int ewald_blocks = 0;
     if (nk_ > 0) ewald_blocks = std::max(1, std::min((nk_ + 2 * kBlock - 1) / (2 * kBlock), 2 * num_cu_));
-    launch_delta(args, pair_blocks, ewald_blocks);
+    if (single && any_molecular_ && model_.g2g_cutoff2 < 1e300 &&
+        !model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(groups[0])])].atomic &&
+        staged_cm_group_ != groups[0])
+        throw std::runtime_error("delta: stage the trial mass centre of a moved molecular group when cutoff_g2g is set");
+    launch_delta(args, pair_blocks, ewald_blocks, nbonds_ > 0 && model_.bonded_term >= 0);
     check_hip(hipStreamSynchronize(stream_), "delta sync");
     volatile DeltaResult* r = h_result_;
     if (r->seq != seq_) throw std::runtime_error("delta: result sequence mismatch");
@@ -861,33 +1022,35 @@ void Engine::delta(const std::vector<int>& groups, const std::vector<char>& all,
         dipole_trial_[2] += M.newp.w * M.newp.z - M.oldp.w * M.oldp.z;
     }
     // Hamiltonian::energy order and early break (energy.cpp:1227-1241)
-    auto fill = [&](double* out, double pair, double rec, const double* dip, bool trial) {
-        int t = 0;
-        std::vector<double> vals;
-        if (model_.has_self_term) {
-            double s = 0;
-            for (size_t g = 0; g < groups.size(); ++g) {
-                const int gi = groups[g];
-                if (all[g]) {
-                    for (int i = 0; i < g_size_[static_cast<size_t>(gi)]; ++i) s += self_energy_of(g_begin_[static_cast<size_t>(gi)] + i);
-                } else {
-                    for (int r2 : rel[g]) s += self_energy_of(g_begin_[static_cast<size_t>(gi)] + r2);
-                }
+    double self = 0;
+    if (model_.has_self_term) {
+        for (size_t g = 0; g < groups.size(); ++g) {
+            const int gi = groups[g];
+            if (all[g]) {
+                for (int i = 0; i < g_size_[static_cast<size_t>(gi)]; ++i) self += self_energy_of(g_begin_[static_cast<size_t>(gi)] + i);
+            } else {
+                for (int r2 : rel[g]) self += self_energy_of(g_begin_[static_cast<size_t>(gi)] + r2);
             }
-            vals.push_back(s);
         }
-        vals.push_back(pair);
-        if (model_.ewald.present) vals.push_back(surface_energy(dip) + rec);
-        (void)trial;
+    }
+    auto fill = [&](double* out, double pair, double rec, double bond, const double* dip) {
         bool stop = false;
-        for (double v : vals) {
-            out[t++] = stop ? 0.0 : v;
+        for (size_t t = 0; t < model_.term_names.size(); ++t) {
+            const std::string& name = model_.term_names[t];
+            double v = 0.0;
+            if (name == "selfenergy") v = self;
+            else if (name == "nonbonded") v = pair;
+            else if (name == "ewald") v = surface_energy(dip) + rec;
+            else if (name == "bonded") v = bond;
+            out[t] = stop ? 0.0 : v;
             if (!stop && (v >= model_.maximum_allowed_energy || std::isnan(v))) stop = true;
         }
     };
-    fill(e_new, r->pair_new, r->rec_new, dipole_trial_, true);
-    fill(e_old, r->pair_old, r->rec_old, dipole_, false);
+    fill(e_new, r->pair_new, r->rec_new, r->bond_new, dipole_trial_);
+    fill(e_old, r->pair_old, r->rec_old, r->bond_old, dipole_);
     pending_moved_ = mv;
+    pending_cm_group_ = (single && staged_cm_group_ == groups[0]) ? staged_cm_group_ : -1;
+    for (int d = 0; d < 3; ++d) pending_cm_[d] = staged_cm_[d];
     pending_ = true;
 }
 
@@ -915,8 +1078,15 @@ void Engine::sync(bool accept) {
         }
         std::swap(d_q_acc_, d_q_trial_);
         for (int d = 0; d < 3; ++d) dipole_[d] = dipole_trial_[d];
+        if (pending_cm_group_ >= 0) {
+            for (int d = 0; d < 3; ++d) g_cm_[static_cast<size_t>(3 * pending_cm_group_ + d)] = pending_cm_[d];
+            check_hip(hipMemcpy(d_gcm_ + 3 * pending_cm_group_, pending_cm_, sizeof(double) * 3, hipMemcpyHostToDevice),
+                      "copy mass centre");
+        }
     }
     pending_ = false;
+    pending_cm_group_ = -1;
+    staged_cm_group_ = -1;
     staged_.clear();
 }
 
@@ -936,44 +1106,56 @@ void Engine::total_energy(double* terms) {
         check_hip(hipMemcpy(&cells_overflow, cells_.overflow, sizeof(int), hipMemcpyDeviceToHost), "copy overflow");
         if (cells_overflow) throw std::runtime_error("cell list overflow: a cell exceeded its capacity");
     }
-    std::vector<double> vals;
+    double self = 0;
     if (model_.has_self_term) {
-        double s = 0;
         for (size_t g = 0; g < g_begin_.size(); ++g) {
             double gs = 0;
             for (int i = 0; i < g_size_[g]; ++i) gs += self_energy_of(g_begin_[g] + i);
-            s += gs;
+            self += gs;
         }
-        vals.push_back(s);
     }
-    // nonbonded: all pairs
+    // nonbonded: all pairs (GroupPairing::all, energy.h:1290-1299)
     const int blocks = (n_ + kBlock - 1) / kBlock;
+    const int bond_blocks = nbonds_ > 0 ? std::min(64, (nbonds_ + kBlock - 1) / kBlock) : 0;
     double* d_part = nullptr;
-    check_hip(hipMalloc(&d_part, sizeof(double) * blocks), "hipMalloc partials");
-    GroupTable gt{nullptr, nullptr, model_.g2g_cutoff2};
-    if (any_molecular_) {
-        (void)hipFree(d_part);
-        throw std::runtime_error("molecular groups are not supported by the full-energy kernel yet");
-    }
-    hipLaunchKernelGGL((all_pairs_kernel<false>), dim3(blocks), dim3(kBlock), 0, stream_, d_pos_, d_meta_, n_, d_pairs_,
-                       model_.ntypes, d_coul_, make_box(model_), gt, d_part);
+    check_hip(hipMalloc(&d_part, sizeof(double) * (blocks + bond_blocks)), "hipMalloc partials");
+    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2};
+    if (any_molecular_)
+        hipLaunchKernelGGL((all_pairs_kernel<true>), dim3(blocks), dim3(kBlock), 0, stream_, d_pos_, d_meta_, n_,
+                           d_pairs_, model_.ntypes, d_coul_, make_box(model_), gt, d_part);
+    else
+        hipLaunchKernelGGL((all_pairs_kernel<false>), dim3(blocks), dim3(kBlock), 0, stream_, d_pos_, d_meta_, n_,
+                           d_pairs_, model_.ntypes, d_coul_, make_box(model_), gt, d_part);
     check_hip(hipGetLastError(), "all_pairs_kernel");
-    std::vector<double> part(static_cast<size_t>(blocks));
-    check_hip(hipMemcpyAsync(part.data(), d_part, sizeof(double) * blocks, hipMemcpyDeviceToHost, stream_), "copy partials");
+    if (bond_blocks > 0) {
+        hipLaunchKernelGGL(bond_total_kernel, dim3(bond_blocks), dim3(kBlock), 0, stream_, d_pos_, d_bond_all_, nbonds_,
+                           make_box(model_), d_part + blocks);
+        check_hip(hipGetLastError(), "bond_total_kernel");
+    }
+    std::vector<double> part(static_cast<size_t>(blocks + bond_blocks));
+    check_hip(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream_),
+              "copy partials");
     check_hip(hipStreamSynchronize(stream_), "sync");
     (void)hipFree(d_part);
-    double e = 0;
-    for (double v : part) e += v;
-    vals.push_back(e);
+    double pair = 0, bond = 0;
+    for (int b = 0; b < blocks; ++b) pair += part[static_cast<size_t>(b)];
+    for (int b = 0; b < bond_blocks; ++b) bond += part[static_cast<size_t>(blocks + b)];
+    double ewald = 0;
     if (model_.ewald.present) {
         std::vector<double2> q(static_cast<size_t>(nk_));
         check_hip(hipMemcpy(q.data(), d_q_acc_, sizeof(double2) * nk_, hipMemcpyDeviceToHost), "copy q");
-        vals.push_back(surface_energy(dipole_) + reciprocal_energy_host(q));
+        ewald = surface_energy(dipole_) + reciprocal_energy_host(q);
     }
     bool stop = false;
-    for (size_t t = 0; t < vals.size(); ++t) {
-        terms[t] = stop ? 0.0 : vals[t];
-        if (!stop && (vals[t] >= model_.maximum_allowed_energy || std::isnan(vals[t]))) stop = true;
+    for (size_t t = 0; t < model_.term_names.size(); ++t) {
+        const std::string& name = model_.term_names[t];
+        double v = 0.0;
+        if (name == "selfenergy") v = self;
+        else if (name == "nonbonded") v = pair;
+        else if (name == "ewald") v = ewald;
+        else if (name == "bonded") v = bond;
+        terms[t] = stop ? 0.0 : v;
+        if (!stop && (v >= model_.maximum_allowed_energy || std::isnan(v))) stop = true;
     }
 }
 

@@ -54,9 +54,20 @@ struct MoveArgs {
 
 // Results of one evaluation, written by the last block into pinned host memory.
 struct DeltaResult {
-    double pair_new, pair_old, rec_new, rec_old;
+    double pair_new, pair_old, rec_new, rec_old, bond_new, bond_old;
     unsigned long long seq;
-    unsigned long long pad[3];
+    unsigned long long pad;
+};
+
+// One bond as seen from one of its atoms (per-particle lists) and one bond of the
+// flat list used for the total energy.
+struct DevBond {
+    int other, pad;
+    double half_k, req;
+};
+struct DevBondPair {
+    int i, j;
+    double half_k, req;
 };
 
 struct CellList {
@@ -88,6 +99,9 @@ class Engine {
 
     // Trial coordinates of the particles a move displaced (absolute indices).
     void stage_trial(int m, const int* idx, const double* xyz);
+    // Trial mass centre of a moved molecular group (needed with a group cutoff).
+    void stage_mass_center(int group, const double* cm);
+    const std::vector<double>& mass_centers() const { return g_cm_; }
     // Energies of the affected pairs/terms of `change` in both states, per term in
     // Hamiltonian order.  Writes the trial Q(k) (Ewald::updateState).
     void delta(const std::vector<int>& groups, const std::vector<char>& all, const std::vector<char>& internal,
@@ -143,6 +157,15 @@ class Engine {
     std::vector<int> g_begin_, g_size_, g_molid_;
     std::vector<double> g_cm_;
     bool any_molecular_ = false;
+    int* d_gflags_ = nullptr;    // bit0 molecular, bit1 rigid
+    double* d_gcm_ = nullptr;    // accepted mass centres, 3 per group
+    int staged_cm_group_ = -1, pending_cm_group_ = -1;
+    double staged_cm_[3] = {0, 0, 0}, pending_cm_[3] = {0, 0, 0};
+    // bonds (intra-molecular harmonic), per particle and as one list
+    int* d_bond_off_ = nullptr;
+    DevBond* d_bond_ = nullptr;
+    DevBondPair* d_bond_all_ = nullptr;
+    int nbonds_ = 0;
     // potentials
     std::vector<DevPair> h_pairs_;
     DevCoulomb h_coul_{};
@@ -178,7 +201,10 @@ class Engine {
     void build_kvectors();
     void full_q(double2* out);
     void build_cell_list();
-    void launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks);
+    void launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks, bool bond_block);
+    void build_group_tables();
+    // PBC-safe mass centre of group g around its first atom (Geometry::massCenter)
+    void group_mass_center_host(int g, double* cm) const;
     double reciprocal_energy_host(const std::vector<double2>& q) const;
     double surface_energy(const double* dipole) const;
 };

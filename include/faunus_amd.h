@@ -9,9 +9,10 @@
  * returned as values, exactly as the reference's terms return them.
  *
  * Energies are in kT, lengths in angstrom, charges in e.  Term energies are
- * reported in Hamiltonian order: [selfenergy], nonbonded, [ewald]
- * (src/energy.cpp:1171-1211; self energy is added before the nonbonded term by
- * PairEnergy::addPairPotentialSelfEnergy, src/energy.h:462-468).
+ * reported in Hamiltonian order: the `energy` list order, with [selfenergy] just
+ * before nonbonded and [ewald] just after it (src/energy.cpp:1171-1211; self energy
+ * is added by PairEnergy::addPairPotentialSelfEnergy, src/energy.h:462-468), and
+ * `bonded` (intra-molecular harmonic bonds, src/energy.cpp:974-1124) where listed.
  */
 #ifndef FAUNUS_AMD_H
 #define FAUNUS_AMD_H
@@ -80,6 +81,17 @@ int fmc_term_name(fmc_engine* engine, int index, char* buffer, int length);
  * each) — the part of the trial Space that Move::move (src/move.cpp:65-76) mutated.
  */
 int fmc_stage_trial(fmc_engine* engine, int count, const int* particle_index, const double* xyz);
+
+/*
+ * Trial mass centre of the moved molecular group (Group::mass_center of the trial
+ * Space after Group::translate/rotate, src/group.cpp:110-123, or the recomputed centre
+ * of AtomicTranslateRotate::translateParticle, src/move.cpp:225-240, and
+ * ChainRotationMove::rotate_segment, src/chainmove.cpp:76-104).  Needed when
+ * cutoff_g2g is set (GroupCutoff::cut, src/energy.h:761-768); kept on accept.
+ */
+int fmc_stage_mass_center(fmc_engine* engine, int group, const double* cm);
+/* Accepted-state mass centre of a group (molecular groups; atomic groups report 0). */
+int fmc_get_mass_center(fmc_engine* engine, int group, double* cm);
 
 /*
  * Ewald::updateState (src/energy.cpp:564-575) followed by Hamiltonian::energy(Change)
