@@ -50,6 +50,8 @@ SIGNATURES = {
     "fmc_batch_destroy": (C.c_int, [C.c_void_p]),
     "fmc_batch_add_replica": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_double,
                                         C.c_ulonglong, C.c_int, I]),
+    "fmc_batch_add_replica_moves": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_double,
+                                              C.c_ulonglong, C.c_int, C.c_int, I, I, C.POINTER(C.c_uint), D, D, I]),
     "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
     "fmc_batch_run_split": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "fmc_batch_run_pipe": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
@@ -340,15 +342,28 @@ class Batch:
         except Exception:
             pass
 
-    def add(self, system, beta=1.0, seed=1, ladder=-1):
+    def add(self, system, beta=1.0, seed=1, ladder=-1, moves=None):
+        """Add a replica; ``moves`` (list of dicts with kind, molid, weight, dp, dprot)
+        selects the general move set, otherwise single-particle translations."""
         cfg, xyz, q, ids, g = system
         text = cfg if isinstance(cfg, str) else json.dumps(cfg)
         xyz, q, ids = _particles(xyz, q, ids)
         gb, gs, gm, _ = _groups(g["begin"], g["size"], g["molid"])
         r = C.c_int()
-        _check(lib().fmc_batch_add_replica(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
-                                           _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), float(beta),
-                                           int(seed), int(ladder), C.byref(r)))
+        if moves is None:
+            _check(lib().fmc_batch_add_replica(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int),
+                                               len(gb), _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int),
+                                               float(beta), int(seed), int(ladder), C.byref(r)))
+        else:
+            kind = np.array([k["kind"] for k in moves], np.int32)
+            molid = np.array([k["molid"] for k in moves], np.int32)
+            weight = np.array([k.get("weight", 1) for k in moves], np.uint32)
+            dp = np.array([float(k.get("dp", 0.0)) for k in moves])
+            dprot = np.array([float(k.get("dprot", 0.0)) for k in moves])
+            _check(lib().fmc_batch_add_replica_moves(
+                self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb), _p(gb, C.c_int),
+                _p(gs, C.c_int), _p(gm, C.c_int), float(beta), int(seed), int(ladder), len(kind), _p(kind, C.c_int),
+                _p(molid, C.c_int), _p(weight, C.c_uint), _p(dp), _p(dprot), C.byref(r)))
         self.sizes.append(len(q))
         return r.value
 

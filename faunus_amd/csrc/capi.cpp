@@ -351,6 +351,27 @@ int fmc_batch_add_replica(fmc_batch* b, const char* config_json, int n, const do
     });
 }
 
+int fmc_batch_add_replica_moves(fmc_batch* b, const char* config_json, int n, const double* xyz, const double* charge,
+                                const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                                double beta, unsigned long long seed, int ladder, int num_kinds, const int* kind,
+                                const int* kind_molid, const unsigned* weight, const double* dp, const double* dprot,
+                                int* replica) {
+    return guarded([&] {
+        if (num_kinds < 1 || !kind || !kind_molid || !weight || !dp || !dprot) throw std::runtime_error("empty move set");
+        std::vector<fmc::MoveKindSpec> kinds(static_cast<size_t>(num_kinds));
+        for (int k = 0; k < num_kinds; ++k) {
+            kinds[static_cast<size_t>(k)].kind = kind[k];
+            kinds[static_cast<size_t>(k)].molid = kind_molid[k];
+            kinds[static_cast<size_t>(k)].weight = weight[k];
+            kinds[static_cast<size_t>(k)].dp = dp[k];
+            kinds[static_cast<size_t>(k)].dprot = dprot[k];
+        }
+        const int r = bat(b).add_replica(config_json, n, xyz, charge, atom_id, num_groups, begin, size, molid, beta,
+                                         seed, ladder, &kinds);
+        if (replica) *replica = r;
+    });
+}
+
 int fmc_batch_run(fmc_batch* b, const long long* moves, int trace_capacity) {
     return guarded([&] {
         std::vector<long long> m(moves, moves + bat(b).size());

@@ -134,6 +134,12 @@ class Engine {
     const DevPair* d_pairs() const { return d_pairs_; }
     const DevCoulomb* d_coul() const { return d_coul_; }
     CellList& cells() { return cells_; }
+    int* d_group_flags() { return d_gflags_; }
+    double* d_mass_centers() { return d_gcm_; }
+    const int* d_bond_offsets() const { return d_bond_off_; }
+    const DevBond* d_bonds() const { return d_bond_; }
+    int num_bonds() const { return nbonds_; }
+    const std::vector<int>& group_molid() const { return g_molid_; }
     double self_energy_of(int idx) const;
     const std::vector<int>& group_of() const { return h_gid_; }
     const std::vector<int>& group_begin() const { return g_begin_; }

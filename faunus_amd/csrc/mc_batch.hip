@@ -418,8 +418,18 @@ Batch::~Batch() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+namespace {
+template <class T> T* upload(const std::vector<T>& v, std::vector<void*>& owned, const char* what) {
+    T* d = nullptr;
+    check_hip(hipMalloc(&d, sizeof(T) * std::max<size_t>(1, v.size())), what);
+    owned.push_back(d);
+    if (!v.empty()) check_hip(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), what);
+    return d;
+}
+} // namespace
+
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
-                                std::vector<void*>& owned) {
+                                std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds) {
     const Model& m = e.model();
     const int n = e.num_particles();
     if (!(m.box[0] == m.box[1] && m.box[1] == m.box[2])) throw std::runtime_error("device MC needs a cubic box");
@@ -429,8 +439,11 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.ncut = m.ewald.present ? static_cast<int>(std::ceil(m.ewald.n_cutoff)) : 0;
     if (s.ncut * s.ncut >= kMaxGk) throw std::runtime_error("ncutoff too large for the device MC (max 32)");
     if (m.ewald.present && !m.ewald.spherical_sum) throw std::runtime_error("device MC needs spherical_sum: true");
-    for (const auto& mol : m.molecules)
-        if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
+    if (!kinds)
+        for (const auto& mol : m.molecules)
+            if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
+    if (m.ewald.present && m.ewald.const_inf > 0.5)
+        throw std::runtime_error("device MC needs tinfoil boundary conditions (epss = 0)");
     s.ntypes = m.ntypes;
     s.L = m.box[0];
     s.half = 0.5 * m.box[0];
@@ -494,6 +507,71 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.replica_id = stream_id;
     s.ladder = ladder;
     s.beta = beta;
+    if (kinds) {
+        if (kinds->empty()) throw std::runtime_error("empty move set");
+        if (m.g2g_cutoff2 < 1e300) throw std::runtime_error("device MC with molecular moves needs cutoff_g2g unset");
+        const auto& gb = e.group_begin();
+        const auto& gs = e.group_size();
+        const auto& gm = e.group_molid();
+        std::vector<MoveKindDev> kd;
+        std::vector<int> kgroups;
+        std::vector<int2> pbonds;
+        unsigned long long cum = 0;
+        int max_moved = 1;
+        std::vector<int> kind_of_molid(m.molecules.size(), -1);
+        for (const MoveKindSpec& k : *kinds) {
+            if (k.kind < 0 || k.kind > 2) throw std::runtime_error("unknown move kind");
+            if (k.molid < 0 || k.molid >= static_cast<int>(m.molecules.size())) throw std::runtime_error("move molid out of range");
+            // translate+rotate keeps the mass centre by translation only: its molecule
+            // type must not also be changed internally by the other kinds
+            int& seen = kind_of_molid[static_cast<size_t>(k.molid)];
+            if (seen >= 0 && ((seen == 1) != (k.kind == 1)))
+                throw std::runtime_error("a molecule type moved by translate+rotate takes no atom or pivot moves");
+            seen = k.kind;
+            MoveKindDev d{};
+            d.kind = k.kind;
+            d.group_first = static_cast<int>(kgroups.size());
+            for (size_t g = 0; g < gm.size(); ++g)
+                if (gm[g] == k.molid) {
+                    kgroups.push_back(static_cast<int>(g));
+                    if (k.kind != 0) max_moved = std::max(max_moved, gs[g]);
+                }
+            d.group_count = static_cast<int>(kgroups.size()) - d.group_first;
+            if (d.group_count == 0) throw std::runtime_error("move kind without groups");
+            const MoleculeCfg& mol = m.molecules[static_cast<size_t>(k.molid)];
+            if (k.kind == 1 && mol.atomic) throw std::runtime_error("translate+rotate needs a molecular group");
+            if (k.kind == 2) {
+                if (mol.atomic) throw std::runtime_error("pivot needs a molecular group");
+                d.bond_first = static_cast<int>(pbonds.size());
+                for (const BondCfg& b : mol.bonds) pbonds.push_back(make_int2(b.i, b.j));
+                d.bond_count = static_cast<int>(pbonds.size()) - d.bond_first;
+            }
+            cum += k.weight;
+            d.cum = cum;
+            d.dp = k.dp;
+            d.dprot = k.dprot;
+            kd.push_back(d);
+        }
+        if (cum == 0 || cum > 0xffffffffull) throw std::runtime_error("move weights must sum to 1..2^32-1");
+        if (max_moved > kGeneralMaxMoved) throw std::runtime_error("molecule too large for the device MC");
+        std::vector<double> dpa(static_cast<size_t>(m.ntypes));
+        for (int t = 0; t < m.ntypes; ++t)
+            dpa[static_cast<size_t>(t)] = m.atoms[static_cast<size_t>(t)].has_dp ? m.atoms[static_cast<size_t>(t)].dp : -1.0;
+        s.nkinds = static_cast<int>(kd.size());
+        s.max_moved = max_moved;
+        s.total_weight = cum;
+        s.kinds = upload(kd, owned, "upload move kinds");
+        s.kind_groups = upload(kgroups, owned, "upload kind groups");
+        s.pivot_bonds = upload(pbonds, owned, "upload pivot bonds");
+        s.gbegin = upload(gb, owned, "upload group begin");
+        s.gsize = upload(gs, owned, "upload group size");
+        s.gflags = e.d_group_flags();
+        s.gcm = e.d_mass_centers();
+        s.dp_atom = upload(dpa, owned, "upload atom dp");
+        const bool bonded = m.bonded_term >= 0 && e.num_bonds() > 0;
+        s.bond_off = bonded ? e.d_bond_offsets() : nullptr;
+        s.bonds = bonded ? e.d_bonds() : nullptr;
+    }
     std::vector<double> terms(static_cast<size_t>(e.num_terms()));
     e.total_energy(terms.data());
     double u0 = 0;
@@ -522,13 +600,18 @@ void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_sta
 
 int Batch::add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                        int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
-                       unsigned long long seed, int ladder) {
+                       unsigned long long seed, int ladder, const std::vector<MoveKindSpec>* kinds) {
     if (finalized_) throw std::runtime_error("batch already finalized");
+    const int general = kinds ? 1 : 0;
+    if (general_ >= 0 && general_ != general)
+        throw std::runtime_error("a batch holds either translate-only replicas or replicas with move sets");
     auto e = std::make_unique<Engine>(config_json);
     e->set_particles(n, xyz, charge, ids);
     e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
     e->init();
-    ReplicaState s = make_replica_state(*e, fp32_ != 0, seed, static_cast<int>(engines_.size()), beta, ladder, owned_);
+    ReplicaState s = make_replica_state(*e, fp32_ != 0, seed, static_cast<int>(engines_.size()), beta, ladder, owned_,
+                                        kinds);
+    general_ = general;
     h_states_.push_back(s);
     engines_.push_back(std::move(e));
     return s.replica_id;
@@ -572,7 +655,8 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
     TraceEntry* tr = trace ? d_trace_ : nullptr;
     const int cap = trace ? trace_cap : 0;
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
-    launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
+    if (general_ == 1) launch_general(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
+    else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 

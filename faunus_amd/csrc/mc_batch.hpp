@@ -15,6 +15,28 @@
 
 namespace fmc {
 
+constexpr int kGeneralMaxMoved = 1024; // atoms moved together by one general-kernel move
+
+// One kind of trial move of the general move set (the reference's MoveCollection
+// entries, move.cpp:513-525, picked with probability weight / total weight).
+struct MoveKindDev {
+    int kind;         // 0 translate one atom, 1 translate+rotate a molecule, 2 pivot
+    int group_first;  // groups of the target molecule type: kind_groups[group_first ...]
+    int group_count;
+    int bond_first;   // kind 2: the molecule's harmonic bonds, pivot_bonds[bond_first ...]
+    int bond_count;
+    int pad;
+    unsigned long long cum; // cumulative weight
+    double dp, dprot;
+};
+
+// Host description of one move kind (see MoveKindDev).
+struct MoveKindSpec {
+    int kind = 0, molid = 0;
+    unsigned weight = 1;
+    double dp = 0.0, dprot = 0.0;
+};
+
 // Per-replica device state (one per workgroup).
 struct ReplicaState {
     int n, nk, ncut, ntypes;
@@ -44,6 +66,19 @@ struct ReplicaState {
     double sum_du;
     int error;
     int pad;
+    // general move set (mc_general_kernel); nkinds == 0 for the translate-only kernel
+    int nkinds, max_moved;
+    unsigned long long total_weight;
+    const MoveKindDev* kinds;
+    const int* kind_groups;
+    const int2* pivot_bonds;  // (i, j) relative to the molecule
+    const int* gbegin;
+    const int* gsize;
+    const int* gflags;        // bit0 molecular, bit1 rigid
+    double* gcm;              // mass centres (kept for translate+rotate targets)
+    const double* dp_atom;    // per type: the atom's dp, or -1 when the move's dp applies
+    const int* bond_off;      // per-particle bond lists (bonded term), or null
+    const DevBond* bonds;
 };
 
 // One recorded move of the trace (tests).
@@ -57,9 +92,12 @@ struct TraceEntry {
 // Device state of one replica (engine already initialised); device buffers it makes
 // are appended to `owned`, which the caller frees.
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
-                                std::vector<void*>& owned);
+                                std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds = nullptr);
 // One persistent launch advancing `count` replicas (states already on the device).
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
+                    TraceEntry* trace, int trace_cap, hipStream_t stream);
+// Same for replicas with a general move set (molecular moves, bonds).
+void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream);
 
 // A set of replicas advanced together on one stream.
@@ -69,7 +107,7 @@ class Batch {
     ~Batch();
     int add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                     int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
-                    unsigned long long seed, int ladder);
+                    unsigned long long seed, int ladder, const std::vector<MoveKindSpec>* kinds = nullptr);
     void finalize();
     // advance every replica by `moves[r]` moves (one launch); trace optional
     void run(const std::vector<long long>& moves, bool trace, int trace_cap);
@@ -96,6 +134,7 @@ class Batch {
 
   private:
     int fp32_ = 0;
+    int general_ = -1; // -1 unset, 0 translate-only replicas, 1 general move sets
     std::vector<std::unique_ptr<Engine>> engines_;
     std::vector<ReplicaState> h_states_;
     std::vector<void*> owned_; // device buffers made for the replicas

@@ -1,0 +1,606 @@
+// Device-resident Metropolis over a general move set: translations of single atoms
+// inside any molecule type, rigid translate+rotate of molecules and pivot rotations
+// of chains.  One workgroup advances one replica; each move's nonbonded pair dU, Ewald
+// reciprocal dU and harmonic bonded dU are evaluated in the same pass (the reference's
+// MoveCollection::sample + performMove loop, move.cpp:513-525 and
+// montecarlo.cpp:139-187, with the proposals of move.cpp:225-345 (atoms),
+// move.cpp:1618-1689 (molecules) and chainmove.cpp:76-104, 216-268 (pivot)).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <stdexcept>
+
+#include "device_math.hpp"
+#include "mc_batch.hpp"
+#include "mc_common.hpp"
+#include "philox.hpp"
+
+namespace fmc {
+namespace {
+
+constexpr int kMaxKinds = 8;   // move kinds per replica
+constexpr int kChunk = 8;      // moved atoms per Ewald phase-table chunk
+
+__device__ __forceinline__ int pick(uint32_t x, int n) {
+    return static_cast<int>((static_cast<uint64_t>(x) * static_cast<uint64_t>(n)) >> 32);
+}
+
+// Isotropic unit vector from z = 2u-1 and phi = 2 pi u'.
+__device__ __forceinline__ void unit_vector(uint32_t zw, uint32_t pw, double* v) {
+    const double z = 2.0 * u01_from32(zw) - 1.0;
+    const double phi = 6.283185307179586 * u01_from32(pw);
+    const double s = sqrt(1.0 - z * z);
+    v[0] = s * cos(phi);
+    v[1] = s * sin(phi);
+    v[2] = z;
+}
+
+// Rotation by `angle` about the unit vector u (Rodrigues' formula), row-major.
+__device__ __forceinline__ void rotation_matrix(const double* u, double angle, double* R) {
+    const double s = sin(angle), c = cos(angle);
+    const double sa0 = u[0] * s, sa1 = u[1] * s, sa2 = u[2] * s;
+    const double ca0 = u[0] * (1.0 - c), ca1 = u[1] * (1.0 - c), ca2 = u[2] * (1.0 - c);
+    R[0] = ca0 * u[0] + c;
+    R[1] = ca0 * u[1] - sa2;
+    R[2] = ca0 * u[2] + sa1;
+    R[3] = ca0 * u[1] + sa2;
+    R[4] = ca1 * u[1] + c;
+    R[5] = ca1 * u[2] - sa0;
+    R[6] = ca0 * u[2] - sa1;
+    R[7] = ca1 * u[2] + sa0;
+    R[8] = ca2 * u[2] + c;
+}
+
+// Proposal of one move, identical on every lane of the workgroup.
+struct GeneralMove {
+    int kind, group, mlo, mhi, internal;
+    double scale, dir[3];       // kind 0 / kind 1 translation
+    bool translate, rotate;     // kind 1
+    double R[9];                // kind 1 and 2 rotation
+    double cm_new[3];           // kind 1 rotation centre after the translation
+    double shift[3], origin[3]; // kind 2 frame
+    double accept_u;
+};
+
+template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
+__global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __restrict__ states,
+                                                               const long long* __restrict__ moves,
+                                                               TraceEntry* __restrict__ trace, int trace_cap) {
+    ReplicaState& S = states[blockIdx.x];
+    const long long M = moves[blockIdx.x];
+    if (M <= 0 || S.error) return;
+    __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
+    __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
+    __shared__ float s_knots_f[kF32 ? kMaxKnots : 1];
+    __shared__ float s_coeff_f[kF32 ? 6 * kMaxKnots : 1];
+    __shared__ DevCoulomb s_coul;
+    __shared__ double s_gk[kMaxGk];
+    __shared__ double s_dpa[kMaxTypes];
+    __shared__ MoveKindDev s_kinds[kMaxKinds];
+    __shared__ double s_red[5][kThreads / 64];
+    __shared__ int s_accept;
+    extern __shared__ int s_dyn[]; // kn | phase chunk tables | new | old coordinates | moved types
+
+    const int tid = threadIdx.x;
+    const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
+    const int width = 2 * ncut + 1;
+    const int kpt = S.kpt;
+    const double L = S.L, half = S.half, inv = S.inv;
+    const int nk8 = (nk + 7) & ~7;
+    int* s_kn = s_dyn;
+    double2* s_phase = reinterpret_cast<double2*>(s_dyn + nk8);
+    double4* s_new = reinterpret_cast<double4*>(s_phase + kChunk * 6 * width);
+    double4* s_old = s_new + S.max_moved;
+    int* s_mt = reinterpret_cast<int*>(s_old + S.max_moved);
+
+    for (int i = tid; i < nt * nt; i += kThreads) {
+        s_pairs[i] = S.pairs[i];
+        if (kF32) {
+            const DevPair& p = S.pairs[i];
+            s_pairs_f[i] = DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4),
+                                    static_cast<float>(p.lj_cut2), static_cast<float>(p.wca_s2),
+                                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2),
+                                    static_cast<float>(p.coul), 0.0f};
+        }
+    }
+    if (tid == 0) s_coul = *S.coul;
+    for (int i = tid; i < nt; i += kThreads) s_dpa[i] = S.dp_atom[i];
+    for (int i = tid; i < S.nkinds; i += kThreads) s_kinds[i] = S.kinds[i];
+    for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
+    for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
+    __syncthreads();
+    if (kF32) {
+        for (int i = tid; i < s_coul.nknots; i += kThreads) s_knots_f[i] = static_cast<float>(s_coul.knots[i]);
+        for (int i = tid; i < 6 * s_coul.nknots; i += kThreads) s_coeff_f[i] = static_cast<float>(s_coul.coeff[i]);
+    }
+
+    const int k0 = tid * kpt;
+    double2 q[kMaxKPT];
+#pragma unroll
+    for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < nk) ? S.q[k0 + s] : make_double2(0.0, 0.0);
+
+    double4* __restrict__ pos = S.pos;
+    float4* __restrict__ pos32 = S.pos32;
+    const uint32_t* __restrict__ meta = S.meta;
+    const unsigned long long move0 = S.moves;
+    const double beta = S.beta;
+    const double pref = S.rec_prefactor;
+    double energy = S.energy, sum_du = S.sum_du;
+    unsigned long long accepted = S.accepted;
+    const uint32_t replica = static_cast<uint32_t>(S.replica_id);
+    const unsigned long long seed = S.seed;
+    const Box box{{L, L, L}, {half, half, half}};
+    __syncthreads();
+
+    for (long long t = 0; t < M; ++t) {
+        // ---- proposal (every lane computes the same values)
+        const unsigned long long mvi = move0 + static_cast<unsigned long long>(t);
+        const uint32_t c0 = static_cast<uint32_t>(mvi), c1 = static_cast<uint32_t>(mvi >> 32);
+        const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
+        const Philox4 ra = philox4x32_10(c0, c1, replica, 0u, key0, key1);
+        const Philox4 rb = philox4x32_10(c0, c1, replica, 1u, key0, key1);
+        const Philox4 rc = philox4x32_10(c0, c1, replica, 2u, key0, key1);
+        GeneralMove P;
+        P.accept_u = (static_cast<double>(rb.v[0] >> 5) * 67108864.0 + static_cast<double>(rb.v[1] >> 6)) *
+                     1.1102230246251565e-16;
+        {
+            const uint64_t sel = (static_cast<uint64_t>(rb.v[2]) * S.total_weight) >> 32;
+            int k = 0;
+            while (k + 1 < S.nkinds && sel >= s_kinds[k].cum) ++k;
+            const MoveKindDev K = s_kinds[k];
+            P.kind = K.kind;
+            P.group = -1;
+            P.mlo = P.mhi = 0;
+            P.internal = 0;
+            P.translate = P.rotate = false;
+            if (K.kind == 0) {
+                const int g = S.kind_groups[K.group_first + (K.group_count == 1 ? 0 : pick(rc.v[3], K.group_count))];
+                P.group = g;
+                const int size = S.gsize[g];
+                if (size > 0) {
+                    const int i = S.gbegin[g] + pick(ra.v[0], size);
+                    const int ti = static_cast<int>(meta[i] & 0xffu);
+                    const double step = s_dpa[ti] >= 0.0 ? s_dpa[ti] : K.dp;
+                    if (step > 0.0) {
+                        unit_vector(ra.v[1], ra.v[2], P.dir);
+                        P.scale = step * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
+                        P.mlo = i;
+                        P.mhi = i + 1;
+                        P.internal = 1;
+                    }
+                }
+            } else if (K.kind == 1) {
+                const int g = S.kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
+                P.group = g;
+                const int size = S.gsize[g];
+                if (size > 0) {
+                    const double* cm = S.gcm + 3 * g;
+                    P.cm_new[0] = cm[0];
+                    P.cm_new[1] = cm[1];
+                    P.cm_new[2] = cm[2];
+                    if (K.dp > 0.0) {
+                        unit_vector(ra.v[1], ra.v[2], P.dir);
+                        P.scale = K.dp * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
+                        P.dir[0] *= P.scale;
+                        P.dir[1] *= P.scale;
+                        P.dir[2] *= P.scale;
+                        P.translate = true;
+                        P.cm_new[0] = wrap(P.cm_new[0] + P.dir[0], L, half, inv);
+                        P.cm_new[1] = wrap(P.cm_new[1] + P.dir[1], L, half, inv);
+                        P.cm_new[2] = wrap(P.cm_new[2] + P.dir[2], L, half, inv);
+                    }
+                    if (K.dprot > DBL_EPSILON) {
+                        double axis[3];
+                        unit_vector(rc.v[0], rc.v[1], axis);
+                        rotation_matrix(axis, K.dprot * (u01_from32(rc.v[2]) - 0.5), P.R);
+                        P.rotate = true;
+                    }
+                    if (P.translate || P.rotate) {
+                        P.mlo = S.gbegin[g];
+                        P.mhi = P.mlo + size;
+                    }
+                }
+            } else {
+                const int g = S.kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
+                P.group = g;
+                const int size = S.gsize[g], b0 = S.gbegin[g];
+                if (size > 2 && K.bond_count > 0 && fabs(K.dprot) > 1e-9) {
+                    const int2 bd = S.pivot_bonds[K.bond_first + pick(rc.v[3], K.bond_count)];
+                    int atom0 = b0 + bd.x, atom1 = b0 + bd.y;
+                    int slo, shi;
+                    if (u01_from32(rb.v[3]) > 0.5) {
+                        slo = atom1 + 1;
+                        shi = b0 + size;
+                        const int tmp = atom0;
+                        atom0 = atom1;
+                        atom1 = tmp;
+                    } else {
+                        slo = b0;
+                        shi = atom0;
+                    }
+                    if (shi > slo) {
+                        const double angle = K.dprot * (u01_from32(rc.v[2]) - 0.5);
+                        const double4 pa = pos[atom0], pb = pos[atom1];
+                        P.shift[0] = pa.x;
+                        P.shift[1] = pa.y;
+                        P.shift[2] = pa.z;
+                        P.origin[0] = wrap(pa.x - P.shift[0], L, half, inv);
+                        P.origin[1] = wrap(pa.y - P.shift[1], L, half, inv);
+                        P.origin[2] = wrap(pa.z - P.shift[2], L, half, inv);
+                        double v[3] = {P.origin[0] - wrap(pb.x - P.shift[0], L, half, inv),
+                                       P.origin[1] - wrap(pb.y - P.shift[1], L, half, inv),
+                                       P.origin[2] - wrap(pb.z - P.shift[2], L, half, inv)};
+                        for (int d = 0; d < 3; ++d) v[d] = min_image(v[d], L, half);
+                        const double norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        const double u[3] = {v[0] / norm, v[1] / norm, v[2] / norm};
+                        rotation_matrix(u, angle, P.R);
+                        P.mlo = slo;
+                        P.mhi = shi;
+                        P.internal = 1;
+                    }
+                }
+            }
+        }
+        const int m = P.mhi - P.mlo;
+        const int g = P.group;
+
+        // ---- trial coordinates of the moved atoms
+        for (int a = tid; a < m; a += kThreads) {
+            const int idx = P.mlo + a;
+            const double4 p = pos[idx];
+            s_old[a] = p;
+            s_mt[a] = static_cast<int>(meta[idx] & 0xffu);
+            double x = p.x, y = p.y, z = p.z;
+            if (P.kind == 0) {
+                x = wrap(x + P.dir[0] * P.scale, L, half, inv);
+                y = wrap(y + P.dir[1] * P.scale, L, half, inv);
+                z = wrap(z + P.dir[2] * P.scale, L, half, inv);
+            } else if (P.kind == 1) {
+                if (P.translate) { // Group::translate (group.cpp:115-123)
+                    x = wrap(x + P.dir[0], L, half, inv);
+                    y = wrap(y + P.dir[1], L, half, inv);
+                    z = wrap(z + P.dir[2], L, half, inv);
+                }
+                if (P.rotate) { // Geometry::rotate about the mass centre (geometry.h:613-626)
+                    x = wrap(x - P.cm_new[0], L, half, inv);
+                    y = wrap(y - P.cm_new[1], L, half, inv);
+                    z = wrap(z - P.cm_new[2], L, half, inv);
+                    const double rx = P.R[0] * x + P.R[1] * y + P.R[2] * z;
+                    const double ry = P.R[3] * x + P.R[4] * y + P.R[5] * z;
+                    const double rz = P.R[6] * x + P.R[7] * y + P.R[8] * z;
+                    x = wrap(rx + P.cm_new[0], L, half, inv);
+                    y = wrap(ry + P.cm_new[1], L, half, inv);
+                    z = wrap(rz + P.cm_new[2], L, half, inv);
+                }
+            } else { // ChainRotationMove::rotate_segment (chainmove.cpp:76-104)
+                const double fx = wrap(x - P.shift[0], L, half, inv) - P.origin[0];
+                const double fy = wrap(y - P.shift[1], L, half, inv) - P.origin[1];
+                const double fz = wrap(z - P.shift[2], L, half, inv) - P.origin[2];
+                const double rx = P.R[0] * fx + P.R[1] * fy + P.R[2] * fz + P.origin[0];
+                const double ry = P.R[3] * fx + P.R[4] * fy + P.R[5] * fz + P.origin[1];
+                const double rz = P.R[6] * fx + P.R[7] * fy + P.R[8] * fz + P.origin[2];
+                x = wrap(rx + P.shift[0], L, half, inv);
+                y = wrap(ry + P.shift[1], L, half, inv);
+                z = wrap(rz + P.shift[2], L, half, inv);
+            }
+            s_new[a] = make_double4(x, y, z, p.w);
+        }
+        __syncthreads();
+
+        // ---- nonbonded pairs: moved x static (GroupPairing, energy.h:885-1027, 1155-1224)
+        double e_new = 0.0, e_old = 0.0;
+        const int gflag = g >= 0 ? S.gflags[g] : 0;
+        const bool same_ok = P.internal && !(gflag & 2); // pairs inside the moved group
+        if (m > 0) {
+            if (!kCells) {
+                constexpr int kB = kF32 ? 8 : 4;
+                for (int j0 = tid; j0 < n; j0 += kThreads * kB) {
+                    uint32_t mj[kB];
+                    double4 pj[kB];
+                    float4 pf[kB];
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j < n) {
+                            mj[u] = meta[j];
+                            if (kF32) pf[u] = pos32[j];
+                            else pj[u] = pos[j];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        const int j = j0 + u * kThreads;
+                        if (j >= n || (j >= P.mlo && j < P.mhi)) continue;
+                        if (static_cast<int>(mj[u] >> 8) == g && !same_ok) continue;
+                        const int tj = static_cast<int>(mj[u] & 0xffu);
+                        for (int a = 0; a < m; ++a) {
+                            const double4 A = s_new[a], B = s_old[a];
+                            const int ta = s_mt[a];
+                            if (kF32) {
+                                const DevPairF& p = s_pairs_f[ta * nt + tj];
+                                const float fL = static_cast<float>(L), fh = static_cast<float>(half);
+                                e_new += static_cast<double>(pair_energy_f(
+                                    p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pf[u].w,
+                                    dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y),
+                                           static_cast<float>(A.z), pf[u].x, pf[u].y, pf[u].z)));
+                                e_old += static_cast<double>(pair_energy_f(
+                                    p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(B.w), pf[u].w,
+                                    dist2f(fL, fh, static_cast<float>(B.x), static_cast<float>(B.y),
+                                           static_cast<float>(B.z), pf[u].x, pf[u].y, pf[u].z)));
+                            } else {
+                                const DevPair& p = s_pairs[ta * nt + tj];
+                                e_new += pair_energy(p, s_coul, A.w, pj[u].w, dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
+                                e_old += pair_energy(p, s_coul, B.w, pj[u].w, dist2(box, B.x, B.y, B.z, pj[u].x, pj[u].y, pj[u].z));
+                            }
+                        }
+                    }
+                }
+            } else {
+                // 27-cell stencils around each moved atom's new (st=0) and old (st=1) position
+                for (int cb = tid; cb < m * 54; cb += kThreads) {
+                    const int a = cb / 54, st = (cb / 27) & 1, c = cb % 27;
+                    const double4 A = st == 0 ? s_new[a] : s_old[a];
+                    const int ta = s_mt[a];
+                    const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                    const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                    const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                    const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                    const int cnt = S.cell_count[cell];
+                    for (int kk = 0; kk < cnt; ++kk) {
+                        const int j = S.cell_members[cell * S.cap + kk];
+                        if (j >= P.mlo && j < P.mhi) continue;
+                        const uint32_t mj = meta[j];
+                        if (static_cast<int>(mj >> 8) == g && !same_ok) continue;
+                        const int tj = static_cast<int>(mj & 0xffu);
+                        double u;
+                        if (kF32) {
+                            const float4 pj = pos32[j];
+                            const float fL = static_cast<float>(L), fh = static_cast<float>(half);
+                            u = static_cast<double>(pair_energy_f(
+                                s_pairs_f[ta * nt + tj], s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pj.w,
+                                dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
+                                       pj.x, pj.y, pj.z)));
+                        } else {
+                            const double4 pj = pos[j];
+                            u = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj.w, dist2(box, A.x, A.y, A.z, pj.x, pj.y, pj.z));
+                        }
+                        if (st == 0) e_new += u;
+                        else e_old += u;
+                    }
+                }
+            }
+            // moved x moved inside the group (internal changes of flexible groups)
+            if (same_ok && m > 1) {
+                for (int idx = tid; idx < m * m; idx += kThreads) {
+                    const int a = idx / m, b = idx % m;
+                    if (a >= b) continue;
+                    const DevPair& p = s_pairs[s_mt[a] * nt + s_mt[b]];
+                    const double4 An = s_new[a], Bn = s_new[b], Ao = s_old[a], Bo = s_old[b];
+                    e_new += pair_energy(p, s_coul, An.w, Bn.w, dist2(box, An.x, An.y, An.z, Bn.x, Bn.y, Bn.z));
+                    e_old += pair_energy(p, s_coul, Ao.w, Bo.w, dist2(box, Ao.x, Ao.y, Ao.z, Bo.x, Bo.y, Bo.z));
+                }
+            }
+        }
+
+        // ---- harmonic bonds with a moved atom (Bonded::internalGroupEnergy, energy.cpp:1068-1088)
+        double b_new = 0.0, b_old = 0.0;
+        if (P.internal && S.bond_off != nullptr) {
+            for (int a = tid; a < m; a += kThreads) {
+                const int ia = P.mlo + a;
+                for (int e = S.bond_off[ia]; e < S.bond_off[ia + 1]; ++e) {
+                    const DevBond B = S.bonds[e];
+                    const bool om = B.other >= P.mlo && B.other < P.mhi;
+                    if (om && B.other < ia) continue; // visited from the lower moved atom
+                    const double4 on = om ? s_new[B.other - P.mlo] : pos[B.other];
+                    const double4 oo = om ? s_old[B.other - P.mlo] : on;
+                    const double4 An = s_new[a], Ao = s_old[a];
+                    const double xn = B.req - sqrt(dist2(box, An.x, An.y, An.z, on.x, on.y, on.z));
+                    const double xo = B.req - sqrt(dist2(box, Ao.x, Ao.y, Ao.z, oo.x, oo.y, oo.z));
+                    b_new += B.half_k * xn * xn;
+                    b_old += B.half_k * xo * xo;
+                }
+            }
+        }
+
+        // ---- Ewald: dQ(k) = sum_a q_a (e^{ik.r_new} - e^{ik.r_old}) over chunks of moved atoms
+        double dqr[kMaxKPT], dqi[kMaxKPT];
+#pragma unroll
+        for (int s = 0; s < kMaxKPT; ++s) dqr[s] = dqi[s] = 0.0;
+        for (int cst = 0; cst < m; cst += kChunk) {
+            const int cm_ = min(kChunk, m - cst);
+            if (cst > 0) __syncthreads(); // previous chunk's tables consumed
+            for (int e = tid; e < cm_ * 6 * width; e += kThreads) {
+                const int nn = e % width - ncut;
+                const int axis = (e / width) % 3;
+                const int st = (e / (3 * width)) & 1;
+                const int a = e / (6 * width);
+                const double4 A = st == 0 ? s_new[cst + a] : s_old[cst + a];
+                const double x = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
+                double sn, cs;
+                sincos(6.283185307179586 * (static_cast<double>(nn) / L) * x, &sn, &cs);
+                s_phase[e] = make_double2(cs, sn);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < kMaxKPT; ++s) {
+                const int k = k0 + s;
+                if (s < kpt && k < nk) {
+                    const int packed = s_kn[k];
+                    const int nx = unpack_n(packed, 0) + ncut, ny = unpack_n(packed, 1) + ncut,
+                              nz = unpack_n(packed, 2) + ncut;
+                    for (int a = 0; a < cm_; ++a) {
+                        const double2* tb = s_phase + a * 6 * width;
+                        const double2 xn = tb[nx], yn = tb[width + ny], zn = tb[2 * width + nz];
+                        const double2 xo = tb[3 * width + nx], yo = tb[4 * width + ny], zo = tb[5 * width + nz];
+                        const double rn = xn.x * yn.x - xn.y * yn.y, in_ = xn.x * yn.y + xn.y * yn.x;
+                        const double ro = xo.x * yo.x - xo.y * yo.y, io = xo.x * yo.y + xo.y * yo.x;
+                        const double qa = s_new[cst + a].w;
+                        dqr[s] += qa * ((rn * zn.x - in_ * zn.y) - (ro * zo.x - io * zo.y));
+                        dqi[s] += qa * ((rn * zn.y + in_ * zn.x) - (ro * zo.y + io * zo.x));
+                    }
+                }
+            }
+        }
+        double de = 0.0;
+#pragma unroll
+        for (int s = 0; s < kMaxKPT; ++s) {
+            const int k = k0 + s;
+            if (s < kpt && k < nk) {
+                const int packed = s_kn[k];
+                const int nx = unpack_n(packed, 0), ny = unpack_n(packed, 1), nz = unpack_n(packed, 2);
+                const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[nx * nx + ny * ny + nz * nz];
+                de += A * (2.0 * (q[s].x * dqr[s] + q[s].y * dqi[s]) + (dqr[s] * dqr[s] + dqi[s] * dqi[s]));
+            }
+        }
+
+        // ---- reduction (fixed order) and the decision on lane 0
+        {
+            double v[5] = {e_new, e_old, de, b_new, b_old};
+#pragma unroll
+            for (int r = 0; r < 5; ++r)
+                for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+            if ((tid & 63) == 0)
+#pragma unroll
+                for (int r = 0; r < 5; ++r) s_red[r][tid >> 6] = v[r];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double tot[5] = {0, 0, 0, 0, 0};
+            for (int w = 0; w < kThreads / 64; ++w)
+                for (int r = 0; r < 5; ++r) tot[r] += s_red[r][w];
+            double du = 0.0;
+            bool acc = true;
+            if (m > 0) {
+                du = energy_change(tot[0] + tot[3], tot[1] + tot[4], pref * tot[2]);
+                acc = metropolis(beta * du, P.accept_u);
+            }
+            s_accept = acc ? 1 : 0;
+            if (trace && t < trace_cap) {
+                TraceEntry& T = trace[static_cast<size_t>(blockIdx.x) * trace_cap + t];
+                T.index = m > 0 ? P.mlo : -1;
+                T.accepted = acc ? 1 : 0;
+                T.du = du;
+                T.x = m > 0 ? s_new[0].x : 0.0;
+                T.y = m > 0 ? s_new[0].y : 0.0;
+                T.z = m > 0 ? s_new[0].z : 0.0;
+            }
+            ++accepted;
+            if (!acc) --accepted;
+            if (acc && m > 0) {
+                energy += du;
+                sum_du += du;
+                if (P.kind == 1 && P.translate)
+                    for (int d = 0; d < 3; ++d) S.gcm[3 * g + d] = P.cm_new[d];
+                if (kCells) {
+                    for (int a = 0; a < m && !S.error; ++a) {
+                        const int i = P.mlo + a;
+                        const double4 A = s_new[a];
+                        const int cnew = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) * S.nc1 +
+                                          cell_of_coord(A.y, half, S.inv_cell, S.nc1)) * S.nc0 +
+                                         cell_of_coord(A.x, half, S.inv_cell, S.nc0);
+                        const int cold = S.cell_of[i];
+                        if (cnew == cold) continue;
+                        const int nold = S.cell_count[cold];
+                        for (int kk = 0; kk < nold; ++kk)
+                            if (S.cell_members[cold * S.cap + kk] == i) {
+                                S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                break;
+                            }
+                        S.cell_count[cold] = nold - 1;
+                        const int nn = S.cell_count[cnew];
+                        if (nn >= S.cap) {
+                            S.error = 1; // cell capacity exceeded: the run stops with an error
+                        } else {
+                            S.cell_members[cnew * S.cap + nn] = i;
+                            S.cell_count[cnew] = nn + 1;
+                            S.cell_of[i] = cnew;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (s_accept && m > 0) {
+            for (int a = tid; a < m; a += kThreads) {
+                const double4 A = s_new[a];
+                pos[P.mlo + a] = A;
+                if (kF32)
+                    pos32[P.mlo + a] = make_float4(static_cast<float>(A.x), static_cast<float>(A.y),
+                                                   static_cast<float>(A.z), static_cast<float>(A.w));
+            }
+#pragma unroll
+            for (int s = 0; s < kMaxKPT; ++s) {
+                q[s].x += dqr[s];
+                q[s].y += dqi[s];
+            }
+        }
+        if (S.error) break;
+        __syncthreads(); // moved coordinates and tables consumed before the next move
+    }
+
+#pragma unroll
+    for (int s = 0; s < kMaxKPT; ++s)
+        if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
+    if (tid == 0) {
+        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.accepted = accepted;
+        S.energy = energy;
+        S.sum_du = sum_du;
+    }
+}
+
+template <int kMaxKPT, bool kCells, bool kF32>
+void launch_one(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap) {
+    auto fn = mc_general_kernel<256, kMaxKPT, kCells, kF32>;
+    static bool attr = false;
+    if (!attr) {
+        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      120 * 1024),
+                  "hipFuncSetAttribute");
+        attr = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, s, m, tr, cap);
+}
+
+template <int kMaxKPT>
+void dispatch(bool cells, bool f32, int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m,
+              TraceEntry* tr, int cap) {
+    if (cells) {
+        if (f32) launch_one<kMaxKPT, true, true>(grid, lds, st, s, m, tr, cap);
+        else launch_one<kMaxKPT, true, false>(grid, lds, st, s, m, tr, cap);
+    } else {
+        if (f32) launch_one<kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap);
+        else launch_one<kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap);
+    }
+}
+
+} // namespace
+
+void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
+                    TraceEntry* trace, int trace_cap, hipStream_t stream) {
+    int maxkpt = 0, maxnk = 0, maxw = 0, maxm = 0;
+    for (int r = 0; r < count; ++r) {
+        const ReplicaState& s = h_states[r];
+        if (s.nkinds < 1 || s.nkinds > kMaxKinds) throw std::runtime_error("general move set: 1 to 8 move kinds");
+        if (s.cells != h_states[0].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
+        if (s.ntypes > kMaxTypes) throw std::runtime_error("too many atom types");
+        maxkpt = std::max(maxkpt, s.kpt);
+        maxnk = std::max(maxnk, s.nk);
+        maxw = std::max(maxw, 2 * s.ncut + 1);
+        maxm = std::max(maxm, s.max_moved);
+    }
+    if (maxkpt > 16) throw std::runtime_error("too many k-vectors for the general device MC (max 16 per thread)");
+    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) +
+                       sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
+                       (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
+    if (lds > 120 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+    // each workgroup lays out its own replica's tables inside this allocation
+    const bool cells = h_states[0].cells != 0;
+    if (maxkpt <= 8) dispatch<8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
+    else dispatch<16>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
+    check_hip(hipGetLastError(), "mc_general_kernel launch");
+}
+
+} // namespace fmc

@@ -327,6 +327,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
 void Batch::run_split(long long moves, int groups, int stride, bool trace, int trace_cap) {
     if (!finalized_) finalize();
     if (size() != 1) throw std::runtime_error("run_split advances a batch of one replica");
+    if (general_ == 1) throw std::runtime_error("run_split handles translate-only replicas");
     const ReplicaState& s = h_states_[0];
     if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error("run_split: 1..64 workgroups");
     if (stride < 1) throw std::runtime_error("run_split: stride >= 1");

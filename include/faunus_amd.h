@@ -169,6 +169,25 @@ int fmc_batch_add_replica(fmc_batch* batch, const char* config_json, int n, cons
                           const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
                           double beta, unsigned long long seed, int ladder, int* replica);
 /* Advance replica r by moves[r] trial moves; trace_capacity > 0 records that many moves per replica. */
+/*
+ * Replica with a general move set: num_kinds kinds (at most 8), each one of
+ *   0 translate one atom of molecule type kind_molid (AtomicTranslateRotate,
+ *     src/move.cpp:225-345; the atom's dp, else dp[k]),
+ *   1 translate by dp[k] and rotate by dprot[k] a molecule of that type
+ *     (TranslateRotate, src/move.cpp:1618-1689),
+ *   2 pivot a chain of that type about a random harmonic bond by dprot[k]
+ *     (PivotMove, src/chainmove.cpp:216-268),
+ * picked with probability weight[k] / sum(weight) as MoveCollection::sample does
+ * (src/move.cpp:513-525).  Proposals come from the replica's Philox stream (counters
+ * (move, replica, 0|1|2)); the Hamiltonian may hold a bonded term.  A batch holds
+ * either replicas added this way or translate-only replicas; cutoff_g2g must be unset
+ * and a molecule type moved by kind 1 takes no kind 0 or 2 moves.
+ */
+int fmc_batch_add_replica_moves(fmc_batch* batch, const char* config_json, int n, const double* xyz,
+                                const double* charge, const int* atom_id, int num_groups, const int* begin,
+                                const int* size, const int* molid, double beta, unsigned long long seed, int ladder,
+                                int num_kinds, const int* kind, const int* kind_molid, const unsigned* weight,
+                                const double* dp, const double* dprot, int* replica);
 int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
 /*
  * Advance a batch of ONE replica by `moves` moves with `groups` cooperating

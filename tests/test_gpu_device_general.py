@@ -1,0 +1,96 @@
+"""Device-resident Metropolis with general move sets (atom translations inside
+molecules, rigid translate+rotate, pivot) against the oracle running the same Philox
+proposal stream: FP64 du within 1e-10 relative to max(|du|, E_scale) with identical
+decisions, final positions, running energy against a full recompute and Q(k) against
+a full rebuild; FP32 pair terms within 1e-6 up to the first near-tie."""
+import numpy as np
+import pytest
+
+import faunus_amd as F
+from oracle import oracle as O
+from tests import systems
+from tests.gpu_helpers import log
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_of(cfg, xyz, q, ids, g):
+    s = O.System(cfg)
+    s.set_particles(xyz, q, ids)
+    s.set_groups(g["begin"], g["size"], g["molid"])
+    return s, s.init()
+
+
+def compare(tr, rec, e_scale):
+    n = len(rec["du"])
+    mism = np.nonzero(tr["accepted"] != rec["accepted"])[0]
+    first = int(mism[0]) if len(mism) else n
+    fin = np.isfinite(rec["du"][:first])
+    assert np.array_equal(np.isinf(tr["du"][:first]), np.isinf(rec["du"][:first]))
+    err = np.abs(tr["du"][:first][fin] - rec["du"][:first][fin]) / np.maximum(np.abs(rec["du"][:first][fin]), e_scale)
+    return dict(first_mismatch=first, n=n, max_rel=float(err.max()) if len(err) else 0.0,
+                accepted=int(rec["accepted"][:first].sum()))
+
+
+def run(sysd, kinds, nm, seed, fp32=False):
+    cfg, xyz, q, ids, g = sysd
+    orc, e0 = oracle_of(cfg, xyz, q, ids, g)
+    b = F.Batch(fp32_pairs=fp32)
+    b.add(sysd, seed=seed, moves=kinds)
+    b.run([nm], trace=nm)
+    b.sync()
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_moves(nm, seed=seed, kinds=kinds, replica=0)
+    return b, orc, e0, compare(tr, rec, max(1.0, abs(e0)))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_polyelectrolyte_general_moves(fp32):
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 400, seed=7, kmax=6)
+    b, orc, e0, rep = run((cfg, xyz, q, ids, g), kinds, 3000, seed=33, fp32=fp32)
+    log("device general polyelectrolyte fp32=%s" % fp32, rep)
+    info = b.info(0)
+    assert info["moves"] == 3000 and info["error"] == 0
+    tot, _ = b.total_energy(0)
+    if not fp32:
+        assert rep["first_mismatch"] == rep["n"] and rep["max_rel"] <= 1e-10, rep
+        np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+        assert abs(info["energy"] - tot) <= 1e-9 * max(1.0, abs(tot))
+    else:
+        assert rep["max_rel"] <= 1e-6 and rep["first_mismatch"] > 500, rep
+        assert abs(info["energy"] - tot) <= 1e-6 * max(1.0, abs(tot))
+    dq, _ = b.check_q(0)
+    assert dq < 1e-9
+
+
+@pytest.mark.parametrize("n_mol,cells", [(216, False), (1000, True)])
+def test_water_general_moves(n_mol, cells):
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=3, kmax=6, lj_cutoff=10.0 if cells else None,
+                                                cutoff=10.0 if cells else 9.0)
+    b, orc, e0, rep = run((cfg, xyz, q, ids, g), kinds, 1500, seed=8)
+    log("device general water", n_mol, rep)
+    assert rep["first_mismatch"] == rep["n"] and rep["max_rel"] <= 1e-10, rep
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+
+
+def test_batch_of_uneven_polyelectrolytes_and_ladder():
+    """Replicas of different chain length and size in one launch, two of them a
+    temperature ladder; each follows its own oracle stream."""
+    specs = [(30, 300, 5), (50, 500, 6), (40, 400, 7)]
+    b = F.Batch()
+    sysds = []
+    for r, (nb, nt, seed) in enumerate(specs):
+        cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(nb, nt, seed=seed, kmax=5)
+        sysds.append(((cfg, xyz, q, ids, g), kinds))
+        b.add((cfg, xyz, q, ids, g), seed=500 + r, moves=kinds)
+    nm = 1200
+    b.run([nm] * len(specs), trace=nm)
+    b.sync()
+    for r, (sysd, kinds) in enumerate(sysds):
+        orc, e0 = oracle_of(*sysd)
+        rec = orc.run_philox_moves(nm, seed=500 + r, kinds=kinds, replica=r)
+        rep = compare(b.trace(r, nm), rec, max(1.0, abs(e0)))
+        log("general batch replica", r, rep)
+        assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10, rep
