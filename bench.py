@@ -240,6 +240,85 @@ def cpu_baseline(seconds=15.0):
 HEADLINE_GROUPS = 16
 
 
+def bench_water(args, fp32=True, replicas=64, n_mol=4096, moves_per_step=3000):
+    """64 rigid-water replicas (4096 molecules, L=49.68 A, Ewald alpha=0.28, Rc=10,
+    kmax=9, LJ on O truncated at 10 A, cell list) on a 280-360 K ladder with molecular
+    translate+rotate moves (dp 0.5 A, dprot 0.5 rad), one persistent launch per step
+    and one round of neighbour swaps after it."""
+    import faunus_amd as F
+    from tests import systems
+    comps = [systems.water(n_mol=n_mol, seed=100 + c) for c in range(4)]
+    temps = np.linspace(280.0, 360.0, replicas)
+    b = F.Batch(fp32_pairs=fp32)
+    for r in range(replicas):
+        cfg, xyz, q, ids, g, kinds = comps[r % len(comps)]
+        b.add((cfg, xyz, q, ids, g), beta=298.15 / temps[r], seed=9000 + r, ladder=0, moves=kinds)
+    b.set_timing(True)
+    b.run([moves_per_step] * replicas)
+    b.sync()
+    kernel_ms = 0.0
+    swaps = (0, 0)
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        b.run([moves_per_step] * replicas)
+        kernel_ms += b.sync()
+        swaps = b.swap(seed=77, rnd=step)
+    dt = time.perf_counter() - t0
+    total = replicas * moves_per_step * args.steps
+    drift = []
+    for r in (0, replicas - 1):
+        tot, _ = b.total_energy(r)
+        drift.append(abs(b.info(r)["energy"] - tot) / abs(tot))
+    acc = np.mean([b.info(r)["accepted"] / b.info(r)["moves"] for r in range(replicas)])
+    b.close()
+    return dict(rate=total / dt, kernel_ms=kernel_ms, seconds=dt, moves=total, swaps=swaps,
+                max_drift=float(max(drift)), acceptance=float(acc))
+
+
+def bench_plain_water(n_mol=4096, nmoves=400, seed=5):
+    """Plain FP64 path on the same water system: one fused launch and one host round
+    trip per molecular move (fmc_replay_moves).  Proposals (translate dp 0.5 A and
+    rotate 0.5 rad about the mass centre) are drawn from the initial configuration."""
+    import faunus_amd as F
+    from tests import systems
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=100)
+    e = F.Engine(cfg)
+    e.set_particles(xyz, q, ids)
+    e.set_groups(g["begin"], g["size"], g["molid"])
+    e.init()
+    L = cfg["geometry"]["length"]
+    rng = np.random.default_rng(seed)
+    grp = rng.integers(0, n_mol, nmoves).astype(np.int32)
+    idx = np.concatenate([3 * w + np.arange(3) for w in grp]).astype(np.int32)
+    new = np.zeros((3 * nmoves, 3))
+    cms = np.zeros((nmoves, 3))
+    for t, w in enumerate(grp):
+        cm = e.mass_center(int(w))
+        d = rng.normal(size=3)
+        d *= 0.5 * rng.random() / np.linalg.norm(d)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        ang = 0.5 * (rng.random() - 0.5)
+        K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+        p = xyz[3 * w:3 * w + 3] - cm
+        p -= L * np.round(p / L)
+        p = p @ R.T + cm + d
+        new[3 * t:3 * t + 3] = p - L * np.round(p / L)
+        c2 = cm + d
+        cms[t] = c2 - L * np.round(c2 / L)
+    off = np.arange(0, 3 * nmoves + 1, 3, dtype=np.int64)
+    u = rng.random(nmoves)
+    e.replay_moves(grp[:20], off[:21], idx[:60], new[:60], u[:20], whole=np.ones(20, np.uint8), mass_center=cms[:20])
+    e.set_timing(True)
+    t0 = time.perf_counter()
+    e.replay_moves(grp[20:], off[20:] - 60, idx[60:], new[60:], u[20:], whole=np.ones(nmoves - 20, np.uint8),
+                   mass_center=cms[20:])
+    dt = time.perf_counter() - t0
+    ms, launches = e.timing()
+    return dict(rate=(nmoves - 20) / dt, kernel_us=ms * 1e3 / max(1, launches))
+
+
 def serve_stream(n_jobs, seed=1, rate=50.0):
     """Poisson arrivals (rate jobs/s) of polyelectrolyte jobs: chain length ~ U[50, 500],
     total N ~ logU[1k, 50k]; the job id seeds the configuration and the Philox stream."""
@@ -254,13 +333,16 @@ def serve_stream(n_jobs, seed=1, rate=50.0):
 
 
 def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
-    """Continuous batching under a Poisson stream: submit each job at its arrival time,
-    run scheduling rounds while anything is queued or resident, and report the
-    per-job time from arrival to completion of `sweeps` sweeps (N moves per sweep)."""
+    """Continuous batching under a Poisson stream of polyelectrolyte jobs (chain of
+    50-500 beads with harmonic bonds, counterions and 1:1 salt, N log-uniform in
+    1k-50k, 5 mM monomer, hard spheres + Ewald alpha=5/L, kmax=8; ion and bead
+    translations and pivots): submit each job at its arrival time, run scheduling
+    rounds while anything is queued or resident, and report the per-job time from
+    arrival to completion of `sweeps` sweeps (N moves per sweep)."""
     import faunus_amd as F
     from tests import systems
     stream = serve_stream(n_jobs, seed)
-    jobs = [systems.polyelectrolyte_job(nb, nt, seed=1000 + j) for j, (_, nb, nt) in enumerate(stream)]
+    jobs = [systems.polyelectrolyte(nb, nt, seed=1000 + j) for j, (_, nb, nt) in enumerate(stream)]
     srv = F.Server(fp32_pairs=fp32, hbm_budget_bytes=200 << 30, max_slots=512)
     t0 = srv.status()["seconds"]
     ids = {}
@@ -270,8 +352,8 @@ def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
     while True:
         now = srv.status()["seconds"] - t0
         while nxt < n_jobs and stream[nxt][0] <= now:
-            sysd = jobs[nxt]
-            ids[nxt] = srv.submit(sysd, sweeps * len(sysd[2]), seed=1000 + nxt)
+            cfg, xyz, q, idl, g, kinds = jobs[nxt]
+            ids[nxt] = srv.submit((cfg, xyz, q, idl, g), sweeps * len(q), seed=1000 + nxt, kinds=kinds)
             nxt += 1
         r, q = srv.step(quantum)
         rounds += 1
@@ -309,8 +391,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
-    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve"],
-                    default=None)
+    ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve",
+                                       "water"], default=None)
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-sweeps", type=int, default=4)
     args = ap.parse_args()
@@ -366,6 +448,15 @@ def main():
         log("batched FP64: %.3e replica-moves/s" % out["batch64"]["rate"])
         out["plain_mix"] = bench_plain_mix(comps)
         log("plain FP64 one move per launch on the mix: %.0f replica-moves/s" % out["plain_mix"])
+    if args.only in (None, "water"):
+        out["water32"] = bench_water(args, fp32=True)
+        log("64 water replicas, FP32 pairs: %.3e replica-moves/s (acceptance %.2f)" % (
+            out["water32"]["rate"], out["water32"]["acceptance"]))
+        out["water64"] = bench_water(args, fp32=False)
+        log("64 water replicas, FP64: %.3e replica-moves/s" % out["water64"]["rate"])
+        out["water_plain"] = bench_plain_water()
+        log("plain FP64 per-move path on water: %.0f moves/s (kernel %.1f us)" % (
+            out["water_plain"]["rate"], out["water_plain"]["kernel_us"]))
     if args.only in (None, "serve"):
         out["serve"] = bench_serve(n_jobs=args.serve_jobs, sweeps=args.serve_sweeps)
         log("serving: p50 %.2f s, p99 %.2f s to %d sweeps, %.3e moves/s" % (
@@ -423,6 +514,14 @@ def main():
             "swaps_attempted_accepted": out["batch32"]["swaps"],
             "sweeps_per_composition": out["batch32"]["sweeps_per_composition"],
             "max_energy_drift": out["batch32"]["max_drift"]}
+    if "water32" in out:
+        line["forms"]["batched_64_water_replicas"] = {
+            "fp32_pairs_fp64_q_replica_moves_per_s": out["water32"]["rate"],
+            "fp64_replica_moves_per_s": out["water64"]["rate"],
+            "plain_fp64_moves_per_s": out["water_plain"]["rate"],
+            "speedup_fp32_batched_vs_plain": out["water32"]["rate"] / out["water_plain"]["rate"],
+            "acceptance": out["water32"]["acceptance"], "swaps_attempted_accepted": out["water32"]["swaps"],
+            "max_energy_drift": out["water32"]["max_drift"]}
     if "serve" in out:
         line["forms"]["serving_poisson_50_per_s"] = out["serve"]
     print(json.dumps(line), flush=True)

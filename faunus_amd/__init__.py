@@ -43,6 +43,7 @@ SIGNATURES = {
     "fmc_check_q": (C.c_int, [C.c_void_p, D, D]),
     "fmc_get_positions": (C.c_int, [C.c_void_p, D]),
     "fmc_replay_translate": (C.c_int, [C.c_void_p, C.c_long, I, I, D, D, D, U8, D, D]),
+    "fmc_replay_moves": (C.c_int, [C.c_void_p, C.c_long, I, C.POINTER(C.c_long), I, D, D, U8, D, D, U8]),
     "fmc_run_translate": (C.c_int, [C.c_void_p, C.c_long, I, I, D, D, D, U8]),
     "fmc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "fmc_get_timing": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_long)]),
@@ -73,6 +74,8 @@ SIGNATURES = {
     "fmc_server_destroy": (C.c_int, [C.c_void_p]),
     "fmc_server_submit": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_longlong,
                                     C.c_ulonglong, I]),
+    "fmc_server_submit_moves": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_longlong,
+                                          C.c_ulonglong, C.c_int, I, I, C.POINTER(C.c_uint), D, D, I]),
     "fmc_server_step": (C.c_int, [C.c_void_p, C.c_longlong, I, I]),
     "fmc_server_job": (C.c_int, [C.c_void_p, C.c_int, I, D, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong),
                                  D, D, D]),
@@ -224,6 +227,25 @@ class Engine:
         index = np.ascontiguousarray(np.atleast_1d(index), np.int32)
         xyz = np.ascontiguousarray(np.atleast_2d(xyz), np.float64)
         _check(lib().fmc_stage_trial(self.h, len(index), _p(index, C.c_int), _p(xyz)))
+
+    def replay_moves(self, group, offsets, index, xyz, uniform, whole=None, mass_center=None):
+        """Metropolis replay of multi-atom moves (see fmc_replay_moves); returns du, accepted."""
+        n = len(uniform)
+        group = np.ascontiguousarray(group, np.int32)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        index = np.ascontiguousarray(index, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        uniform = np.ascontiguousarray(uniform, np.float64)
+        if len(group) != n or len(offsets) != n + 1 or len(index) < offsets[-1] or xyz.size < 3 * offsets[-1]:
+            raise ValueError("replay_moves: inconsistent array lengths")
+        wh = None if whole is None else np.ascontiguousarray(whole, np.uint8)
+        cm = None if mass_center is None else np.ascontiguousarray(mass_center, np.float64)
+        du = np.zeros(n)
+        acc = np.zeros(n, np.uint8)
+        _check(lib().fmc_replay_moves(self.h, n, _p(group, C.c_int), _p(offsets, C.c_long), _p(index, C.c_int),
+                                      _p(xyz), _p(cm), _p(wh, C.c_ubyte), _p(uniform), _p(du),
+                                      _p(acc, C.c_ubyte)))
+        return du, acc
 
     def stage_mass_center(self, group, cm):
         cm = np.ascontiguousarray(cm, np.float64)
@@ -450,15 +472,28 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, system, moves, seed):
+    def submit(self, system, moves, seed, kinds=None):
+        """Queue a job of ``moves`` trial moves; ``kinds`` (list of dicts with kind,
+        molid, weight, dp, dprot) selects the general move set."""
         cfg, xyz, q, ids, g = system
         text = cfg if isinstance(cfg, str) else json.dumps(cfg)
         xyz, q, ids = _particles(xyz, q, ids)
         gb, gs, gm, _ = _groups(g["begin"], g["size"], g["molid"])
         j = C.c_int()
-        _check(lib().fmc_server_submit(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
-                                       _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), int(moves), int(seed),
-                                       C.byref(j)))
+        if kinds is None:
+            _check(lib().fmc_server_submit(self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb),
+                                           _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), int(moves), int(seed),
+                                           C.byref(j)))
+        else:
+            kind = np.array([k["kind"] for k in kinds], np.int32)
+            molid = np.array([k["molid"] for k in kinds], np.int32)
+            weight = np.array([k.get("weight", 1) for k in kinds], np.uint32)
+            dp = np.array([float(k.get("dp", 0.0)) for k in kinds])
+            dprot = np.array([float(k.get("dprot", 0.0)) for k in kinds])
+            _check(lib().fmc_server_submit_moves(
+                self.h, text.encode(), len(q), _p(xyz), _p(q), _p(ids, C.c_int), len(gb), _p(gb, C.c_int),
+                _p(gs, C.c_int), _p(gm, C.c_int), int(moves), int(seed), len(kind), _p(kind, C.c_int),
+                _p(molid, C.c_int), _p(weight, C.c_uint), _p(dp), _p(dprot), C.byref(j)))
         self.sizes[j.value] = len(q)
         return j.value
 

@@ -241,6 +241,50 @@ int fmc_replay_translate(fmc_engine* e, long count, const int* group, const int*
     });
 }
 
+int fmc_replay_moves(fmc_engine* e, long count, const int* group, const long* offsets, const int* index,
+                     const double* xyz, const double* mass_center, const unsigned char* whole, const double* uniform,
+                     double* du_out, unsigned char* accepted) {
+    return guarded([&] {
+        fmc::Engine& E = eng(e);
+        const int nt = E.num_terms();
+        std::vector<double> en(static_cast<size_t>(nt)), eo(static_cast<size_t>(nt));
+        const auto& gb = E.group_begin();
+        const auto& gsz = E.group_size();
+        std::vector<int> groups(1);
+        std::vector<char> all(1, 0), internal(1, 1);
+        std::vector<std::vector<int>> rel(1);
+        for (long n = 0; n < count; ++n) {
+            const long o0 = offsets[n], o1 = offsets[n + 1];
+            if (o1 < o0) throw std::runtime_error("replay: offsets must not decrease");
+            if (o1 == o0) { // a move that changed nothing
+                if (du_out) du_out[n] = 0.0;
+                if (accepted) accepted[n] = 1;
+                continue;
+            }
+            const int g = group[n];
+            if (g < 0 || g >= static_cast<int>(gb.size())) throw std::runtime_error("replay: group out of range");
+            E.stage_trial(static_cast<int>(o1 - o0), index + o0, xyz + 3 * o0);
+            if (mass_center) E.stage_mass_center(g, mass_center + 3 * n);
+            groups[0] = g;
+            all[0] = whole && whole[n] ? 1 : 0;
+            internal[0] = all[0] ? 0 : 1;
+            rel[0].clear();
+            if (!all[0])
+                for (long k = o0; k < o1; ++k) {
+                    const int r = index[k] - gb[static_cast<size_t>(g)];
+                    if (r < 0 || r >= gsz[static_cast<size_t>(g)]) throw std::runtime_error("replay: atom outside its group");
+                    rel[0].push_back(r);
+                }
+            E.delta(groups, all, internal, rel, en.data(), eo.data());
+            const double du = energy_change(sum_terms(en.data(), nt), sum_terms(eo.data(), nt));
+            const bool acc = metropolis(du, uniform[n]);
+            E.sync(acc);
+            if (du_out) du_out[n] = du;
+            if (accepted) accepted[n] = acc ? 1 : 0;
+        }
+    });
+}
+
 int fmc_run_translate(fmc_engine* e, long count, const int* group, const int* index, const double* displacement,
                       const double* uniform, double* du_out, unsigned char* accepted) {
     return guarded([&] {
@@ -506,6 +550,22 @@ int fmc_server_submit(fmc_server* s, const char* config_json, int n, const doubl
                       long long moves, unsigned long long seed, int* job) {
     return guarded([&] {
         const int id = srv(s).submit(config_json, n, xyz, charge, atom_id, num_groups, begin, size, molid, moves, seed);
+        if (job) *job = id;
+    });
+}
+
+int fmc_server_submit_moves(fmc_server* s, const char* config_json, int n, const double* xyz, const double* charge,
+                            const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
+                            long long moves, unsigned long long seed, int num_kinds, const int* kind,
+                            const int* kind_molid, const unsigned* weight, const double* dp, const double* dprot,
+                            int* job) {
+    return guarded([&] {
+        if (num_kinds < 1 || !kind || !kind_molid || !weight || !dp || !dprot) throw std::runtime_error("empty move set");
+        std::vector<fmc::MoveKindSpec> kinds(static_cast<size_t>(num_kinds));
+        for (int k = 0; k < num_kinds; ++k)
+            kinds[static_cast<size_t>(k)] = fmc::MoveKindSpec{kind[k], kind_molid[k], weight[k], dp[k], dprot[k]};
+        const int id = srv(s).submit(config_json, n, xyz, charge, atom_id, num_groups, begin, size, molid, moves, seed,
+                                     &kinds);
         if (job) *job = id;
     });
 }

@@ -49,7 +49,7 @@ size_t Server::estimate_footprint(const Model& m, int n) {
 
 int Server::submit(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                    int ngroups, const int* gbegin, const int* gsize, const int* gmolid, long long moves,
-                   unsigned long long seed) {
+                   unsigned long long seed, const std::vector<MoveKindSpec>* kinds) {
     if (n <= 0 || moves < 0) throw std::runtime_error("server: bad job");
     const Model m = Model::from_json(config_json); // validates the input before queueing
     auto j = std::make_unique<JobRecord>();
@@ -66,6 +66,10 @@ int Server::submit(const std::string& config_json, int n, const double* xyz, con
     j->gbegin.assign(gbegin, gbegin + ngroups);
     j->gsize.assign(gsize, gsize + ngroups);
     j->gmolid.assign(gmolid, gmolid + ngroups);
+    if (kinds) {
+        if (kinds->empty()) throw std::runtime_error("server: empty move set");
+        j->kinds = *kinds;
+    }
     const int id = j->id;
     jobs_.push_back(std::move(j));
     queue_.push_back(id);
@@ -86,9 +90,11 @@ void Server::admit() {
                                  nullptr);
             s.engine->init();
             // the stream is keyed by the job seed alone, so a job runs the same chain in any mix
-            s.state = make_replica_state(*s.engine, fp32_ != 0, j.seed, 0, 1.0, -1, s.owned);
+            s.general = !j.kinds.empty();
+            s.state = make_replica_state(*s.engine, fp32_ != 0, j.seed, 0, 1.0, -1, s.owned,
+                                         s.general ? &j.kinds : nullptr);
             s.state.kpt = (s.state.nk + 255) / 256;
-            if (s.state.kpt > 68) throw std::runtime_error("too many k-vectors for the device MC");
+            if (s.state.kpt > (s.general ? 16 : 68)) throw std::runtime_error("too many k-vectors for the device MC");
         } catch (const std::exception& e) {
             for (void* p : s.owned) (void)hipFree(p);
             j.state = 3;
@@ -136,13 +142,15 @@ void Server::step(long long quantum) {
         check_hip(hipMalloc(&d_states_, sizeof(ReplicaState) * capacity_), "hipMalloc states");
         check_hip(hipMalloc(&d_moves_, sizeof(long long) * capacity_), "hipMalloc moves");
     }
-    // jobs with and without a cell list run in separate launches
+    // jobs with and without a cell list, and with and without a general move set, run
+    // in separate launches
     std::vector<ReplicaState> states(static_cast<size_t>(count));
     std::vector<long long> moves(static_cast<size_t>(count));
     std::vector<int> order;
-    for (int pass = 0; pass < 2; ++pass)
+    auto launch_class = [&](const Slot& s) { return (s.state.cells != 0 ? 1 : 0) + (s.general ? 2 : 0); };
+    for (int pass = 0; pass < 4; ++pass)
         for (int k = 0; k < count; ++k)
-            if ((active_[static_cast<size_t>(k)].state.cells != 0) == (pass == 1)) order.push_back(k);
+            if (launch_class(active_[static_cast<size_t>(k)]) == pass) order.push_back(k);
     for (int r = 0; r < count; ++r) {
         const Slot& s = active_[static_cast<size_t>(order[static_cast<size_t>(r)])];
         const JobRecord& j = *jobs_[static_cast<size_t>(s.job)];
@@ -157,10 +165,15 @@ void Server::step(long long quantum) {
               "copy moves");
     int first = 0;
     while (first < count) {
+        const int cls = launch_class(active_[static_cast<size_t>(order[static_cast<size_t>(first)])]);
         int last = first;
-        while (last < count && states[static_cast<size_t>(last)].cells == states[static_cast<size_t>(first)].cells) ++last;
-        launch_advance(states.data() + first, last - first, d_states_ + first, d_moves_ + first, fp32_ != 0, nullptr, 0,
-                       stream_);
+        while (last < count && launch_class(active_[static_cast<size_t>(order[static_cast<size_t>(last)])]) == cls) ++last;
+        if (cls >= 2)
+            launch_general(states.data() + first, last - first, d_states_ + first, d_moves_ + first, fp32_ != 0,
+                           nullptr, 0, stream_);
+        else
+            launch_advance(states.data() + first, last - first, d_states_ + first, d_moves_ + first, fp32_ != 0,
+                           nullptr, 0, stream_);
         first = last;
     }
     check_hip(hipMemcpyAsync(states.data(), d_states_, sizeof(ReplicaState) * count, hipMemcpyDeviceToHost, stream_),

@@ -27,6 +27,7 @@ struct JobRecord {
     std::string config;
     std::vector<double> xyz, charge;
     std::vector<int> ids, gbegin, gsize, gmolid;
+    std::vector<MoveKindSpec> kinds; // empty: single-particle translations
     // results
     double energy = 0;
     unsigned long long moves_done = 0, accepted = 0;
@@ -40,7 +41,7 @@ class Server {
     ~Server();
     int submit(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                int ngroups, const int* gbegin, const int* gsize, const int* gmolid, long long moves,
-               unsigned long long seed);
+               unsigned long long seed, const std::vector<MoveKindSpec>* kinds = nullptr);
     // One scheduling round: admit, advance every resident job by up to `quantum` moves
     // (scaled by per-move cost so rounds take similar time per job), retire finished jobs.
     void step(long long quantum);
@@ -54,6 +55,7 @@ class Server {
   private:
     struct Slot {
         int job;
+        bool general;
         std::unique_ptr<Engine> engine;
         ReplicaState state;
         std::vector<void*> owned;

@@ -137,6 +137,20 @@ int fmc_replay_translate(fmc_engine* engine, long count, const int* group, const
                          double* old_terms);
 
 /*
+ * The same Metropolis step for a stream of multi-atom moves: move n changes the
+ * atoms index[offsets[n] .. offsets[n+1]) (absolute, ascending, all in group[n]) to
+ * the coordinates xyz (3 per atom); whole[n] != 0 marks a rigid-body change of the
+ * whole group (Change all=true, internal=false, as TranslateRotate reports it,
+ * src/move.cpp:1676-1680), otherwise the change is internal with relative indices (as
+ * AtomicTranslateRotate and PivotMove report it, src/move.cpp:311-313,
+ * src/chainmove.cpp:107-121).  mass_center (3 per move, may be NULL) gives the trial
+ * mass centre of the moved group.  An empty move counts as accepted with du = 0.
+ */
+int fmc_replay_moves(fmc_engine* engine, long count, const int* group, const long* offsets, const int* index,
+                     const double* xyz, const double* mass_center, const unsigned char* whole, const double* uniform,
+                     double* du, unsigned char* accepted);
+
+/*
  * Same Metropolis step for translations given as displacements from the current
  * accepted position: new = boundary(old + displacement) as in
  * AtomicTranslateRotate::translateParticle (src/move.cpp:225-240) and
@@ -240,6 +254,12 @@ int fmc_server_destroy(fmc_server* server);
 int fmc_server_submit(fmc_server* server, const char* config_json, int n, const double* xyz, const double* charge,
                       const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
                       long long moves, unsigned long long seed, int* job);
+/* A job with a general move set (same move kinds as fmc_batch_add_replica_moves). */
+int fmc_server_submit_moves(fmc_server* server, const char* config_json, int n, const double* xyz,
+                            const double* charge, const int* atom_id, int num_groups, const int* begin,
+                            const int* size, const int* molid, long long moves, unsigned long long seed,
+                            int num_kinds, const int* kind, const int* kind_molid, const unsigned* weight,
+                            const double* dp, const double* dprot, int* job);
 int fmc_server_step(fmc_server* server, long long quantum, int* resident, int* queued);
 /* Times are seconds since the server was created (host steady clock). */
 int fmc_server_job(fmc_server* server, int job, int* state, double* energy, unsigned long long* moves,

@@ -261,20 +261,17 @@ def polyelectrolyte(n_beads, n_total, seed, kmax=8, dp=10.0, bead_dp=2.0, dprot=
     if a < 4.0:
         raise RuntimeError("too dense")
     sites = rng.permutation(m ** 3)
-    ions = []
-    for site in sites:
-        c = (np.array([site % m, (site // m) % m, site // (m * m)], np.float64) + 0.5) * a - box / 2
-        c += (a - 4.0) / 2 * (2 * rng.random(3) - 1)
-        d = chain - c
+    cand = (np.stack([sites % m, (sites // m) % m, sites // (m * m)], axis=1).astype(np.float64) + 0.5) * a - box / 2
+    cand += (a - 4.0) / 2 * (2 * rng.random(cand.shape) - 1)
+    ok = np.ones(len(cand), bool)
+    for b in chain:  # drop candidates closer than 4.4 A to any bead
+        d = cand - b
         d -= box * np.round(d / box)
-        if (np.einsum("ij,ij->i", d, d) < 4.4 ** 2).any():
-            continue
-        ions.append(c)
-        if len(ions) == n_ions:
-            break
+        ok &= np.einsum("ij,ij->i", d, d) >= 4.4 ** 2
+    ions = cand[ok][:n_ions]
     if len(ions) < n_ions:
         raise RuntimeError("ion placement failed")
-    xyz = np.concatenate([chain, np.array(ions)])
+    xyz = np.concatenate([chain, ions])
     ids = np.concatenate([np.zeros(n_beads, np.int32), np.ones(n_beads + n_salt, np.int32),
                           np.full(n_salt, 2, np.int32)])
     q = np.where(ids == 1, 1.0, -1.0)

@@ -129,3 +129,51 @@ def test_job_too_many_kvectors_fails_cleanly():
     assert s.job(bad)["state"] == F.Server.FAILED
     assert s.job(good)["state"] == F.Server.DONE
     s.close()
+
+
+def chain_jobs():
+    out = []
+    for r, (nb, nt) in enumerate([(30, 300), (60, 500), (45, 350)]):
+        cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(nb, nt, seed=40 + r, kmax=5)
+        out.append(("chain%d" % r, (cfg, xyz, q, ids, g), kinds, 2000 + 500 * r, 60 + r))
+    return out
+
+
+def test_chain_jobs_alone_equal_mixed_and_match_oracle():
+    """Polyelectrolyte jobs (ion and bead translations, pivots, bonds) share launches
+    with translate-only jobs; each result is bitwise the same as alone, and the FP64
+    result ends at the oracle's positions for the same stream."""
+    chains = chain_jobs()
+    alone = {}
+    for name, sysd, kinds, moves, seed in chains:
+        s = F.Server(fp32_pairs=False)
+        j = s.submit(sysd, moves, seed, kinds=kinds)
+        drain(s, quantum=10**6)
+        info = s.job(j)
+        assert info["state"] == F.Server.DONE and info["moves"] == moves, info
+        alone[name] = (info, s.positions(j))
+        s.close()
+        cfg, xyz, q, ids, g = sysd
+        orc = O.System(cfg)
+        orc.set_particles(xyz, q, ids)
+        orc.set_groups(g["begin"], g["size"], g["molid"])
+        orc.init()
+        rec = orc.run_philox_moves(moves, seed=seed, kinds=kinds, replica=0)
+        log("chain job", name, info["accepted"], "oracle", int(rec["accepted"].sum()))
+        assert info["accepted"] == int(rec["accepted"].sum())
+        np.testing.assert_allclose(alone[name][1], orc.positions(), rtol=0, atol=1e-9)
+    s = F.Server(fp32_pairs=False)
+    ids = {}
+    atomic = job_systems()[:2]
+    for name, sysd, kinds, moves, seed in chains:
+        ids[name] = s.submit(sysd, moves, seed, kinds=kinds)
+    for name, sysd, moves, seed in atomic:
+        s.submit(sysd, moves, seed)
+    drain(s, quantum=200)
+    for name, j in ids.items():
+        info = s.job(j)
+        a_info, a_xyz = alone[name]
+        assert info["state"] == F.Server.DONE
+        assert info["energy"] == a_info["energy"] and info["accepted"] == a_info["accepted"], name
+        assert np.array_equal(s.positions(j), a_xyz), name
+    s.close()
