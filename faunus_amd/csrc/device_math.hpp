@@ -101,6 +101,21 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return s; // valid in thread 0
 }
 
+// Intra-molecular pair exclusion (MoleculeData::isPairExcluded) from per-particle
+// sorted partner lists; no lists means no exclusions.
+__device__ __forceinline__ bool pair_excluded(const int* __restrict__ off, const int* __restrict__ list, int a, int b) {
+    if (off == nullptr) return false;
+    int lo = off[a], hi = off[a + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = list[mid];
+        if (v == b) return true;
+        if (v < b) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
+
 __device__ __forceinline__ int unpack_n(int packed, int which) {
     return static_cast<int>(static_cast<signed char>((packed >> (8 * which)) & 0xff));
 }

@@ -26,6 +26,8 @@ struct GroupTable {
     const int* flags;     // bit0 molecular (not atomic), bit1 rigid
     const double* cm;     // 3 per group (accepted state)
     double g2g2;          // group-to-group cutoff squared
+    const int* excl_off;  // per-particle excluded partners (null: none)
+    const int* excl;
 };
 
 struct CellArgs {
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
             if (jm < 0) {
                 for (int a = 0; a < m; ++a) {
                     const MovedParticle& M = mp[a];
+                    if (kGroups && M.gid == gj && pair_excluded(groups.excl_off, groups.excl, M.idx, j)) continue;
                     const DevPair& p = s_pairs[M.type * ntypes + tj];
                     if (allowed_other(M.gid, gj, true))
                         acc_new += pair_energy(p, s_coul, M.newp.w, pj.w,
@@ -122,7 +125,9 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
                 for (int a = 0; a < jm; ++a) {
                     const MovedParticle& A = mp[a];
                     bool ok;
-                    if (A.gid == B.gid) ok = single && mv.internal && (!kGroups || !(groups.flags[A.gid] & 2));
+                    if (A.gid == B.gid)
+                        ok = single && mv.internal && (!kGroups || (!(groups.flags[A.gid] & 2) &&
+                                                                    !pair_excluded(groups.excl_off, groups.excl, A.idx, B.idx)));
                     else ok = true; // several moved groups: atomic (molecular multi-group changes are rejected on the host)
                     if (!ok) continue;
                     const DevPair& p = s_pairs[A.type * ntypes + B.type];
@@ -161,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
                     const int tj = static_cast<int>(mj & 0xffu);
                     const int gj = static_cast<int>(mj >> 8);
                     if (!allowed_other(M.gid, gj, s == 0)) continue;
+                    if (kGroups && M.gid == gj && pair_excluded(groups.excl_off, groups.excl, M.idx, j)) continue;
                     const DevPair& p = s_pairs[M.type * ntypes + tj];
                     const double u = pair_energy(p, s_coul, P.w, pj.w, dist2(box, P.x, P.y, P.z, pj.x, pj.y, pj.z));
                     if (s == 0) acc_new += u;
@@ -173,7 +179,11 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
                     if (a >= b) continue;
                     const MovedParticle& A = mp[a];
                     const MovedParticle& B = mp[b];
-                    bool ok = (A.gid == B.gid) ? (single && mv.internal && (!kGroups || !(groups.flags[A.gid] & 2))) : true;
+                    bool ok = (A.gid == B.gid)
+                                  ? (single && mv.internal &&
+                                     (!kGroups || (!(groups.flags[A.gid] & 2) &&
+                                                   !pair_excluded(groups.excl_off, groups.excl, A.idx, B.idx))))
+                                  : true;
                     if (!ok) continue;
                     const DevPair& p = s_pairs[A.type * ntypes + B.type];
                     acc_new += pair_energy(p, s_coul, A.newp.w, B.newp.w,
@@ -442,6 +452,7 @@ __global__ __launch_bounds__(kBlock) void all_pairs_kernel(const double4* __rest
                 if (kGroups) {
                     if (gi == gj) {
                         if (groups.flags[gi] & 2) continue; // rigid: no internal pairs
+                        if (pair_excluded(groups.excl_off, groups.excl, i, j)) continue;
                     } else if ((groups.flags[gi] & 1) && (groups.flags[gj] & 1)) {
                         const double* a = groups.cm + 3 * gi;
                         const double* b = groups.cm + 3 * gj;
@@ -521,8 +532,6 @@ Engine::Engine(const std::string& config_json) : model_(Model::from_json(config_
     num_cu_ = prop.multiProcessorCount;
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     if (model_.ntypes > kMaxTypes) throw std::runtime_error("too many atom types for the GPU engine");
-    for (const auto& mol : model_.molecules)
-        if (!mol.excluded.empty()) throw std::runtime_error("pair exclusions are not supported by the GPU engine");
     max_blocks_ = 4096;
     check_hip(hipMalloc(&d_partials_, sizeof(double2) * max_blocks_), "hipMalloc partials");
     check_hip(hipMalloc(&d_ticket_, sizeof(unsigned)), "hipMalloc ticket");
@@ -556,6 +565,8 @@ Engine::~Engine() {
     (void)hipFree(d_bond_off_);
     (void)hipFree(d_bond_);
     (void)hipFree(d_bond_all_);
+    (void)hipFree(d_excl_off_);
+    (void)hipFree(d_excl_);
     if (h_result_) (void)hipHostFree(h_result_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -690,6 +701,36 @@ void Engine::build_group_tables() {
                 flat.push_back(DevBondPair{b0 + bd.i, b0 + bd.j, bd.half_k, bd.req});
             }
         }
+    }
+    // exclusions (MoleculeData::isPairExcluded): per-particle sorted absolute partners
+    std::vector<std::vector<int>> ex(static_cast<size_t>(n_));
+    bool any_excl = false;
+    for (int g = 0; g < ng; ++g) {
+        const MoleculeCfg& mol = model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(g)])];
+        if (mol.atomic) continue;
+        const int b0 = g_begin_[static_cast<size_t>(g)], sz = g_size_[static_cast<size_t>(g)];
+        for (auto [i, j] : mol.excluded) {
+            if (i >= sz || j >= sz) continue; // pairs beyond the group's atoms do not apply
+            ex[static_cast<size_t>(b0 + i)].push_back(b0 + j);
+            ex[static_cast<size_t>(b0 + j)].push_back(b0 + i);
+            any_excl = true;
+        }
+    }
+    (void)hipFree(d_excl_off_);
+    (void)hipFree(d_excl_);
+    d_excl_off_ = d_excl_ = nullptr;
+    if (any_excl) {
+        std::vector<int> eoff(static_cast<size_t>(n_) + 1, 0), elist;
+        for (int i = 0; i < n_; ++i) {
+            auto& v = ex[static_cast<size_t>(i)];
+            std::sort(v.begin(), v.end());
+            elist.insert(elist.end(), v.begin(), v.end());
+            eoff[static_cast<size_t>(i) + 1] = static_cast<int>(elist.size());
+        }
+        check_hip(hipMalloc(&d_excl_off_, sizeof(int) * eoff.size()), "hipMalloc exclusion offsets");
+        check_hip(hipMalloc(&d_excl_, sizeof(int) * elist.size()), "hipMalloc exclusions");
+        check_hip(hipMemcpy(d_excl_off_, eoff.data(), sizeof(int) * eoff.size(), hipMemcpyHostToDevice), "copy exclusions");
+        check_hip(hipMemcpy(d_excl_, elist.data(), sizeof(int) * elist.size(), hipMemcpyHostToDevice), "copy exclusions");
     }
     std::vector<int> off(static_cast<size_t>(n_) + 1, 0);
     std::vector<DevBond> list;
@@ -889,7 +930,7 @@ void Engine::timing(double* total_ms, long* launches) {
 
 void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks, bool bond_block) {
     const Box box = make_box(model_);
-    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2};
+    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2, d_excl_off_, d_excl_};
     CellArgs ca{cells_.count, cells_.members, cells_.nc[0], cells_.nc[1], cells_.nc[2], cells_.cap,
                 cells_.inv_cell[0], cells_.inv_cell[1], cells_.inv_cell[2]};
     const int ewald_end = pair_blocks + ewald_blocks;
@@ -1119,7 +1160,7 @@ void Engine::total_energy(double* terms) {
     const int bond_blocks = nbonds_ > 0 ? std::min(64, (nbonds_ + kBlock - 1) / kBlock) : 0;
     double* d_part = nullptr;
     check_hip(hipMalloc(&d_part, sizeof(double) * (blocks + bond_blocks)), "hipMalloc partials");
-    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2};
+    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2, d_excl_off_, d_excl_};
     if (any_molecular_)
         hipLaunchKernelGGL((all_pairs_kernel<true>), dim3(blocks), dim3(kBlock), 0, stream_, d_pos_, d_meta_, n_,
                            d_pairs_, model_.ntypes, d_coul_, make_box(model_), gt, d_part);

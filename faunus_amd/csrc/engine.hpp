@@ -139,6 +139,8 @@ class Engine {
     const int* d_bond_offsets() const { return d_bond_off_; }
     const DevBond* d_bonds() const { return d_bond_; }
     int num_bonds() const { return nbonds_; }
+    const int* d_exclusion_offsets() const { return d_excl_off_; }
+    const int* d_exclusions() const { return d_excl_; }
     const std::vector<int>& group_molid() const { return g_molid_; }
     double self_energy_of(int idx) const;
     const std::vector<int>& group_of() const { return h_gid_; }
@@ -172,6 +174,8 @@ class Engine {
     DevBond* d_bond_ = nullptr;
     DevBondPair* d_bond_all_ = nullptr;
     int nbonds_ = 0;
+    int* d_excl_off_ = nullptr; // excluded partners per particle (null: no exclusions)
+    int* d_excl_ = nullptr;
     // potentials
     std::vector<DevPair> h_pairs_;
     DevCoulomb h_coul_{};

@@ -571,6 +571,8 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
         const bool bonded = m.bonded_term >= 0 && e.num_bonds() > 0;
         s.bond_off = bonded ? e.d_bond_offsets() : nullptr;
         s.bonds = bonded ? e.d_bonds() : nullptr;
+        s.excl_off = e.d_exclusion_offsets();
+        s.excl = e.d_exclusions();
     }
     std::vector<double> terms(static_cast<size_t>(e.num_terms()));
     e.total_energy(terms.data());

@@ -79,6 +79,8 @@ struct ReplicaState {
     const double* dp_atom;    // per type: the atom's dp, or -1 when the move's dp applies
     const int* bond_off;      // per-particle bond lists (bonded term), or null
     const DevBond* bonds;
+    const int* excl_off;      // per-particle excluded partners, or null
+    const int* excl;
 };
 
 // One recorded move of the trace (tests).

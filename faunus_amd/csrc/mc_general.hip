@@ -313,9 +313,11 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                     for (int u = 0; u < kB; ++u) {
                         const int j = j0 + u * kThreads;
                         if (j >= n || (j >= P.mlo && j < P.mhi)) continue;
-                        if (static_cast<int>(mj[u] >> 8) == g && !same_ok) continue;
+                        const bool same = static_cast<int>(mj[u] >> 8) == g;
+                        if (same && !same_ok) continue;
                         const int tj = static_cast<int>(mj[u] & 0xffu);
                         for (int a = 0; a < m; ++a) {
+                            if (same && pair_excluded(S.excl_off, S.excl, P.mlo + a, j)) continue;
                             const double4 A = s_new[a], B = s_old[a];
                             const int ta = s_mt[a];
                             if (kF32) {
@@ -352,7 +354,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int j = S.cell_members[cell * S.cap + kk];
                         if (j >= P.mlo && j < P.mhi) continue;
                         const uint32_t mj = meta[j];
-                        if (static_cast<int>(mj >> 8) == g && !same_ok) continue;
+                        if (static_cast<int>(mj >> 8) == g &&
+                            (!same_ok || pair_excluded(S.excl_off, S.excl, P.mlo + a, j)))
+                            continue;
                         const int tj = static_cast<int>(mj & 0xffu);
                         double u;
                         if (kF32) {
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (same_ok && m > 1) {
                 for (int idx = tid; idx < m * m; idx += kThreads) {
                     const int a = idx / m, b = idx % m;
-                    if (a >= b) continue;
+                    if (a >= b || pair_excluded(S.excl_off, S.excl, P.mlo + a, P.mlo + b)) continue;
                     const DevPair& p = s_pairs[s_mt[a] * nt + s_mt[b]];
                     const double4 An = s_new[a], Bn = s_new[b], Ao = s_old[a], Bo = s_old[b];
                     e_new += pair_energy(p, s_coul, An.w, Bn.w, dist2(box, An.x, An.y, An.z, Bn.x, Bn.y, Bn.z));

@@ -207,12 +207,13 @@ def water(n_mol=4096, box=None, seed=1, alpha=0.28, cutoff=10.0, kmax=9, lj_cuto
 
 
 def polyelectrolyte(n_beads, n_total, seed, kmax=8, dp=10.0, bead_dp=2.0, dprot=3.0, monomer_molar=0.005,
-                    bond_k=0.5, bond_req=4.0):
+                    bond_k=0.5, bond_req=4.0, excluded_neighbours=0):
     """Bead-spring polyelectrolyte (n_beads, q=-1, sigma=4 A, harmonic bonds k=0.5
     kJ/mol/A^2, req=4 A) with n_beads counterions (q=+1) and 1:1 salt up to n_total
     particles; hard spheres + Ewald alpha=5/L, Rc=L/2, kmax=8; cubic box at the given
-    monomer concentration.  Groups: the chain (molecular), then one atomic group of
-    ions.  The move set translates ions (dp) and beads (bead_dp) and pivots the chain
+    monomer concentration; excluded_neighbours > 0 removes the nonbonded pairs of
+    beads within that many bonds.  Groups: the chain (molecular), then one atomic
+    group of ions.  The move set translates ions (dp) and beads (bead_dp) and pivots the chain
     (dprot), weighted by the number of ions, beads and bonds."""
     n_salt = max(0, (n_total - 2 * n_beads) // 2)
     n_ions = n_beads + 2 * n_salt
@@ -223,7 +224,8 @@ def polyelectrolyte(n_beads, n_total, seed, kmax=8, dp=10.0, bead_dp=2.0, dprot=
         "geometry": {"type": "cuboid", "length": box},
         "atomlist": [{"P": {"q": -1.0, "sigma": 4.0, "dp": bead_dp}}, {"C": {"q": 1.0, "sigma": 4.0, "dp": dp}},
                      {"A": {"q": -1.0, "sigma": 4.0, "dp": dp}}],
-        "moleculelist": [{"chain": {"atoms": ["P"] * n_beads, "bondlist": bonds}},
+        "moleculelist": [{"chain": {"atoms": ["P"] * n_beads, "bondlist": bonds,
+                                    "excluded_neighbours": excluded_neighbours}},
                          {"ions": {"atoms": ["C", "A"], "atomic": True}}],
         "energy": [{"bonded": {}},
                    {"nonbonded": {"default": [

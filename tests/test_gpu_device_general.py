@@ -44,9 +44,9 @@ def run(sysd, kinds, nm, seed, fp32=False):
     return b, orc, e0, compare(tr, rec, max(1.0, abs(e0)))
 
 
-@pytest.mark.parametrize("fp32", [False, True])
-def test_polyelectrolyte_general_moves(fp32):
-    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 400, seed=7, kmax=6)
+@pytest.mark.parametrize("fp32,excl", [(False, 0), (True, 0), (False, 1)])
+def test_polyelectrolyte_general_moves(fp32, excl):
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 400, seed=7, kmax=6, excluded_neighbours=excl)
     b, orc, e0, rep = run((cfg, xyz, q, ids, g), kinds, 3000, seed=33, fp32=fp32)
     log("device general polyelectrolyte fp32=%s" % fp32, rep)
     info = b.info(0)

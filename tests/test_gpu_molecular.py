@@ -99,8 +99,9 @@ def run_case(sysd, kinds, nmoves, seed):
     return rep, eng, orc, rec
 
 
-def test_polyelectrolyte_pivot_and_translations():
-    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 300, seed=7, kmax=6)
+@pytest.mark.parametrize("excl", [0, 2])
+def test_polyelectrolyte_pivot_and_translations(excl):
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 300, seed=7, kmax=6, excluded_neighbours=excl)
     rep, eng, orc, rec = run_case((cfg, xyz, q, ids, g), kinds, 1500, seed=21)
     log("polyelectrolyte", rep, "kinds", np.bincount(rec["kind"]))
     assert rep["first_mismatch"] == rep["n"], rep
