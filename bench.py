@@ -343,14 +343,20 @@ def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
     from tests import systems
     stream = serve_stream(n_jobs, seed)
     jobs = [systems.polyelectrolyte(nb, nt, seed=1000 + j) for j, (_, nb, nt) in enumerate(stream)]
+    log("serve: %d jobs built, %d particles in total" % (n_jobs, sum(len(j[2]) for j in jobs)))
     srv = F.Server(fp32_pairs=fp32, hbm_budget_bytes=200 << 30, max_slots=512)
     t0 = srv.status()["seconds"]
     ids = {}
     nxt = 0
     rounds = 0
     peak = 0
+    last_report = 0.0
     while True:
         now = srv.status()["seconds"] - t0
+        if now - last_report > 5.0:
+            st = srv.status()
+            log("serve: %.1f s, %d submitted, %d rounds, %.3e moves done" % (now, nxt, rounds, st["moves_done"]))
+            last_report = now
         while nxt < n_jobs and stream[nxt][0] <= now:
             cfg, xyz, q, idl, g, kinds = jobs[nxt]
             ids[nxt] = srv.submit((cfg, xyz, q, idl, g), sweeps * len(q), seed=1000 + nxt, kinds=kinds)

@@ -28,14 +28,20 @@ __device__ __forceinline__ double dist2(const Box& b, double ax, double ay, doub
     return dx * dx + dy * dy + dz * dz;
 }
 
-// tabulate.h:184-202: interval by lower_bound, Horner from the top coefficient.
+// tabulate.h:184-202: interval by lower_bound, Horner from the top coefficient.  The
+// lower_bound starts one knot before the first knot of x's uniform bucket and takes a
+// fixed number of compare-and-step rounds (enough for the fullest bucket), which finds
+// the same index as a binary search without data-dependent branches.
 template <class R>
-__device__ __forceinline__ R spline_eval(const R* __restrict__ knots, const R* __restrict__ coeff, int n, R x) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (knots[mid] < x) lo = mid + 1;
-        else hi = mid;
+__device__ __forceinline__ R spline_eval(const R* __restrict__ knots, const R* __restrict__ coeff, int n, R x,
+                                         const unsigned short* __restrict__ bucket, int steps) {
+    int b = static_cast<int>(x * static_cast<R>(kSplineBuckets));
+    b = b < 0 ? 0 : (b > kSplineBuckets - 1 ? kSplineBuckets - 1 : b);
+    int lo = static_cast<int>(bucket[b]) - 1;
+    lo = lo < 0 ? 0 : lo;
+    for (int s = 0; s < steps; ++s) {
+        const R k = knots[lo < n ? lo : n - 1];
+        lo += (lo < n && k < x) ? 1 : 0;
     }
     int pos = lo - 1;
     pos = pos < 0 ? 0 : (pos > n - 2 ? n - 2 : pos);
@@ -58,7 +64,7 @@ __device__ __forceinline__ double coulomb_pair(const DevCoulomb& c, double qa, d
     if (!(r < c.rc)) return 0.0;
     const double q = r * c.rc_inv;
     double s;
-    if (c.splined) s = spline_eval<double>(c.knots, c.coeff, c.nknots, q);
+    if (c.splined) s = spline_eval<double>(c.knots, c.coeff, c.nknots, q, c.bucket, c.spline_steps);
     else if (c.scheme == kCoulEwald) s = erfc(c.alpha * c.rc * q);
     else {
         const double q5 = q * q * q * q * q;

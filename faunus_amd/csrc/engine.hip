@@ -591,6 +591,19 @@ void Engine::upload_potentials() {
         h_coul_.nknots = static_cast<int>(c.S.knots.size());
         std::copy(c.S.knots.begin(), c.S.knots.end(), h_coul_.knots);
         std::copy(c.S.coeff.begin(), c.S.coeff.end(), h_coul_.coeff);
+        for (int b = 0; b <= kSplineBuckets; ++b) {
+            const double x = static_cast<double>(b) / kSplineBuckets;
+            h_coul_.bucket[b] = static_cast<unsigned short>(
+                std::lower_bound(c.S.knots.begin(), c.S.knots.end(), x) - c.S.knots.begin());
+        }
+        // lower_bound(x) for x in bucket b lies in [bucket[b], bucket[b+1]]; the search
+        // starts one knot early and gets one extra step for FP32 rounding of the knots
+        int steps = 1;
+        for (int b = 0; b < kSplineBuckets; ++b) {
+            const int start = std::max(0, static_cast<int>(h_coul_.bucket[b]) - 1);
+            steps = std::max(steps, static_cast<int>(h_coul_.bucket[b + 1]) - start + 1);
+        }
+        h_coul_.spline_steps = steps;
     }
     check_hip(hipMalloc(&d_pairs_, sizeof(DevPair) * h_pairs_.size()), "hipMalloc pairs");
     check_hip(hipMalloc(&d_coul_, sizeof(DevCoulomb)), "hipMalloc coulomb");

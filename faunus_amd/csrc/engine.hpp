@@ -15,6 +15,7 @@ namespace fmc {
 constexpr int kMaxInlineMoved = 4;     // moved particles passed in the kernel arguments
 constexpr int kMaxMoved = 4096;        // moved particles per Change
 constexpr int kMaxKnots = 64;          // Coulomb S(q) spline knots
+constexpr int kSplineBuckets = 1024;   // uniform buckets of q in [0, 1) indexing the knots
 constexpr int kMaxTypes = 16;          // atom types handled by the device kernels
 constexpr int kPhaseTableMoved = 8;    // moved particles with LDS phase tables (Ewald)
 constexpr int kBlock = 256;
@@ -29,10 +30,12 @@ struct DevCoulomb {
     int scheme;   // CoulombScheme
     int splined;
     int nknots;
-    int pad;
+    int spline_steps; // forward steps from a bucket's start that always reach the interval
     double lB, rc, rc_inv, alpha;
     double knots[kMaxKnots];
     double coeff[6 * kMaxKnots];
+    // bucket[b]: index of the first knot >= b / kSplineBuckets (start of the interval search)
+    unsigned short bucket[kSplineBuckets + 1];
 };
 
 // One moved particle: absolute index, type/group, new and old coordinates (x,y,z,q).

@@ -24,40 +24,38 @@ __device__ __forceinline__ float dist2f(float L, float half, float ax, float ay,
     return dx * dx + dy * dy + dz * dz;
 }
 
-// FP32 pair energy (same algebra as pair_energy, single precision).
+// FP32 pair energy: the algebra of pair_energy in single precision, written without
+// per-lane branches (every term is evaluated and selected) and with the hardware
+// reciprocal and reciprocal square root (1 ulp), which the FP32 form's tolerance covers.
 __device__ __forceinline__ float pair_energy_f(const DevPairF& p, const DevCoulomb& c, const float* knots,
                                                const float* coeff, float qa, float qb, float r2) {
-    float u = 0.0f;
-    if (p.lj_e4 != 0.0f && r2 < p.lj_cut2) {
-        float x = p.lj_s2 / r2;
-        x = x * x * x;
-        u = u + p.lj_e4 * (x * x - x);
-    }
-    if (p.wca_e4 != 0.0f && !(r2 > p.wca_s2 * 1.2599210498948732f)) {
-        float x = p.wca_s2 / r2;
-        x = x * x * x;
-        u = u + p.wca_e4 * (x * x - x + 0.25f);
-    }
-    if (r2 < p.hs_s2) u = u + __builtin_huge_valf();
-    if (p.coul != 0.0f) {
+    const float ir2 = __builtin_amdgcn_rcpf(r2);
+    float x = p.lj_s2 * ir2;
+    x = x * x * x;
+    float u = (r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0f;
+    float w = p.wca_s2 * ir2;
+    w = w * w * w;
+    u += (r2 > p.wca_s2 * 1.2599210498948732f) ? 0.0f : p.wca_e4 * (w * w - w + 0.25f);
+    u += (r2 < p.hs_s2) ? __builtin_huge_valf() : 0.0f;
+    if (c.scheme != kCoulNone) { // the same scheme for every lane
         const float lB = static_cast<float>(c.lB);
-        if (c.scheme == kCoulPM) {
-            u = u + lB * qa * qb / sqrtf(r2);
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        float e;
+        if (c.scheme == kCoulPM || c.scheme == kCoulPlain) {
+            e = lB * (qa * qb * ir);
         } else {
-            const float r = sqrtf(r2);
-            if (c.scheme == kCoulPlain) u = u + lB * (qa * qb / r);
-            else if (r < static_cast<float>(c.rc)) {
-                const float qq = r * static_cast<float>(c.rc_inv);
-                float s;
-                if (c.splined) s = spline_eval<float>(knots, coeff, c.nknots, qq);
-                else if (c.scheme == kCoulEwald) s = erfcf(static_cast<float>(c.alpha * c.rc) * qq);
-                else {
-                    const float q5 = qq * qq * qq * qq * qq;
-                    s = 1.0f - 1.75f * qq + 5.25f * q5 - 7.0f * q5 * qq + 2.5f * q5 * qq * qq;
-                }
-                u = u + lB * (qa * qb / r * s);
+            const float r = r2 * ir;
+            const float qq = r * static_cast<float>(c.rc_inv);
+            float s;
+            if (c.splined) s = spline_eval<float>(knots, coeff, c.nknots, qq, c.bucket, c.spline_steps);
+            else if (c.scheme == kCoulEwald) s = erfcf(static_cast<float>(c.alpha * c.rc) * qq);
+            else {
+                const float q5 = qq * qq * qq * qq * qq;
+                s = 1.0f - 1.75f * qq + 5.25f * q5 - 7.0f * q5 * qq + 2.5f * q5 * qq * qq;
             }
+            e = (r < static_cast<float>(c.rc)) ? lB * (qa * qb * ir * s) : 0.0f;
         }
+        u += (p.coul != 0.0f) ? e : 0.0f;
     }
     return u;
 }
@@ -97,5 +95,27 @@ __device__ __forceinline__ bool metropolis(double du, double u) {
     return u <= exp(-du);
 }
 
+
+// Exclusive prefix sum of one int per thread over a block of kThreads threads
+// (wavefront shuffles, then the wave totals in LDS); *total receives the block sum.
+template <int kThreads>
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wave) base += s_wave[w];
+        tot += s_wave[w];
+    }
+    __syncthreads(); // s_wave may be reused by the caller
+    *total = tot;
+    return base + x - v;
+}
 
 } // namespace fmc

@@ -22,6 +22,7 @@ namespace {
 
 constexpr int kMaxKinds = 8;   // move kinds per replica
 constexpr int kChunk = 8;      // moved atoms per Ewald phase-table chunk
+constexpr int kMaxComb = 512;  // (moved atom, state, cell) combinations of the flattened stencil walk
 
 __device__ __forceinline__ int pick(uint32_t x, int n) {
     return static_cast<int>((static_cast<uint64_t>(x) * static_cast<uint64_t>(n)) >> 32);
@@ -81,6 +82,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
     __shared__ int s_accept;
+    __shared__ int s_ccell[kCells ? kMaxComb : 1];
+    __shared__ int s_cpre[kCells ? kMaxComb + 1 : 1];
+    __shared__ int s_wsum[kThreads / 64];
     extern __shared__ int s_dyn[]; // kn | phase chunk tables | new | old coordinates | moved types
 
     const int tid = threadIdx.x;
@@ -339,8 +343,87 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         }
                     }
                 }
+            } else if (m * 54 <= kMaxComb) {
+                // 27-cell stencils around each moved atom's new (st=0) and old (st=1)
+                // position, flattened: the members of all (atom, state, cell)
+                // combinations are numbered by a prefix sum and dealt to the lanes in
+                // strides, with the loads of several members issued together
+                const int ncomb = m * 54;
+                int carry = 0;
+                for (int base = 0; base < ncomb; base += kThreads) {
+                    const int cb = base + tid;
+                    int cnt = 0;
+                    if (cb < ncomb) {
+                        const int a = cb / 54, st = (cb / 27) & 1, c = cb % 27;
+                        const double4 A = st == 0 ? s_new[a] : s_old[a];
+                        const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                        const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                        const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                        const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                        s_ccell[cb] = cell;
+                        cnt = S.cell_count[cell];
+                    }
+                    int tot;
+                    const int ex = block_exclusive_scan<kThreads>(cnt, s_wsum, &tot);
+                    if (cb < ncomb) s_cpre[cb] = carry + ex;
+                    carry += tot;
+                }
+                if (tid == 0) s_cpre[ncomb] = carry;
+                __syncthreads();
+                const int W = carry;
+                constexpr int kU = 4;
+                int c = 0;
+                for (int w0 = tid; w0 < W; w0 += kThreads * kU) {
+                    int jj[kU], cc[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int w = w0 + u * kThreads;
+                        jj[u] = -1;
+                        cc[u] = 0;
+                        if (w < W) {
+                            while (s_cpre[c + 1] <= w) ++c;
+                            cc[u] = c;
+                            jj[u] = S.cell_members[s_ccell[c] * S.cap + (w - s_cpre[c])];
+                        }
+                    }
+                    uint32_t mj[kU];
+                    double4 pj[kU];
+                    float4 pf[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (jj[u] >= 0) {
+                            mj[u] = meta[jj[u]];
+                            if (kF32) pf[u] = pos32[jj[u]];
+                            else pj[u] = pos[jj[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const int j = jj[u];
+                        if (j < 0 || (j >= P.mlo && j < P.mhi)) continue;
+                        const int a = cc[u] / 54, st = (cc[u] / 27) & 1;
+                        if (static_cast<int>(mj[u] >> 8) == g &&
+                            (!same_ok || pair_excluded(S.excl_off, S.excl, P.mlo + a, j)))
+                            continue;
+                        const double4 A = st == 0 ? s_new[a] : s_old[a];
+                        const int ta = s_mt[a], tj = static_cast<int>(mj[u] & 0xffu);
+                        double e;
+                        if (kF32) {
+                            const float fL = static_cast<float>(L), fh = static_cast<float>(half);
+                            e = static_cast<double>(pair_energy_f(
+                                s_pairs_f[ta * nt + tj], s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pf[u].w,
+                                dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
+                                       pf[u].x, pf[u].y, pf[u].z)));
+                        } else {
+                            e = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
+                                            dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
+                        }
+                        if (st == 0) e_new += e;
+                        else e_old += e;
+                    }
+                }
             } else {
-                // 27-cell stencils around each moved atom's new (st=0) and old (st=1) position
+                // many moved atoms: one (atom, state, cell) combination per lane
                 for (int cb = tid; cb < m * 54; cb += kThreads) {
                     const int a = cb / 54, st = (cb / 27) & 1, c = cb % 27;
                     const double4 A = st == 0 ? s_new[a] : s_old[a];
@@ -561,7 +644,7 @@ void launch_one(int grid, size_t lds, hipStream_t st, ReplicaState* s, const lon
     static bool attr = false;
     if (!attr) {
         check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      120 * 1024),
+                                      112 * 1024),
                   "hipFuncSetAttribute");
         attr = true;
     }
@@ -599,7 +682,7 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) +
                        sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
                        (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
-    if (lds > 120 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+    if (lds > 112 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
     // each workgroup lays out its own replica's tables inside this allocation
     const bool cells = h_states[0].cells != 0;
     if (maxkpt <= 8) dispatch<8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
