@@ -399,8 +399,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
     ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve",
                                        "water"], default=None)
-    ap.add_argument("--serve-jobs", type=int, default=48)
-    ap.add_argument("--serve-sweeps", type=int, default=4)
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-sweeps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "

@@ -26,8 +26,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     if (M <= 0 || S.error) return;
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
-    __shared__ float s_knots_f[kF32 ? kMaxKnots : 1];
-    __shared__ float s_coeff_f[kF32 ? 6 * kMaxKnots : 1];
+    __shared__ float4 s_stab[kF32 ? kSTable : 1];
     __shared__ DevCoulomb s_coul;
     __shared__ double s_dp[kMaxTypes];
     __shared__ double s_gk[kMaxGk];
@@ -59,9 +58,11 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
     __syncthreads();
     if (kF32) {
-        for (int i = tid; i < s_coul.nknots; i += kThreads) s_knots_f[i] = static_cast<float>(s_coul.knots[i]);
-        for (int i = tid; i < 6 * s_coul.nknots; i += kThreads) s_coeff_f[i] = static_cast<float>(s_coul.coeff[i]);
+        if (S.stab)
+            for (int i = tid; i < kSTable; i += kThreads) s_stab[i] = S.stab[i];
     }
+    const CoulF cf{s_coul.scheme, static_cast<float>(s_coul.lB), static_cast<float>(s_coul.rc),
+                   static_cast<float>(s_coul.rc_inv)};
 
     // this thread's chunk of k-vectors and their running structure factors
     const int k0 = tid * kpt;
@@ -108,58 +109,16 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         // nonbonded pair sums of the moved particle in the new and old position
         double e_new = 0.0, e_old = 0.0;
         if (!kCells) {
-            // the loads of a batch of neighbours are issued together so that their latency overlaps
-            constexpr int kB = kF32 ? 8 : 4;
-            for (int j0 = tid; j0 < n; j0 += kThreads * kB) {
-                if (kF32) {
-                    float4 pj[kB];
-                    uint32_t mj[kB];
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j < n) {
-                            pj[u] = pos32[j];
-                            mj[u] = meta[j];
-                        }
-                    }
-                    const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j < n && j != i) {
-                            const DevPairF& p = s_pairs_f[ti * nt + static_cast<int>(mj[u] & 0xffu)];
-                            e_new += static_cast<double>(pair_energy_f(
-                                p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj[u].w,
-                                dist2f(fL, fh, static_cast<float>(nx_), static_cast<float>(ny_), static_cast<float>(nz_),
-                                       pj[u].x, pj[u].y, pj[u].z)));
-                            e_old += static_cast<double>(pair_energy_f(
-                                p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(po.w), pj[u].w,
-                                dist2f(fL, fh, static_cast<float>(po.x), static_cast<float>(po.y), static_cast<float>(po.z),
-                                       pj[u].x, pj[u].y, pj[u].z)));
-                        }
-                    }
-                } else {
-                    double4 pj[kB];
-                    uint32_t mj[kB];
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j < n) {
-                            pj[u] = pos[j];
-                            mj[u] = meta[j];
-                        }
-                    }
-                    const Box b{{L, L, L}, {half, half, half}};
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j < n && j != i) {
-                            const DevPair& p = s_pairs[ti * nt + static_cast<int>(mj[u] & 0xffu)];
-                            e_new += pair_energy(p, s_coul, po.w, pj[u].w, dist2(b, nx_, ny_, nz_, pj[u].x, pj[u].y, pj[u].z));
-                            e_old += pair_energy(p, s_coul, po.w, pj[u].w, dist2(b, po.x, po.y, po.z, pj[u].x, pj[u].y, pj[u].z));
-                        }
-                    }
-                }
+            if (kF32) {
+                pair_row_f32<kThreads>(pos32, meta, n, i, -1, true, s_pairs_f + ti * nt, cf, s_stab,
+                                       S.terms, static_cast<float>(L), static_cast<float>(inv), static_cast<float>(nx_),
+                                       static_cast<float>(ny_), static_cast<float>(nz_), static_cast<float>(po.x),
+                                       static_cast<float>(po.y), static_cast<float>(po.z), static_cast<float>(po.w),
+                                       e_new, e_old);
+            } else {
+                const Box b{{L, L, L}, {half, half, half}};
+                pair_row_f64<kThreads>(pos, meta, n, i, -1, true, s_pairs + ti * nt, s_coul, S.terms, b, nx_, ny_, nz_,
+                                       po.x, po.y, po.z, po.w, e_new, e_old);
             }
         } else {
             // 27-cell stencils around the new (st=0) and old (st=1) position
@@ -181,10 +140,10 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                     double u;
                     if (kF32) {
                         const float4 pj = pos32[j];
-                        const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-                        u = static_cast<double>(pair_energy_f(s_pairs_f[ti * nt + tj], s_coul, s_knots_f, s_coeff_f,
+                        const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                        u = static_cast<double>(pair_energy_ft(s_pairs_f[ti * nt + tj], cf, s_stab, S.terms,
                                                               static_cast<float>(po.w), pj.w,
-                                                              dist2f(fL, fh, static_cast<float>(px), static_cast<float>(py),
+                                                              dist2f_round(fL, fIL, static_cast<float>(px), static_cast<float>(py),
                                                                      static_cast<float>(pz), pj.x, pj.y, pj.z)));
                     } else {
                         const double4 pj = pos[j];
@@ -375,7 +334,7 @@ void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const
     auto fn = mc_advance_kernel<kThreads, kMaxKPT, kCells, kF32>;
     static bool attr_set = false;
     if (!attr_set) {
-        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 40 * 1024),
+        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024),
                   "hipFuncSetAttribute");
         attr_set = true;
     }
@@ -418,6 +377,7 @@ Batch::~Batch() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+
 namespace {
 template <class T> T* upload(const std::vector<T>& v, std::vector<void*>& owned, const char* what) {
     T* d = nullptr;
@@ -427,6 +387,44 @@ template <class T> T* upload(const std::vector<T>& v, std::vector<void*>& owned,
     return d;
 }
 } // namespace
+
+// S(q) and dS/dq of the engine's Coulomb model: the Andrea spline (value and the
+// derivative of its interval polynomial) or the exact function.
+static void short_range_and_slope(const DevCoulomb& c, double q, double* s, double* ds) {
+    if (c.splined) {
+        const int n = c.nknots;
+        const int lo = static_cast<int>(std::lower_bound(c.knots, c.knots + n, q) - c.knots);
+        int pos = std::min(std::max(lo - 1, 0), n - 2);
+        const double* k = c.coeff + 6 * pos;
+        const double z = q - c.knots[pos];
+        *s = ((((k[5] * z + k[4]) * z + k[3]) * z + k[2]) * z + k[1]) * z + k[0];
+        *ds = (((5 * k[5] * z + 4 * k[4]) * z + 3 * k[3]) * z + 2 * k[2]) * z + k[1];
+    } else if (c.scheme == kCoulEwald) {
+        const double a = c.alpha * c.rc;
+        *s = std::erfc(a * q);
+        *ds = -2.0 * a / std::sqrt(kPi) * std::exp(-a * a * q * q);
+    } else {
+        const double q5 = q * q * q * q * q;
+        *s = 1.0 - 1.75 * q + 5.25 * q5 - 7.0 * q5 * q + 2.5 * q5 * q * q;
+        *ds = -1.75 + 26.25 * q * q * q * q - 42.0 * q5 + 17.5 * q5 * q;
+    }
+}
+
+// Cubic Hermite coefficients (in the local coordinate of each interval) of S(q).
+static std::vector<float4> short_range_table(const DevCoulomb& c) {
+    std::vector<float4> t(static_cast<size_t>(kSTable));
+    const double h = 1.0 / kSTable;
+    for (int i = 0; i < kSTable; ++i) {
+        double f0, d0, f1, d1;
+        short_range_and_slope(c, i * h, &f0, &d0);
+        short_range_and_slope(c, (i + 1) * h, &f1, &d1);
+        const double c2 = 3.0 * (f1 - f0) - h * (2.0 * d0 + d1);
+        const double c3 = 2.0 * (f0 - f1) + h * (d0 + d1);
+        t[static_cast<size_t>(i)] = make_float4(static_cast<float>(f0), static_cast<float>(h * d0),
+                                                static_cast<float>(c2), static_cast<float>(c3));
+    }
+    return t;
+}
 
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
                                 std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds) {
@@ -503,6 +501,14 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
         check_hip(hipMemcpy(d_p32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
     }
     s.pos32 = d_p32;
+    const DevCoulomb& dc = e.dev_coulomb();
+    if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
+    s.terms = 0;
+    for (const DevPair& p : e.dev_pairs()) {
+        if (p.lj_e4 != 0.0) s.terms |= kTermLJ;
+        if (p.wca_e4 != 0.0) s.terms |= kTermWCA;
+        if (p.hs_s2 > 0.0) s.terms |= kTermHS;
+    }
     s.seed = seed;
     s.replica_id = stream_id;
     s.ladder = ladder;
@@ -592,7 +598,7 @@ void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_sta
         if (h_states[r].cells != h_states[0].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
     }
     const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
-    if (lds > 120 * 1024) throw std::runtime_error("k-vector table too large for LDS");
+    if (lds > 96 * 1024) throw std::runtime_error("k-vector table too large for LDS");
     const bool cells = h_states[0].cells != 0;
     if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
     else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);

@@ -65,7 +65,7 @@ struct ReplicaState {
     unsigned long long accepted;
     double sum_du;
     int error;
-    int pad;
+    int terms;          // pair terms present in the pair table (kTermLJ | kTermWCA | kTermHS)
     // general move set (mc_general_kernel); nkinds == 0 for the translate-only kernel
     int nkinds, max_moved;
     unsigned long long total_weight;
@@ -81,6 +81,7 @@ struct ReplicaState {
     const DevBond* bonds;
     const int* excl_off;      // per-particle excluded partners, or null
     const int* excl;
+    const float4* stab;       // FP32 forms: cubic table of S(q) (kSTable intervals), or null
 };
 
 // One recorded move of the trace (tests).

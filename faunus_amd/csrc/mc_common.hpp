@@ -15,49 +15,152 @@ struct DevPairF {
     float lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
 };
 
-__device__ __forceinline__ float dist2f(float L, float half, float ax, float ay, float az, float bx, float by,
-                                        float bz) {
+// Pair terms present anywhere in a replica's pair table (a uniform switch per kernel).
+constexpr int kTermLJ = 1, kTermWCA = 2, kTermHS = 4, kTermAll = 7;
+
+// FP64 pair energy with the same term switch (exact arithmetic of pair_energy).
+__device__ __forceinline__ double pair_energy_t(const DevPair& p, const DevCoulomb& c, double qa, double qb, double r2,
+                                                int terms) {
+    double u = 0.0;
+    if (terms & kTermLJ) {
+        double x = p.lj_s2 / r2;
+        x = x * x * x;
+        u = u + ((p.lj_e4 != 0.0 && r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0);
+    }
+    if (terms & kTermWCA) {
+        double x = p.wca_s2 / r2;
+        x = x * x * x;
+        u = u + ((p.wca_e4 != 0.0 && !(r2 > p.wca_s2 * 1.2599210498948732)) ? p.wca_e4 * (x * x - x + 0.25) : 0.0);
+    }
+    if (terms & kTermHS) u = u + ((r2 < p.hs_s2) ? __builtin_huge_val() : 0.0);
+    if (c.scheme != kCoulNone) {
+        const double e = coulomb_pair(c, qa, qb, r2);
+        u = u + ((p.coul != 0.0) ? e : 0.0);
+    }
+    return u;
+}
+
+// FP32 forms: the Coulomb short-range function S(q) of the splined (or exact)
+// Ewald/Fanourgakis schemes as a uniform cubic Hermite table on q in [0, 1], built on
+// the host from the FP64 form's S(q) and S'(q) (interpolation error ~1e-13, far below
+// FP32 rounding), and the scheme constants held in registers.
+constexpr int kSTable = 1024;
+
+struct CoulF {
+    int scheme;
+    float lB, rc, rc_inv;
+};
+
+// Minimum image by rounding (|dx| <= L for particles inside the box).
+__device__ __forceinline__ float dist2f_round(float L, float invL, float ax, float ay, float az, float bx, float by,
+                                              float bz) {
     float dx = ax - bx, dy = ay - by, dz = az - bz;
-    dx = dx > half ? dx - L : (dx < -half ? dx + L : dx);
-    dy = dy > half ? dy - L : (dy < -half ? dy + L : dy);
-    dz = dz > half ? dz - L : (dz < -half ? dz + L : dz);
+    dx -= L * rintf(dx * invL);
+    dy -= L * rintf(dy * invL);
+    dz -= L * rintf(dz * invL);
     return dx * dx + dy * dy + dz * dz;
 }
 
-// FP32 pair energy: the algebra of pair_energy in single precision, written without
-// per-lane branches (every term is evaluated and selected) and with the hardware
-// reciprocal and reciprocal square root (1 ulp), which the FP32 form's tolerance covers.
-__device__ __forceinline__ float pair_energy_f(const DevPairF& p, const DevCoulomb& c, const float* knots,
-                                               const float* coeff, float qa, float qb, float r2) {
-    const float ir2 = __builtin_amdgcn_rcpf(r2);
-    float x = p.lj_s2 * ir2;
-    x = x * x * x;
-    float u = (r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0f;
-    float w = p.wca_s2 * ir2;
-    w = w * w * w;
-    u += (r2 > p.wca_s2 * 1.2599210498948732f) ? 0.0f : p.wca_e4 * (w * w - w + 0.25f);
-    u += (r2 < p.hs_s2) ? __builtin_huge_valf() : 0.0f;
-    if (c.scheme != kCoulNone) { // the same scheme for every lane
-        const float lB = static_cast<float>(c.lB);
+__device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
+                                                int terms, float qa, float qb, float r2) {
+    float u = 0.0f;
+    if (terms & (kTermLJ | kTermWCA)) {
+        const float ir2 = __builtin_amdgcn_rcpf(r2);
+        if (terms & kTermLJ) {
+            float x = p.lj_s2 * ir2;
+            x = x * x * x;
+            u += (r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0f;
+        }
+        if (terms & kTermWCA) {
+            float w = p.wca_s2 * ir2;
+            w = w * w * w;
+            u += (r2 > p.wca_s2 * 1.2599210498948732f) ? 0.0f : p.wca_e4 * (w * w - w + 0.25f);
+        }
+    }
+    if (terms & kTermHS) u += (r2 < p.hs_s2) ? __builtin_huge_valf() : 0.0f;
+    if (c.scheme != kCoulNone) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float e;
         if (c.scheme == kCoulPM || c.scheme == kCoulPlain) {
-            e = lB * (qa * qb * ir);
+            e = c.lB * (qa * qb * ir);
         } else {
             const float r = r2 * ir;
-            const float qq = r * static_cast<float>(c.rc_inv);
-            float s;
-            if (c.splined) s = spline_eval<float>(knots, coeff, c.nknots, qq, c.bucket, c.spline_steps);
-            else if (c.scheme == kCoulEwald) s = erfcf(static_cast<float>(c.alpha * c.rc) * qq);
-            else {
-                const float q5 = qq * qq * qq * qq * qq;
-                s = 1.0f - 1.75f * qq + 5.25f * q5 - 7.0f * q5 * qq + 2.5f * q5 * qq * qq;
-            }
-            e = (r < static_cast<float>(c.rc)) ? lB * (qa * qb * ir * s) : 0.0f;
+            const float t = r * c.rc_inv * static_cast<float>(kSTable);
+            int i = static_cast<int>(t);
+            i = i > kSTable - 1 ? kSTable - 1 : i;
+            const float f = t - static_cast<float>(i);
+            const float4 k = tab[i];
+            const float sv = ((k.w * f + k.z) * f + k.y) * f + k.x;
+            e = (r < c.rc) ? c.lB * (qa * qb * ir * sv) : 0.0f;
         }
         u += (p.coul != 0.0f) ? e : 0.0f;
     }
     return u;
+}
+
+// Pair sums of one moved particle, at its new (a) and old (b) position, against every
+// particle j in [0, n) except `skip`; pairs inside group g count only if same_ok.
+// Lanes stride over j; each lane loads kU neighbours before evaluating them.
+template <int kThreads>
+__device__ __forceinline__ void pair_row_f32(const float4* __restrict__ pos32, const uint32_t* __restrict__ meta, int n,
+                                             int skip, int g, bool same_ok, const DevPairF* row, const CoulF& c,
+                                             const float4* tab, int terms, float L, float invL, float ax, float ay,
+                                             float az, float bx, float by, float bz, float qa, double& e_new,
+                                             double& e_old) {
+    constexpr int kU = 4;
+    for (int j0 = threadIdx.x; j0 < n; j0 += kThreads * kU) {
+        float4 pj[kU];
+        uint32_t mj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int j = j0 + u * kThreads;
+            pj[u] = j < n ? pos32[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            mj[u] = j < n ? meta[j] : 0u;
+        }
+        float sn = 0.0f, so = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int j = j0 + u * kThreads;
+            const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
+            const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
+            const float un = pair_energy_ft(p, c, tab, terms, qa, pj[u].w,
+                                            dist2f_round(L, invL, ax, ay, az, pj[u].x, pj[u].y, pj[u].z));
+            const float uo = pair_energy_ft(p, c, tab, terms, qa, pj[u].w,
+                                            dist2f_round(L, invL, bx, by, bz, pj[u].x, pj[u].y, pj[u].z));
+            sn += ok ? un : 0.0f;
+            so += ok ? uo : 0.0f;
+        }
+        e_new += static_cast<double>(sn);
+        e_old += static_cast<double>(so);
+    }
+}
+
+template <int kThreads>
+__device__ __forceinline__ void pair_row_f64(const double4* __restrict__ pos, const uint32_t* __restrict__ meta, int n,
+                                             int skip, int g, bool same_ok, const DevPair* row, const DevCoulomb& c,
+                                             int terms, const Box& box, double ax, double ay, double az, double bx,
+                                             double by, double bz, double qa, double& e_new, double& e_old) {
+    constexpr int kU = 4;
+    for (int j0 = threadIdx.x; j0 < n; j0 += kThreads * kU) {
+        double4 pj[kU];
+        uint32_t mj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int j = j0 + u * kThreads;
+            pj[u] = j < n ? pos[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+            mj[u] = j < n ? meta[j] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int j = j0 + u * kThreads;
+            const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
+            const DevPair& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
+            const double un = pair_energy_t(p, c, qa, pj[u].w, dist2(box, ax, ay, az, pj[u].x, pj[u].y, pj[u].z), terms);
+            const double uo = pair_energy_t(p, c, qa, pj[u].w, dist2(box, bx, by, bz, pj[u].x, pj[u].y, pj[u].z), terms);
+            e_new += ok ? un : 0.0;
+            e_old += ok ? uo : 0.0;
+        }
+    }
 }
 
 __device__ __forceinline__ int cell_of_coord(double x, double half, double inv_cell, int nc) {

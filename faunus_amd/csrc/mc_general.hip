@@ -74,8 +74,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     if (M <= 0 || S.error) return;
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
-    __shared__ float s_knots_f[kF32 ? kMaxKnots : 1];
-    __shared__ float s_coeff_f[kF32 ? 6 * kMaxKnots : 1];
+    __shared__ float4 s_stab[kF32 ? kSTable : 1];
     __shared__ DevCoulomb s_coul;
     __shared__ double s_gk[kMaxGk];
     __shared__ double s_dpa[kMaxTypes];
@@ -116,9 +115,11 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
     __syncthreads();
     if (kF32) {
-        for (int i = tid; i < s_coul.nknots; i += kThreads) s_knots_f[i] = static_cast<float>(s_coul.knots[i]);
-        for (int i = tid; i < 6 * s_coul.nknots; i += kThreads) s_coeff_f[i] = static_cast<float>(s_coul.coeff[i]);
+        if (S.stab)
+            for (int i = tid; i < kSTable; i += kThreads) s_stab[i] = S.stab[i];
     }
+    const CoulF cf{s_coul.scheme, static_cast<float>(s_coul.lB), static_cast<float>(s_coul.rc),
+                   static_cast<float>(s_coul.rc_inv)};
 
     const int k0 = tid * kpt;
     double2 q[kMaxKPT];
@@ -297,7 +298,21 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         double e_new = 0.0, e_old = 0.0;
         const int gflag = g >= 0 ? S.gflags[g] : 0;
         const bool same_ok = P.internal && !(gflag & 2); // pairs inside the moved group
-        if (m > 0) {
+        if (m == 1 && !kCells && S.excl_off == nullptr) {
+            // one moved atom: the row sums with the moved coordinates in registers
+            const double4 A = s_new[0], B = s_old[0];
+            const int ta = s_mt[0];
+            if (kF32) {
+                pair_row_f32<kThreads>(pos32, meta, n, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab,
+                                       S.terms, static_cast<float>(L), static_cast<float>(inv),
+                                       static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
+                                       static_cast<float>(B.x), static_cast<float>(B.y), static_cast<float>(B.z),
+                                       static_cast<float>(A.w), e_new, e_old);
+            } else {
+                pair_row_f64<kThreads>(pos, meta, n, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, S.terms, box, A.x,
+                                       A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
+            }
+        } else if (m > 0) {
             if (!kCells) {
                 constexpr int kB = kF32 ? 8 : 4;
                 for (int j0 = tid; j0 < n; j0 += kThreads * kB) {
@@ -326,14 +341,14 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             const int ta = s_mt[a];
                             if (kF32) {
                                 const DevPairF& p = s_pairs_f[ta * nt + tj];
-                                const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-                                e_new += static_cast<double>(pair_energy_f(
-                                    p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pf[u].w,
-                                    dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y),
+                                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                                e_new += static_cast<double>(pair_energy_ft(
+                                    p, cf, s_stab, S.terms, static_cast<float>(A.w), pf[u].w,
+                                    dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y),
                                            static_cast<float>(A.z), pf[u].x, pf[u].y, pf[u].z)));
-                                e_old += static_cast<double>(pair_energy_f(
-                                    p, s_coul, s_knots_f, s_coeff_f, static_cast<float>(B.w), pf[u].w,
-                                    dist2f(fL, fh, static_cast<float>(B.x), static_cast<float>(B.y),
+                                e_old += static_cast<double>(pair_energy_ft(
+                                    p, cf, s_stab, S.terms, static_cast<float>(B.w), pf[u].w,
+                                    dist2f_round(fL, fIL, static_cast<float>(B.x), static_cast<float>(B.y),
                                            static_cast<float>(B.z), pf[u].x, pf[u].y, pf[u].z)));
                             } else {
                                 const DevPair& p = s_pairs[ta * nt + tj];
@@ -409,10 +424,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int ta = s_mt[a], tj = static_cast<int>(mj[u] & 0xffu);
                         double e;
                         if (kF32) {
-                            const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-                            e = static_cast<double>(pair_energy_f(
-                                s_pairs_f[ta * nt + tj], s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pf[u].w,
-                                dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
+                            const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                            e = static_cast<double>(pair_energy_ft(
+                                s_pairs_f[ta * nt + tj], cf, s_stab, S.terms, static_cast<float>(A.w), pf[u].w,
+                                dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        pf[u].x, pf[u].y, pf[u].z)));
                         } else {
                             e = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
@@ -444,10 +459,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         double u;
                         if (kF32) {
                             const float4 pj = pos32[j];
-                            const float fL = static_cast<float>(L), fh = static_cast<float>(half);
-                            u = static_cast<double>(pair_energy_f(
-                                s_pairs_f[ta * nt + tj], s_coul, s_knots_f, s_coeff_f, static_cast<float>(A.w), pj.w,
-                                dist2f(fL, fh, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
+                            const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                            u = static_cast<double>(pair_energy_ft(
+                                s_pairs_f[ta * nt + tj], cf, s_stab, S.terms, static_cast<float>(A.w), pj.w,
+                                dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        pj.x, pj.y, pj.z)));
                         } else {
                             const double4 pj = pos[j];
@@ -644,7 +659,7 @@ void launch_one(int grid, size_t lds, hipStream_t st, ReplicaState* s, const lon
     static bool attr = false;
     if (!attr) {
         check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      112 * 1024),
+                                      96 * 1024),
                   "hipFuncSetAttribute");
         attr = true;
     }
@@ -682,7 +697,7 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) +
                        sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
                        (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
-    if (lds > 112 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+    if (lds > 96 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
     // each workgroup lays out its own replica's tables inside this allocation
     const bool cells = h_states[0].cells != 0;
     if (maxkpt <= 8) dispatch<8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
