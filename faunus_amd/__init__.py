@@ -72,6 +72,7 @@ SIGNATURES = {
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
     "fmc_server_create": (C.c_int, [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]),
     "fmc_server_destroy": (C.c_int, [C.c_void_p]),
+    "fmc_server_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fmc_server_submit": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_longlong,
                                     C.c_ulonglong, I]),
     "fmc_server_submit_moves": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, D, D, I, C.c_int, I, I, I, C.c_longlong,
@@ -455,11 +456,15 @@ class Server:
 
     QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, fp32_pairs=True, hbm_budget_bytes=64 << 30, max_slots=1024):
+    def __init__(self, fp32_pairs=True, hbm_budget_bytes=64 << 30, max_slots=1024, split_atoms=0, max_parts=16):
+        """``split_atoms`` > 0 shares each job with a general move set and no cell list
+        among ceil(n / split_atoms) workgroups (at most ``max_parts``)."""
         h = C.c_void_p()
         _check(lib().fmc_server_create(int(bool(fp32_pairs)), int(hbm_budget_bytes), int(max_slots), C.byref(h)))
         self.h = h
         self.sizes = {}
+        if split_atoms:
+            _check(lib().fmc_server_set_split(self.h, int(split_atoms), int(max_parts)))
 
     def close(self):
         if getattr(self, "h", None):

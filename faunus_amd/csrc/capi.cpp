@@ -545,6 +545,10 @@ int fmc_server_destroy(fmc_server* s) {
     return guarded([&] { delete s; });
 }
 
+int fmc_server_set_split(fmc_server* s, int atoms_per_workgroup, int max_workgroups) {
+    return guarded([&] { srv(s).set_split(atoms_per_workgroup, max_workgroups); });
+}
+
 int fmc_server_submit(fmc_server* s, const char* config_json, int n, const double* xyz, const double* charge,
                       const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
                       long long moves, unsigned long long seed, int* job) {

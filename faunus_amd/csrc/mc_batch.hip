@@ -110,14 +110,14 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         double e_new = 0.0, e_old = 0.0;
         if (!kCells) {
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, n, i, -1, true, s_pairs_f + ti * nt, cf, s_stab,
+                pair_row_f32<kThreads>(pos32, meta, 0, n, i, -1, true, s_pairs_f + ti * nt, cf, s_stab,
                                        S.terms, static_cast<float>(L), static_cast<float>(inv), static_cast<float>(nx_),
                                        static_cast<float>(ny_), static_cast<float>(nz_), static_cast<float>(po.x),
                                        static_cast<float>(po.y), static_cast<float>(po.z), static_cast<float>(po.w),
                                        e_new, e_old);
             } else {
                 const Box b{{L, L, L}, {half, half, half}};
-                pair_row_f64<kThreads>(pos, meta, n, i, -1, true, s_pairs + ti * nt, s_coul, S.terms, b, nx_, ny_, nz_,
+                pair_row_f64<kThreads>(pos, meta, 0, n, i, -1, true, s_pairs + ti * nt, s_coul, S.terms, b, nx_, ny_, nz_,
                                        po.x, po.y, po.z, po.w, e_new, e_old);
             }
         } else {
@@ -372,6 +372,7 @@ Batch::~Batch() {
     (void)hipFree(split_slots_);
     (void)hipFree(pipe_slots_);
     (void)hipFree(split_err_);
+    (void)hipFree(d_map_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -501,6 +502,8 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
         check_hip(hipMemcpy(d_p32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
     }
     s.pos32 = d_p32;
+    s.parts = 1;
+    s.gslots = nullptr;
     const DevCoulomb& dc = e.dev_coulomb();
     if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
     s.terms = 0;
@@ -663,7 +666,12 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
     TraceEntry* tr = trace ? d_trace_ : nullptr;
     const int cap = trace ? trace_cap : 0;
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
-    if (general_ == 1) launch_general(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
+    if (general_ == 1) {
+        // one workgroup per replica, also for replicas that earlier ran split
+        std::vector<ReplicaState> one(h_states_);
+        for (auto& s : one) s.parts = 1;
+        launch_general(one.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
+    }
     else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }

@@ -82,6 +82,9 @@ struct ReplicaState {
     const int* excl_off;      // per-particle excluded partners, or null
     const int* excl;
     const float4* stab;       // FP32 forms: cubic table of S(q) (kSTable intervals), or null
+    int parts;                // workgroups sharing this replica in a split general launch (1 otherwise)
+    int pad2;
+    void* gslots;             // split general launch: 2 x parts exchange slots
 };
 
 // One recorded move of the trace (tests).
@@ -99,9 +102,19 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
 // One persistent launch advancing `count` replicas (states already on the device).
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream);
-// Same for replicas with a general move set (molecular moves, bonds).
+// Same for replicas with a general move set (molecular moves, bonds).  Replicas with
+// parts > 1 are shared by that many workgroups: the launch is then cooperative, its
+// grid is the sum of the parts and block b advances part map[b].y of replica map[b].x
+// (d_blockmap, as laid out by general_blockmap, on the device).
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream);
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap = nullptr);
+// Share a replica among `parts` workgroups of later general launches (1 undoes it);
+// allocates its exchange slots into `owned`.
+void set_replica_parts(ReplicaState& s, int parts, std::vector<void*>& owned);
+// Host block map of a general launch over these replicas; returns the grid size.
+int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map);
+// Workgroups of the split general kernel that fit on the device at once.
+int general_coresident_blocks(bool f32);
 
 // A set of replicas advanced together on one stream.
 class Batch {
@@ -114,7 +127,8 @@ class Batch {
     void finalize();
     // advance every replica by `moves[r]` moves (one launch); trace optional
     void run(const std::vector<long long>& moves, bool trace, int trace_cap);
-    // one replica spread over `groups` workgroups (every `stride`-th block of the grid)
+    // one replica spread over `groups` workgroups (every `stride`-th block of the grid;
+    // replicas with a general move set take stride 1)
     void run_split(long long moves, int groups, int stride, bool trace, int trace_cap);
     int split_error();
     // one replica over `groups` workgroups with up to `depth` moves in flight
@@ -155,6 +169,11 @@ class Batch {
     void* split_slots_ = nullptr;
     void* pipe_slots_ = nullptr;
     int* split_err_ = nullptr;
+    int2* d_map_ = nullptr; // block map of a split general launch
+    int map_cap_ = 0;
+    std::vector<int2> h_map_;
+    long long h_split_moves_ = 0;
+    void run_split_general(long long moves, int groups, bool trace, int trace_cap);
 };
 
 } // namespace fmc

@@ -99,16 +99,16 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
 }
 
 // Pair sums of one moved particle, at its new (a) and old (b) position, against every
-// particle j in [0, n) except `skip`; pairs inside group g count only if same_ok.
+// particle j in [jb, n) except `skip`; pairs inside group g count only if same_ok.
 // Lanes stride over j; each lane loads kU neighbours before evaluating them.
 template <int kThreads>
-__device__ __forceinline__ void pair_row_f32(const float4* __restrict__ pos32, const uint32_t* __restrict__ meta, int n,
-                                             int skip, int g, bool same_ok, const DevPairF* row, const CoulF& c,
+__device__ __forceinline__ void pair_row_f32(const float4* __restrict__ pos32, const uint32_t* __restrict__ meta, int jb,
+                                             int n, int skip, int g, bool same_ok, const DevPairF* row, const CoulF& c,
                                              const float4* tab, int terms, float L, float invL, float ax, float ay,
                                              float az, float bx, float by, float bz, float qa, double& e_new,
                                              double& e_old) {
     constexpr int kU = 4;
-    for (int j0 = threadIdx.x; j0 < n; j0 += kThreads * kU) {
+    for (int j0 = jb + static_cast<int>(threadIdx.x); j0 < n; j0 += kThreads * kU) {
         float4 pj[kU];
         uint32_t mj[kU];
 #pragma unroll
@@ -136,12 +136,13 @@ __device__ __forceinline__ void pair_row_f32(const float4* __restrict__ pos32, c
 }
 
 template <int kThreads>
-__device__ __forceinline__ void pair_row_f64(const double4* __restrict__ pos, const uint32_t* __restrict__ meta, int n,
-                                             int skip, int g, bool same_ok, const DevPair* row, const DevCoulomb& c,
-                                             int terms, const Box& box, double ax, double ay, double az, double bx,
-                                             double by, double bz, double qa, double& e_new, double& e_old) {
+__device__ __forceinline__ void pair_row_f64(const double4* __restrict__ pos, const uint32_t* __restrict__ meta, int jb,
+                                             int n, int skip, int g, bool same_ok, const DevPair* row,
+                                             const DevCoulomb& c, int terms, const Box& box, double ax, double ay,
+                                             double az, double bx, double by, double bz, double qa, double& e_new,
+                                             double& e_old) {
     constexpr int kU = 4;
-    for (int j0 = threadIdx.x; j0 < n; j0 += kThreads * kU) {
+    for (int j0 = jb + static_cast<int>(threadIdx.x); j0 < n; j0 += kThreads * kU) {
         double4 pj[kU];
         uint32_t mj[kU];
 #pragma unroll

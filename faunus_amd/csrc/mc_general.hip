@@ -5,6 +5,16 @@
 // MoveCollection::sample + performMove loop, move.cpp:513-525 and
 // montecarlo.cpp:139-187, with the proposals of move.cpp:225-345 (atoms),
 // move.cpp:1618-1689 (molecules) and chainmove.cpp:76-104, 216-268 (pivot)).
+//
+// Split form: a large replica is shared by `parts` workgroups.  Part p owns the p-th
+// slice of the k-vectors (its Q(k) stays in registers) and the p-th slice of the
+// static particles of the pair sums.  Every part draws the same proposal, publishes
+// its five partial sums in an exchange slot (agent-scope release store of the move's
+// tag), waits for the other parts' slots (acquire loads, bounded spin), adds the
+// partials in part order and so takes the same decision as every other part.  Every
+// part writes accepted coordinates and mass centres itself, so a part only ever reads
+// particle data it wrote; the slot exchange orders each part's writes of one move
+// before any part's writes of a later move.  The launch is cooperative.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +33,14 @@ namespace {
 constexpr int kMaxKinds = 8;   // move kinds per replica
 constexpr int kChunk = 8;      // moved atoms per Ewald phase-table chunk
 constexpr int kMaxComb = 512;  // (moved atom, state, cell) combinations of the flattened stencil walk
+constexpr int kMaxParts = 64;  // workgroups per replica in the split form (one waiting lane each)
+
+// Exchange slot of one part for one move parity: the partial sums (pair new/old,
+// Ewald, bonds new/old) and the tag of the move they belong to (move index + 1).
+struct alignas(64) GeneralSlot {
+    double v[5];
+    unsigned long long seq;
+};
 
 __device__ __forceinline__ int pick(uint32_t x, int n) {
     return static_cast<int>((static_cast<uint64_t>(x) * static_cast<uint64_t>(n)) >> 32);
@@ -65,13 +83,20 @@ struct GeneralMove {
     double accept_u;
 };
 
-template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
+template <int kThreads, int kMaxKPT, bool kCells, bool kF32, bool kSplit>
 __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __restrict__ states,
                                                                const long long* __restrict__ moves,
-                                                               TraceEntry* __restrict__ trace, int trace_cap) {
-    ReplicaState& S = states[blockIdx.x];
-    const long long M = moves[blockIdx.x];
+                                                               TraceEntry* __restrict__ trace, int trace_cap,
+                                                               const int2* __restrict__ blockmap,
+                                                               long long spin_limit) {
+    static_assert(!(kSplit && kCells), "the split form walks all particles");
+    // block -> (replica, part); without split every block is part 0 of its own replica
+    const int2 bm = kSplit ? blockmap[blockIdx.x] : make_int2(static_cast<int>(blockIdx.x), 0);
+    const int rep = bm.x, part = bm.y;
+    ReplicaState& S = states[rep];
+    const long long M = moves[rep];
     if (M <= 0 || S.error) return;
+    const int G = kSplit ? max(1, S.parts) : 1; // replicas of one part run unsplit inside a split launch
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
     __shared__ float4 s_stab[kF32 ? kSTable : 1];
@@ -80,7 +105,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ double s_dpa[kMaxTypes];
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
-    __shared__ int s_accept;
+    __shared__ double s_part[5][kSplit ? kMaxParts : 1];
+    __shared__ int s_accept, s_abort;
     __shared__ int s_ccell[kCells ? kMaxComb : 1];
     __shared__ int s_cpre[kCells ? kMaxComb + 1 : 1];
     __shared__ int s_wsum[kThreads / 64];
@@ -89,7 +115,15 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int tid = threadIdx.x;
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
     const int width = 2 * ncut + 1;
-    const int kpt = S.kpt;
+    // this part's k-vectors [kb, ke) (kpt consecutive ones per thread) and static
+    // particles [jb, je) of the pair sums
+    const int kw = kSplit ? (nk + G - 1) / G : nk;
+    const int kb = kSplit ? part * kw : 0;
+    const int ke = kSplit ? min(nk, kb + kw) : nk;
+    const int kpt = kSplit ? (kw + kThreads - 1) / kThreads : S.kpt;
+    const int jw = kSplit ? (n + G - 1) / G : n;
+    const int jb = kSplit ? min(n, part * jw) : 0;
+    const int je = kSplit ? min(n, jb + jw) : n;
     const double L = S.L, half = S.half, inv = S.inv;
     const int nk8 = (nk + 7) & ~7;
     int* s_kn = s_dyn;
@@ -108,7 +142,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                                     static_cast<float>(p.coul), 0.0f};
         }
     }
-    if (tid == 0) s_coul = *S.coul;
+    if (tid == 0) {
+        s_coul = *S.coul;
+        s_abort = 0;
+    }
     for (int i = tid; i < nt; i += kThreads) s_dpa[i] = S.dp_atom[i];
     for (int i = tid; i < S.nkinds; i += kThreads) s_kinds[i] = S.kinds[i];
     for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
@@ -121,10 +158,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const CoulF cf{s_coul.scheme, static_cast<float>(s_coul.lB), static_cast<float>(s_coul.rc),
                    static_cast<float>(s_coul.rc_inv)};
 
-    const int k0 = tid * kpt;
+    const int k0 = kb + tid * kpt;
     double2 q[kMaxKPT];
 #pragma unroll
-    for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < nk) ? S.q[k0 + s] : make_double2(0.0, 0.0);
+    for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < ke) ? S.q[k0 + s] : make_double2(0.0, 0.0);
 
     double4* __restrict__ pos = S.pos;
     float4* __restrict__ pos32 = S.pos32;
@@ -303,26 +340,26 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             const double4 A = s_new[0], B = s_old[0];
             const int ta = s_mt[0];
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, n, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab,
+                pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab,
                                        S.terms, static_cast<float>(L), static_cast<float>(inv),
                                        static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        static_cast<float>(B.x), static_cast<float>(B.y), static_cast<float>(B.z),
                                        static_cast<float>(A.w), e_new, e_old);
             } else {
-                pair_row_f64<kThreads>(pos, meta, n, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, S.terms, box, A.x,
+                pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, S.terms, box, A.x,
                                        A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
             }
         } else if (m > 0) {
             if (!kCells) {
                 constexpr int kB = kF32 ? 8 : 4;
-                for (int j0 = tid; j0 < n; j0 += kThreads * kB) {
+                for (int j0 = jb + tid; j0 < je; j0 += kThreads * kB) {
                     uint32_t mj[kB];
                     double4 pj[kB];
                     float4 pf[kB];
 #pragma unroll
                     for (int u = 0; u < kB; ++u) {
                         const int j = j0 + u * kThreads;
-                        if (j < n) {
+                        if (j < je) {
                             mj[u] = meta[j];
                             if (kF32) pf[u] = pos32[j];
                             else pj[u] = pos[j];
@@ -331,7 +368,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 #pragma unroll
                     for (int u = 0; u < kB; ++u) {
                         const int j = j0 + u * kThreads;
-                        if (j >= n || (j >= P.mlo && j < P.mhi)) continue;
+                        if (j >= je || (j >= P.mlo && j < P.mhi)) continue;
                         const bool same = static_cast<int>(mj[u] >> 8) == g;
                         if (same && !same_ok) continue;
                         const int tj = static_cast<int>(mj[u] & 0xffu);
@@ -475,7 +512,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             }
             // moved x moved inside the group (internal changes of flexible groups)
             if (same_ok && m > 1) {
-                for (int idx = tid; idx < m * m; idx += kThreads) {
+                for (int idx = part * kThreads + tid; idx < m * m; idx += G * kThreads) {
                     const int a = idx / m, b = idx % m;
                     if (a >= b || pair_excluded(S.excl_off, S.excl, P.mlo + a, P.mlo + b)) continue;
                     const DevPair& p = s_pairs[s_mt[a] * nt + s_mt[b]];
@@ -488,7 +525,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 
         // ---- harmonic bonds with a moved atom (Bonded::internalGroupEnergy, energy.cpp:1068-1088)
         double b_new = 0.0, b_old = 0.0;
-        if (P.internal && S.bond_off != nullptr) {
+        if (P.internal && S.bond_off != nullptr && part == 0) {
             for (int a = tid; a < m; a += kThreads) {
                 const int ia = P.mlo + a;
                 for (int e = S.bond_off[ia]; e < S.bond_off[ia + 1]; ++e) {
@@ -528,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
                 const int k = k0 + s;
-                if (s < kpt && k < nk) {
+                if (s < kpt && k < ke) {
                     const int packed = s_kn[k];
                     const int nx = unpack_n(packed, 0) + ncut, ny = unpack_n(packed, 1) + ncut,
                               nz = unpack_n(packed, 2) + ncut;
@@ -549,7 +586,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 #pragma unroll
         for (int s = 0; s < kMaxKPT; ++s) {
             const int k = k0 + s;
-            if (s < kpt && k < nk) {
+            if (s < kpt && k < ke) {
                 const int packed = s_kn[k];
                 const int nx = unpack_n(packed, 0), ny = unpack_n(packed, 1), nz = unpack_n(packed, 2);
                 const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[nx * nx + ny * ny + nz * nz];
@@ -568,10 +605,53 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 for (int r = 0; r < 5; ++r) s_red[r][tid >> 6] = v[r];
         }
         __syncthreads();
+        if (kSplit && G > 1) {
+            // publish this part's partials, then gather every part's (lane p of wave 0
+            // waits for part p)
+            GeneralSlot* slots = static_cast<GeneralSlot*>(S.gslots);
+            const int parity = static_cast<int>(mvi & 1ull);
+            const unsigned long long tag = mvi + 1ull;
+            if (tid == 0) {
+                GeneralSlot* sl = slots + parity * G + part;
+                for (int r = 0; r < 5; ++r) {
+                    double x = 0.0;
+                    for (int w = 0; w < kThreads / 64; ++w) x += s_red[r][w];
+                    sl->v[r] = x;
+                }
+                __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (tid < 64) {
+                bool ok = true;
+                if (tid < G) {
+                    GeneralSlot* sl = slots + parity * G + tid;
+                    long long spins = 0;
+                    while (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+                        if (++spins > spin_limit) {
+                            ok = false;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (ok)
+                        for (int r = 0; r < 5; ++r) s_part[r][tid] = sl->v[r];
+                }
+                if (!ok) s_abort = 1;
+            }
+            __syncthreads();
+            if (s_abort) {
+                if (tid == 0) atomicExch(&S.error, 2); // a part did not arrive: the replica stops with an error
+                break;
+            }
+        }
         if (tid == 0) {
             double tot[5] = {0, 0, 0, 0, 0};
-            for (int w = 0; w < kThreads / 64; ++w)
-                for (int r = 0; r < 5; ++r) tot[r] += s_red[r][w];
+            if (kSplit && G > 1) {
+                for (int p = 0; p < G; ++p)
+                    for (int r = 0; r < 5; ++r) tot[r] += s_part[r][p];
+            } else {
+                for (int w = 0; w < kThreads / 64; ++w)
+                    for (int r = 0; r < 5; ++r) tot[r] += s_red[r][w];
+            }
             double du = 0.0;
             bool acc = true;
             if (m > 0) {
@@ -579,8 +659,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 acc = metropolis(beta * du, P.accept_u);
             }
             s_accept = acc ? 1 : 0;
-            if (trace && t < trace_cap) {
-                TraceEntry& T = trace[static_cast<size_t>(blockIdx.x) * trace_cap + t];
+            if (trace && t < trace_cap && part == 0) {
+                TraceEntry& T = trace[static_cast<size_t>(rep) * trace_cap + t];
                 T.index = m > 0 ? P.mlo : -1;
                 T.accepted = acc ? 1 : 0;
                 T.du = du;
@@ -632,20 +712,21 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                     pos32[P.mlo + a] = make_float4(static_cast<float>(A.x), static_cast<float>(A.y),
                                                    static_cast<float>(A.z), static_cast<float>(A.w));
             }
+            if (kSplit && G > 1) __threadfence(); // this move's writes complete before the next publish
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
                 q[s].x += dqr[s];
                 q[s].y += dqi[s];
             }
         }
-        if (S.error) break;
+        if (!kSplit && S.error) break;
         __syncthreads(); // moved coordinates and tables consumed before the next move
     }
 
 #pragma unroll
     for (int s = 0; s < kMaxKPT; ++s)
-        if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
-    if (tid == 0) {
+        if (s < kpt && k0 + s < ke) S.q[k0 + s] = q[s];
+    if (tid == 0 && part == 0) {
         S.moves = move0 + static_cast<unsigned long long>(M);
         S.accepted = accepted;
         S.energy = energy;
@@ -653,56 +734,129 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     }
 }
 
-template <int kMaxKPT, bool kCells, bool kF32>
-void launch_one(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap) {
-    auto fn = mc_general_kernel<256, kMaxKPT, kCells, kF32>;
-    static bool attr = false;
-    if (!attr) {
-        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      96 * 1024),
-                  "hipFuncSetAttribute");
-        attr = true;
-    }
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, s, m, tr, cap);
+template <int kMaxKPT, bool kCells, bool kF32, bool kSplit>
+const void* kernel_of() {
+    return reinterpret_cast<const void*>(mc_general_kernel<256, kMaxKPT, kCells, kF32, kSplit>);
 }
 
-template <int kMaxKPT>
-void dispatch(bool cells, bool f32, int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m,
-              TraceEntry* tr, int cap) {
-    if (cells) {
-        if (f32) launch_one<kMaxKPT, true, true>(grid, lds, st, s, m, tr, cap);
-        else launch_one<kMaxKPT, true, false>(grid, lds, st, s, m, tr, cap);
-    } else {
-        if (f32) launch_one<kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap);
-        else launch_one<kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap);
+const void* select_kernel(int maxkpt, bool cells, bool f32, bool split) {
+    if (maxkpt <= 8) {
+        if (split) return f32 ? kernel_of<8, false, true, true>() : kernel_of<8, false, false, true>();
+        if (cells) return f32 ? kernel_of<8, true, true, false>() : kernel_of<8, true, false, false>();
+        return f32 ? kernel_of<8, false, true, false>() : kernel_of<8, false, false, false>();
     }
+    if (split) return f32 ? kernel_of<16, false, true, true>() : kernel_of<16, false, false, true>();
+    if (cells) return f32 ? kernel_of<16, true, true, false>() : kernel_of<16, true, false, false>();
+    return f32 ? kernel_of<16, false, true, false>() : kernel_of<16, false, false, false>();
 }
 
-} // namespace
+constexpr size_t kGeneralLds = 96 * 1024; // dynamic LDS limit (static LDS stays below 64 KB)
 
-void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream) {
-    int maxkpt = 0, maxnk = 0, maxw = 0, maxm = 0;
+void prepare(const void* fn) {
+    static std::vector<const void*> done;
+    if (std::find(done.begin(), done.end(), fn) != done.end()) return;
+    check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kGeneralLds)),
+              "hipFuncSetAttribute");
+    done.push_back(fn);
+}
+
+// k-vectors per thread of a replica's part (its slice of ceil(nk / parts) k-vectors).
+int part_kpt(const ReplicaState& s) {
+    const int parts = s.parts > 1 ? s.parts : 1;
+    const int kw = (s.nk + parts - 1) / parts;
+    return (kw + 255) / 256;
+}
+
+struct GeneralLaunch {
+    int maxkpt;
+    bool cells, split;
+    size_t lds;
+    int grid;
+};
+
+GeneralLaunch plan(const ReplicaState* h_states, int count) {
+    GeneralLaunch g{0, h_states[0].cells != 0, false, 0, 0};
+    int maxnk = 0, maxw = 0, maxm = 0;
     for (int r = 0; r < count; ++r) {
         const ReplicaState& s = h_states[r];
         if (s.nkinds < 1 || s.nkinds > kMaxKinds) throw std::runtime_error("general move set: 1 to 8 move kinds");
         if (s.cells != h_states[0].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
         if (s.ntypes > kMaxTypes) throw std::runtime_error("too many atom types");
-        maxkpt = std::max(maxkpt, s.kpt);
+        if (s.parts > 1) {
+            if (s.parts > kMaxParts) throw std::runtime_error("split general launch: at most 64 workgroups per replica");
+            if (s.cells) throw std::runtime_error("split general launch: replicas without a cell list only");
+            if (s.gslots == nullptr) throw std::runtime_error("split general launch: exchange slots missing");
+            g.split = true;
+        }
+        g.grid += s.parts > 1 ? s.parts : 1;
+        g.maxkpt = std::max(g.maxkpt, s.parts > 1 ? part_kpt(s) : s.kpt);
         maxnk = std::max(maxnk, s.nk);
         maxw = std::max(maxw, 2 * s.ncut + 1);
         maxm = std::max(maxm, s.max_moved);
     }
-    if (maxkpt > 16) throw std::runtime_error("too many k-vectors for the general device MC (max 16 per thread)");
-    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) +
-                       sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
-                       (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
-    if (lds > 96 * 1024) throw std::runtime_error("k-vector and molecule tables too large for LDS");
-    // each workgroup lays out its own replica's tables inside this allocation
-    const bool cells = h_states[0].cells != 0;
-    if (maxkpt <= 8) dispatch<8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
-    else dispatch<16>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
-    check_hip(hipGetLastError(), "mc_general_kernel launch");
+    if (g.maxkpt > 16) throw std::runtime_error("too many k-vectors for the general device MC (max 16 per thread)");
+    g.lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) + sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
+            (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
+    if (g.lds > kGeneralLds) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+    return g;
+}
+
+} // namespace
+
+int general_coresident_blocks(bool f32) {
+    const void* fn = select_kernel(16, false, f32, true);
+    prepare(fn);
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, kGeneralLds), "occupancy");
+    return per_cu * prop.multiProcessorCount;
+}
+
+void set_replica_parts(ReplicaState& s, int parts, std::vector<void*>& owned) {
+    if (parts < 1 || parts > kMaxParts) throw std::runtime_error("split general launch: 1 to 64 workgroups per replica");
+    if (parts > 1 && s.cells) throw std::runtime_error("split general launch: replicas without a cell list only");
+    s.parts = parts;
+    s.gslots = nullptr;
+    if (parts == 1) return;
+    if (part_kpt(s) > 16) throw std::runtime_error("too many k-vectors per workgroup for the split general launch");
+    void* p = nullptr;
+    const size_t bytes = sizeof(GeneralSlot) * 2 * static_cast<size_t>(parts);
+    check_hip(hipMalloc(&p, bytes), "hipMalloc exchange slots");
+    owned.push_back(p);
+    // tags are move indices + 1, so zeroed slots never match a move
+    check_hip(hipMemset(p, 0, bytes), "clear exchange slots");
+    s.gslots = p;
+}
+
+int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map) {
+    map.clear();
+    for (int r = 0; r < count; ++r) {
+        const int parts = h_states[r].parts > 1 ? h_states[r].parts : 1;
+        for (int p = 0; p < parts; ++p) map.push_back(make_int2(r, p));
+    }
+    return static_cast<int>(map.size());
+}
+
+void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap) {
+    const GeneralLaunch g = plan(h_states, count);
+    const void* fn = select_kernel(g.maxkpt, g.cells, f32, g.split);
+    prepare(fn);
+    // each workgroup lays out its own replica's tables inside the dynamic LDS
+    long long spin = 1LL << 24;
+    if (!g.split) {
+        const int2* none = nullptr;
+        void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &none, &spin};
+        check_hip(hipLaunchKernel(fn, dim3(count), dim3(256), args, g.lds, stream), "mc_general_kernel launch");
+        return;
+    }
+    if (d_blockmap == nullptr) throw std::runtime_error("split general launch: block map missing");
+    if (g.grid > general_coresident_blocks(f32)) throw std::runtime_error("split general launch: the grid cannot be co-resident");
+    void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin};
+    check_hip(hipLaunchCooperativeKernel(fn, dim3(g.grid), dim3(256), args, static_cast<unsigned>(g.lds), stream),
+              "mc_general_kernel cooperative launch");
 }
 
 } // namespace fmc

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <stdexcept>
 
@@ -327,7 +328,11 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
 void Batch::run_split(long long moves, int groups, int stride, bool trace, int trace_cap) {
     if (!finalized_) finalize();
     if (size() != 1) throw std::runtime_error("run_split advances a batch of one replica");
-    if (general_ == 1) throw std::runtime_error("run_split handles translate-only replicas");
+    if (general_ == 1) {
+        if (stride != 1) throw std::runtime_error("run_split: replicas with a general move set take stride 1");
+        run_split_general(moves, groups, trace, trace_cap);
+        return;
+    }
     const ReplicaState& s = h_states_[0];
     if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error("run_split: 1..64 workgroups");
     if (stride < 1) throw std::runtime_error("run_split: stride >= 1");
@@ -371,7 +376,47 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 
+// The replica's general-move kernel over `groups` workgroups (cooperative launch).
+void Batch::run_split_general(long long moves, int groups, bool trace, int trace_cap) {
+    ReplicaState& h = h_states_[0];
+    if (h.parts != groups || (groups > 1 && h.gslots == nullptr)) {
+        check_hip(hipStreamSynchronize(stream_), "batch sync");
+        set_replica_parts(h, groups, owned_);
+        // the device copy holds the chain's dynamic state: update only the split fields
+        const size_t off = offsetof(ReplicaState, parts);
+        const size_t len = offsetof(ReplicaState, gslots) + sizeof(void*) - off;
+        check_hip(hipMemcpy(reinterpret_cast<char*>(d_states_) + off, reinterpret_cast<const char*>(&h) + off, len,
+                            hipMemcpyHostToDevice),
+                  "copy split fields");
+    }
+    const int grid = general_blockmap(h_states_.data(), 1, h_map_);
+    if (grid > map_cap_) {
+        check_hip(hipStreamSynchronize(stream_), "batch sync");
+        (void)hipFree(d_map_);
+        d_map_ = nullptr;
+        check_hip(hipMalloc(&d_map_, sizeof(int2) * static_cast<size_t>(grid)), "hipMalloc block map");
+        map_cap_ = grid;
+    }
+    if (trace && trace_cap > trace_cap_) {
+        check_hip(hipStreamSynchronize(stream_), "batch sync");
+        (void)hipFree(d_trace_);
+        check_hip(hipMalloc(&d_trace_, sizeof(TraceEntry) * static_cast<size_t>(trace_cap)), "hipMalloc trace");
+        trace_cap_ = trace_cap;
+    }
+    check_hip(hipStreamSynchronize(stream_), "batch sync"); // the host copies below are read by the stream
+    h_split_moves_ = moves;
+    check_hip(hipMemcpyAsync(d_map_, h_map_.data(), sizeof(int2) * static_cast<size_t>(grid), hipMemcpyHostToDevice,
+                             stream_),
+              "copy block map");
+    check_hip(hipMemcpyAsync(d_moves_, &h_split_moves_, sizeof(long long), hipMemcpyHostToDevice, stream_), "copy moves");
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    launch_general(h_states_.data(), 1, d_states_, d_moves_, fp32_ != 0, trace ? d_trace_ : nullptr,
+                   trace ? trace_cap : 0, stream_, groups > 1 ? d_map_ : nullptr);
+    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
 int Batch::split_error() {
+    if (general_ == 1) return state(0).error;
     if (!split_err_) return 0;
     int e = 0;
     check_hip(hipStreamSynchronize(stream_), "sync");

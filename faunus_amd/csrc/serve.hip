@@ -23,7 +23,25 @@ Server::~Server() {
         for (void* p : s.owned) (void)hipFree(p);
     (void)hipFree(d_states_);
     (void)hipFree(d_moves_);
+    (void)hipFree(d_map_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Server::set_split(int atoms_per_part, int max_parts) {
+    if (atoms_per_part < 0) throw std::runtime_error("server: atoms per workgroup must be >= 0");
+    if (max_parts < 1 || max_parts > 64) throw std::runtime_error("server: 1 to 64 workgroups per job");
+    if (!active_.empty() || !queue_.empty()) throw std::runtime_error("server: set the split policy before submitting jobs");
+    split_atoms_ = atoms_per_part;
+    max_parts_ = max_parts;
+}
+
+// Workgroups of a job: from its size alone, so that its chain does not depend on the mix.
+int Server::parts_for(const ReplicaState& s) const {
+    if (s.nkinds == 0 || s.cells) return 1;
+    int parts = 1;
+    if (split_atoms_ > 0) parts = std::min(max_parts_, (s.n + split_atoms_ - 1) / split_atoms_);
+    const int kmax_parts = (s.nk + 16 * 256 - 1) / (16 * 256); // at most 16 k-vectors per thread
+    return std::max(std::max(parts, kmax_parts), 1);
 }
 
 double Server::now() const {
@@ -94,7 +112,14 @@ void Server::admit() {
             s.state = make_replica_state(*s.engine, fp32_ != 0, j.seed, 0, 1.0, -1, s.owned,
                                          s.general ? &j.kinds : nullptr);
             s.state.kpt = (s.state.nk + 255) / 256;
-            if (s.state.kpt > (s.general ? 16 : 68)) throw std::runtime_error("too many k-vectors for the device MC");
+            s.parts = s.general ? parts_for(s.state) : 1;
+            if (s.parts > 1) {
+                if (coresident_ == 0) coresident_ = general_coresident_blocks(fp32_ != 0);
+                if (s.parts > coresident_) throw std::runtime_error("job needs more workgroups than fit on the device");
+                set_replica_parts(s.state, s.parts, s.owned);
+            } else if (s.state.kpt > (s.general ? 16 : 68)) {
+                throw std::runtime_error("too many k-vectors for the device MC");
+            }
         } catch (const std::exception& e) {
             for (void* p : s.owned) (void)hipFree(p);
             j.state = 3;
@@ -102,7 +127,9 @@ void Server::admit() {
             j.t_done = now();
             continue;
         }
-        s.cost = static_cast<double>(s.state.n) + 2.0 * s.state.nk;
+        // per-move time scale: the pair and Ewald work of one workgroup, plus the
+        // exchange between workgroups of a split job
+        s.cost = (static_cast<double>(s.state.n) + 2.0 * s.state.nk) / s.parts + (s.parts > 1 ? 1500.0 : 0.0);
         j.state = 1;
         j.t_admit = now();
         j.config.clear();
@@ -124,7 +151,8 @@ void Server::retire(size_t k) {
     s.engine->refresh_host_positions();
     s.engine->get_positions(j.final_xyz.data());
     j.state = s.state.error ? 3 : 2;
-    if (s.state.error) j.error = "device error (cell capacity)";
+    if (s.state.error == 2) j.error = "device error (a workgroup of the split job did not arrive)";
+    else if (s.state.error) j.error = "device error (cell capacity)";
     j.t_done = now();
     for (void* p : s.owned) (void)hipFree(p);
     used_ -= j.footprint;
@@ -148,9 +176,13 @@ void Server::step(long long quantum) {
     std::vector<long long> moves(static_cast<size_t>(count));
     std::vector<int> order;
     auto launch_class = [&](const Slot& s) { return (s.state.cells != 0 ? 1 : 0) + (s.general ? 2 : 0); };
+    // split jobs first within their class, so that they lead the cooperative launches
     for (int pass = 0; pass < 4; ++pass)
-        for (int k = 0; k < count; ++k)
-            if (launch_class(active_[static_cast<size_t>(k)]) == pass) order.push_back(k);
+        for (int split = 1; split >= 0; --split)
+            for (int k = 0; k < count; ++k) {
+                const Slot& s = active_[static_cast<size_t>(k)];
+                if (launch_class(s) == pass && (s.parts > 1 ? 1 : 0) == split) order.push_back(k);
+            }
     for (int r = 0; r < count; ++r) {
         const Slot& s = active_[static_cast<size_t>(order[static_cast<size_t>(r)])];
         const JobRecord& j = *jobs_[static_cast<size_t>(s.job)];
@@ -163,18 +195,60 @@ void Server::step(long long quantum) {
               "copy states");
     check_hip(hipMemcpyAsync(d_moves_, moves.data(), sizeof(long long) * count, hipMemcpyHostToDevice, stream_),
               "copy moves");
+    // launches: [first, last) replicas, with their block map at map_off (-1: no split)
+    struct Launch {
+        int first, last, cls, map_off;
+    };
+    std::vector<Launch> launches;
+    h_map_.clear();
     int first = 0;
     while (first < count) {
         const int cls = launch_class(active_[static_cast<size_t>(order[static_cast<size_t>(first)])]);
         int last = first;
         while (last < count && launch_class(active_[static_cast<size_t>(order[static_cast<size_t>(last)])]) == cls) ++last;
-        if (cls >= 2)
-            launch_general(states.data() + first, last - first, d_states_ + first, d_moves_ + first, fp32_ != 0,
-                           nullptr, 0, stream_);
-        else
-            launch_advance(states.data() + first, last - first, d_states_ + first, d_moves_ + first, fp32_ != 0,
-                           nullptr, 0, stream_);
+        int a = first;
+        while (a < last) {
+            // a cooperative launch takes split jobs (and then single ones) while the
+            // grid stays co-resident; the rest go in one ordinary launch
+            int b = a;
+            int map_off = -1;
+            if (states[static_cast<size_t>(a)].parts > 1) {
+                int blocks = 0;
+                while (b < last) {
+                    const int p = std::max(1, states[static_cast<size_t>(b)].parts);
+                    if (blocks + p > coresident_) break;
+                    blocks += p;
+                    ++b;
+                }
+                std::vector<int2> map;
+                general_blockmap(states.data() + a, b - a, map);
+                map_off = static_cast<int>(h_map_.size());
+                h_map_.insert(h_map_.end(), map.begin(), map.end());
+            } else {
+                b = last;
+            }
+            launches.push_back(Launch{a, b, cls, map_off});
+            a = b;
+        }
         first = last;
+    }
+    if (!h_map_.empty()) {
+        if (static_cast<int>(h_map_.size()) > map_cap_) {
+            (void)hipFree(d_map_);
+            d_map_ = nullptr;
+            map_cap_ = std::max(static_cast<int>(h_map_.size()), 2 * map_cap_);
+            check_hip(hipMalloc(&d_map_, sizeof(int2) * static_cast<size_t>(map_cap_)), "hipMalloc block map");
+        }
+        check_hip(hipMemcpyAsync(d_map_, h_map_.data(), sizeof(int2) * h_map_.size(), hipMemcpyHostToDevice, stream_),
+                  "copy block map");
+    }
+    for (const Launch& l : launches) {
+        if (l.cls >= 2)
+            launch_general(states.data() + l.first, l.last - l.first, d_states_ + l.first, d_moves_ + l.first,
+                           fp32_ != 0, nullptr, 0, stream_, l.map_off >= 0 ? d_map_ + l.map_off : nullptr);
+        else
+            launch_advance(states.data() + l.first, l.last - l.first, d_states_ + l.first, d_moves_ + l.first,
+                           fp32_ != 0, nullptr, 0, stream_);
     }
     check_hip(hipMemcpyAsync(states.data(), d_states_, sizeof(ReplicaState) * count, hipMemcpyDeviceToHost, stream_),
               "copy states back");

@@ -4,7 +4,10 @@
 // persistent launch (one workgroup per job), and finished jobs leave the batch so
 // queued jobs take their slots in the next round.  A job's Markov chain depends only
 // on its own Philox stream (keyed by its seed, with replica counter 0), so its result
-// is the same alone and in any mix.
+// is the same alone and in any mix.  Large jobs with a general move set can be shared
+// by several workgroups (the split form of the general kernel); their number of
+// workgroups is a function of the job alone (its particle and k-vector counts), never
+// of the mix, so the equality holds for them too.
 #pragma once
 #include <chrono>
 #include <deque>
@@ -45,6 +48,10 @@ class Server {
     // One scheduling round: admit, advance every resident job by up to `quantum` moves
     // (scaled by per-move cost so rounds take similar time per job), retire finished jobs.
     void step(long long quantum);
+    // Jobs with a general move set and no cell list get ceil(n / atoms_per_part)
+    // workgroups (at most max_parts, at least enough for 16 k-vectors per thread);
+    // atoms_per_part 0 keeps one workgroup per job.
+    void set_split(int atoms_per_part, int max_parts);
     int queued() const { return static_cast<int>(queue_.size()); }
     int resident() const { return static_cast<int>(active_.size()); }
     const JobRecord& job(int id) const { return *jobs_.at(static_cast<size_t>(id)); }
@@ -56,6 +63,7 @@ class Server {
     struct Slot {
         int job;
         bool general;
+        int parts;
         std::unique_ptr<Engine> engine;
         ReplicaState state;
         std::vector<void*> owned;
@@ -73,9 +81,15 @@ class Server {
     ReplicaState* d_states_ = nullptr;
     long long* d_moves_ = nullptr;
     int capacity_ = 0;
+    int split_atoms_ = 0, max_parts_ = 16;
+    int coresident_ = 0;            // workgroups of the split general kernel that fit at once
+    int2* d_map_ = nullptr;         // block maps of the split launches
+    int map_cap_ = 0;
+    std::vector<int2> h_map_;
     std::chrono::steady_clock::time_point t0_;
 
     void admit();
+    int parts_for(const ReplicaState& s) const;
     void retire(size_t k);
     static size_t estimate_footprint(const Model& m, int n);
 };

@@ -209,7 +209,9 @@ int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
  * takes part, so with stride = number of XCDs the participants share one XCD).
  * The workgroups exchange per-move partial sums through device memory; a wait that
  * exceeds its spin bound stops the run and sets the error read by
- * fmc_batch_split_error instead of hanging.
+ * fmc_batch_split_error instead of hanging.  A replica with a general move set
+ * (fmc_batch_add_replica_moves, no cell list, stride 1, 1..64 groups) runs the split
+ * form of the general kernel: each workgroup also takes a slice of the pair sums.
  */
 int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int stride, int trace_capacity);
 /*
@@ -251,6 +253,16 @@ int fmc_batch_check_q(fmc_batch* batch, int replica, double* max_abs_dq, double*
 typedef struct fmc_server fmc_server;
 int fmc_server_create(int fp32_pairs, long long hbm_budget_bytes, int max_slots, fmc_server** out);
 int fmc_server_destroy(fmc_server* server);
+/*
+ * Split policy (before the first submit): a job with a general move set and no cell
+ * list is shared by ceil(n / atoms_per_workgroup) workgroups, at most max_workgroups
+ * (1..64) and at least enough to hold its k-vectors at 16 per thread; 0 atoms keeps
+ * one workgroup per job.  The workgroups of a job split its k-vectors and pair sums
+ * and exchange partial sums every move in a cooperative launch.  The count depends on
+ * the job alone, so a job's chain stays the same in any mix (it differs from the
+ * one-workgroup chain only by the summation order of the partial sums).
+ */
+int fmc_server_set_split(fmc_server* server, int atoms_per_workgroup, int max_workgroups);
 int fmc_server_submit(fmc_server* server, const char* config_json, int n, const double* xyz, const double* charge,
                       const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,
                       long long moves, unsigned long long seed, int* job);
