@@ -131,6 +131,32 @@ def bench_split(sysd, args, groups, stride, steps=None):
                 groups=groups, stride=stride)
 
 
+def bench_general_split(n_beads=500, n_total=43000, moves=4000, parts=(1, 2, 4, 8, 16, 32, 64), fp32=True):
+    """One polyelectrolyte replica (the largest serving job: 500-bead chain with
+    counterions and salt, N=43k, Ewald kmax=8, ion and bead translations and pivots)
+    advanced by the split general kernel over 1..64 workgroups: kernel time per move."""
+    import faunus_amd as F
+    from tests import systems
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(n_beads, n_total, seed=5)
+    out = {}
+    for G in parts:
+        b = F.Batch(fp32_pairs=fp32)
+        b.add((cfg, xyz, q, ids, g), seed=77, moves=kinds)
+        b.set_timing(True)
+        b.run_split(200, groups=G)
+        b.sync()
+        b.run_split(moves, groups=G)
+        ms = b.sync()
+        err = b.split_error()
+        info = b.info(0)
+        b.close()
+        if err:
+            raise RuntimeError("split general run reported an error")
+        out[G] = dict(us_per_move=ms * 1e3 / moves, acceptance=info["accepted"] / max(1, info["moves"]))
+        log("general split N=%d parts=%d: %.2f us/move" % (len(q), G, out[G]["us_per_move"]))
+    return dict(n=len(q), fp32=fp32, moves=moves, per_parts=out)
+
+
 def bench_pipe(sysd, args, groups, depth, steps=None, fp32=False):
     """One replica over `groups` workgroups with up to `depth` moves in flight."""
     import faunus_amd as F
@@ -336,19 +362,22 @@ def serve_stream(n_jobs, seed=1, rate=50.0):
     return out
 
 
-def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
+def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1, split_atoms=2048, max_parts=16):
     """Continuous batching under a Poisson stream of polyelectrolyte jobs (chain of
     50-500 beads with harmonic bonds, counterions and 1:1 salt, N log-uniform in
     1k-50k, 5 mM monomer, hard spheres + Ewald alpha=5/L, kmax=8; ion and bead
     translations and pivots): submit each job at its arrival time, run scheduling
     rounds while anything is queued or resident, and report the per-job time from
-    arrival to completion of `sweeps` sweeps (N moves per sweep)."""
+    arrival to completion of `sweeps` sweeps (N moves per sweep).  Jobs above
+    `split_atoms` particles are shared by ceil(N / split_atoms) workgroups (at most
+    `max_parts`)."""
     import faunus_amd as F
     from tests import systems
     stream = serve_stream(n_jobs, seed)
     jobs = [systems.polyelectrolyte(nb, nt, seed=1000 + j) for j, (_, nb, nt) in enumerate(stream)]
     log("serve: %d jobs built, %d particles in total" % (n_jobs, sum(len(j[2]) for j in jobs)))
-    srv = F.Server(fp32_pairs=fp32, hbm_budget_bytes=200 << 30, max_slots=512)
+    srv = F.Server(fp32_pairs=fp32, hbm_budget_bytes=200 << 30, max_slots=512, split_atoms=split_atoms,
+                   max_parts=max_parts)
     t0 = srv.status()["seconds"]
     ids = {}
     nxt = 0
@@ -386,7 +415,7 @@ def bench_serve(n_jobs=48, sweeps=4, fp32=True, quantum=5000, seed=1):
     srv.close()
     lat = np.array(lat)
     ns = np.array([len(sysd[2]) for sysd in jobs])
-    return dict(jobs=n_jobs, sweeps_per_job=sweeps, p50_s=float(np.percentile(lat, 50)),
+    return dict(jobs=n_jobs, sweeps_per_job=sweeps, split_atoms=split_atoms, max_parts=max_parts, p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), moves_per_s=moves / (t_end - stream[0][0]),
                 wall_s=t_end - stream[0][0], rounds=rounds, peak_resident=peak,
                 n_min=int(ns.min()), n_median=int(np.median(ns)), n_max=int(ns.max()),
@@ -402,9 +431,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
     ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve",
+                                       "gsplit",
                                        "water"], default=None)
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-sweeps", type=int, default=2)
+    ap.add_argument("--serve-split-atoms", type=int, default=2048,
+                    help="particles per workgroup of a serving job (0: one workgroup per job)")
+    ap.add_argument("--serve-max-parts", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
@@ -467,8 +500,11 @@ def main():
         out["water_plain"] = bench_plain_water()
         log("plain FP64 per-move path on water: %.0f moves/s (kernel %.1f us)" % (
             out["water_plain"]["rate"], out["water_plain"]["kernel_us"]))
+    if args.only in (None, "gsplit"):
+        out["gsplit"] = bench_general_split()
     if args.only in (None, "serve"):
-        out["serve"] = bench_serve(n_jobs=args.serve_jobs, sweeps=args.serve_sweeps)
+        out["serve"] = bench_serve(n_jobs=args.serve_jobs, sweeps=args.serve_sweeps,
+                                   split_atoms=args.serve_split_atoms, max_parts=args.serve_max_parts)
         log("serving: p50 %.2f s, p99 %.2f s to %d sweeps, %.3e moves/s" % (
             out["serve"]["p50_s"], out["serve"]["p99_s"], out["serve"]["sweeps_per_job"], out["serve"]["moves_per_s"]))
     if args.only is not None:

@@ -105,14 +105,18 @@ void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_sta
 // Same for replicas with a general move set (molecular moves, bonds).  Replicas with
 // parts > 1 are shared by that many workgroups: the launch is then cooperative, its
 // grid is the sum of the parts and block b advances part map[b].y of replica map[b].x
-// (d_blockmap, as laid out by general_blockmap, on the device).
+// (d_blockmap, as laid out by general_blockmap, on the device; map_blocks its length,
+// 0 for one block per part).
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap = nullptr);
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap = nullptr,
+                    int map_blocks = 0);
 // Share a replica among `parts` workgroups of later general launches (1 undoes it);
 // allocates its exchange slots into `owned`.
 void set_replica_parts(ReplicaState& s, int parts, std::vector<void*>& owned);
-// Host block map of a general launch over these replicas; returns the grid size.
-int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map);
+// Host block map of a general launch over these replicas; returns the grid size.  With
+// stride > 1 the parts sit on every stride-th block (idle blocks between them), so that
+// with stride = the XCD count the parts of a replica share one XCD and its L2.
+int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map, int stride = 1);
 // Workgroups of the split general kernel that fit on the device at once.
 int general_coresident_blocks(bool f32);
 
@@ -127,8 +131,7 @@ class Batch {
     void finalize();
     // advance every replica by `moves[r]` moves (one launch); trace optional
     void run(const std::vector<long long>& moves, bool trace, int trace_cap);
-    // one replica spread over `groups` workgroups (every `stride`-th block of the grid;
-    // replicas with a general move set take stride 1)
+    // one replica spread over `groups` workgroups (every `stride`-th block of the grid)
     void run_split(long long moves, int groups, int stride, bool trace, int trace_cap);
     int split_error();
     // one replica over `groups` workgroups with up to `depth` moves in flight
@@ -173,7 +176,7 @@ class Batch {
     int map_cap_ = 0;
     std::vector<int2> h_map_;
     long long h_split_moves_ = 0;
-    void run_split_general(long long moves, int groups, bool trace, int trace_cap);
+    void run_split_general(long long moves, int groups, int stride, bool trace, int trace_cap);
 };
 
 } // namespace fmc

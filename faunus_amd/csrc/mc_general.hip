@@ -57,8 +57,7 @@ __device__ __forceinline__ void unit_vector(uint32_t zw, uint32_t pw, double* v)
 }
 
 // Rotation by `angle` about the unit vector u (Rodrigues' formula), row-major.
-__device__ __forceinline__ void rotation_matrix(const double* u, double angle, double* R) {
-    const double s = sin(angle), c = cos(angle);
+__device__ __forceinline__ void rotation_matrix_sc(const double* u, double s, double c, double* R) {
     const double sa0 = u[0] * s, sa1 = u[1] * s, sa2 = u[2] * s;
     const double ca0 = u[0] * (1.0 - c), ca1 = u[1] * (1.0 - c), ca2 = u[2] * (1.0 - c);
     R[0] = ca0 * u[0] + c;
@@ -72,6 +71,127 @@ __device__ __forceinline__ void rotation_matrix(const double* u, double angle, d
     R[8] = ca2 * u[2] + c;
 }
 
+__device__ __forceinline__ void rotation_matrix(const double* u, double angle, double* R) {
+    rotation_matrix_sc(u, sin(angle), cos(angle), R);
+}
+
+// Random part of one proposal: everything that does not depend on the current
+// coordinates (move kind, target group and atoms, displacement, rotation, acceptance
+// deviate).  Drawn kDraw moves ahead, one lane of wave 0 per move, so that the
+// Philox blocks, the transcendental functions and the table lookups of 64 proposals
+// overlap instead of forming one serial chain per move.
+constexpr int kDraw = 64;
+
+struct MoveDraw {
+    int kind;                // -1: nothing moves (m = 0)
+    int group, mlo, mhi;
+    int internal, flags;     // flags: 1 translate, 2 rotate (kind 1); 4 the moved atom has bonds (kind 0)
+    int atom0, atom1;        // kind 2: pivot origin and the bond partner
+    double accept_u, scale;
+    double dir[3];           // kind 0: unit direction (times scale at the trial); kind 1: displacement
+    double R[9];             // kind 1 rotation; kind 2: R[0], R[1] = sin, cos of the angle
+};
+
+// move.cpp:225-345 (atoms), move.cpp:1618-1689 (molecules), chainmove.cpp:216-268 (pivot)
+__device__ void draw_move(MoveDraw& D, unsigned long long mvi, uint32_t replica, unsigned long long seed,
+                          const MoveKindDev* kinds, int nkinds, unsigned long long total_weight,
+                          const int* __restrict__ kind_groups, const int* __restrict__ gbegin,
+                          const int* __restrict__ gsize, const int2* __restrict__ pivot_bonds,
+                          const uint32_t* __restrict__ meta, const double* dpa, const int* __restrict__ bond_off) {
+    const uint32_t c0 = static_cast<uint32_t>(mvi), c1 = static_cast<uint32_t>(mvi >> 32);
+    const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
+    const Philox4 ra = philox4x32_10(c0, c1, replica, 0u, key0, key1);
+    const Philox4 rb = philox4x32_10(c0, c1, replica, 1u, key0, key1);
+    const Philox4 rc = philox4x32_10(c0, c1, replica, 2u, key0, key1);
+    D.accept_u = (static_cast<double>(rb.v[0] >> 5) * 67108864.0 + static_cast<double>(rb.v[1] >> 6)) *
+                 1.1102230246251565e-16;
+    D.kind = -1;
+    D.group = -1;
+    D.mlo = D.mhi = 0;
+    D.internal = 0;
+    D.flags = 0;
+    D.atom0 = D.atom1 = 0;
+    D.scale = 0.0;
+    const uint64_t sel = (static_cast<uint64_t>(rb.v[2]) * total_weight) >> 32;
+    int k = 0;
+    while (k + 1 < nkinds && sel >= kinds[k].cum) ++k;
+    const MoveKindDev K = kinds[k];
+    if (K.kind == 0) {
+        const int g = kind_groups[K.group_first + (K.group_count == 1 ? 0 : pick(rc.v[3], K.group_count))];
+        D.group = g;
+        const int size = gsize[g];
+        if (size > 0) {
+            const int i = gbegin[g] + pick(ra.v[0], size);
+            const int ti = static_cast<int>(meta[i] & 0xffu);
+            const double step = dpa[ti] >= 0.0 ? dpa[ti] : K.dp;
+            if (step > 0.0) {
+                unit_vector(ra.v[1], ra.v[2], D.dir);
+                D.scale = step * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
+                D.kind = 0;
+                D.mlo = i;
+                D.mhi = i + 1;
+                D.internal = 1;
+                if (bond_off != nullptr && bond_off[i + 1] > bond_off[i]) D.flags = 4;
+            }
+        }
+    } else if (K.kind == 1) {
+        const int g = kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
+        D.group = g;
+        const int size = gsize[g];
+        if (size > 0) {
+            if (K.dp > 0.0) {
+                unit_vector(ra.v[1], ra.v[2], D.dir);
+                D.scale = K.dp * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
+                D.dir[0] *= D.scale;
+                D.dir[1] *= D.scale;
+                D.dir[2] *= D.scale;
+                D.flags |= 1;
+            }
+            if (K.dprot > DBL_EPSILON) {
+                double axis[3];
+                unit_vector(rc.v[0], rc.v[1], axis);
+                rotation_matrix(axis, K.dprot * (u01_from32(rc.v[2]) - 0.5), D.R);
+                D.flags |= 2;
+            }
+            if (D.flags) {
+                D.kind = 1;
+                D.mlo = gbegin[g];
+                D.mhi = D.mlo + size;
+            }
+        }
+    } else {
+        const int g = kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
+        D.group = g;
+        const int size = gsize[g], b0 = gbegin[g];
+        if (size > 2 && K.bond_count > 0 && fabs(K.dprot) > 1e-9) {
+            const int2 bd = pivot_bonds[K.bond_first + pick(rc.v[3], K.bond_count)];
+            int atom0 = b0 + bd.x, atom1 = b0 + bd.y;
+            int slo, shi;
+            if (u01_from32(rb.v[3]) > 0.5) {
+                slo = atom1 + 1;
+                shi = b0 + size;
+                const int tmp = atom0;
+                atom0 = atom1;
+                atom1 = tmp;
+            } else {
+                slo = b0;
+                shi = atom0;
+            }
+            if (shi > slo) {
+                const double angle = K.dprot * (u01_from32(rc.v[2]) - 0.5);
+                D.R[0] = sin(angle);
+                D.R[1] = cos(angle);
+                D.atom0 = atom0;
+                D.atom1 = atom1;
+                D.kind = 2;
+                D.mlo = slo;
+                D.mhi = shi;
+                D.internal = 1;
+            }
+        }
+    }
+}
+
 // Proposal of one move, identical on every lane of the workgroup.
 struct GeneralMove {
     int kind, group, mlo, mhi, internal;
@@ -83,6 +203,89 @@ struct GeneralMove {
     double accept_u;
 };
 
+// The atoms of one move: [mlo, mhi) in the moved group g, their new and old
+// coordinates and types in LDS; pairs inside g count only if same_ok, minus exclusions.
+struct MovedSet {
+    int mlo, mhi, m, g;
+    bool same_ok;
+    const double4* s_new;
+    const double4* s_old;
+    const int* s_mt;
+    const int* excl_off;
+    const int* excl;
+};
+
+// Pair sums of the moved atoms (new and old coordinates) against the particles
+// base + tid + u * kThreads (u < kB) below je: the kB neighbours sit in registers and
+// the moved atoms stream from LDS, kB independent pair terms per atom and state.
+template <int kThreads, int kB, bool kF32>
+__device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms, const uint32_t* __restrict__ meta,
+                                            const float4* __restrict__ pos32, const double4* __restrict__ pos,
+                                            const DevPairF* s_pairs_f, const DevPair* s_pairs, int nt, const CoulF& cf,
+                                            const float4* s_stab, const DevCoulomb& s_coul, int terms, const Box& box,
+                                            float fL, float fIL, double& e_new, double& e_old) {
+    const int j0 = base + static_cast<int>(threadIdx.x);
+    uint32_t mj[kB];
+    double4 pj[kB];
+    float4 pf[kB];
+    int use[kB]; // 0 no pair, 1 pair, 2 pair inside the moved group (exclusions apply)
+    int any = 0;
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+        const int j = j0 + u * kThreads;
+        const bool in = j < je && !(j >= ms.mlo && j < ms.mhi);
+        mj[u] = in ? meta[j] : 0u;
+        if (kF32) pf[u] = in ? pos32[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else pj[u] = in ? pos[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const bool same = static_cast<int>(mj[u] >> 8) == ms.g;
+        use[u] = !in ? 0 : (same ? (ms.same_ok ? 2 : 0) : 1);
+        any |= use[u];
+    }
+    if (!any) return;
+    for (int a = 0; a < ms.m; ++a) {
+        const double4 A = ms.s_new[a], B = ms.s_old[a];
+        const int ta = ms.s_mt[a];
+        if (kF32) {
+            const DevPairF* row = s_pairs_f + ta * nt;
+            const float ax = static_cast<float>(A.x), ay = static_cast<float>(A.y), az = static_cast<float>(A.z),
+                        qa = static_cast<float>(A.w);
+            const float bx = static_cast<float>(B.x), by = static_cast<float>(B.y), bz = static_cast<float>(B.z);
+            float sn = 0.0f, so = 0.0f;
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const bool ok = use[u] == 1 ||
+                                (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a, j0 + u * kThreads));
+                const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
+                const float un = pair_energy_ft(p, cf, s_stab, terms, qa, pf[u].w,
+                                                dist2f_round(fL, fIL, ax, ay, az, pf[u].x, pf[u].y, pf[u].z));
+                const float uo = pair_energy_ft(p, cf, s_stab, terms, qa, pf[u].w,
+                                                dist2f_round(fL, fIL, bx, by, bz, pf[u].x, pf[u].y, pf[u].z));
+                sn += ok ? un : 0.0f;
+                so += ok ? uo : 0.0f;
+            }
+            e_new += static_cast<double>(sn);
+            e_old += static_cast<double>(so);
+        } else {
+            const DevPair* row = s_pairs + ta * nt;
+            double sn = 0.0, so = 0.0;
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const bool ok = use[u] == 1 ||
+                                (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a, j0 + u * kThreads));
+                const DevPair& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
+                const double un =
+                    pair_energy_t(p, s_coul, A.w, pj[u].w, dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z), terms);
+                const double uo =
+                    pair_energy_t(p, s_coul, B.w, pj[u].w, dist2(box, B.x, B.y, B.z, pj[u].x, pj[u].y, pj[u].z), terms);
+                sn += ok ? un : 0.0;
+                so += ok ? uo : 0.0;
+            }
+            e_new += sn;
+            e_old += so;
+        }
+    }
+}
+
 template <int kThreads, int kMaxKPT, bool kCells, bool kF32, bool kSplit>
 __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __restrict__ states,
                                                                const long long* __restrict__ moves,
@@ -93,6 +296,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     // block -> (replica, part); without split every block is part 0 of its own replica
     const int2 bm = kSplit ? blockmap[blockIdx.x] : make_int2(static_cast<int>(blockIdx.x), 0);
     const int rep = bm.x, part = bm.y;
+    if (kSplit && rep < 0) return; // an idle block of a placement map
     ReplicaState& S = states[rep];
     const long long M = moves[rep];
     if (M <= 0 || S.error) return;
@@ -105,12 +309,14 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ double s_dpa[kMaxTypes];
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
-    __shared__ double s_part[5][kSplit ? kMaxParts : 1];
     __shared__ int s_accept, s_abort;
     __shared__ int s_ccell[kCells ? kMaxComb : 1];
     __shared__ int s_cpre[kCells ? kMaxComb + 1 : 1];
     __shared__ int s_wsum[kThreads / 64];
-    extern __shared__ int s_dyn[]; // kn | phase chunk tables | new | old coordinates | moved types
+    __shared__ MoveDraw s_draw[kDraw];
+    // kn | phase chunk tables | new coordinates (two buffers in the split form) | old
+    // coordinates | moved types
+    extern __shared__ int s_dyn[];
 
     const int tid = threadIdx.x;
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
@@ -128,9 +334,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int nk8 = (nk + 7) & ~7;
     int* s_kn = s_dyn;
     double2* s_phase = reinterpret_cast<double2*>(s_dyn + nk8);
-    double4* s_new = reinterpret_cast<double4*>(s_phase + kChunk * 6 * width);
-    double4* s_old = s_new + S.max_moved;
-    int* s_mt = reinterpret_cast<int*>(s_old + S.max_moved);
+    const int max_moved = S.max_moved;
+    double4* s_nbuf = reinterpret_cast<double4*>(s_phase + kChunk * 6 * width);
+    double4* s_old = s_nbuf + (kSplit ? 2 : 1) * max_moved;
+    int* s_mt = reinterpret_cast<int*>(s_old + max_moved);
 
     for (int i = tid; i < nt * nt; i += kThreads) {
         s_pairs[i] = S.pairs[i];
@@ -174,116 +381,90 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const uint32_t replica = static_cast<uint32_t>(S.replica_id);
     const unsigned long long seed = S.seed;
     const Box box{{L, L, L}, {half, half, half}};
+    const int nkinds = S.nkinds;
+    const unsigned long long total_weight = S.total_weight;
+    const int* __restrict__ kind_groups = S.kind_groups;
+    const int* __restrict__ gbegin = S.gbegin;
+    const int* __restrict__ gsize = S.gsize;
+    const int* __restrict__ gflags = S.gflags;
+    const int2* __restrict__ pivot_bonds = S.pivot_bonds;
+    double* __restrict__ gcm = S.gcm;
+    const int* __restrict__ bond_off = S.bond_off;
+    const DevBond* __restrict__ bonds = S.bonds;
+    const int* __restrict__ excl_off = S.excl_off;
+    const int* __restrict__ excl = S.excl;
+    const int terms = S.terms;
+    // split form: a part sees the coordinates (and mass centre) the previous move
+    // committed in other parts only after that move's successor has been exchanged, so
+    // reads of the previous move's accepted atoms take this part's own copy
+    int prev_lo = 0, prev_hi = 0, prev_cg = -1;
+    double prev_cm[3] = {0.0, 0.0, 0.0};
+    auto load_pos = [&](int idx, const double4* s_prev) -> double4 {
+        if (kSplit && idx >= prev_lo && idx < prev_hi) return s_prev[idx - prev_lo];
+        return pos[idx];
+    };
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
-        // ---- proposal (every lane computes the same values)
+        // ---- proposal: the drawn part, completed with the current coordinates
         const unsigned long long mvi = move0 + static_cast<unsigned long long>(t);
-        const uint32_t c0 = static_cast<uint32_t>(mvi), c1 = static_cast<uint32_t>(mvi >> 32);
-        const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
-        const Philox4 ra = philox4x32_10(c0, c1, replica, 0u, key0, key1);
-        const Philox4 rb = philox4x32_10(c0, c1, replica, 1u, key0, key1);
-        const Philox4 rc = philox4x32_10(c0, c1, replica, 2u, key0, key1);
+        const int slot = static_cast<int>(t % kDraw);
+        if (slot == 0) {
+            // the previous draws were last read before this move's barriers
+            if (tid < kDraw)
+                draw_move(s_draw[tid], mvi + static_cast<unsigned long long>(tid), replica, seed, s_kinds, nkinds,
+                          total_weight, kind_groups, gbegin, gsize, pivot_bonds, meta, s_dpa, bond_off);
+            __syncthreads();
+        }
+        double4* s_new = s_nbuf + (kSplit ? (t & 1) * max_moved : 0);
+        const double4* s_prev = s_nbuf + (kSplit ? ((t + 1) & 1) * max_moved : 0);
+        const MoveDraw& D = s_draw[slot];
         GeneralMove P;
-        P.accept_u = (static_cast<double>(rb.v[0] >> 5) * 67108864.0 + static_cast<double>(rb.v[1] >> 6)) *
-                     1.1102230246251565e-16;
-        {
-            const uint64_t sel = (static_cast<uint64_t>(rb.v[2]) * S.total_weight) >> 32;
-            int k = 0;
-            while (k + 1 < S.nkinds && sel >= s_kinds[k].cum) ++k;
-            const MoveKindDev K = s_kinds[k];
-            P.kind = K.kind;
-            P.group = -1;
-            P.mlo = P.mhi = 0;
-            P.internal = 0;
-            P.translate = P.rotate = false;
-            if (K.kind == 0) {
-                const int g = S.kind_groups[K.group_first + (K.group_count == 1 ? 0 : pick(rc.v[3], K.group_count))];
-                P.group = g;
-                const int size = S.gsize[g];
-                if (size > 0) {
-                    const int i = S.gbegin[g] + pick(ra.v[0], size);
-                    const int ti = static_cast<int>(meta[i] & 0xffu);
-                    const double step = s_dpa[ti] >= 0.0 ? s_dpa[ti] : K.dp;
-                    if (step > 0.0) {
-                        unit_vector(ra.v[1], ra.v[2], P.dir);
-                        P.scale = step * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
-                        P.mlo = i;
-                        P.mhi = i + 1;
-                        P.internal = 1;
-                    }
-                }
-            } else if (K.kind == 1) {
-                const int g = S.kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
-                P.group = g;
-                const int size = S.gsize[g];
-                if (size > 0) {
-                    const double* cm = S.gcm + 3 * g;
-                    P.cm_new[0] = cm[0];
-                    P.cm_new[1] = cm[1];
-                    P.cm_new[2] = cm[2];
-                    if (K.dp > 0.0) {
-                        unit_vector(ra.v[1], ra.v[2], P.dir);
-                        P.scale = K.dp * (static_cast<double>(ra.v[3]) * 2.3283064365386963e-10);
-                        P.dir[0] *= P.scale;
-                        P.dir[1] *= P.scale;
-                        P.dir[2] *= P.scale;
-                        P.translate = true;
-                        P.cm_new[0] = wrap(P.cm_new[0] + P.dir[0], L, half, inv);
-                        P.cm_new[1] = wrap(P.cm_new[1] + P.dir[1], L, half, inv);
-                        P.cm_new[2] = wrap(P.cm_new[2] + P.dir[2], L, half, inv);
-                    }
-                    if (K.dprot > DBL_EPSILON) {
-                        double axis[3];
-                        unit_vector(rc.v[0], rc.v[1], axis);
-                        rotation_matrix(axis, K.dprot * (u01_from32(rc.v[2]) - 0.5), P.R);
-                        P.rotate = true;
-                    }
-                    if (P.translate || P.rotate) {
-                        P.mlo = S.gbegin[g];
-                        P.mhi = P.mlo + size;
-                    }
-                }
+        P.accept_u = D.accept_u;
+        P.kind = D.kind;
+        P.group = D.group;
+        P.mlo = D.mlo;
+        P.mhi = D.mhi;
+        P.internal = D.internal;
+        P.translate = (D.flags & 1) != 0;
+        P.rotate = (D.flags & 2) != 0;
+        P.scale = D.scale;
+        P.dir[0] = D.dir[0];
+        P.dir[1] = D.dir[1];
+        P.dir[2] = D.dir[2];
+        if (P.kind == 1) {
+            const int g = P.group;
+            if (kSplit && g == prev_cg) {
+                P.cm_new[0] = prev_cm[0];
+                P.cm_new[1] = prev_cm[1];
+                P.cm_new[2] = prev_cm[2];
             } else {
-                const int g = S.kind_groups[K.group_first + pick(ra.v[0], K.group_count)];
-                P.group = g;
-                const int size = S.gsize[g], b0 = S.gbegin[g];
-                if (size > 2 && K.bond_count > 0 && fabs(K.dprot) > 1e-9) {
-                    const int2 bd = S.pivot_bonds[K.bond_first + pick(rc.v[3], K.bond_count)];
-                    int atom0 = b0 + bd.x, atom1 = b0 + bd.y;
-                    int slo, shi;
-                    if (u01_from32(rb.v[3]) > 0.5) {
-                        slo = atom1 + 1;
-                        shi = b0 + size;
-                        const int tmp = atom0;
-                        atom0 = atom1;
-                        atom1 = tmp;
-                    } else {
-                        slo = b0;
-                        shi = atom0;
-                    }
-                    if (shi > slo) {
-                        const double angle = K.dprot * (u01_from32(rc.v[2]) - 0.5);
-                        const double4 pa = pos[atom0], pb = pos[atom1];
-                        P.shift[0] = pa.x;
-                        P.shift[1] = pa.y;
-                        P.shift[2] = pa.z;
-                        P.origin[0] = wrap(pa.x - P.shift[0], L, half, inv);
-                        P.origin[1] = wrap(pa.y - P.shift[1], L, half, inv);
-                        P.origin[2] = wrap(pa.z - P.shift[2], L, half, inv);
-                        double v[3] = {P.origin[0] - wrap(pb.x - P.shift[0], L, half, inv),
-                                       P.origin[1] - wrap(pb.y - P.shift[1], L, half, inv),
-                                       P.origin[2] - wrap(pb.z - P.shift[2], L, half, inv)};
-                        for (int d = 0; d < 3; ++d) v[d] = min_image(v[d], L, half);
-                        const double norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                        const double u[3] = {v[0] / norm, v[1] / norm, v[2] / norm};
-                        rotation_matrix(u, angle, P.R);
-                        P.mlo = slo;
-                        P.mhi = shi;
-                        P.internal = 1;
-                    }
-                }
+                P.cm_new[0] = gcm[3 * g];
+                P.cm_new[1] = gcm[3 * g + 1];
+                P.cm_new[2] = gcm[3 * g + 2];
             }
+            if (P.translate) {
+                P.cm_new[0] = wrap(P.cm_new[0] + P.dir[0], L, half, inv);
+                P.cm_new[1] = wrap(P.cm_new[1] + P.dir[1], L, half, inv);
+                P.cm_new[2] = wrap(P.cm_new[2] + P.dir[2], L, half, inv);
+            }
+            if (P.rotate)
+                for (int e = 0; e < 9; ++e) P.R[e] = D.R[e];
+        } else if (P.kind == 2) {
+            const double4 pa = load_pos(D.atom0, s_prev), pb = load_pos(D.atom1, s_prev);
+            P.shift[0] = pa.x;
+            P.shift[1] = pa.y;
+            P.shift[2] = pa.z;
+            P.origin[0] = wrap(pa.x - P.shift[0], L, half, inv);
+            P.origin[1] = wrap(pa.y - P.shift[1], L, half, inv);
+            P.origin[2] = wrap(pa.z - P.shift[2], L, half, inv);
+            double v[3] = {P.origin[0] - wrap(pb.x - P.shift[0], L, half, inv),
+                           P.origin[1] - wrap(pb.y - P.shift[1], L, half, inv),
+                           P.origin[2] - wrap(pb.z - P.shift[2], L, half, inv)};
+            for (int d = 0; d < 3; ++d) v[d] = min_image(v[d], L, half);
+            const double norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const double u[3] = {v[0] / norm, v[1] / norm, v[2] / norm};
+            rotation_matrix_sc(u, D.R[0], D.R[1], P.R);
         }
         const int m = P.mhi - P.mlo;
         const int g = P.group;
@@ -291,7 +472,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         // ---- trial coordinates of the moved atoms
         for (int a = tid; a < m; a += kThreads) {
             const int idx = P.mlo + a;
-            const double4 p = pos[idx];
+            const double4 p = load_pos(idx, s_prev);
             s_old[a] = p;
             s_mt[a] = static_cast<int>(meta[idx] & 0xffu);
             double x = p.x, y = p.y, z = p.z;
@@ -333,66 +514,47 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 
         // ---- nonbonded pairs: moved x static (GroupPairing, energy.h:885-1027, 1155-1224)
         double e_new = 0.0, e_old = 0.0;
-        const int gflag = g >= 0 ? S.gflags[g] : 0;
+        const int gflag = g >= 0 ? gflags[g] : 0;
         const bool same_ok = P.internal && !(gflag & 2); // pairs inside the moved group
-        if (m == 1 && !kCells && S.excl_off == nullptr) {
+        if (m == 1 && !kCells && excl_off == nullptr) {
             // one moved atom: the row sums with the moved coordinates in registers
             const double4 A = s_new[0], B = s_old[0];
             const int ta = s_mt[0];
             if (kF32) {
                 pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab,
-                                       S.terms, static_cast<float>(L), static_cast<float>(inv),
+                                       terms, static_cast<float>(L), static_cast<float>(inv),
                                        static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        static_cast<float>(B.x), static_cast<float>(B.y), static_cast<float>(B.z),
                                        static_cast<float>(A.w), e_new, e_old);
             } else {
-                pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, S.terms, box, A.x,
+                pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, terms, box, A.x,
                                        A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
             }
         } else if (m > 0) {
             if (!kCells) {
-                constexpr int kB = kF32 ? 8 : 4;
-                for (int j0 = jb + tid; j0 < je; j0 += kThreads * kB) {
-                    uint32_t mj[kB];
-                    double4 pj[kB];
-                    float4 pf[kB];
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j < je) {
-                            mj[u] = meta[j];
-                            if (kF32) pf[u] = pos32[j];
-                            else pj[u] = pos[j];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kB; ++u) {
-                        const int j = j0 + u * kThreads;
-                        if (j >= je || (j >= P.mlo && j < P.mhi)) continue;
-                        const bool same = static_cast<int>(mj[u] >> 8) == g;
-                        if (same && !same_ok) continue;
-                        const int tj = static_cast<int>(mj[u] & 0xffu);
-                        for (int a = 0; a < m; ++a) {
-                            if (same && pair_excluded(S.excl_off, S.excl, P.mlo + a, j)) continue;
-                            const double4 A = s_new[a], B = s_old[a];
-                            const int ta = s_mt[a];
-                            if (kF32) {
-                                const DevPairF& p = s_pairs_f[ta * nt + tj];
-                                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                                e_new += static_cast<double>(pair_energy_ft(
-                                    p, cf, s_stab, S.terms, static_cast<float>(A.w), pf[u].w,
-                                    dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y),
-                                           static_cast<float>(A.z), pf[u].x, pf[u].y, pf[u].z)));
-                                e_old += static_cast<double>(pair_energy_ft(
-                                    p, cf, s_stab, S.terms, static_cast<float>(B.w), pf[u].w,
-                                    dist2f_round(fL, fIL, static_cast<float>(B.x), static_cast<float>(B.y),
-                                           static_cast<float>(B.z), pf[u].x, pf[u].y, pf[u].z)));
-                            } else {
-                                const DevPair& p = s_pairs[ta * nt + tj];
-                                e_new += pair_energy(p, s_coul, A.w, pj[u].w, dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
-                                e_old += pair_energy(p, s_coul, B.w, pj[u].w, dist2(box, B.x, B.y, B.z, pj[u].x, pj[u].y, pj[u].z));
-                            }
-                        }
+                // blocks of kB neighbours per lane (kB from 8 down to 1 as the slice
+                // runs out, so that few lanes sit idle on short slices)
+                const MovedSet ms{P.mlo, P.mhi, m, g, same_ok, s_new, s_old, s_mt, excl_off, excl};
+                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                int base = jb;
+                while (base < je) {
+                    const int rem = je - base;
+                    if (kF32 && rem >= 8 * kThreads) {
+                        moved_block<kThreads, 8, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
+                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
+                        base += 8 * kThreads;
+                    } else if (rem >= 4 * kThreads) {
+                        moved_block<kThreads, 4, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
+                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
+                        base += 4 * kThreads;
+                    } else if (rem > kThreads) {
+                        moved_block<kThreads, 2, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
+                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
+                        base += 2 * kThreads;
+                    } else {
+                        moved_block<kThreads, 1, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
+                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
+                        base += kThreads;
                     }
                 }
             } else if (m * 54 <= kMaxComb) {
@@ -455,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         if (j < 0 || (j >= P.mlo && j < P.mhi)) continue;
                         const int a = cc[u] / 54, st = (cc[u] / 27) & 1;
                         if (static_cast<int>(mj[u] >> 8) == g &&
-                            (!same_ok || pair_excluded(S.excl_off, S.excl, P.mlo + a, j)))
+                            (!same_ok || pair_excluded(excl_off, excl, P.mlo + a, j)))
                             continue;
                         const double4 A = st == 0 ? s_new[a] : s_old[a];
                         const int ta = s_mt[a], tj = static_cast<int>(mj[u] & 0xffu);
@@ -463,7 +625,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         if (kF32) {
                             const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
                             e = static_cast<double>(pair_energy_ft(
-                                s_pairs_f[ta * nt + tj], cf, s_stab, S.terms, static_cast<float>(A.w), pf[u].w,
+                                s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pf[u].w,
                                 dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        pf[u].x, pf[u].y, pf[u].z)));
                         } else {
@@ -490,7 +652,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         if (j >= P.mlo && j < P.mhi) continue;
                         const uint32_t mj = meta[j];
                         if (static_cast<int>(mj >> 8) == g &&
-                            (!same_ok || pair_excluded(S.excl_off, S.excl, P.mlo + a, j)))
+                            (!same_ok || pair_excluded(excl_off, excl, P.mlo + a, j)))
                             continue;
                         const int tj = static_cast<int>(mj & 0xffu);
                         double u;
@@ -498,7 +660,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             const float4 pj = pos32[j];
                             const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
                             u = static_cast<double>(pair_energy_ft(
-                                s_pairs_f[ta * nt + tj], cf, s_stab, S.terms, static_cast<float>(A.w), pj.w,
+                                s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pj.w,
                                 dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
                                        pj.x, pj.y, pj.z)));
                         } else {
@@ -514,7 +676,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (same_ok && m > 1) {
                 for (int idx = part * kThreads + tid; idx < m * m; idx += G * kThreads) {
                     const int a = idx / m, b = idx % m;
-                    if (a >= b || pair_excluded(S.excl_off, S.excl, P.mlo + a, P.mlo + b)) continue;
+                    if (a >= b || pair_excluded(excl_off, excl, P.mlo + a, P.mlo + b)) continue;
                     const DevPair& p = s_pairs[s_mt[a] * nt + s_mt[b]];
                     const double4 An = s_new[a], Bn = s_new[b], Ao = s_old[a], Bo = s_old[b];
                     e_new += pair_energy(p, s_coul, An.w, Bn.w, dist2(box, An.x, An.y, An.z, Bn.x, Bn.y, Bn.z));
@@ -525,14 +687,14 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
 
         // ---- harmonic bonds with a moved atom (Bonded::internalGroupEnergy, energy.cpp:1068-1088)
         double b_new = 0.0, b_old = 0.0;
-        if (P.internal && S.bond_off != nullptr && part == 0) {
+        if (P.internal && bond_off != nullptr && part == 0 && (P.kind != 0 || (D.flags & 4))) {
             for (int a = tid; a < m; a += kThreads) {
                 const int ia = P.mlo + a;
-                for (int e = S.bond_off[ia]; e < S.bond_off[ia + 1]; ++e) {
-                    const DevBond B = S.bonds[e];
+                for (int e = bond_off[ia]; e < bond_off[ia + 1]; ++e) {
+                    const DevBond B = bonds[e];
                     const bool om = B.other >= P.mlo && B.other < P.mhi;
                     if (om && B.other < ia) continue; // visited from the lower moved atom
-                    const double4 on = om ? s_new[B.other - P.mlo] : pos[B.other];
+                    const double4 on = om ? s_new[B.other - P.mlo] : load_pos(B.other, s_prev);
                     const double4 oo = om ? s_old[B.other - P.mlo] : on;
                     const double4 An = s_new[a], Ao = s_old[a];
                     const double xn = B.req - sqrt(dist2(box, An.x, An.y, An.z, on.x, on.y, on.z));
@@ -605,9 +767,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 for (int r = 0; r < 5; ++r) s_red[r][tid >> 6] = v[r];
         }
         __syncthreads();
+        double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0}; // totals, valid in thread 0
         if (kSplit && G > 1) {
             // publish this part's partials, then gather every part's (lane p of wave 0
-            // waits for part p)
+            // waits for part p) and add them with the same butterfly in every part
             GeneralSlot* slots = static_cast<GeneralSlot*>(S.gslots);
             const int parity = static_cast<int>(mvi & 1ull);
             const unsigned long long tag = mvi + 1ull;
@@ -621,37 +784,38 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (tid < 64) {
-                bool ok = true;
+                double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                int ok = 1;
                 if (tid < G) {
                     GeneralSlot* sl = slots + parity * G + tid;
                     long long spins = 0;
-                    while (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+                    // relaxed polling, then one acquire fence (an acquire load per poll
+                    // would invalidate the caches on every iteration)
+                    while (__hip_atomic_load(&sl->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != tag) {
                         if (++spins > spin_limit) {
-                            ok = false;
+                            ok = 0;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     if (ok)
-                        for (int r = 0; r < 5; ++r) s_part[r][tid] = sl->v[r];
+                        for (int r = 0; r < 5; ++r) v[r] = sl->v[r];
                 }
-                if (!ok) s_abort = 1;
+#pragma unroll
+                for (int r = 0; r < 5; ++r)
+                    for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                const int all_ok = __all(ok);
+                if (tid == 0) {
+                    for (int r = 0; r < 5; ++r) tot[r] = v[r];
+                    if (!all_ok) s_abort = 1; // a part did not arrive: the replica stops with an error
+                }
             }
-            __syncthreads();
-            if (s_abort) {
-                if (tid == 0) atomicExch(&S.error, 2); // a part did not arrive: the replica stops with an error
-                break;
-            }
+        } else if (tid == 0) {
+            for (int w = 0; w < kThreads / 64; ++w)
+                for (int r = 0; r < 5; ++r) tot[r] += s_red[r][w];
         }
         if (tid == 0) {
-            double tot[5] = {0, 0, 0, 0, 0};
-            if (kSplit && G > 1) {
-                for (int p = 0; p < G; ++p)
-                    for (int r = 0; r < 5; ++r) tot[r] += s_part[r][p];
-            } else {
-                for (int w = 0; w < kThreads / 64; ++w)
-                    for (int r = 0; r < 5; ++r) tot[r] += s_red[r][w];
-            }
             double du = 0.0;
             bool acc = true;
             if (m > 0) {
@@ -673,8 +837,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (acc && m > 0) {
                 energy += du;
                 sum_du += du;
-                if (P.kind == 1 && P.translate)
-                    for (int d = 0; d < 3; ++d) S.gcm[3 * g + d] = P.cm_new[d];
+                if (P.kind == 1 && P.translate && part == 0)
+                    for (int d = 0; d < 3; ++d) gcm[3 * g + d] = P.cm_new[d];
                 if (kCells) {
                     for (int a = 0; a < m && !S.error; ++a) {
                         const int i = P.mlo + a;
@@ -704,20 +868,36 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             }
         }
         __syncthreads();
-        if (s_accept && m > 0) {
+        if (kSplit && s_abort) {
+            if (tid == 0) atomicExch(&S.error, 2);
+            break;
+        }
+        const bool committed = s_accept && m > 0;
+        if (committed) {
+            // each particle is written by the part whose pair slice holds it (all of them
+            // without split); the next exchange orders these writes before later reads
             for (int a = tid; a < m; a += kThreads) {
+                const int idx = P.mlo + a;
+                if (kSplit && (idx < jb || idx >= je)) continue;
                 const double4 A = s_new[a];
-                pos[P.mlo + a] = A;
+                pos[idx] = A;
                 if (kF32)
-                    pos32[P.mlo + a] = make_float4(static_cast<float>(A.x), static_cast<float>(A.y),
-                                                   static_cast<float>(A.z), static_cast<float>(A.w));
+                    pos32[idx] = make_float4(static_cast<float>(A.x), static_cast<float>(A.y),
+                                             static_cast<float>(A.z), static_cast<float>(A.w));
             }
-            if (kSplit && G > 1) __threadfence(); // this move's writes complete before the next publish
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
                 q[s].x += dqr[s];
                 q[s].y += dqi[s];
             }
+        }
+        if (kSplit) {
+            prev_lo = committed ? P.mlo : 0;
+            prev_hi = committed ? P.mhi : 0;
+            prev_cg = (committed && P.kind == 1 && P.translate) ? g : -1;
+            prev_cm[0] = P.cm_new[0];
+            prev_cm[1] = P.cm_new[1];
+            prev_cm[2] = P.cm_new[2];
         }
         if (!kSplit && S.error) break;
         __syncthreads(); // moved coordinates and tables consumed before the next move
@@ -750,14 +930,21 @@ const void* select_kernel(int maxkpt, bool cells, bool f32, bool split) {
     return f32 ? kernel_of<16, false, true, false>() : kernel_of<16, false, false, false>();
 }
 
-constexpr size_t kGeneralLds = 96 * 1024; // dynamic LDS limit (static LDS stays below 64 KB)
+constexpr size_t kLdsPerCU = 160 * 1024;
 
-void prepare(const void* fn) {
-    static std::vector<const void*> done;
-    if (std::find(done.begin(), done.end(), fn) != done.end()) return;
-    check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kGeneralLds)),
+// Dynamic LDS a kernel may take: what its static LDS leaves of the CU's 160 KB, at
+// most 96 KB (set once per kernel).
+size_t prepare(const void* fn) {
+    static std::vector<std::pair<const void*, size_t>> done;
+    for (const auto& d : done)
+        if (d.first == fn) return d.second;
+    hipFuncAttributes attr;
+    check_hip(hipFuncGetAttributes(&attr, fn), "hipFuncGetAttributes");
+    const size_t limit = std::min<size_t>(96 * 1024, kLdsPerCU - attr.sharedSizeBytes);
+    check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(limit)),
               "hipFuncSetAttribute");
-    done.push_back(fn);
+    done.emplace_back(fn, limit);
+    return limit;
 }
 
 // k-vectors per thread of a replica's part (its slice of ceil(nk / parts) k-vectors).
@@ -795,9 +982,9 @@ GeneralLaunch plan(const ReplicaState* h_states, int count) {
         maxm = std::max(maxm, s.max_moved);
     }
     if (g.maxkpt > 16) throw std::runtime_error("too many k-vectors for the general device MC (max 16 per thread)");
+    // the split form keeps the previous move's coordinates in a second buffer
     g.lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) + sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
-            (2 * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
-    if (g.lds > kGeneralLds) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+            ((g.split ? 3 : 2) * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
     return g;
 }
 
@@ -805,12 +992,12 @@ GeneralLaunch plan(const ReplicaState* h_states, int count) {
 
 int general_coresident_blocks(bool f32) {
     const void* fn = select_kernel(16, false, f32, true);
-    prepare(fn);
+    const size_t lds = prepare(fn);
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, kGeneralLds), "occupancy");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds), "occupancy");
     return per_cu * prop.multiProcessorCount;
 }
 
@@ -830,20 +1017,24 @@ void set_replica_parts(ReplicaState& s, int parts, std::vector<void*>& owned) {
     s.gslots = p;
 }
 
-int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map) {
+int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>& map, int stride) {
+    if (stride < 1) throw std::runtime_error("block map: stride >= 1");
     map.clear();
     for (int r = 0; r < count; ++r) {
         const int parts = h_states[r].parts > 1 ? h_states[r].parts : 1;
-        for (int p = 0; p < parts; ++p) map.push_back(make_int2(r, p));
+        for (int p = 0; p < parts; ++p) {
+            map.push_back(make_int2(r, p));
+            for (int i = 1; i < stride; ++i) map.push_back(make_int2(-1, -1));
+        }
     }
     return static_cast<int>(map.size());
 }
 
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap) {
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap, int map_blocks) {
     const GeneralLaunch g = plan(h_states, count);
     const void* fn = select_kernel(g.maxkpt, g.cells, f32, g.split);
-    prepare(fn);
+    if (g.lds > prepare(fn)) throw std::runtime_error("k-vector and molecule tables too large for LDS");
     // each workgroup lays out its own replica's tables inside the dynamic LDS
     long long spin = 1LL << 24;
     if (!g.split) {
@@ -853,9 +1044,11 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
         return;
     }
     if (d_blockmap == nullptr) throw std::runtime_error("split general launch: block map missing");
-    if (g.grid > general_coresident_blocks(f32)) throw std::runtime_error("split general launch: the grid cannot be co-resident");
+    const int grid = map_blocks > 0 ? map_blocks : g.grid;
+    if (grid < g.grid) throw std::runtime_error("split general launch: block map shorter than the parts");
+    if (grid > general_coresident_blocks(f32)) throw std::runtime_error("split general launch: the grid cannot be co-resident");
     void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin};
-    check_hip(hipLaunchCooperativeKernel(fn, dim3(g.grid), dim3(256), args, static_cast<unsigned>(g.lds), stream),
+    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, static_cast<unsigned>(g.lds), stream),
               "mc_general_kernel cooperative launch");
 }
 

@@ -329,8 +329,7 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     if (!finalized_) finalize();
     if (size() != 1) throw std::runtime_error("run_split advances a batch of one replica");
     if (general_ == 1) {
-        if (stride != 1) throw std::runtime_error("run_split: replicas with a general move set take stride 1");
-        run_split_general(moves, groups, trace, trace_cap);
+        run_split_general(moves, groups, stride, trace, trace_cap);
         return;
     }
     const ReplicaState& s = h_states_[0];
@@ -377,7 +376,7 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
 }
 
 // The replica's general-move kernel over `groups` workgroups (cooperative launch).
-void Batch::run_split_general(long long moves, int groups, bool trace, int trace_cap) {
+void Batch::run_split_general(long long moves, int groups, int stride, bool trace, int trace_cap) {
     ReplicaState& h = h_states_[0];
     if (h.parts != groups || (groups > 1 && h.gslots == nullptr)) {
         check_hip(hipStreamSynchronize(stream_), "batch sync");
@@ -389,7 +388,8 @@ void Batch::run_split_general(long long moves, int groups, bool trace, int trace
                             hipMemcpyHostToDevice),
                   "copy split fields");
     }
-    const int grid = general_blockmap(h_states_.data(), 1, h_map_);
+    if (stride < 1) throw std::runtime_error("run_split: stride >= 1");
+    const int grid = general_blockmap(h_states_.data(), 1, h_map_, groups > 1 ? stride : 1);
     if (grid > map_cap_) {
         check_hip(hipStreamSynchronize(stream_), "batch sync");
         (void)hipFree(d_map_);
@@ -411,7 +411,7 @@ void Batch::run_split_general(long long moves, int groups, bool trace, int trace
     check_hip(hipMemcpyAsync(d_moves_, &h_split_moves_, sizeof(long long), hipMemcpyHostToDevice, stream_), "copy moves");
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
     launch_general(h_states_.data(), 1, d_states_, d_moves_, fp32_ != 0, trace ? d_trace_ : nullptr,
-                   trace ? trace_cap : 0, stream_, groups > 1 ? d_map_ : nullptr);
+                   trace ? trace_cap : 0, stream_, groups > 1 ? d_map_ : nullptr, groups > 1 ? grid : 0);
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 

@@ -210,8 +210,8 @@ int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
  * The workgroups exchange per-move partial sums through device memory; a wait that
  * exceeds its spin bound stops the run and sets the error read by
  * fmc_batch_split_error instead of hanging.  A replica with a general move set
- * (fmc_batch_add_replica_moves, no cell list, stride 1, 1..64 groups) runs the split
- * form of the general kernel: each workgroup also takes a slice of the pair sums.
+ * (fmc_batch_add_replica_moves, no cell list, 1..64 groups) runs the split form of the
+ * general kernel: each workgroup also takes a slice of the pair sums.
  */
 int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int stride, int trace_capacity);
 /*
