@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int kw = kSplit ? (nk + G - 1) / G : nk;
     const int kb = kSplit ? part * kw : 0;
     const int ke = kSplit ? min(nk, kb + kw) : nk;
-    const int kpt = kSplit ? (kw + kThreads - 1) / kThreads : S.kpt;
+    const int kpt = (kw + kThreads - 1) / kThreads;
     const int jw = kSplit ? (n + G - 1) / G : n;
     const int jb = kSplit ? min(n, part * jw) : 0;
     const int je = kSplit ? min(n, jb + jw) : n;
@@ -914,20 +914,29 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     }
 }
 
-template <int kMaxKPT, bool kCells, bool kF32, bool kSplit>
+// Workgroup sizes: 512 threads (two waves per SIMD) for the kernels that walk all
+// particles, 256 for the cell-list kernels (their stencil tables take more LDS).
+constexpr int kWide = 512, kNarrow = 256;
+
+int threads_of(bool cells) { return cells ? kNarrow : kWide; }
+int kpt_cap(bool cells) { return cells ? 16 : 8; } // 4096 k-vectors per workgroup either way
+
+template <int kThreads, int kMaxKPT, bool kCells, bool kF32, bool kSplit>
 const void* kernel_of() {
-    return reinterpret_cast<const void*>(mc_general_kernel<256, kMaxKPT, kCells, kF32, kSplit>);
+    return reinterpret_cast<const void*>(mc_general_kernel<kThreads, kMaxKPT, kCells, kF32, kSplit>);
 }
 
 const void* select_kernel(int maxkpt, bool cells, bool f32, bool split) {
-    if (maxkpt <= 8) {
-        if (split) return f32 ? kernel_of<8, false, true, true>() : kernel_of<8, false, false, true>();
-        if (cells) return f32 ? kernel_of<8, true, true, false>() : kernel_of<8, true, false, false>();
-        return f32 ? kernel_of<8, false, true, false>() : kernel_of<8, false, false, false>();
+    if (cells) {
+        if (maxkpt <= 8) return f32 ? kernel_of<kNarrow, 8, true, true, false>() : kernel_of<kNarrow, 8, true, false, false>();
+        return f32 ? kernel_of<kNarrow, 16, true, true, false>() : kernel_of<kNarrow, 16, true, false, false>();
     }
-    if (split) return f32 ? kernel_of<16, false, true, true>() : kernel_of<16, false, false, true>();
-    if (cells) return f32 ? kernel_of<16, true, true, false>() : kernel_of<16, true, false, false>();
-    return f32 ? kernel_of<16, false, true, false>() : kernel_of<16, false, false, false>();
+    if (maxkpt <= 4) {
+        if (split) return f32 ? kernel_of<kWide, 4, false, true, true>() : kernel_of<kWide, 4, false, false, true>();
+        return f32 ? kernel_of<kWide, 4, false, true, false>() : kernel_of<kWide, 4, false, false, false>();
+    }
+    if (split) return f32 ? kernel_of<kWide, 8, false, true, true>() : kernel_of<kWide, 8, false, false, true>();
+    return f32 ? kernel_of<kWide, 8, false, true, false>() : kernel_of<kWide, 8, false, false, false>();
 }
 
 constexpr size_t kLdsPerCU = 160 * 1024;
@@ -948,10 +957,11 @@ size_t prepare(const void* fn) {
 }
 
 // k-vectors per thread of a replica's part (its slice of ceil(nk / parts) k-vectors).
-int part_kpt(const ReplicaState& s) {
-    const int parts = s.parts > 1 ? s.parts : 1;
+int part_kpt(const ReplicaState& s, int parts) {
+    parts = parts > 1 ? parts : 1;
     const int kw = (s.nk + parts - 1) / parts;
-    return (kw + 255) / 256;
+    const int threads = threads_of(s.cells != 0);
+    return (kw + threads - 1) / threads;
 }
 
 struct GeneralLaunch {
@@ -976,12 +986,13 @@ GeneralLaunch plan(const ReplicaState* h_states, int count) {
             g.split = true;
         }
         g.grid += s.parts > 1 ? s.parts : 1;
-        g.maxkpt = std::max(g.maxkpt, s.parts > 1 ? part_kpt(s) : s.kpt);
+        g.maxkpt = std::max(g.maxkpt, part_kpt(s, s.parts));
         maxnk = std::max(maxnk, s.nk);
         maxw = std::max(maxw, 2 * s.ncut + 1);
         maxm = std::max(maxm, s.max_moved);
     }
-    if (g.maxkpt > 16) throw std::runtime_error("too many k-vectors for the general device MC (max 16 per thread)");
+    if (g.maxkpt > kpt_cap(g.cells))
+        throw std::runtime_error("too many k-vectors for the general device MC (max 4096 per workgroup)");
     // the split form keeps the previous move's coordinates in a second buffer
     g.lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) + sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
             ((g.split ? 3 : 2) * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
@@ -991,23 +1002,24 @@ GeneralLaunch plan(const ReplicaState* h_states, int count) {
 } // namespace
 
 int general_coresident_blocks(bool f32) {
-    const void* fn = select_kernel(16, false, f32, true);
+    const void* fn = select_kernel(8, false, f32, true);
     const size_t lds = prepare(fn);
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds), "occupancy");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWide, lds), "occupancy");
     return per_cu * prop.multiProcessorCount;
 }
 
 void set_replica_parts(ReplicaState& s, int parts, std::vector<void*>& owned) {
     if (parts < 1 || parts > kMaxParts) throw std::runtime_error("split general launch: 1 to 64 workgroups per replica");
     if (parts > 1 && s.cells) throw std::runtime_error("split general launch: replicas without a cell list only");
+    if (parts > 1 && part_kpt(s, parts) > kpt_cap(false))
+        throw std::runtime_error("too many k-vectors per workgroup for the split general launch");
     s.parts = parts;
     s.gslots = nullptr;
     if (parts == 1) return;
-    if (part_kpt(s) > 16) throw std::runtime_error("too many k-vectors per workgroup for the split general launch");
     void* p = nullptr;
     const size_t bytes = sizeof(GeneralSlot) * 2 * static_cast<size_t>(parts);
     check_hip(hipMalloc(&p, bytes), "hipMalloc exchange slots");
@@ -1040,7 +1052,8 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     if (!g.split) {
         const int2* none = nullptr;
         void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &none, &spin};
-        check_hip(hipLaunchKernel(fn, dim3(count), dim3(256), args, g.lds, stream), "mc_general_kernel launch");
+        check_hip(hipLaunchKernel(fn, dim3(count), dim3(threads_of(g.cells)), args, g.lds, stream),
+                  "mc_general_kernel launch");
         return;
     }
     if (d_blockmap == nullptr) throw std::runtime_error("split general launch: block map missing");
@@ -1048,7 +1061,7 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     if (grid < g.grid) throw std::runtime_error("split general launch: block map shorter than the parts");
     if (grid > general_coresident_blocks(f32)) throw std::runtime_error("split general launch: the grid cannot be co-resident");
     void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin};
-    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, static_cast<unsigned>(g.lds), stream),
+    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWide), args, static_cast<unsigned>(g.lds), stream),
               "mc_general_kernel cooperative launch");
 }
 
