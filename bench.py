@@ -477,7 +477,7 @@ def main():
             log("pipe G=%d D=%d: %.0f moves/s (kernel %.2f us/move)" % (
                 groups, depth, r["rate"], r["kernel_ms"] * 1e3 / r["moves"]))
     if args.only == "split":
-        for groups, stride in [(12, 1), (16, 1), (24, 1), (32, 1), (16, 8)]:
+        for groups, stride in [(12, 1), (16, 1), (24, 1), (32, 1), (48, 1), (64, 1), (16, 8)]:
             r = bench_split(sysd, args, groups, stride, steps=3)
             out[f"split_{groups}_{stride}"] = r
             log("split G=%d stride=%d: %.0f moves/s (kernel %.2f us/move)" % (

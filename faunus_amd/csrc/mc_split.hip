@@ -57,8 +57,8 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
     __shared__ double s_dp[kMaxTypes];
     __shared__ double s_gk[kGkMax];
     __shared__ double s_red[3][4];
-    __shared__ double s_slot[3][kSplitMaxG];
     __shared__ int s_accept, s_abort;
+    __shared__ Proposal s_prop[64]; // proposals of the next 64 moves, one lane of wave 0 each
     __shared__ double2 s_phase[6 * 65];
 
     const int tid = threadIdx.x;
@@ -103,7 +103,12 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
 
     for (long long t = 0; t < M; ++t) {
         const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
-        const Proposal P = make_proposal(seed, replica, mid, n);
+        if ((t & 63) == 0) {
+            // the previous proposals were last read before this move's barriers
+            if (tid < 64) s_prop[tid] = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
+            __syncthreads();
+        }
+        const Proposal P = s_prop[t & 63];
         const int i = P.index;
         double4 po = pos[i];
         // most recent accepted write of this particle wins
@@ -199,39 +204,48 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
             sl->po = ((s_red[2][0] + s_red[2][1]) + s_red[2][2]) + s_red[2][3];
             __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // gather every workgroup's partial (lane l of wave 0 waits for workgroup l)
+        // gather every workgroup's partial (lane l of wave 0 waits for workgroup l) and
+        // add them with the same butterfly in every workgroup
+        double sde = 0.0, spn = 0.0, spo = 0.0; // totals, valid in thread 0
         if (tid < 64) {
-            bool ok = true;
+            int ok = 1;
+            double a = 0.0, b = 0.0, c = 0.0;
             if (tid < G) {
                 SplitSlot* sl = slots + parity * G + tid;
                 long long spins = 0;
-                while (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+                // relaxed polling, then one acquire fence (an acquire load per poll would
+                // invalidate the caches on every iteration)
+                while (__hip_atomic_load(&sl->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != tag) {
                     if (++spins > spin_limit) {
-                        ok = false;
+                        ok = 0;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 if (ok) {
-                    s_slot[0][tid] = sl->de;
-                    s_slot[1][tid] = sl->pn;
-                    s_slot[2][tid] = sl->po;
+                    a = sl->de;
+                    b = sl->pn;
+                    c = sl->po;
                 }
             }
-            if (!ok) {
-                atomicExch(err, 1);
-                s_abort = 1;
+            for (int off = 32; off > 0; off >>= 1) {
+                a += __shfl_xor(a, off, 64);
+                b += __shfl_xor(b, off, 64);
+                c += __shfl_xor(c, off, 64);
+            }
+            const int all_ok = __all(ok);
+            if (tid == 0) {
+                sde = a;
+                spn = b;
+                spo = c;
+                if (!all_ok) {
+                    atomicExch(err, 1);
+                    s_abort = 1;
+                }
             }
         }
-        __syncthreads();
-        if (s_abort) break;
-        if (tid == 0) {
-            double sde = 0.0, spn = 0.0, spo = 0.0;
-            for (int w = 0; w < G; ++w) {
-                sde += s_slot[0][w];
-                spn += s_slot[1][w];
-                spo += s_slot[2][w];
-            }
+        if (tid == 0 && !s_abort) {
             const double drec = pref * sde;
             const double e_tot_new = spn + drec;
             double du;
@@ -290,6 +304,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
             }
         }
         __syncthreads();
+        if (s_abort) break;
         if (s_accept) {
 #pragma unroll
             for (int s = 0; s < kSplitMaxKPT; ++s) {
