@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 240.4e6
+HEADLINE_TRAFFIC_BYTES = 248.4e6
 FP64_PEAK_TFLOPS = 78.6
 STEP_MOVES = 20000
 PLAIN_MOVES = 2000
@@ -572,6 +572,8 @@ def main():
             "max_energy_drift": out["water32"]["max_drift"]}
     if "serve" in out:
         line["forms"]["serving_poisson_50_per_s"] = out["serve"]
+    if "gsplit" in out:
+        line["forms"]["general_split_polyelectrolyte_43k"] = out["gsplit"]
     print(json.dumps(line), flush=True)
 
 
