@@ -32,12 +32,13 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     __shared__ double s_gk[kMaxGk];
     __shared__ double s_red[3][kThreads / 64];
     __shared__ int s_accept;
+    __shared__ Proposal s_prop[64]; // the next 64 proposals, drawn one per lane of wave 0
     extern __shared__ int s_dyn[]; // kn[nk] followed by the phase tables (double2)
 
     const int tid = threadIdx.x;
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
     const int width = 2 * ncut + 1;
-    const int kpt = S.kpt;
+    const int kpt = (nk + kThreads - 1) / kThreads;
     const double L = S.L, half = S.half, inv = S.inv;
     int* s_kn = s_dyn;
     double2* s_phase = reinterpret_cast<double2*>(s_dyn + ((nk + 3) & ~3));
@@ -83,7 +84,16 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
-        const Proposal P = make_proposal(seed, static_cast<uint32_t>(replica), move0 + static_cast<unsigned long long>(t), n);
+        if ((t & 63) == 0) {
+            // the previous proposals were last read before this move's barriers; the
+            // Philox blocks and sin/cos of 64 moves overlap instead of forming one serial
+            // chain per move
+            if (tid < 64)
+                s_prop[tid] = make_proposal(seed, static_cast<uint32_t>(replica),
+                                            move0 + static_cast<unsigned long long>(t + tid), n);
+            __syncthreads();
+        }
+        const Proposal P = s_prop[t & 63];
         const int i = P.index;
         const double4 po = pos[i];
         const int ti = static_cast<int>(meta[i] & 0xffu);
@@ -591,22 +601,41 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     return s;
 }
 
+int advance_threads(const ReplicaState& s) { return (!s.cells && s.nk <= 8 * 512) ? 512 : 256; }
+
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream) {
-    int maxkpt = 0, maxnk = 0, maxw = 0;
-    for (int r = 0; r < count; ++r) {
-        maxkpt = std::max(maxkpt, h_states[r].kpt);
-        maxnk = std::max(maxnk, h_states[r].nk);
-        maxw = std::max(maxw, 2 * h_states[r].ncut + 1);
-        if (h_states[r].cells != h_states[0].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
+    // one launch per run of replicas with the same workgroup size (a function of each
+    // replica alone, so that a replica's arithmetic does not depend on its batch)
+    int first = 0;
+    while (first < count) {
+        const int threads = advance_threads(h_states[first]);
+        int last = first;
+        int maxkpt = 0, maxnk = 0, maxw = 0;
+        while (last < count && advance_threads(h_states[last]) == threads) {
+            const ReplicaState& s = h_states[last];
+            if (s.cells != h_states[first].cells) throw std::runtime_error("replicas of one launch must agree on the cell list");
+            maxkpt = std::max(maxkpt, (s.nk + threads - 1) / threads);
+            maxnk = std::max(maxnk, s.nk);
+            maxw = std::max(maxw, 2 * s.ncut + 1);
+            ++last;
+        }
+        if (maxkpt > 68) throw std::runtime_error("too many k-vectors per replica for the device MC (max 68 per thread)");
+        const size_t lds =
+            sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
+        if (lds > 96 * 1024) throw std::runtime_error("k-vector table too large for LDS");
+        const bool cells = h_states[first].cells != 0;
+        const int grid = last - first;
+        ReplicaState* ds = d_states + first;
+        const long long* dm = d_moves + first;
+        TraceEntry* tr = trace ? trace + static_cast<size_t>(first) * trace_cap : nullptr;
+        if (threads == 512) dispatch_cells<512, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
+        else if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
+        else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
+        else dispatch_cells<256, 68>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
+        check_hip(hipGetLastError(), "mc_advance_kernel launch");
+        first = last;
     }
-    const size_t lds = sizeof(int) * static_cast<size_t>((maxnk + 3) & ~3) + sizeof(double2) * 6 * static_cast<size_t>(maxw);
-    if (lds > 96 * 1024) throw std::runtime_error("k-vector table too large for LDS");
-    const bool cells = h_states[0].cells != 0;
-    if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
-    else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
-    else dispatch_cells<256, 68>(cells, f32, count, lds, stream, d_states, d_moves, trace, trace_cap);
-    check_hip(hipGetLastError(), "mc_advance_kernel launch");
 }
 
 int Batch::add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,

@@ -99,7 +99,11 @@ struct TraceEntry {
 // are appended to `owned`, which the caller frees.
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
                                 std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds = nullptr);
-// One persistent launch advancing `count` replicas (states already on the device).
+// Workgroup size of the translate-only kernel for a replica: 512 threads (two waves per
+// SIMD) without a cell list and with K <= 4096, else 256.
+int advance_threads(const ReplicaState& s);
+// Persistent launches advancing `count` replicas (states already on the device), one
+// per run of replicas with the same workgroup size.
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream);
 // Same for replicas with a general move set (molecular moves, bonds).  Replicas with

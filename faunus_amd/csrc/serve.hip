@@ -175,9 +175,12 @@ void Server::step(long long quantum) {
     std::vector<ReplicaState> states(static_cast<size_t>(count));
     std::vector<long long> moves(static_cast<size_t>(count));
     std::vector<int> order;
-    auto launch_class = [&](const Slot& s) { return (s.state.cells != 0 ? 1 : 0) + (s.general ? 2 : 0); };
+    // translate-only jobs also split by workgroup size (one launch each)
+    auto launch_class = [&](const Slot& s) {
+        return (s.state.cells != 0 ? 1 : 0) + (s.general ? 2 : 0) + (!s.general && advance_threads(s.state) == 512 ? 4 : 0);
+    };
     // split jobs first within their class, so that they lead the cooperative launches
-    for (int pass = 0; pass < 4; ++pass)
+    for (int pass = 0; pass < 8; ++pass)
         for (int split = 1; split >= 0; --split)
             for (int k = 0; k < count; ++k) {
                 const Slot& s = active_[static_cast<size_t>(k)];
@@ -243,7 +246,7 @@ void Server::step(long long quantum) {
                   "copy block map");
     }
     for (const Launch& l : launches) {
-        if (l.cls >= 2)
+        if (l.cls == 2 || l.cls == 3)
             launch_general(states.data() + l.first, l.last - l.first, d_states_ + l.first, d_moves_ + l.first,
                            fp32_ != 0, nullptr, 0, stream_, l.map_off >= 0 ? d_map_ + l.map_off : nullptr);
         else
