@@ -66,6 +66,10 @@ SIGNATURES = {
     "fmc_batch_positions": (C.c_int, [C.c_void_p, C.c_int, D]),
     "fmc_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, I, U8, D, D]),
     "fmc_batch_total_energy": (C.c_int, [C.c_void_p, C.c_int, D, D]),
+    "fmc_batch_sample_energy": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "fmc_batch_energy_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_ulonglong), D, D]),
+    "fmc_batch_rdf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                C.POINTER(C.c_ulonglong)]),
     "fmc_batch_check_q": (C.c_int, [C.c_void_p, C.c_int, D, D]),
     "fmc_copy_probe": (C.c_int, [C.c_longlong, C.c_int, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
@@ -399,6 +403,31 @@ class Batch:
 
     def run_pipe(self, moves, groups=16, depth=2, trace=0):
         _check(lib().fmc_batch_run_pipe(self.h, int(moves), int(groups), int(depth), int(trace)))
+
+    def sample_energy(self, every):
+        """Device energy samples every ``every`` moves (0: off); clears the accumulators."""
+        _check(lib().fmc_batch_sample_energy(self.h, int(every)))
+
+    def energy_stats(self, r):
+        """Replica r's device energy samples: count, sum, sum of squares, mean and
+        variance (kT at T_ref)."""
+        n, s1, s2 = C.c_ulonglong(), C.c_double(), C.c_double()
+        _check(lib().fmc_batch_energy_stats(self.h, int(r), C.byref(n), C.byref(s1), C.byref(s2)))
+        k = n.value
+        mean = s1.value / k if k else float("nan")
+        return dict(samples=k, sum=s1.value, sum_sq=s2.value, mean=mean,
+                    variance=(s2.value / k - mean * mean) if k else float("nan"))
+
+    def rdf(self, r, type_a, type_b, dr, nbins, counts=None):
+        """Add the a-b pair-distance histogram of replica r's current configuration to
+        ``counts`` (uint64, nbins; a new array if None) and return it."""
+        if counts is None:
+            counts = np.zeros(int(nbins), np.uint64)
+        if counts.dtype != np.uint64 or len(counts) != int(nbins) or not counts.flags["C_CONTIGUOUS"]:
+            raise ValueError("counts must be a contiguous uint64 array of nbins")
+        _check(lib().fmc_batch_rdf(self.h, int(r), int(type_a), int(type_b), float(dr), int(nbins),
+                                   counts.ctypes.data_as(C.POINTER(C.c_ulonglong))))
+        return counts
 
     def split_error(self):
         e = C.c_int()

@@ -468,6 +468,27 @@ int fmc_batch_replica_info(fmc_batch* b, int replica, double* energy, double* be
     });
 }
 
+int fmc_batch_sample_energy(fmc_batch* b, long long every) {
+    return guarded([&] { bat(b).sample_energy(every); });
+}
+
+int fmc_batch_energy_stats(fmc_batch* b, int replica, unsigned long long* samples, double* sum, double* sum_sq) {
+    return guarded([&] {
+        if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");
+        const fmc::ReplicaState s = bat(b).state(replica);
+        if (samples) *samples = s.e_samples;
+        if (sum) *sum = s.e_sum;
+        if (sum_sq) *sum_sq = s.e_sum2;
+    });
+}
+
+int fmc_batch_rdf(fmc_batch* b, int replica, int type_a, int type_b, double dr, int nbins, unsigned long long* counts) {
+    return guarded([&] {
+        if (!counts) throw std::runtime_error("rdf: null counts");
+        bat(b).rdf(replica, type_a, type_b, dr, nbins, counts);
+    });
+}
+
 int fmc_batch_positions(fmc_batch* b, int replica, double* xyz) {
     return guarded([&] {
         if (replica < 0 || replica >= bat(b).size()) throw std::runtime_error("replica out of range");

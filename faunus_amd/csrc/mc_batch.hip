@@ -81,6 +81,9 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     unsigned long long accepted = S.accepted;
     const int replica = S.replica_id;
     const unsigned long long seed = S.seed;
+    const long long sample_every = S.sample_every;
+    unsigned long long e_samples = S.e_samples;
+    double e_sum = S.e_sum, e_sum2 = S.e_sum2;
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -264,6 +267,12 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                     }
                 }
             }
+            if (sample_every > 0 && isfinite(energy) &&
+                (move0 + static_cast<unsigned long long>(t) + 1ull) % static_cast<unsigned long long>(sample_every) == 0ull) {
+                e_sum += energy;
+                e_sum2 += energy * energy;
+                ++e_samples;
+            }
         }
         __syncthreads();
         if (s_accept) {
@@ -305,6 +314,9 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.e_samples = e_samples;
+        S.e_sum = e_sum;
+        S.e_sum2 = e_sum2;
     }
 }
 
@@ -361,6 +373,38 @@ void dispatch_cells(bool cells, bool f32, int grid, size_t lds, hipStream_t st, 
         if (f32) launch_advance<kThreads, kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap);
         else launch_advance<kThreads, kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap);
     }
+}
+
+// Pair-distance histogram of one configuration (AtomRDF::_sample, analysis.cpp:1556-1600):
+// types a != b: every (a, b) pair; a == b: pairs i < j.  r = |minimum-image distance|,
+// bin floor(r / dr) as Equidistant2DTable::to_bin (equidistant_table.h:32-39), bins >= nbins
+// dropped.  Block b takes particles i = b, b + grid, ...; its lanes stride over j; counts
+// go to an LDS histogram, then once per block to the global one.
+__global__ __launch_bounds__(256) void rdf_kernel(const double4* __restrict__ pos, const uint32_t* __restrict__ meta,
+                                                  int n, int ta, int tb, double L, double half, double inv_dr,
+                                                  int nbins, unsigned long long* __restrict__ counts) {
+    extern __shared__ unsigned int s_hist[];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0u;
+    __syncthreads();
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int ti = static_cast<int>(meta[i] & 0xffu);
+        if (ti != ta) continue; // uniform over the block
+        const double4 pi = pos[i];
+        for (int j = (ta == tb ? i + 1 : 0) + static_cast<int>(threadIdx.x); j < n; j += blockDim.x) {
+            if (static_cast<int>(meta[j] & 0xffu) != tb) continue;
+            const double4 pj = pos[j];
+            const double dx = min_image(pi.x - pj.x, L, half);
+            const double dy = min_image(pi.y - pj.y, L, half);
+            const double dz = min_image(pi.z - pj.z, L, half);
+            // x^2 + y^2 + z^2 rounded term by term, as the reference's norm
+            const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+            const double x = floor(__dmul_rn(r, inv_dr));
+            if (x < static_cast<double>(nbins)) atomicAdd(&s_hist[static_cast<int>(x)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (s_hist[b]) atomicAdd(&counts[b], static_cast<unsigned long long>(s_hist[b]));
 }
 
 } // namespace
@@ -514,6 +558,9 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.pos32 = d_p32;
     s.parts = 1;
     s.gslots = nullptr;
+    s.sample_every = 0;
+    s.e_samples = 0;
+    s.e_sum = s.e_sum2 = 0.0;
     const DevCoulomb& dc = e.dev_coulomb();
     if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
     s.terms = 0;
@@ -703,6 +750,51 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
     }
     else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
+void Batch::sample_energy(long long every) {
+    if (every < 0) throw std::runtime_error("sample_energy: interval >= 0");
+    if (!finalized_) finalize();
+    check_hip(hipStreamSynchronize(stream_), "batch sync");
+    std::vector<ReplicaState> cur(static_cast<size_t>(size()));
+    check_hip(hipMemcpy(cur.data(), d_states_, sizeof(ReplicaState) * size(), hipMemcpyDeviceToHost), "copy states");
+    for (int r = 0; r < size(); ++r) {
+        for (ReplicaState* s : {&cur[static_cast<size_t>(r)], &h_states_[static_cast<size_t>(r)]}) {
+            s->sample_every = every;
+            s->e_samples = 0;
+            s->e_sum = s->e_sum2 = 0.0;
+        }
+    }
+    check_hip(hipMemcpy(d_states_, cur.data(), sizeof(ReplicaState) * size(), hipMemcpyHostToDevice), "copy states");
+}
+
+void Batch::rdf(int r, int type_a, int type_b, double dr, int nbins, unsigned long long* counts) {
+    if (r < 0 || r >= size()) throw std::runtime_error("replica index out of range");
+    if (!(dr > 0.0)) throw std::runtime_error("rdf: dr must be positive");
+    if (nbins < 1 || nbins > 8192) throw std::runtime_error("rdf: 1 to 8192 bins");
+    const ReplicaState& h = h_states_[static_cast<size_t>(r)];
+    if (type_a < 0 || type_b < 0 || type_a >= h.ntypes || type_b >= h.ntypes) throw std::runtime_error("rdf: bad atom type");
+    if (!finalized_) finalize();
+    unsigned long long* d_counts = nullptr;
+    check_hip(hipMalloc(&d_counts, sizeof(unsigned long long) * static_cast<size_t>(nbins)), "hipMalloc rdf");
+    std::vector<unsigned long long> part(static_cast<size_t>(nbins));
+    try {
+        check_hip(hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * static_cast<size_t>(nbins), stream_),
+                  "clear rdf");
+        const int grid = std::max(1, std::min(h.n, 2048));
+        hipLaunchKernelGGL(rdf_kernel, dim3(grid), dim3(256), sizeof(unsigned int) * static_cast<size_t>(nbins), stream_,
+                           h.pos, h.meta, h.n, type_a, type_b, h.L, h.half, 1.0 / dr, nbins, d_counts);
+        check_hip(hipGetLastError(), "rdf_kernel launch");
+        check_hip(hipMemcpyAsync(part.data(), d_counts, sizeof(unsigned long long) * static_cast<size_t>(nbins),
+                                 hipMemcpyDeviceToHost, stream_),
+                  "copy rdf");
+        check_hip(hipStreamSynchronize(stream_), "rdf sync");
+    } catch (...) {
+        (void)hipFree(d_counts);
+        throw;
+    }
+    check_hip(hipFree(d_counts), "hipFree rdf");
+    for (int b = 0; b < nbins; ++b) counts[b] += part[static_cast<size_t>(b)];
 }
 
 void Batch::sync() {

@@ -85,6 +85,11 @@ struct ReplicaState {
     int parts;                // workgroups sharing this replica in a split general launch (1 otherwise)
     int pad2;
     void* gslots;             // split general launch: 2 x parts exchange slots
+    // energy samples (SystemEnergy, analysis.h:547-575): after every move whose index + 1 is a
+    // multiple of sample_every (0: off), the running energy is added to e_sum and its square to e_sum2
+    long long sample_every;
+    unsigned long long e_samples;
+    double e_sum, e_sum2;
 };
 
 // One recorded move of the trace (tests).
@@ -141,6 +146,11 @@ class Batch {
     // one replica over `groups` workgroups with up to `depth` moves in flight
     void run_pipe(long long moves, int groups, int depth, bool trace, int trace_cap);
     void swap_ladders(unsigned long long seed, unsigned long long round);
+    // energy samples every `every` moves (0: off) for all replicas, counters cleared
+    void sample_energy(long long every);
+    // pair-distance histogram of types a and b (a == b: pairs i < j) of replica r's current
+    // configuration, bins [k dr, (k+1) dr), k < nbins, added to counts
+    void rdf(int r, int type_a, int type_b, double dr, int nbins, unsigned long long* counts);
     void sync();
     int size() const { return static_cast<int>(engines_.size()); }
     ReplicaState state(int r);

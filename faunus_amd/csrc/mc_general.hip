@@ -394,6 +394,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int* __restrict__ excl_off = S.excl_off;
     const int* __restrict__ excl = S.excl;
     const int terms = S.terms;
+    const long long sample_every = S.sample_every;
+    unsigned long long e_samples = S.e_samples;
+    double e_sum = S.e_sum, e_sum2 = S.e_sum2;
     // split form: a part sees the coordinates (and mass centre) the previous move
     // committed in other parts only after that move's successor has been exchanged, so
     // reads of the previous move's accepted atoms take this part's own copy
@@ -837,6 +840,14 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (acc && m > 0) {
                 energy += du;
                 sum_du += du;
+            }
+            if (sample_every > 0 && isfinite(energy) &&
+                (mvi + 1ull) % static_cast<unsigned long long>(sample_every) == 0ull) {
+                e_sum += energy;
+                e_sum2 += energy * energy;
+                ++e_samples;
+            }
+            if (acc && m > 0) {
                 if (P.kind == 1 && P.translate && part == 0)
                     for (int d = 0; d < 3; ++d) gcm[3 * g + d] = P.cm_new[d];
                 if (kCells) {
@@ -911,6 +922,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.e_samples = e_samples;
+        S.e_sum = e_sum;
+        S.e_sum2 = e_sum2;
     }
 }
 

@@ -95,6 +95,9 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
     unsigned long long accepted = S.accepted;
     const unsigned long long seed = S.seed;
     const uint32_t replica = static_cast<uint32_t>(S.replica_id);
+    const long long sample_every = S.sample_every;
+    unsigned long long e_samples = S.e_samples;
+    double e_sum = S.e_sum, e_sum2 = S.e_sum2;
     // the last two moves (identical in every workgroup): their writes to pos may not
     // be visible yet, so reads of those particles take these values
     int h_idx[2] = {-1, -1};
@@ -301,6 +304,12 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
                         }
                     }
                 }
+                if (sample_every > 0 && isfinite(energy) &&
+                    (mid + 1ull) % static_cast<unsigned long long>(sample_every) == 0ull) {
+                    e_sum += energy;
+                    e_sum2 += energy * energy;
+                    ++e_samples;
+                }
             }
         }
         __syncthreads();
@@ -332,6 +341,9 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.e_samples = e_samples;
+        S.e_sum = e_sum;
+        S.e_sum2 = e_sum2;
     }
 }
 

@@ -236,6 +236,24 @@ int fmc_batch_trace(fmc_batch* batch, int replica, int count, int* index, unsign
                     double* xyz);
 /* Change::everything energy of a replica recomputed from its positions, per term. */
 int fmc_batch_total_energy(fmc_batch* batch, int replica, double* terms, double* total);
+/*
+ * On-device observables.
+ * fmc_batch_sample_energy: from now on every replica adds its running energy (kT at
+ * T_ref) and its square to device accumulators after each move whose index + 1 is a
+ * multiple of `every` (0: off), skipping non-finite energies, as SystemEnergy::_sample
+ * (src/analysis.cpp:339-347) does at its sampling interval; the accumulators are cleared.
+ * The one-workgroup, split and general kernels sample; fmc_batch_run_pipe refuses to run
+ * while sampling is on.
+ * fmc_batch_energy_stats reads a replica's sample count, sum and sum of squares.
+ * fmc_batch_rdf adds the pair-distance histogram of atom types a and b of a replica's
+ * current configuration (a == b: pairs i < j) to counts[nbins], bin floor(r / dr), as
+ * AtomRDF::_sample (src/analysis.cpp:1556-1600) fills its Equidistant2DTable
+ * (src/aux/equidistant_table.h:32-39); normalisation to g(r) stays with the caller.
+ */
+int fmc_batch_sample_energy(fmc_batch* batch, long long every);
+int fmc_batch_energy_stats(fmc_batch* batch, int replica, unsigned long long* samples, double* sum, double* sum_sq);
+int fmc_batch_rdf(fmc_batch* batch, int replica, int type_a, int type_b, double dr, int nbins,
+                  unsigned long long* counts);
 int fmc_batch_check_q(fmc_batch* batch, int replica, double* max_abs_dq, double* rel_reciprocal);
 
 /*
