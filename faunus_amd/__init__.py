@@ -74,6 +74,8 @@ SIGNATURES = {
     "fmc_copy_probe": (C.c_int, [C.c_longlong, C.c_int, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
+    "fmc_parse_state": (C.c_int, [C.c_char_p, I, I, D, D, D, I, I, I, I, D]),
+    "fmc_load_state": (C.c_int, [C.c_void_p, C.c_char_p]),
     "fmc_server_create": (C.c_int, [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]),
     "fmc_server_destroy": (C.c_int, [C.c_void_p]),
     "fmc_server_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -208,6 +210,15 @@ class Engine:
         begin, size, molid, cm = _groups(begin, size, molid, mass_center)
         _check(lib().fmc_set_groups(self.h, len(begin), _p(begin, C.c_int), _p(size, C.c_int),
                                     _p(molid, C.c_int), _p(cm)))
+
+    def load_state(self, text):
+        """Particles and groups from a reference state file (Space JSON text or dict)."""
+        if not isinstance(text, str):
+            text = json.dumps(text)
+        _check(lib().fmc_load_state(self.h, text.encode()))
+        n = C.c_int()
+        _check(lib().fmc_parse_state(text.encode(), C.byref(n), None, None, None, None, None, None, None, None, None))
+        self.n = n.value
 
     def init(self):
         e = C.c_double()
@@ -477,6 +488,24 @@ class Batch:
         a, b = C.c_double(), C.c_double()
         _check(lib().fmc_batch_check_q(self.h, r, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+def parse_state(text):
+    """Read a reference state file (Space JSON): box, particles and groups as arrays."""
+    if not isinstance(text, str):
+        text = json.dumps(text)
+    n, ng = C.c_int(), C.c_int()
+    _check(lib().fmc_parse_state(text.encode(), C.byref(n), C.byref(ng), None, None, None, None, None, None, None,
+                                 None))
+    box = np.zeros(3)
+    xyz = np.zeros((n.value, 3))
+    q = np.zeros(n.value)
+    ids = np.zeros(n.value, np.int32)
+    gb, gs, gm = (np.zeros(ng.value, np.int32) for _ in range(3))
+    cm = np.zeros((ng.value, 3))
+    _check(lib().fmc_parse_state(text.encode(), C.byref(n), C.byref(ng), _p(box), _p(xyz), _p(q), _p(ids, C.c_int),
+                                 _p(gb, C.c_int), _p(gs, C.c_int), _p(gm, C.c_int), _p(cm)))
+    return dict(box=box, xyz=xyz, q=q, ids=ids, groups=dict(begin=gb, size=gs, molid=gm), mass_center=cm)
 
 
 class Server:

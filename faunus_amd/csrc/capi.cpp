@@ -121,6 +121,39 @@ int fmc_set_groups(fmc_engine* e, int ng, const int* begin, const int* size, con
     return guarded([&] { eng(e).set_groups(ng, begin, size, molid, cm); });
 }
 
+int fmc_parse_state(const char* state_json, int* n, int* num_groups, double* box, double* xyz, double* charge,
+                    int* atom_id, int* begin, int* size, int* molid, double* mass_center) {
+    return guarded([&] {
+        if (!state_json) throw std::runtime_error("null state text");
+        const fmc::StateCfg st = fmc::StateCfg::from_json(state_json);
+        const int np = static_cast<int>(st.charge.size()), ng = static_cast<int>(st.size.size());
+        if (n) *n = np;
+        if (num_groups) *num_groups = ng;
+        if (box) std::copy(st.box, st.box + 3, box);
+        if (xyz) std::copy(st.xyz.begin(), st.xyz.end(), xyz);
+        if (charge) std::copy(st.charge.begin(), st.charge.end(), charge);
+        if (atom_id) std::copy(st.atom_id.begin(), st.atom_id.end(), atom_id);
+        if (begin) std::copy(st.begin.begin(), st.begin.end(), begin);
+        if (size) std::copy(st.size.begin(), st.size.end(), size);
+        if (molid) std::copy(st.molid.begin(), st.molid.end(), molid);
+        if (mass_center) std::copy(st.mass_center.begin(), st.mass_center.end(), mass_center);
+    });
+}
+
+int fmc_load_state(fmc_engine* e, const char* state_json) {
+    return guarded([&] {
+        if (!state_json) throw std::runtime_error("null state text");
+        const fmc::StateCfg st = fmc::StateCfg::from_json(state_json);
+        const double* box = eng(e).model().box;
+        for (int d = 0; d < 3; ++d)
+            if (std::fabs(st.box[d] - box[d]) > 1e-12 * std::fabs(box[d]))
+                throw std::runtime_error("state: box differs from the input's geometry");
+        eng(e).set_particles(static_cast<int>(st.charge.size()), st.xyz.data(), st.charge.data(), st.atom_id.data());
+        eng(e).set_groups(static_cast<int>(st.size.size()), st.begin.data(), st.size.data(), st.molid.data(),
+                          st.mass_center.data());
+    });
+}
+
 int fmc_init(fmc_engine* e, double* initial_energy) {
     return guarded([&] {
         eng(e).init();

@@ -426,4 +426,37 @@ Model Model::from_json(const std::string& text) {
     return m;
 }
 
+StateCfg StateCfg::from_json(const std::string& text) {
+    const jsonlite::Value j = jsonlite::parse(text);
+    StateCfg st;
+    const jsonlite::Value& geo = j.at("geometry");
+    if (geo.value("type", "cuboid") != "cuboid") throw std::runtime_error("state: only cuboid geometry is supported");
+    const jsonlite::Value& len = geo.at("length");
+    for (int d = 0; d < 3; ++d) st.box[d] = len.is_array() ? len.at(static_cast<size_t>(d)).as_number() : len.as_number();
+    if (!j.contains("groups")) throw std::runtime_error("state: 'groups' missing (insertmolecules is not supported)");
+    for (const auto& p : j.at("particles").items()) {
+        const jsonlite::Value& pos = p.at("pos");
+        if (pos.size() != 3) throw std::runtime_error("state: particle position needs three coordinates");
+        for (int d = 0; d < 3; ++d) st.xyz.push_back(pos.at(static_cast<size_t>(d)).as_number());
+        st.charge.push_back(p.at("q").as_number());
+        st.atom_id.push_back(static_cast<int>(p.at("id").as_number()));
+    }
+    int begin = 0;
+    for (const auto& g : j.at("groups").items()) {
+        const int size = static_cast<int>(g.at("size").as_number());
+        const int capacity = static_cast<int>(g.value("capacity", static_cast<double>(size)));
+        if (size < 0 || capacity != size)
+            throw std::runtime_error("state: groups with inactive particles (capacity > size) are not supported");
+        st.begin.push_back(begin);
+        st.size.push_back(size);
+        st.molid.push_back(static_cast<int>(g.at("id").as_number()));
+        const jsonlite::Value& cm = g.at("cm");
+        for (int d = 0; d < 3; ++d) st.mass_center.push_back(cm.at(static_cast<size_t>(d)).as_number());
+        begin += capacity;
+    }
+    if (begin != static_cast<int>(st.charge.size()))
+        throw std::runtime_error("state: the groups do not cover the particles exactly (load error)");
+    return st;
+}
+
 } // namespace fmc

@@ -109,4 +109,16 @@ struct Model {
     static Model from_json(const std::string& text);
 };
 
+// A reference state file (the Space JSON of Space::to_json / from_json, space.cpp:449-500,
+// with Group to_json / from_json, group.cpp:231-253): cuboid geometry, particles in order
+// (pos, q, atom id) and groups laid end to end (molecule id, size, mass centre).
+struct StateCfg {
+    double box[3] = {0, 0, 0};
+    std::vector<double> xyz, charge;
+    std::vector<int> atom_id;
+    std::vector<int> begin, size, molid;
+    std::vector<double> mass_center; // 3 per group
+    static StateCfg from_json(const std::string& text);
+};
+
 } // namespace fmc

@@ -62,6 +62,23 @@ int fmc_destroy(fmc_engine* engine);
 /* Space::particles (src/particle.h:220-227): positions (3n), charges, atom ids. */
 int fmc_set_particles(fmc_engine* engine, int n, const double* xyz, const double* charge, const int* atom_id);
 
+/*
+ * A reference state file: the Space JSON written by Space::to_json and read by
+ * MetropolisMonteCarlo::restore (src/montecarlo.cpp:118-137; src/space.cpp:449-500;
+ * src/group.cpp:231-253).  Reads the cuboid box, the particles (pos, q, id) and the
+ * groups (molecule id, size, mass centre, laid end to end).  fmc_parse_state fills
+ * whichever arrays are non-null (call it first with nulls for n and num_groups): box[3],
+ * xyz[3n], charge[n], atom_id[n], begin/size/molid[num_groups], mass_center[3 num_groups].
+ * fmc_load_state checks the box against the input's geometry and sets the engine's
+ * particles and groups (fmc_init follows).  Groups with inactive particles
+ * (capacity > size) and states without groups are errors, as is any mismatch between
+ * the group sizes and the particle count (the reference's "load error").  The random
+ * generator states in the file are not used: proposals come from the caller or Philox.
+ */
+int fmc_parse_state(const char* state_json, int* n, int* num_groups, double* box, double* xyz, double* charge,
+                    int* atom_id, int* begin, int* size, int* molid, double* mass_center);
+int fmc_load_state(fmc_engine* engine, const char* state_json);
+
 /* Space::groups (src/group.h:177-194): contiguous ranges in particle order. */
 int fmc_set_groups(fmc_engine* engine, int num_groups, const int* begin, const int* size, const int* molid,
                    const double* mass_center);

@@ -7,6 +7,8 @@ the reference checkout and writes small ``.npz`` fixtures next to this script:
   ``systemenergy.init`` from examples/minimal/minimal.out.json:10
 * ``bulk.npz``     examples/bulk/bulk.state.json (2304 ions) + golden
   ``systemenergy.init`` from examples/bulk/bulk.out.json:28
+* ``minimal.state.json``  examples/minimal/minimal.state.json as data (the input of the
+  state-file loader tests)
 
 Usage: python tests/golden/make_fixtures.py /path/to/reference
 """
@@ -43,7 +45,15 @@ def extract(ref, name):
     print(name, len(q), init, mean)
 
 
+def copy_state(ref, name):
+    with open(os.path.join(ref, "examples", name, f"{name}.state.json")) as f:
+        state = json.load(f)
+    with open(os.path.join(HERE, f"{name}.state.json"), "w") as f:
+        json.dump(state, f, indent=1)
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     extract(ref, "minimal")
     extract(ref, "bulk")
+    copy_state(ref, "minimal")
