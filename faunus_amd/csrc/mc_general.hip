@@ -928,12 +928,12 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     }
 }
 
-// Workgroup sizes: 512 threads (two waves per SIMD) for the kernels that walk all
-// particles, 256 for the cell-list kernels (their stencil tables take more LDS).
-constexpr int kWide = 512, kNarrow = 256;
+// Workgroup size: 512 threads, two waves per SIMD (the second wave hides the latency
+// of the first; some cold paths spill to scratch), up to 8 k-vectors per thread.
+constexpr int kWide = 512;
 
-int threads_of(bool cells) { return cells ? kNarrow : kWide; }
-int kpt_cap(bool cells) { return cells ? 16 : 8; } // 4096 k-vectors per workgroup either way
+int threads_of(bool) { return kWide; }
+int kpt_cap(bool) { return 8; } // 4096 k-vectors per workgroup
 
 template <int kThreads, int kMaxKPT, bool kCells, bool kF32, bool kSplit>
 const void* kernel_of() {
@@ -942,8 +942,8 @@ const void* kernel_of() {
 
 const void* select_kernel(int maxkpt, bool cells, bool f32, bool split) {
     if (cells) {
-        if (maxkpt <= 8) return f32 ? kernel_of<kNarrow, 8, true, true, false>() : kernel_of<kNarrow, 8, true, false, false>();
-        return f32 ? kernel_of<kNarrow, 16, true, true, false>() : kernel_of<kNarrow, 16, true, false, false>();
+        if (maxkpt <= 4) return f32 ? kernel_of<kWide, 4, true, true, false>() : kernel_of<kWide, 4, true, false, false>();
+        return f32 ? kernel_of<kWide, 8, true, true, false>() : kernel_of<kWide, 8, true, false, false>();
     }
     if (maxkpt <= 4) {
         if (split) return f32 ? kernel_of<kWide, 4, false, true, true>() : kernel_of<kWide, 4, false, false, true>();
