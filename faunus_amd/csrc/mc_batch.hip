@@ -561,6 +561,7 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.sample_every = 0;
     s.e_samples = 0;
     s.e_sum = s.e_sum2 = 0.0;
+    s.pair_range = e.model().pair_range;
     const DevCoulomb& dc = e.dev_coulomb();
     if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
     s.terms = 0;

@@ -90,6 +90,7 @@ struct ReplicaState {
     long long sample_every;
     unsigned long long e_samples;
     double e_sum, e_sum2;
+    double pair_range;        // distance beyond which every pair term vanishes (cell-list forms)
 };
 
 // One recorded move of the trace (tests).

@@ -34,6 +34,7 @@ constexpr int kMaxKinds = 8;   // move kinds per replica
 constexpr int kChunk = 8;      // moved atoms per Ewald phase-table chunk
 constexpr int kMaxComb = 512;  // (moved atom, state, cell) combinations of the flattened stencil walk
 constexpr int kMaxParts = 64;  // workgroups per replica in the split form (one waiting lane each)
+constexpr int kMolStencil = 16; // moved atoms that may share one stencil (each member against all of them)
 
 // Exchange slot of one part for one move parity: the partial sums (pair new/old,
 // Ewald, bonds new/old) and the tag of the move they belong to (move index + 1).
@@ -564,8 +565,23 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 // 27-cell stencils around each moved atom's new (st=0) and old (st=1)
                 // position, flattened: the members of all (atom, state, cell)
                 // combinations are numbered by a prefix sum and dealt to the lanes in
-                // strides, with the loads of several members issued together
-                const int ncomb = m * 54;
+                // strides, with the loads of several members issued together.  When
+                // the pair range plus the moved atoms' largest distance from the first
+                // one fits in a cell (a small molecule), the first atom's stencils hold
+                // every partner of every moved atom: one stencil per state is walked and
+                // each member is paired with all moved atoms (pairs beyond the range
+                // add exactly zero)
+                bool mol = false;
+                if (m > 1 && m <= kMolStencil) {
+                    double ext2 = 0.0;
+                    for (int a = 1; a < m; ++a) {
+                        ext2 = fmax(ext2, dist2(box, s_new[a].x, s_new[a].y, s_new[a].z, s_new[0].x, s_new[0].y, s_new[0].z));
+                        ext2 = fmax(ext2, dist2(box, s_old[a].x, s_old[a].y, s_old[a].z, s_old[0].x, s_old[0].y, s_old[0].z));
+                    }
+                    mol = S.pair_range + sqrt(ext2) <= 1.0 / S.inv_cell;
+                }
+                const int na = mol ? 1 : m; // stencil anchors per state
+                const int ncomb = na * 54;
                 int carry = 0;
                 for (int base = 0; base < ncomb; base += kThreads) {
                     const int cb = base + tid;
@@ -618,22 +634,26 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                     for (int u = 0; u < kU; ++u) {
                         const int j = jj[u];
                         if (j < 0 || (j >= P.mlo && j < P.mhi)) continue;
-                        const int a = cc[u] / 54, st = (cc[u] / 27) & 1;
-                        if (static_cast<int>(mj[u] >> 8) == g &&
-                            (!same_ok || pair_excluded(excl_off, excl, P.mlo + a, j)))
-                            continue;
-                        const double4 A = st == 0 ? s_new[a] : s_old[a];
-                        const int ta = s_mt[a], tj = static_cast<int>(mj[u] & 0xffu);
-                        double e;
-                        if (kF32) {
-                            const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                            e = static_cast<double>(pair_energy_ft(
-                                s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pf[u].w,
-                                dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
-                                       pf[u].x, pf[u].y, pf[u].z)));
-                        } else {
-                            e = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
-                                            dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
+                        const int st = (cc[u] / 27) & 1;
+                        const int alo = mol ? 0 : cc[u] / 54, ahi = mol ? m : alo + 1;
+                        const bool same = static_cast<int>(mj[u] >> 8) == g;
+                        if (same && !same_ok) continue;
+                        const int tj = static_cast<int>(mj[u] & 0xffu);
+                        double e = 0.0;
+                        for (int a = alo; a < ahi; ++a) {
+                            if (same && pair_excluded(excl_off, excl, P.mlo + a, j)) continue;
+                            const double4 A = st == 0 ? s_new[a] : s_old[a];
+                            const int ta = s_mt[a];
+                            if (kF32) {
+                                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                                e += static_cast<double>(pair_energy_ft(
+                                    s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pf[u].w,
+                                    dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y),
+                                                 static_cast<float>(A.z), pf[u].x, pf[u].y, pf[u].z)));
+                            } else {
+                                e += pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
+                                                 dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
+                            }
                         }
                         if (st == 0) e_new += e;
                         else e_old += e;

@@ -63,12 +63,19 @@ def test_polyelectrolyte_general_moves(fp32, excl):
     assert dq < 1e-9
 
 
-@pytest.mark.parametrize("n_mol,cells", [(216, False), (1000, True)])
-def test_water_general_moves(n_mol, cells):
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=3, kmax=6, lj_cutoff=10.0 if cells else None,
-                                                cutoff=10.0 if cells else 9.0)
+@pytest.mark.parametrize("n_mol,cells,rc", [(216, False, 9.0), (1000, True, 10.0), (1000, True, 8.0)])
+def test_water_general_moves(n_mol, cells, rc):
+    """Cell list at Rc = 10 A (cells of L/3 = 10.35 A: one stencil per moved atom) and at
+    Rc = 8 A (Rc + the O-H distance fits in a cell: one stencil per state for the whole
+    molecule)."""
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=3, kmax=6, lj_cutoff=rc if cells else None,
+                                                cutoff=rc)
+    if cells:
+        box = cfg["geometry"]["length"]
+        cell = box / np.floor(box / rc)
+        assert (rc + 1.0 <= cell) == (rc < 9.0)  # which stencil form the case exercises
     b, orc, e0, rep = run((cfg, xyz, q, ids, g), kinds, 1500, seed=8)
-    log("device general water", n_mol, rep)
+    log("device general water", n_mol, rc, rep)
     assert rep["first_mismatch"] == rep["n"] and rep["max_rel"] <= 1e-10, rep
     np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
     tot, _ = b.total_energy(0)
