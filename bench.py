@@ -272,14 +272,15 @@ HEADLINE_GROUPS = 16
 
 def bench_water(args, fp32=True, replicas=64, n_mol=4096, moves_per_step=3000):
     """64 rigid-water replicas (4096 molecules, L=49.68 A, Ewald alpha=0.28, Rc=10,
-    kmax=9, LJ on O truncated at 10 A, cell list) on a 280-360 K ladder with molecular
-    translate+rotate moves (dp 0.5 A, dprot 0.5 rad), one persistent launch per step
-    and one round of neighbour swaps after it."""
+    kmax=9, LJ on O truncated at 10 A, cell list with cells of at least Rc /
+    args.water_cell_divisions) on a 280-360 K ladder with molecular translate+rotate
+    moves (dp 0.5 A, dprot 0.5 rad), one persistent launch per step and one round of
+    neighbour swaps after it."""
     import faunus_amd as F
     from tests import systems
     comps = [systems.water(n_mol=n_mol, seed=100 + c) for c in range(4)]
     temps = np.linspace(280.0, 360.0, replicas)
-    b = F.Batch(fp32_pairs=fp32)
+    b = F.Batch(fp32_pairs=fp32, cell_divisions=args.water_cell_divisions)
     for r in range(replicas):
         cfg, xyz, q, ids, g, kinds = comps[r % len(comps)]
         b.add((cfg, xyz, q, ids, g), beta=298.15 / temps[r], seed=9000 + r, ladder=0, moves=kinds)
@@ -438,6 +439,8 @@ def main():
     ap.add_argument("--serve-split-atoms", type=int, default=2048,
                     help="particles per workgroup of a serving job (0: one workgroup per job)")
     ap.add_argument("--serve-max-parts", type=int, default=16)
+    ap.add_argument("--water-cell-divisions", type=int, default=2,
+                    help="water replicas: cells of at least Rc / d (1: 27-cell stencils, 2: 125)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
@@ -569,7 +572,7 @@ def main():
             "plain_fp64_moves_per_s": out["water_plain"]["rate"],
             "speedup_fp32_batched_vs_plain": out["water32"]["rate"] / out["water_plain"]["rate"],
             "acceptance": out["water32"]["acceptance"], "swaps_attempted_accepted": out["water32"]["swaps"],
-            "max_energy_drift": out["water32"]["max_drift"]}
+            "max_energy_drift": out["water32"]["max_drift"], "cell_divisions": args.water_cell_divisions}
     if "serve" in out:
         line["forms"]["serving_poisson_50_per_s"] = out["serve"]
     if "gsplit" in out:

@@ -67,6 +67,7 @@ SIGNATURES = {
     "fmc_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, I, U8, D, D]),
     "fmc_batch_total_energy": (C.c_int, [C.c_void_p, C.c_int, D, D]),
     "fmc_batch_sample_energy": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "fmc_batch_set_cell_divisions": (C.c_int, [C.c_void_p, C.c_int]),
     "fmc_batch_energy_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_ulonglong), D, D]),
     "fmc_batch_rdf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                 C.POINTER(C.c_ulonglong)]),
@@ -362,12 +363,16 @@ class Engine:
 class Batch:
     """Replicas advanced by the device-resident Metropolis kernel (one workgroup each)."""
 
-    def __init__(self, fp32_pairs=False):
+    def __init__(self, fp32_pairs=False, cell_divisions=1):
+        """``cell_divisions`` 2 gives replicas with a cell list cells of half the pair
+        range and 125-cell stencils (move-set replicas only)."""
         h = C.c_void_p()
         _check(lib().fmc_batch_create(int(bool(fp32_pairs)), C.byref(h)))
         self.h = h
         self.sizes = []
         self.nterms = []
+        if cell_divisions != 1:
+            _check(lib().fmc_batch_set_cell_divisions(self.h, int(cell_divisions)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -404,6 +404,10 @@ fmc::Batch& bat(fmc_batch* b) {
 
 extern "C" {
 
+int fmc_batch_set_cell_divisions(fmc_batch* b, int divisions) {
+    return guarded([&] { bat(b).set_cell_divisions(divisions); });
+}
+
 int fmc_batch_create(int fp32_pairs, fmc_batch** out) {
     return guarded([&] {
         int n = 0;

@@ -816,16 +816,28 @@ void Engine::full_q(double2* out) {
     check_hip(hipGetLastError(), "full_q_kernel");
 }
 
+void Engine::set_cell_divisions(int d) {
+    if (d < 1 || d > 2) throw std::runtime_error("cell divisions: 1 or 2");
+    if (d_pos_ != nullptr) throw std::runtime_error("cell divisions must be set before init");
+    cell_divisions_ = d;
+}
+
 void Engine::build_cell_list() {
     cells_.enabled = false;
     const double range = model_.pair_range;
     if (!(range > 0) || !std::isfinite(range)) return;
     int nc[3];
+    // cells of at least range / r with a (2r+1)^3 stencil; the finer grid needs 2r+1
+    // cells per side, else the 27-cell grid is used
+    int r = cell_divisions_;
+    for (int d = 0; d < 3; ++d)
+        if (std::floor(model_.box[d] * r / range) < 2 * r + 1) r = 1;
     for (int d = 0; d < 3; ++d) {
         if (range > 0.5 * model_.box[d]) return;
-        nc[d] = static_cast<int>(std::floor(model_.box[d] / range));
-        if (nc[d] < 3) return;
+        nc[d] = static_cast<int>(std::floor(model_.box[d] * r / range));
+        if (nc[d] < 2 * r + 1) return;
     }
+    cells_.radius = r;
     const long ncell = static_cast<long>(nc[0]) * nc[1] * nc[2];
     if (ncell > (1L << 24)) return;
     std::vector<int> count(static_cast<size_t>(ncell), 0), cell_of(static_cast<size_t>(n_));
@@ -976,6 +988,8 @@ void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_block
         if (staged_cm_group_ == args.change_group)
             for (int d = 0; d < 3; ++d) cn[d] = staged_cm_[d];
     }
+    if (cells_.enabled && cells_.radius != 1)
+        throw std::runtime_error("the per-move path walks 27-cell stencils (cell divisions 1)");
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, stream_, d_pos_, d_meta_, n_, d_pairs_,
                            model_.ntypes, d_coul_, box, gt, ca, args, pair_blocks, d_kn_, d_ak_, d_q_acc_, d_q_trial_,

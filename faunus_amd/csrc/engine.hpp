@@ -77,6 +77,7 @@ struct CellList {
     bool enabled = false;
     int nc[3] = {0, 0, 0};
     int cap = 0;
+    int radius = 1;         // stencil half-width in cells: cells >= pair range / radius
     double inv_cell[3] = {0, 0, 0};
     int* count = nullptr;   // [ncell]
     int* members = nullptr; // [ncell * cap]
@@ -123,6 +124,10 @@ class Engine {
     void timing(double* total_ms, long* launches);
     long launches() const { return launches_; }
     bool cell_list_enabled() const { return cells_.enabled; }
+    // Cells of at least pair range / d (d = 1 or 2) with a (2d+1)^3 stencil; before init().
+    // Only the general device kernel walks d = 2; the other paths need d = 1.
+    void set_cell_divisions(int d);
+    int cell_radius() const { return cells_.radius; }
 
     // Pair-parameter and spline tables (as uploaded).
     const std::vector<DevPair>& dev_pairs() const { return h_pairs_; }
@@ -200,6 +205,7 @@ class Engine {
     std::vector<MovedParticle> staged_;
     // cell list
     CellList cells_;
+    int cell_divisions_ = 1;
     // pending change (between delta and sync)
     bool pending_ = false;
     std::vector<MovedParticle> pending_moved_;

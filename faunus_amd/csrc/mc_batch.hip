@@ -562,6 +562,8 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.e_samples = 0;
     s.e_sum = s.e_sum2 = 0.0;
     s.pair_range = e.model().pair_range;
+    s.cell_radius = e.cell_radius();
+    s.pad3 = 0;
     const DevCoulomb& dc = e.dev_coulomb();
     if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
     s.terms = 0;
@@ -696,13 +698,22 @@ int Batch::add_replica(const std::string& config_json, int n, const double* xyz,
     auto e = std::make_unique<Engine>(config_json);
     e->set_particles(n, xyz, charge, ids);
     e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
+    e->set_cell_divisions(cell_divisions_);
     e->init();
+    if (!general && e->cell_list_enabled() && e->cell_radius() != 1)
+        throw std::runtime_error("the finer cell grid is walked by the general move-set kernel only");
     ReplicaState s = make_replica_state(*e, fp32_ != 0, seed, static_cast<int>(engines_.size()), beta, ladder, owned_,
                                         kinds);
     general_ = general;
     h_states_.push_back(s);
     engines_.push_back(std::move(e));
     return s.replica_id;
+}
+
+void Batch::set_cell_divisions(int d) {
+    if (d < 1 || d > 2) throw std::runtime_error("cell divisions: 1 or 2");
+    if (!engines_.empty()) throw std::runtime_error("set the cell divisions before adding replicas");
+    cell_divisions_ = d;
 }
 
 int Batch::threads_for(int r) const {

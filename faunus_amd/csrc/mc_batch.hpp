@@ -91,6 +91,8 @@ struct ReplicaState {
     unsigned long long e_samples;
     double e_sum, e_sum2;
     double pair_range;        // distance beyond which every pair term vanishes (cell-list forms)
+    int cell_radius;          // stencil half-width in cells (1: 27 cells, 2: 125 cells)
+    int pad3;
 };
 
 // One recorded move of the trace (tests).
@@ -134,6 +136,9 @@ int general_coresident_blocks(bool f32);
 class Batch {
   public:
     explicit Batch(int precision_fp32);
+    // cells of at least pair range / d for the replicas added afterwards (d = 1 or 2; the
+    // finer grid is walked by the general kernel only)
+    void set_cell_divisions(int d);
     ~Batch();
     int add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                     int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
@@ -170,6 +175,7 @@ class Batch {
   private:
     int fp32_ = 0;
     int general_ = -1; // -1 unset, 0 translate-only replicas, 1 general move sets
+    int cell_divisions_ = 1;
     std::vector<std::unique_ptr<Engine>> engines_;
     std::vector<ReplicaState> h_states_;
     std::vector<void*> owned_; // device buffers made for the replicas

@@ -561,37 +561,39 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         base += kThreads;
                     }
                 }
-            } else if (m * 54 <= kMaxComb) {
-                // 27-cell stencils around each moved atom's new (st=0) and old (st=1)
-                // position, flattened: the members of all (atom, state, cell)
-                // combinations are numbered by a prefix sum and dealt to the lanes in
-                // strides, with the loads of several members issued together.  When
-                // the pair range plus the moved atoms' largest distance from the first
-                // one fits in a cell (a small molecule), the first atom's stencils hold
-                // every partner of every moved atom: one stencil per state is walked and
-                // each member is paired with all moved atoms (pairs beyond the range
-                // add exactly zero)
-                bool mol = false;
-                if (m > 1 && m <= kMolStencil) {
-                    double ext2 = 0.0;
-                    for (int a = 1; a < m; ++a) {
-                        ext2 = fmax(ext2, dist2(box, s_new[a].x, s_new[a].y, s_new[a].z, s_new[0].x, s_new[0].y, s_new[0].z));
-                        ext2 = fmax(ext2, dist2(box, s_old[a].x, s_old[a].y, s_old[a].z, s_old[0].x, s_old[0].y, s_old[0].z));
-                    }
-                    mol = S.pair_range + sqrt(ext2) <= 1.0 / S.inv_cell;
-                }
-                const int na = mol ? 1 : m; // stencil anchors per state
-                const int ncomb = na * 54;
+            } else {
+              // stencils of (2r+1)^3 cells (cells of at least range / r) around each moved
+              // atom's new (st=0) and old (st=1) position.  When the pair range plus the
+              // moved atoms' largest distance from the first one fits in r cells (a small
+              // molecule), the first atom's stencils hold every partner of every moved
+              // atom: one stencil per state is walked and each member is paired with all
+              // moved atoms (pairs beyond the range add exactly zero)
+              const int cr = S.cell_radius, cw = 2 * cr + 1, ns = cw * cw * cw;
+              bool mol = false;
+              if (m > 1 && m <= kMolStencil) {
+                  double ext2 = 0.0;
+                  for (int a = 1; a < m; ++a) {
+                      ext2 = fmax(ext2, dist2(box, s_new[a].x, s_new[a].y, s_new[a].z, s_new[0].x, s_new[0].y, s_new[0].z));
+                      ext2 = fmax(ext2, dist2(box, s_old[a].x, s_old[a].y, s_old[a].z, s_old[0].x, s_old[0].y, s_old[0].z));
+                  }
+                  mol = S.pair_range + sqrt(ext2) <= static_cast<double>(cr) / S.inv_cell;
+              }
+              const int na = mol ? 1 : m; // stencil anchors per state
+              if (na * 2 * ns <= kMaxComb) {
+                // flattened: the members of all (anchor, state, cell) combinations are
+                // numbered by a prefix sum and dealt to the lanes in strides, with the
+                // loads of several members issued together
+                const int ncomb = na * 2 * ns;
                 int carry = 0;
                 for (int base = 0; base < ncomb; base += kThreads) {
                     const int cb = base + tid;
                     int cnt = 0;
                     if (cb < ncomb) {
-                        const int a = cb / 54, st = (cb / 27) & 1, c = cb % 27;
+                        const int a = cb / (2 * ns), st = (cb / ns) & 1, c = cb % ns;
                         const double4 A = st == 0 ? s_new[a] : s_old[a];
-                        const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
-                        const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
-                        const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                        const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % cw) - cr + S.nc0) % S.nc0;
+                        const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / cw) % cw) - cr + S.nc1) % S.nc1;
+                        const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / (cw * cw)) - cr + S.nc2) % S.nc2;
                         const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
                         s_ccell[cb] = cell;
                         cnt = S.cell_count[cell];
@@ -634,8 +636,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                     for (int u = 0; u < kU; ++u) {
                         const int j = jj[u];
                         if (j < 0 || (j >= P.mlo && j < P.mhi)) continue;
-                        const int st = (cc[u] / 27) & 1;
-                        const int alo = mol ? 0 : cc[u] / 54, ahi = mol ? m : alo + 1;
+                        const int st = (cc[u] / ns) & 1;
+                        const int alo = mol ? 0 : cc[u] / (2 * ns), ahi = mol ? m : alo + 1;
                         const bool same = static_cast<int>(mj[u] >> 8) == g;
                         if (same && !same_ok) continue;
                         const int tj = static_cast<int>(mj[u] & 0xffu);
@@ -659,15 +661,15 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         else e_old += e;
                     }
                 }
-            } else {
+              } else {
                 // many moved atoms: one (atom, state, cell) combination per lane
-                for (int cb = tid; cb < m * 54; cb += kThreads) {
-                    const int a = cb / 54, st = (cb / 27) & 1, c = cb % 27;
+                for (int cb = tid; cb < m * 2 * ns; cb += kThreads) {
+                    const int a = cb / (2 * ns), st = (cb / ns) & 1, c = cb % ns;
                     const double4 A = st == 0 ? s_new[a] : s_old[a];
                     const int ta = s_mt[a];
-                    const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
-                    const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
-                    const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                    const int cx = (cell_of_coord(A.x, half, S.inv_cell, S.nc0) + (c % cw) - cr + S.nc0) % S.nc0;
+                    const int cy = (cell_of_coord(A.y, half, S.inv_cell, S.nc1) + ((c / cw) % cw) - cr + S.nc1) % S.nc1;
+                    const int cz = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) + (c / (cw * cw)) - cr + S.nc2) % S.nc2;
                     const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
                     const int cnt = S.cell_count[cell];
                     for (int kk = 0; kk < cnt; ++kk) {
@@ -694,6 +696,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         else e_old += u;
                     }
                 }
+              }
             }
             // moved x moved inside the group (internal changes of flexible groups)
             if (same_ok && m > 1) {

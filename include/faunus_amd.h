@@ -195,6 +195,14 @@ int fmc_get_timing(fmc_engine* engine, double* total_ms, long* launches);
  */
 typedef struct fmc_batch fmc_batch;
 int fmc_batch_create(int fp32_pairs, fmc_batch** out);
+/*
+ * Cell-list grid of the replicas added afterwards: cells of at least pair range / d with
+ * a (2d+1)^3 stencil (d = 1: 27 cells, the default; d = 2: 125 smaller cells, about 2.4x
+ * fewer candidates).  d = 2 is walked by the general move-set kernel only; translate-only
+ * replicas and the per-move path refuse it.  No reference counterpart (the reference
+ * loops over all pairs, src/energy.h:1155-1224).
+ */
+int fmc_batch_set_cell_divisions(fmc_batch* batch, int divisions);
 int fmc_batch_destroy(fmc_batch* batch);
 int fmc_batch_add_replica(fmc_batch* batch, const char* config_json, int n, const double* xyz, const double* charge,
                           const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,

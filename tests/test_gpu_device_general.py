@@ -32,10 +32,10 @@ def compare(tr, rec, e_scale):
                 accepted=int(rec["accepted"][:first].sum()))
 
 
-def run(sysd, kinds, nm, seed, fp32=False):
+def run(sysd, kinds, nm, seed, fp32=False, cell_divisions=1):
     cfg, xyz, q, ids, g = sysd
     orc, e0 = oracle_of(cfg, xyz, q, ids, g)
-    b = F.Batch(fp32_pairs=fp32)
+    b = F.Batch(fp32_pairs=fp32, cell_divisions=cell_divisions)
     b.add(sysd, seed=seed, moves=kinds)
     b.run([nm], trace=nm)
     b.sync()
@@ -80,6 +80,30 @@ def test_water_general_moves(n_mol, cells, rc):
     np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
     tot, _ = b.total_energy(0)
     assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+
+
+@pytest.mark.parametrize("rc,molecule_stencil", [(9.0, True), (10.0, False)])
+def test_water_on_the_finer_cell_grid(rc, molecule_stencil):
+    """Cells of at least Rc / 2 and 125-cell stencils (L = 31.05 A, 6 cells of 5.18 A):
+    at Rc = 9 A, Rc plus the O-H distance fits two cells and one stencil per state serves
+    all three atoms; at Rc = 10 A each atom walks its own stencil, one (atom, state,
+    cell) combination per lane."""
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=3, kmax=6, lj_cutoff=rc, cutoff=rc)
+    box = cfg["geometry"]["length"]
+    cell = box / np.floor(2 * box / rc)
+    assert (rc + 1.0 <= 2 * cell) == molecule_stencil
+    b, orc, e0, rep = run((cfg, xyz, q, ids, g), kinds, 1500, seed=8, cell_divisions=2)
+    log("device general water, finer cells", rc, rep)
+    assert rep["first_mismatch"] == rep["n"] and rep["max_rel"] <= 1e-10, rep
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+
+
+def test_finer_cell_grid_is_refused_by_the_translate_kernel():
+    b = F.Batch(cell_divisions=2)
+    with pytest.raises(RuntimeError, match="general"):
+        b.add(systems.nacl_lj(n_pairs=800, box=255.0 * (1600 / 20000) ** (1 / 3), seed=2, kmax=9), seed=1)
 
 
 def test_batch_of_uneven_polyelectrolytes_and_ladder():

@@ -64,6 +64,24 @@ def test_split_polyelectrolyte_fp64(groups):
     check_fp64(b, orc, rep, "split polyelectrolyte groups=%d" % groups)
 
 
+def test_split_polyelectrolyte_with_exclusions_fp64():
+    """Bonded neighbours up to two bonds apart excluded from the nonbonded pairs; pivots
+    move up to 39 beads whose exclusion lists span several workgroups' slices."""
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 400, seed=7, kmax=6, excluded_neighbours=2)
+    sysd = (cfg, xyz, q, ids, g)
+    orc, e0 = oracle_of(*sysd)
+    nm = 3000
+    b = F.Batch(fp32_pairs=False)
+    b.add(sysd, seed=34, moves=kinds)
+    print("split polyelectrolyte exclusions groups=8", flush=True)
+    b.run_split(nm, groups=8, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    rec = orc.run_philox_moves(nm, seed=34, kinds=kinds, replica=0)
+    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)))
+    check_fp64(b, orc, rep, "split polyelectrolyte exclusions")
+
+
 def test_split_polyelectrolyte_fp32():
     cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(40, 400, seed=7, kmax=6)
     sysd = (cfg, xyz, q, ids, g)
