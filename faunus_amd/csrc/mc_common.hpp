@@ -11,9 +11,15 @@ namespace fmc {
 
 constexpr int kMaxGk = 1025; // |n|^2 table (ncut <= 32)
 
-struct DevPairF {
+struct alignas(16) DevPairF {
     float lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
 };
+
+// An empty asm statement that takes and returns the values: the loads that produce
+// them happen here, unconditionally, and are not moved into the branches the compiler
+// would otherwise make of the selects that use them (a branch per load serialises the
+// LDS latency of every pair term).
+__device__ __forceinline__ void materialize(float4& v) { asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
 
 // Pair terms present anywhere in a replica's pair table (a uniform switch per kernel).
 constexpr int kTermLJ = 1, kTermWCA = 2, kTermHS = 4, kTermAll = 7;
@@ -63,21 +69,26 @@ __device__ __forceinline__ float dist2f_round(float L, float invL, float ax, flo
 
 __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
                                                 int terms, float qa, float qb, float r2) {
+    // both halves of the pair parameters in two vector loads, then register arithmetic
+    float4 pa = reinterpret_cast<const float4*>(&p)[0]; // lj_s2, lj_e4, lj_cut2, wca_s2
+    float4 pb = reinterpret_cast<const float4*>(&p)[1]; // wca_e4, hs_s2, coul, -
+    materialize(pa);
+    materialize(pb);
     float u = 0.0f;
     if (terms & (kTermLJ | kTermWCA)) {
         const float ir2 = __builtin_amdgcn_rcpf(r2);
         if (terms & kTermLJ) {
-            float x = p.lj_s2 * ir2;
+            float x = pa.x * ir2;
             x = x * x * x;
-            u += (r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0f;
+            u += (r2 < pa.z) ? pa.y * (x * x - x) : 0.0f;
         }
         if (terms & kTermWCA) {
-            float w = p.wca_s2 * ir2;
+            float w = pa.w * ir2;
             w = w * w * w;
-            u += (r2 > p.wca_s2 * 1.2599210498948732f) ? 0.0f : p.wca_e4 * (w * w - w + 0.25f);
+            u += (r2 > pa.w * 1.2599210498948732f) ? 0.0f : pb.x * (w * w - w + 0.25f);
         }
     }
-    if (terms & kTermHS) u += (r2 < p.hs_s2) ? __builtin_huge_valf() : 0.0f;
+    if (terms & kTermHS) u += (r2 < pb.y) ? __builtin_huge_valf() : 0.0f;
     if (c.scheme != kCoulNone) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float e;
@@ -88,12 +99,14 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
             const float t = r * c.rc_inv * static_cast<float>(kSTable);
             int i = static_cast<int>(t);
             i = i > kSTable - 1 ? kSTable - 1 : i;
+            i = i < 0 ? 0 : i;
             const float f = t - static_cast<float>(i);
-            const float4 k = tab[i];
+            float4 k = tab[i];
+            materialize(k);
             const float sv = ((k.w * f + k.z) * f + k.y) * f + k.x;
             e = (r < c.rc) ? c.lB * (qa * qb * ir * sv) : 0.0f;
         }
-        u += (p.coul != 0.0f) ? e : 0.0f;
+        u += (pb.z != 0.0f) ? e : 0.0f;
     }
     return u;
 }

@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 
 #include "device_math.hpp"
 #include "mc_batch.hpp"
@@ -219,7 +220,9 @@ struct MovedSet {
 // Pair sums of the moved atoms (new and old coordinates) against the particles
 // base + tid + u * kThreads (u < kB) below je: the kB neighbours sit in registers and
 // the moved atoms stream from LDS, kB independent pair terms per atom and state.
-template <int kThreads, int kB, bool kF32>
+// kExcl: the system has pair exclusions (looked up per pair inside the moved group);
+// without them the loop over the neighbours is straight-line code.
+template <int kThreads, int kB, bool kF32, bool kExcl>
 __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms, const uint32_t* __restrict__ meta,
                                             const float4* __restrict__ pos32, const double4* __restrict__ pos,
                                             const DevPairF* s_pairs_f, const DevPair* s_pairs, int nt, const CoulF& cf,
@@ -254,8 +257,9 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
             float sn = 0.0f, so = 0.0f;
 #pragma unroll
             for (int u = 0; u < kB; ++u) {
-                const bool ok = use[u] == 1 ||
-                                (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a, j0 + u * kThreads));
+                const bool ok = kExcl ? (use[u] == 1 || (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
+                                                                                        j0 + u * kThreads)))
+                                      : use[u] != 0;
                 const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
                 const float un = pair_energy_ft(p, cf, s_stab, terms, qa, pf[u].w,
                                                 dist2f_round(fL, fIL, ax, ay, az, pf[u].x, pf[u].y, pf[u].z));
@@ -271,8 +275,9 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
             double sn = 0.0, so = 0.0;
 #pragma unroll
             for (int u = 0; u < kB; ++u) {
-                const bool ok = use[u] == 1 ||
-                                (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a, j0 + u * kThreads));
+                const bool ok = kExcl ? (use[u] == 1 || (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
+                                                                                        j0 + u * kThreads)))
+                                      : use[u] != 0;
                 const DevPair& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
                 const double un =
                     pair_energy_t(p, s_coul, A.w, pj[u].w, dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z), terms);
@@ -540,27 +545,32 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 // runs out, so that few lanes sit idle on short slices)
                 const MovedSet ms{P.mlo, P.mhi, m, g, same_ok, s_new, s_old, s_mt, excl_off, excl};
                 const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                int base = jb;
-                while (base < je) {
-                    const int rem = je - base;
-                    if (kF32 && rem >= 8 * kThreads) {
-                        moved_block<kThreads, 8, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
-                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
-                        base += 8 * kThreads;
-                    } else if (rem >= 4 * kThreads) {
-                        moved_block<kThreads, 4, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
-                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
-                        base += 4 * kThreads;
-                    } else if (rem > kThreads) {
-                        moved_block<kThreads, 2, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
-                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
-                        base += 2 * kThreads;
-                    } else {
-                        moved_block<kThreads, 1, kF32>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf, s_stab,
-                                                       s_coul, terms, box, fL, fIL, e_new, e_old);
-                        base += kThreads;
+                auto blocks = [&](auto excl_tag) {
+                    constexpr bool kX = decltype(excl_tag)::value;
+                    int base = jb;
+                    while (base < je) {
+                        const int rem = je - base;
+                        if (kF32 && rem >= 8 * kThreads) {
+                            moved_block<kThreads, 8, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                            base += 8 * kThreads;
+                        } else if (rem >= 4 * kThreads) {
+                            moved_block<kThreads, 4, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                            base += 4 * kThreads;
+                        } else if (rem > kThreads) {
+                            moved_block<kThreads, 2, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                            base += 2 * kThreads;
+                        } else {
+                            moved_block<kThreads, 1, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                            base += kThreads;
+                        }
                     }
-                }
+                };
+                if (excl_off != nullptr) blocks(std::true_type{});
+                else blocks(std::false_type{});
             } else {
               // stencils of (2r+1)^3 cells (cells of at least range / r) around each moved
               // atom's new (st=0) and old (st=1) position.  When the pair range plus the
