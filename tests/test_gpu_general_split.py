@@ -176,3 +176,34 @@ def test_split_serving_jobs_alone_equal_mixed(fp32):
         assert info["energy"] == a_info["energy"] and info["accepted"] == a_info["accepted"], (name, rep)
         assert np.array_equal(s.positions(j), a_xyz), (name, rep)
     s.close()
+
+
+def test_split_jobs_beyond_the_coresident_grid_run_in_several_launches():
+    """Six chain jobs of 300 particles at 5 particles per workgroup take 60 workgroups
+    each, 360 together: more than fit on the device at once (one 512-thread workgroup
+    per CU), so the server runs them in more than one cooperative launch per round.
+    Each result is bitwise the same as the job alone (one launch of 60)."""
+    jobs = []
+    for r in range(6):
+        cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(30, 300, seed=70 + r, kmax=5)
+        jobs.append(((cfg, xyz, q, ids, g), kinds, 600, 90 + r))
+    alone = []
+    for sysd, kinds, moves, seed in jobs:
+        s = F.Server(fp32_pairs=False, split_atoms=5, max_parts=64)
+        j = s.submit(sysd, moves, seed, kinds=kinds)
+        print("wide split job alone", seed, flush=True)
+        drain(s, quantum=10**6)
+        info = s.job(j)
+        assert info["state"] == F.Server.DONE, info
+        alone.append((info, s.positions(j)))
+        s.close()
+    s = F.Server(fp32_pairs=False, split_atoms=5, max_parts=64)
+    ids = [s.submit(sysd, moves, seed, kinds=kinds) for sysd, kinds, moves, seed in jobs]
+    print("wide split jobs mixed", flush=True)
+    drain(s, quantum=100)
+    for j, (a_info, a_xyz) in zip(ids, alone):
+        info = s.job(j)
+        assert info["state"] == F.Server.DONE, info
+        assert info["energy"] == a_info["energy"] and info["accepted"] == a_info["accepted"]
+        assert np.array_equal(s.positions(j), a_xyz)
+    s.close()
