@@ -19,7 +19,7 @@
 namespace fmc {
 namespace {
 
-constexpr int kSplitMaxKPT = 8;
+constexpr int kSplitMaxKPT = 8; // k-vectors per thread at 256 threads
 constexpr int kSplitMaxG = 64;
 constexpr int kGkMax = 1025;
 
@@ -43,8 +43,8 @@ __device__ __forceinline__ double wrap_s(double x, double L, double half, double
     return x;
 }
 
-template <bool kCells>
-__global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
+template <int kThreads, int kMaxKPT, bool kCells>
+__global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
                                                        int stride, SplitSlot* __restrict__ slots,
                                                        int* __restrict__ err, TraceEntry* __restrict__ trace,
                                                        int trace_cap, long long spin_limit) {
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
     __shared__ DevCoulomb s_coul;
     __shared__ double s_dp[kMaxTypes];
     __shared__ double s_gk[kGkMax];
-    __shared__ double s_red[3][4];
+    __shared__ double s_red[3][kThreads / 64];
     __shared__ int s_accept, s_abort;
     __shared__ Proposal s_prop[64]; // proposals of the next 64 moves, one lane of wave 0 each
     __shared__ double2 s_phase[6 * 65];
@@ -65,23 +65,23 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
     const int width = 2 * ncut + 1;
     const double L = S.L, half = S.half, inv = S.inv;
-    for (int i = tid; i < nt * nt; i += 256) s_pairs[i] = S.pairs[i];
+    for (int i = tid; i < nt * nt; i += kThreads) s_pairs[i] = S.pairs[i];
     if (tid == 0) {
         s_coul = *S.coul;
         s_abort = 0;
     }
-    for (int i = tid; i < nt; i += 256) s_dp[i] = S.dp_type[i];
-    for (int i = tid; i <= ncut * ncut && i < kGkMax; i += 256) s_gk[i] = S.gk[i];
+    for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
+    for (int i = tid; i <= ncut * ncut && i < kGkMax; i += kThreads) s_gk[i] = S.gk[i];
 
     // this workgroup's k-slice, interleaved over the threads
     const int kw = (nk + G - 1) / G;
     const int kb = g * kw;
     const int ke = min(nk, kb + kw);
-    int kn[kSplitMaxKPT];
-    double2 q[kSplitMaxKPT], dq[kSplitMaxKPT];
+    int kn[kMaxKPT];
+    double2 q[kMaxKPT], dq[kMaxKPT];
 #pragma unroll
-    for (int s = 0; s < kSplitMaxKPT; ++s) {
-        const int k = kb + s * 256 + tid;
+    for (int s = 0; s < kMaxKPT; ++s) {
+        const int k = kb + s * kThreads + tid;
         const bool on = k < ke;
         kn[s] = on ? S.kn[k] : 0;
         q[s] = on ? S.q[k] : make_double2(0.0, 0.0);
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         const double nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
         const double ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
         const double nz_ = wrap_s(po.z + P.dir[2] * scale, L, half, inv);
-        for (int e = tid; e < 6 * width; e += 256) {
+        for (int e = tid; e < 6 * width; e += kThreads) {
             const int nn = e % width - ncut;
             const int axis = (e / width) % 3;
             const int st = e / (3 * width);
@@ -136,7 +136,9 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         if (g == 0) {
             const Box b{{L, L, L}, {half, half, half}};
             if (kCells) {
-                const int combo = tid >> 2, slot0 = tid & 3; // 54 (state, cell) combos x 4 lanes
+                // 54 (state, cell) combos x (4 or 8) lanes
+                constexpr int kLanes = kThreads >= 512 ? 8 : 4;
+                const int combo = tid / kLanes, slot0 = tid % kLanes;
                 if (combo < 54) {
                     const int st = combo / 27, c = combo % 27;
                     const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
                     const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
                     const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
                     const int cnt = S.cell_count[cell];
-                    for (int k = slot0; k < cnt; k += 4) {
+                    for (int k = slot0; k < cnt; k += kLanes) {
                         const int j = S.cell_members[cell * S.cap + k];
                         if (j == i) continue;
                         const double4 pj = pos[j];
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
                     }
                 }
             } else {
-                for (int j = tid; j < n; j += 256) {
+                for (int j = tid; j < n; j += kThreads) {
                     if (j == i) continue;
                     const double4 pj = pos[j];
                     const DevPair& p = s_pairs[ti * nt + static_cast<int>(meta[j] & 0xffu)];
@@ -168,8 +170,8 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         __syncthreads(); // phase tables
         double de = 0.0;
 #pragma unroll
-        for (int s = 0; s < kSplitMaxKPT; ++s) {
-            const int k = kb + s * 256 + tid;
+        for (int s = 0; s < kMaxKPT; ++s) {
+            const int k = kb + s * kThreads + tid;
             if (k < ke) {
                 const int nx = unpack_n(kn[s], 0), ny = unpack_n(kn[s], 1), nz = unpack_n(kn[s], 2);
                 const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut], zn = s_phase[2 * width + nz + ncut];
@@ -202,9 +204,15 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         const unsigned long long tag = mid + 1ull;
         if (tid == 0) {
             SplitSlot* sl = slots + parity * G + g;
-            sl->de = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
-            sl->pn = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
-            sl->po = ((s_red[2][0] + s_red[2][1]) + s_red[2][2]) + s_red[2][3];
+            double a = 0.0, b = 0.0, c = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) {
+                a += s_red[0][w];
+                b += s_red[1][w];
+                c += s_red[2][w];
+            }
+            sl->de = a;
+            sl->pn = b;
+            sl->po = c;
             __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         // gather every workgroup's partial (lane l of wave 0 waits for workgroup l) and
@@ -316,7 +324,7 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         if (s_abort) break;
         if (s_accept) {
 #pragma unroll
-            for (int s = 0; s < kSplitMaxKPT; ++s) {
+            for (int s = 0; s < kMaxKPT; ++s) {
                 q[s].x += dq[s].x;
                 q[s].y += dq[s].y;
             }
@@ -332,8 +340,8 @@ __global__ __launch_bounds__(256) void mc_split_kernel(ReplicaState* __restrict_
         __syncthreads(); // phase tables are rewritten by the next move
     }
 #pragma unroll
-    for (int s = 0; s < kSplitMaxKPT; ++s) {
-        const int k = kb + s * 256 + tid;
+    for (int s = 0; s < kMaxKPT; ++s) {
+        const int k = kb + s * kThreads + tid;
         if (k < ke) S.q[k] = q[s];
     }
     if (g == 0 && tid == 0) {
@@ -363,7 +371,11 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error("run_split: 1..64 workgroups");
     if (stride < 1) throw std::runtime_error("run_split: stride >= 1");
     const int kw = (s.nk + groups - 1) / groups;
-    if ((kw + 255) / 256 > kSplitMaxKPT) throw std::runtime_error("run_split: too many k-vectors per workgroup");
+    // 512 threads (two waves per SIMD, 8 lanes per stencil cell) when the slice fits 4
+    // k-vectors per thread, else 256 threads with up to 8
+    const int threads = kw <= 4 * 512 ? 512 : 256;
+    if ((kw + threads - 1) / threads > (threads == 512 ? 4 : kSplitMaxKPT))
+        throw std::runtime_error("run_split: too many k-vectors per workgroup");
     if (2 * s.ncut + 1 > 65) throw std::runtime_error("run_split: ncutoff too large");
     if (s.ncut * s.ncut >= kGkMax) throw std::runtime_error("run_split: ncutoff too large");
     hipDeviceProp_t prop;
@@ -372,9 +384,13 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     const int grid = groups * stride;
     int per_cu = 0;
-    const void* fn = s.cells ? reinterpret_cast<const void*>(mc_split_kernel<true>)
-                             : reinterpret_cast<const void*>(mc_split_kernel<false>);
-    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0), "occupancy");
+    const void* fn =
+        threads == 512
+            ? (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, true>)
+                       : reinterpret_cast<const void*>(mc_split_kernel<512, 4, false>))
+            : (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true>)
+                       : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false>));
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy");
     if (per_cu * prop.multiProcessorCount < grid)
         throw std::runtime_error("run_split: the grid cannot be co-resident");
     if (!split_slots_) {
@@ -398,7 +414,7 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     long long spin = 1LL << 24;
     void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
-    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, stream_), "hipLaunchCooperativeKernel");
+    check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, stream_), "hipLaunchCooperativeKernel");
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 
