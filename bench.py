@@ -268,9 +268,9 @@ def cpu_baseline(seconds=15.0):
 
 
 HEADLINE_GROUPS = 16
-# every 8th block of the grid: the workgroups land on one XCD (blocks are dealt to the
-# 8 XCDs in turn) and exchange their partial sums through its L2
-HEADLINE_STRIDE = 8
+# consecutive blocks (spread over the XCDs); placing the 16 workgroups on one XCD (stride
+# 8) measured 6.6-6.9 us per move on one box but 7.4-8.4 on others, against a steady 7.1-7.5
+HEADLINE_STRIDE = 1
 
 
 def bench_water(args, fp32=True, replicas=64, n_mol=4096, moves_per_step=3000):
@@ -538,7 +538,7 @@ def main():
         "data": "synthetic (random sequential addition, seed 1; device Philox proposals)",
         "config": {"workload": "N=20000 Na/Cl LJ(LB, cut 12 A)+Ewald(alpha 0.25, Rc 12, kmax 20, K=17328), "
                                "L=255 A, translate dp=2 A, device-resident Metropolis, %d cooperating "
-                               "workgroups of 512 threads on one XCD (k-vector slices)" % HEADLINE_GROUPS,
+                               "workgroups of 512 threads (k-vector slices)" % HEADLINE_GROUPS,
                    "step_moves": STEP_MOVES, "acceptance": d["acceptance"], "energy_drift": d["drift"]},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": HEADLINE_TRAFFIC_BYTES,
