@@ -60,6 +60,14 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     __shared__ int s_accept, s_abort;
     __shared__ Proposal s_prop[64]; // proposals of the next 64 moves, one lane of wave 0 each
     __shared__ double2 s_phase[6 * 65];
+    // Workgroup 0 at 512 threads with a cell list: waves 1-7 load the next move's
+    // particle and its stencil neighbours while wave 0 waits for the other workgroups'
+    // partials.  The loads are kept (s_pf_ok) unless the accepted move was the next
+    // particle or changed one of the 54 loaded cells (s_mv_cell).
+    constexpr bool kPre = kCells && kThreads >= 512;
+    __shared__ double s_pf_pos[4];
+    __shared__ int s_pf_ti, s_pf_ok;
+    __shared__ int s_mv_cell[2];
 
     const int tid = threadIdx.x;
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
@@ -69,6 +77,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     if (tid == 0) {
         s_coul = *S.coul;
         s_abort = 0;
+        s_pf_ok = 0;
     }
     for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
     for (int i = tid; i <= ncut * ncut && i < kGkMax; i += kThreads) s_gk[i] = S.gk[i];
@@ -102,6 +111,11 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     // be visible yet, so reads of those particles take these values
     int h_idx[2] = {-1, -1};
     double h_pos[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    // this lane's loaded stencil cell of the next move (waves 1-7 of workgroup 0):
+    // the state's position, the cell, its count and its first two members
+    double f_px = 0.0, f_py = 0.0, f_pz = 0.0;
+    int f_cell = -1, f_cnt = 0, f_j0 = -1, f_j1 = -1, f_t0 = 0, f_t1 = 0;
+    double4 f_p0 = make_double4(0.0, 0.0, 0.0, 0.0), f_p1 = f_p0;
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -113,11 +127,19 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         }
         const Proposal P = s_prop[t & 63];
         const int i = P.index;
-        double4 po = pos[i];
-        // most recent accepted write of this particle wins
-        if (h_idx[1] == i) { po.x = h_pos[1][0]; po.y = h_pos[1][1]; po.z = h_pos[1][2]; }
-        else if (h_idx[0] == i) { po.x = h_pos[0][0]; po.y = h_pos[0][1]; po.z = h_pos[0][2]; }
-        const int ti = static_cast<int>(meta[i] & 0xffu);
+        const bool pre = kPre && g == 0 && s_pf_ok;
+        double4 po;
+        int ti;
+        if (pre) {
+            po = make_double4(s_pf_pos[0], s_pf_pos[1], s_pf_pos[2], s_pf_pos[3]);
+            ti = s_pf_ti;
+        } else {
+            po = pos[i];
+            // most recent accepted write of this particle wins
+            if (h_idx[1] == i) { po.x = h_pos[1][0]; po.y = h_pos[1][1]; po.z = h_pos[1][2]; }
+            else if (h_idx[0] == i) { po.x = h_pos[0][0]; po.y = h_pos[0][1]; po.z = h_pos[0][2]; }
+            ti = static_cast<int>(meta[i] & 0xffu);
+        }
         const double scale = s_dp[ti] * P.length_u;
         const double nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
         const double ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
@@ -135,7 +157,55 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         double e_new = 0.0, e_old = 0.0;
         if (g == 0) {
             const Box b{{L, L, L}, {half, half, half}};
-            if (kCells) {
+            if (kPre) {
+                // 54 (state, cell) combos x 8 lanes on waves 1-7; with the loads of the
+                // previous move kept, the first two members of each lane come from registers
+                const int combo = (tid - 64) >> 3, slot0 = (tid - 64) & 7;
+                if (tid >= 64 && combo < 54) {
+                    double px, py, pz;
+                    int cell, cnt;
+                    if (pre) {
+                        px = f_px;
+                        py = f_py;
+                        pz = f_pz;
+                        cell = f_cell;
+                        cnt = f_cnt;
+                    } else {
+                        const int st = combo / 27, c = combo % 27;
+                        px = st == 0 ? nx_ : po.x;
+                        py = st == 0 ? ny_ : po.y;
+                        pz = st == 0 ? nz_ : po.z;
+                        const int cx = (cell_coord_s(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                        const int cy = (cell_coord_s(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                        const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                        cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                        cnt = S.cell_count[cell];
+                    }
+                    double acc_u = 0.0;
+                    for (int k = slot0, u = 0; k < cnt; k += 8, ++u) {
+                        int j, tj;
+                        double4 pj;
+                        if (pre && u == 0) {
+                            j = f_j0;
+                            pj = f_p0;
+                            tj = f_t0;
+                        } else if (pre && u == 1) {
+                            j = f_j1;
+                            pj = f_p1;
+                            tj = f_t1;
+                        } else {
+                            j = S.cell_members[cell * S.cap + k];
+                            if (j == i) continue;
+                            pj = pos[j];
+                            tj = static_cast<int>(meta[j] & 0xffu);
+                        }
+                        if (j == i) continue;
+                        acc_u += pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w, dist2(b, px, py, pz, pj.x, pj.y, pj.z));
+                    }
+                    if (combo < 27) e_new = acc_u;
+                    else e_old = acc_u;
+                }
+            } else if (kCells) {
                 // 54 (state, cell) combos x (4 or 8) lanes
                 constexpr int kLanes = kThreads >= 512 ? 8 : 4;
                 const int combo = tid / kLanes, slot0 = tid % kLanes;
@@ -274,6 +344,12 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             else acc = P.accept_u <= exp(-bdu);
             s_accept = acc ? 1 : 0;
             if (g == 0) {
+                if (kPre) {
+                    const long long tn = t + 1;
+                    s_pf_ok = (tn < M && (tn & 63) != 0 && !(acc && s_prop[tn & 63].index == i)) ? 1 : 0;
+                    s_mv_cell[0] = -1;
+                    s_mv_cell[1] = -1;
+                }
                 if (trace && t < trace_cap) {
                     TraceEntry& T = trace[t];
                     T.index = i;
@@ -293,6 +369,10 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                                           cell_coord_s(ny_, half, S.inv_cell, S.nc1)) * S.nc0 +
                                          cell_coord_s(nx_, half, S.inv_cell, S.nc0);
                         const int cold = S.cell_of[i];
+                        if (kPre) {
+                            s_mv_cell[0] = cold;
+                            s_mv_cell[1] = cnew;
+                        }
                         if (cnew != cold) {
                             const int nold = S.cell_count[cold];
                             for (int kk = 0; kk < nold; ++kk)
@@ -320,8 +400,56 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 }
             }
         }
+        if (kPre && g == 0 && tid >= 64) {
+            // load the next move's particle and stencil while wave 0 waits
+            const long long tn = t + 1;
+            const int combo = (tid - 64) >> 3, slot0 = (tid - 64) & 7;
+            f_cell = -1;
+            if (tn < M && (tn & 63) != 0 && combo < 54) {
+                const Proposal& Q = s_prop[tn & 63];
+                const int in = Q.index;
+                const double4 qo = pos[in];
+                const int tq = static_cast<int>(meta[in] & 0xffu);
+                const double sc = s_dp[tq] * Q.length_u;
+                const int st = combo / 27, c = combo % 27;
+                if (st == 0) {
+                    f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
+                    f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
+                    f_pz = wrap_s(qo.z + Q.dir[2] * sc, L, half, inv);
+                } else {
+                    f_px = qo.x;
+                    f_py = qo.y;
+                    f_pz = qo.z;
+                }
+                if (tid == 64) {
+                    s_pf_pos[0] = qo.x;
+                    s_pf_pos[1] = qo.y;
+                    s_pf_pos[2] = qo.z;
+                    s_pf_pos[3] = qo.w;
+                    s_pf_ti = tq;
+                }
+                const int cx = (cell_coord_s(f_px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
+                const int cy = (cell_coord_s(f_py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
+                const int cz = (cell_coord_s(f_pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
+                f_cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                f_cnt = S.cell_count[f_cell];
+                f_j0 = slot0 < f_cnt ? S.cell_members[f_cell * S.cap + slot0] : -1;
+                f_j1 = slot0 + 8 < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + 8] : -1;
+                if (f_j0 >= 0) {
+                    f_p0 = pos[f_j0];
+                    f_t0 = static_cast<int>(meta[f_j0] & 0xffu);
+                }
+                if (f_j1 >= 0) {
+                    f_p1 = pos[f_j1];
+                    f_t1 = static_cast<int>(meta[f_j1] & 0xffu);
+                }
+            }
+        }
         __syncthreads();
         if (s_abort) break;
+        // the accepted move changed a loaded cell: load again at the next move
+        if (kPre && g == 0 && tid >= 64 && f_cell >= 0 && (f_cell == s_mv_cell[0] || f_cell == s_mv_cell[1]))
+            s_pf_ok = 0;
         if (s_accept) {
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
