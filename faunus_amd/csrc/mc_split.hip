@@ -67,6 +67,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     constexpr bool kPre = kCells && kThreads >= 512;
     __shared__ double s_pf_pos[4];
     __shared__ int s_pf_ti, s_pf_ok;
+    __shared__ int s_pf_cell_of[2]; // the next particle's cell_of entry, by move parity
     __shared__ int s_mv_cell[2];
 
     const int tid = threadIdx.x;
@@ -368,7 +369,9 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                         const int cnew = (cell_coord_s(nz_, half, S.inv_cell, S.nc2) * S.nc1 +
                                           cell_coord_s(ny_, half, S.inv_cell, S.nc1)) * S.nc0 +
                                          cell_coord_s(nx_, half, S.inv_cell, S.nc0);
-                        const int cold = S.cell_of[i];
+                        // cell_of changes only when its particle moves, so the loaded entry is
+                        // current whenever the other loads of this move were kept
+                        const int cold = pre ? s_pf_cell_of[t & 1] : S.cell_of[i];
                         if (kPre) {
                             s_mv_cell[0] = cold;
                             s_mv_cell[1] = cnew;
@@ -427,6 +430,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     s_pf_pos[2] = qo.z;
                     s_pf_pos[3] = qo.w;
                     s_pf_ti = tq;
+                    s_pf_cell_of[tn & 1] = S.cell_of[in];
                 }
                 const int cx = (cell_coord_s(f_px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
                 const int cy = (cell_coord_s(f_py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
