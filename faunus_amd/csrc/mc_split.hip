@@ -2,9 +2,12 @@
 // slice of the k-vectors (its Q(k) stays in registers for the whole launch);
 // workgroup 0 also evaluates the pair dU through the cell list and commits
 // accepted moves.  After each move every workgroup publishes its partial sums with
-// an agent-scope release store, waits (acquire loads, bounded spin) for all others,
-// sums the partials in the same fixed order and so takes the same decision as every
-// other workgroup.  The launch is cooperative, so all workgroups are resident.
+// an agent-scope release store, waits for all others (relaxed polling with a bounded
+// spin, then one acquire fence), sums the partials with the same butterfly and so
+// takes the same decision as every other workgroup.  While wave 0 of workgroup 0
+// waits, its other waves load the next move's particle and stencil neighbours; the
+// next move uses them unless the accepted move changed one of the loaded cells.
+// The launch is cooperative, so all workgroups are resident.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
