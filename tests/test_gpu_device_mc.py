@@ -229,3 +229,26 @@ def test_temperature_ladder_swaps_permute_betas():
     for r in range(4):
         tot, _ = b.total_energy(r)
         assert abs(b.info(r)["energy"] - tot) <= 1e-9 * max(1.0, abs(tot))
+
+
+def test_split_three_cell_grid_matches_oracle():
+    """A 40 A box with 12 A cells: a 3x3x3 grid, so every stencil holds every cell and
+    each accepted move changes cells that the next move's early-loaded neighbours came
+    from.  Those loads are then discarded and taken again; decisions stay exact."""
+    sysd = systems.nacl_lj(n_pairs=300, box=40.0, seed=3, kmax=6)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=41)
+    nm = 1500
+    print("split three-cell grid", flush=True)
+    b.run_split(nm, groups=8, stride=1, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=41, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    log("split three-cell grid", rep, "accepted", int(rec["accepted"].sum()))
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
