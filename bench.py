@@ -446,8 +446,9 @@ def main():
                     help="water replicas: cells of at least Rc / d (1: 27-cell stencils, 2: 125)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would shard over several "
-              "GPUs is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the engine has no multi-GPU path; "
+              "independent replicas and serving jobs would be spread over GPUs by the caller, one process each",
+              file=sys.stderr)
         sys.exit(2)
     import faunus_amd as F
     if F.device_count() < 1:
