@@ -53,6 +53,14 @@ def pair_bytes_cells(n, box, cutoff):
     return 27.0 * n / nc ** 3 * 36 + 2 * 27 * 8
 
 
+def check_replicas(b, count, what):
+    """Every replica of a batch ended its launches without an error (a full cell, a
+    workgroup that did not arrive)."""
+    bad = [(r, b.info(r)["error"]) for r in range(count) if b.info(r)["error"]]
+    if bad:
+        raise RuntimeError(f"{what}: replicas stopped with an error: {bad[:8]}")
+
+
 def bench_plain_single(sysd, moves, seed=7):
     import faunus_amd as F
     cfg, xyz, q, ids, g = sysd
@@ -94,6 +102,7 @@ def bench_device_single(sysd, args, fp32):
         b.run([STEP_MOVES])
         kernel_ms += b.sync()
     dt = time.perf_counter() - t0
+    check_replicas(b, 1, "device-resident replica")
     info = b.info(0)
     tot, _ = b.total_energy(0)
     dq, drec = b.check_q(0)
@@ -124,7 +133,7 @@ def bench_split(sysd, args, groups, stride, steps=None):
     tot, _ = b.total_energy(0)
     b.close()
     if err:
-        raise RuntimeError("split run reported a synchronisation time-out")
+        raise RuntimeError("split run stopped with error %d (1: full cell, 2: synchronisation time-out)" % err)
     moves = steps * STEP_MOVES
     return dict(rate=moves / dt, ms_per_step=dt * 1e3 / steps, kernel_ms=kernel_ms, moves=moves,
                 acceptance=info["accepted"] / max(1, info["moves"]), drift=(info["energy"] - tot) / abs(tot),
@@ -222,6 +231,7 @@ def bench_batch(comps, temps, fp32, args, budget_moves):
         kernel_ms += b.sync()
         swaps = b.swap(seed=4242, rnd=step)
     dt = time.perf_counter() - t0
+    check_replicas(b, len(cost), "batched salt replicas")
     total_moves = int(moves.sum()) * args.steps
     # per-move drift check on a few replicas
     drifts = []
@@ -298,6 +308,7 @@ def bench_water(args, fp32=True, replicas=64, n_mol=4096, moves_per_step=3000):
         kernel_ms += b.sync()
         swaps = b.swap(seed=77, rnd=step)
     dt = time.perf_counter() - t0
+    check_replicas(b, replicas, "water replicas")
     total = replicas * moves_per_step * args.steps
     drift = []
     for r in (0, replicas - 1):

@@ -68,6 +68,11 @@ SIGNATURES = {
     "fmc_batch_total_energy": (C.c_int, [C.c_void_p, C.c_int, D, D]),
     "fmc_batch_sample_energy": (C.c_int, [C.c_void_p, C.c_longlong]),
     "fmc_batch_set_cell_divisions": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_batch_set_cell_capacity": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_batch_get_replica_state": (C.c_int, [C.c_void_p, C.c_int, D, D,
+                                              C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
+    "fmc_batch_set_replica_state": (C.c_int, [C.c_void_p, C.c_int, D, D, C.c_ulonglong,
+                                              C.c_ulonglong]),
     "fmc_batch_energy_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_ulonglong), D, D]),
     "fmc_batch_rdf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                 C.POINTER(C.c_ulonglong)]),
@@ -363,16 +368,20 @@ class Engine:
 class Batch:
     """Replicas advanced by the device-resident Metropolis kernel (one workgroup each)."""
 
-    def __init__(self, fp32_pairs=False, cell_divisions=1):
+    def __init__(self, fp32_pairs=False, cell_divisions=1, cell_capacity=0):
         """``cell_divisions`` 2 gives replicas with a cell list cells of half the pair
-        range and 125-cell stencils (move-set replicas only)."""
+        range and 125-cell stencils (move-set replicas only); ``cell_capacity`` > 0 fixes
+        the members per cell (0: automatic)."""
         h = C.c_void_p()
         _check(lib().fmc_batch_create(int(bool(fp32_pairs)), C.byref(h)))
         self.h = h
         self.sizes = []
+        self.ngroups = []
         self.nterms = []
         if cell_divisions != 1:
             _check(lib().fmc_batch_set_cell_divisions(self.h, int(cell_divisions)))
+        if cell_capacity:
+            _check(lib().fmc_batch_set_cell_capacity(self.h, int(cell_capacity)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -408,6 +417,7 @@ class Batch:
                 _p(gs, C.c_int), _p(gm, C.c_int), float(beta), int(seed), int(ladder), len(kind), _p(kind, C.c_int),
                 _p(molid, C.c_int), _p(weight, C.c_uint), _p(dp), _p(dprot), C.byref(r)))
         self.sizes.append(len(q))
+        self.ngroups.append(len(gb))
         return r.value
 
     def run(self, moves, trace=0):
@@ -474,6 +484,22 @@ class Batch:
         xyz = np.zeros((self.sizes[r], 3))
         _check(lib().fmc_batch_positions(self.h, r, _p(xyz)))
         return xyz
+
+    def get_state(self, r):
+        """Checkpoint of replica r: positions, mass centres, move counter, accepted count."""
+        xyz = np.zeros((self.sizes[r], 3))
+        cm = np.zeros((self.ngroups[r], 3))
+        m, a = C.c_ulonglong(), C.c_ulonglong()
+        _check(lib().fmc_batch_get_replica_state(self.h, int(r), _p(xyz), _p(cm), C.byref(m), C.byref(a)))
+        return dict(xyz=xyz, mass_center=cm, moves=m.value, accepted=a.value)
+
+    def set_state(self, r, state):
+        """Resume replica r from a checkpoint made by get_state (mass_center may be None)."""
+        xyz = np.ascontiguousarray(state["xyz"], np.float64)
+        cm = state.get("mass_center")
+        cm = None if cm is None else np.ascontiguousarray(cm, np.float64)
+        _check(lib().fmc_batch_set_replica_state(self.h, int(r), _p(xyz), None if cm is None else _p(cm),
+                                                 int(state["moves"]), int(state.get("accepted", 0))))
 
     def trace(self, r, count):
         idx = np.zeros(count, np.int32)

@@ -408,6 +408,26 @@ int fmc_batch_set_cell_divisions(fmc_batch* b, int divisions) {
     return guarded([&] { bat(b).set_cell_divisions(divisions); });
 }
 
+int fmc_batch_set_cell_capacity(fmc_batch* b, int capacity) {
+    return guarded([&] { bat(b).set_cell_capacity(capacity); });
+}
+
+int fmc_batch_get_replica_state(fmc_batch* b, int replica, double* xyz, double* mass_center, unsigned long long* moves,
+                                unsigned long long* accepted) {
+    return guarded([&] {
+        if (!xyz) throw std::runtime_error("get_replica_state: positions buffer is null");
+        bat(b).get_replica_state(replica, xyz, mass_center, moves, accepted);
+    });
+}
+
+int fmc_batch_set_replica_state(fmc_batch* b, int replica, const double* xyz, const double* mass_center,
+                                unsigned long long moves, unsigned long long accepted) {
+    return guarded([&] {
+        if (!xyz) throw std::runtime_error("set_replica_state: positions are null");
+        bat(b).set_replica_state(replica, xyz, mass_center, moves, accepted);
+    });
+}
+
 int fmc_batch_create(int fp32_pairs, fmc_batch** out) {
     return guarded([&] {
         int n = 0;

@@ -816,6 +816,11 @@ void Engine::full_q(double2* out) {
     check_hip(hipGetLastError(), "full_q_kernel");
 }
 
+void Engine::set_cell_capacity(int cap) {
+    if (cap < 0) throw std::runtime_error("cell capacity must be >= 0");
+    cell_capacity_ = cap;
+}
+
 void Engine::set_cell_divisions(int d) {
     if (d < 1 || d > 2) throw std::runtime_error("cell divisions: 1 or 2");
     if (d_pos_ != nullptr) throw std::runtime_error("cell divisions must be set before init");
@@ -856,6 +861,10 @@ void Engine::build_cell_list() {
     const double mean = static_cast<double>(n_) / static_cast<double>(ncell);
     int cap = std::max(maxcount * 2, static_cast<int>(4 * mean + 32));
     cap = (cap + 7) & ~7;
+    if (cell_capacity_ > 0) {
+        if (cell_capacity_ < maxcount) throw std::runtime_error("cell capacity below the fullest initial cell");
+        cap = cell_capacity_;
+    }
     std::vector<int> members(static_cast<size_t>(ncell * cap), -1);
     std::fill(count.begin(), count.end(), 0);
     for (int i = 0; i < n_; ++i) {
@@ -1249,6 +1258,39 @@ void Engine::check_q(double* max_abs_dq, double* rel_rec) {
 }
 
 void Engine::get_positions(double* xyz) const { std::copy(h_xyz_.begin(), h_xyz_.end(), xyz); }
+
+void Engine::reset_positions(const double* xyz, const double* mass_center) {
+    if (!d_pos_) throw std::runtime_error("reset_positions: engine not initialised");
+    if (pending_) throw std::runtime_error("reset_positions: a move is pending (sync first)");
+    check_hip(hipStreamSynchronize(stream_), "sync");
+    h_xyz_.assign(xyz, xyz + 3 * static_cast<size_t>(n_));
+    std::vector<double4> p(static_cast<size_t>(n_));
+    for (int i = 0; i < n_; ++i)
+        p[static_cast<size_t>(i)] = make_double4(h_xyz_[3 * i], h_xyz_[3 * i + 1], h_xyz_[3 * i + 2], h_q_[static_cast<size_t>(i)]);
+    check_hip(hipMemcpy(d_pos_, p.data(), sizeof(double4) * n_, hipMemcpyHostToDevice), "copy pos");
+    const int ng = static_cast<int>(g_begin_.size());
+    if (mass_center) g_cm_.assign(mass_center, mass_center + 3 * static_cast<size_t>(ng));
+    else
+        for (int g = 0; g < ng; ++g)
+            if (!model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(g)])].atomic && g_size_[static_cast<size_t>(g)] > 0)
+                group_mass_center_host(g, &g_cm_[static_cast<size_t>(3 * g)]);
+    check_hip(hipMemcpy(d_gcm_, g_cm_.data(), sizeof(double) * 3 * ng, hipMemcpyHostToDevice), "copy group centres");
+    for (int d = 0; d < 3; ++d) {
+        dipole_[d] = 0;
+        for (int i = 0; i < n_; ++i) dipole_[d] += h_xyz_[3 * i + d] * h_q_[static_cast<size_t>(i)];
+    }
+    if (model_.ewald.present) {
+        full_q(d_q_acc_);
+        check_hip(hipStreamSynchronize(stream_), "full Q");
+    }
+    (void)hipFree(cells_.count);
+    (void)hipFree(cells_.members);
+    (void)hipFree(cells_.cell_of);
+    (void)hipFree(cells_.overflow);
+    cells_ = CellList{};
+    cell_capacity_ = 0; // a fixed capacity applies to the first build only
+    build_cell_list();
+}
 
 void Engine::refresh_host_positions() {
     std::vector<double4> p(static_cast<size_t>(n_));

@@ -128,6 +128,10 @@ class Engine {
     // Only the general device kernel walks d = 2; the other paths need d = 1.
     void set_cell_divisions(int d);
     int cell_radius() const { return cells_.radius; }
+    // Members per cell of the cell list built by init(): 0 chooses max(2 x the fullest
+    // cell, 4 x the mean + 32); a fixed capacity must hold the fullest initial cell.
+    void set_cell_capacity(int cap);
+    int cell_capacity() const { return cells_.cap; }
 
     // Pair-parameter and spline tables (as uploaded).
     const std::vector<DevPair>& dev_pairs() const { return h_pairs_; }
@@ -157,6 +161,11 @@ class Engine {
     const std::vector<double>& host_charges() const { return h_q_; }
     const std::vector<int>& host_types() const { return h_type_; }
     void refresh_host_positions();
+    // Replace the accepted positions (3n) of an initialised engine and rebuild what
+    // depends on them: Q(k) (full update), the cell list (automatic capacity), the
+    // dipole, and the molecular mass centres (from `mass_center`, 3 per group, or
+    // recomputed when null).  MetropolisMonteCarlo::restore + init (montecarlo.cpp:118-137).
+    void reset_positions(const double* xyz, const double* mass_center);
 
   private:
     Model model_;
@@ -206,6 +215,7 @@ class Engine {
     // cell list
     CellList cells_;
     int cell_divisions_ = 1;
+    int cell_capacity_ = 0;
     // pending change (between delta and sync)
     bool pending_ = false;
     std::vector<MovedParticle> pending_moved_;

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
+    long long t_done = M; // moves decided by this launch
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -249,17 +250,19 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                     const int cnew = (czn * S.nc1 + cyn) * S.nc0 + cxn;
                     const int cold = S.cell_of[i];
                     if (cnew != cold) {
-                        const int nold = S.cell_count[cold];
-                        for (int kk = 0; kk < nold; ++kk)
-                            if (S.cell_members[cold * S.cap + kk] == i) {
-                                S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
-                                break;
-                            }
-                        S.cell_count[cold] = nold - 1;
                         const int nn = S.cell_count[cnew];
                         if (nn >= S.cap) {
-                            S.error = 1; // cell capacity exceeded: the run stops with an error
+                            // full cell: the lists are left as they are and the chain stops
+                            // after this move
+                            S.error = 1;
                         } else {
+                            const int nold = S.cell_count[cold];
+                            for (int kk = 0; kk < nold; ++kk)
+                                if (S.cell_members[cold * S.cap + kk] == i) {
+                                    S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                    break;
+                                }
+                            S.cell_count[cold] = nold - 1;
                             S.cell_members[cnew * S.cap + nn] = i;
                             S.cell_count[cnew] = nn + 1;
                             S.cell_of[i] = cnew;
@@ -302,7 +305,10 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                 }
             }
         }
-        if (S.error) break;
+        if (S.error) {
+            t_done = t + 1;
+            break;
+        }
         __syncthreads(); // phase tables and the accepted position are consumed before the next move
     }
 
@@ -310,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     for (int s = 0; s < kMaxKPT; ++s)
         if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
     if (tid == 0) {
-        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.moves = move0 + static_cast<unsigned long long>(t_done);
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
@@ -497,6 +503,11 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
             if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
     if (m.ewald.present && m.ewald.const_inf > 0.5)
         throw std::runtime_error("device MC needs tinfoil boundary conditions (epss = 0)");
+    // Hamiltonian::energy stops summing a state's terms at the first one >= maxenergy
+    // (energy.cpp:1227-1241); the device kernels form dU as one difference, so a finite
+    // maxenergy is left to the per-move path
+    if (std::isfinite(m.maximum_allowed_energy))
+        throw std::runtime_error("device MC: maxenergy is supported by the per-move path only");
     s.ntypes = m.ntypes;
     s.L = m.box[0];
     s.half = 0.5 * m.box[0];
@@ -699,6 +710,7 @@ int Batch::add_replica(const std::string& config_json, int n, const double* xyz,
     e->set_particles(n, xyz, charge, ids);
     e->set_groups(ngroups, gbegin, gsize, gmolid, nullptr);
     e->set_cell_divisions(cell_divisions_);
+    e->set_cell_capacity(cell_capacity_);
     e->init();
     if (!general && e->cell_list_enabled() && e->cell_radius() != 1)
         throw std::runtime_error("the finer cell grid is walked by the general move-set kernel only");
@@ -708,6 +720,11 @@ int Batch::add_replica(const std::string& config_json, int n, const double* xyz,
     h_states_.push_back(s);
     engines_.push_back(std::move(e));
     return s.replica_id;
+}
+
+void Batch::set_cell_capacity(int cap) {
+    if (cap < 0) throw std::runtime_error("cell capacity must be >= 0");
+    cell_capacity_ = cap;
 }
 
 void Batch::set_cell_divisions(int d) {
@@ -864,6 +881,57 @@ void Batch::swap_ladders(unsigned long long seed, unsigned long long round) {
     (void)hipFree(d_ls);
     (void)hipFree(d_slots);
     (void)hipFree(d_cnt);
+}
+
+void Batch::get_replica_state(int r, double* xyz, double* mass_center, unsigned long long* moves,
+                              unsigned long long* accepted) {
+    const ReplicaState s = state(r);
+    positions(r, xyz);
+    Engine& e = *engines_[static_cast<size_t>(r)];
+    if (mass_center) {
+        const size_t ng = e.group_begin().size();
+        check_hip(hipMemcpy(mass_center, e.d_mass_centers(), sizeof(double) * 3 * ng, hipMemcpyDeviceToHost),
+                  "copy mass centres");
+    }
+    if (moves) *moves = s.moves;
+    if (accepted) *accepted = s.accepted;
+}
+
+void Batch::set_replica_state(int r, const double* xyz, const double* mass_center, unsigned long long moves,
+                              unsigned long long accepted) {
+    ReplicaState cur = state(r); // finalizes and waits for the stream
+    Engine& e = *engines_[static_cast<size_t>(r)];
+    e.reset_positions(xyz, mass_center);
+    const CellList& cl = e.cells();
+    ReplicaState& h = h_states_[static_cast<size_t>(r)];
+    for (ReplicaState* s : {&cur, &h}) {
+        if (s->cells != (cl.enabled ? 1 : 0)) throw std::runtime_error("set_replica_state: cell list changed kind");
+        if (cl.enabled) {
+            s->cap = cl.cap;
+            s->cell_count = cl.count;
+            s->cell_members = cl.members;
+            s->cell_of = cl.cell_of;
+        }
+    }
+    if (cur.pos32) {
+        const auto& q = e.host_charges();
+        const int n = e.num_particles();
+        std::vector<float4> p32(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            p32[static_cast<size_t>(i)] = make_float4(static_cast<float>(xyz[3 * i]), static_cast<float>(xyz[3 * i + 1]),
+                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(q[static_cast<size_t>(i)]));
+        check_hip(hipMemcpy(cur.pos32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
+    }
+    std::vector<double> terms(static_cast<size_t>(e.num_terms()));
+    e.total_energy(terms.data());
+    double u = 0.0;
+    for (double v : terms) u += v;
+    cur.energy = u;
+    cur.moves = moves;
+    cur.accepted = accepted;
+    cur.sum_du = 0.0;
+    cur.error = 0;
+    check_hip(hipMemcpy(d_states_ + r, &cur, sizeof(ReplicaState), hipMemcpyHostToDevice), "copy state");
 }
 
 ReplicaState Batch::state(int r) {

@@ -139,6 +139,8 @@ class Batch {
     // cells of at least pair range / d for the replicas added afterwards (d = 1 or 2; the
     // finer grid is walked by the general kernel only)
     void set_cell_divisions(int d);
+    // members per cell of the cell lists of the replicas added afterwards (0: automatic)
+    void set_cell_capacity(int cap);
     ~Batch();
     int add_replica(const std::string& config_json, int n, const double* xyz, const double* charge, const int* ids,
                     int ngroups, const int* gbegin, const int* gsize, const int* gmolid, double beta,
@@ -161,6 +163,15 @@ class Batch {
     int size() const { return static_cast<int>(engines_.size()); }
     ReplicaState state(int r);
     void positions(int r, double* xyz);
+    // Checkpoint of replica r: positions (3n), molecular mass centres (3 per group; may
+    // be null), Philox move counter and accepted count.
+    void get_replica_state(int r, double* xyz, double* mass_center, unsigned long long* moves,
+                           unsigned long long* accepted);
+    // Resume replica r from a checkpoint: positions, mass centres (null: recomputed),
+    // move counter and accepted count; Q(k), the cell list and the running energy are
+    // rebuilt and the replica's error is cleared.
+    void set_replica_state(int r, const double* xyz, const double* mass_center, unsigned long long moves,
+                           unsigned long long accepted);
     std::vector<TraceEntry> trace(int r, int count);
     Engine& engine(int r) { return *engines_[static_cast<size_t>(r)]; }
     hipStream_t stream() const { return stream_; }
@@ -176,6 +187,7 @@ class Batch {
     int fp32_ = 0;
     int general_ = -1; // -1 unset, 0 translate-only replicas, 1 general move sets
     int cell_divisions_ = 1;
+    int cell_capacity_ = 0;
     std::vector<std::unique_ptr<Engine>> engines_;
     std::vector<ReplicaState> h_states_;
     std::vector<void*> owned_; // device buffers made for the replicas

@@ -412,6 +412,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         if (kSplit && idx >= prev_lo && idx < prev_hi) return s_prev[idx - prev_lo];
         return pos[idx];
     };
+    long long t_done = M; // moves decided by this launch
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -884,12 +885,28 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 if (P.kind == 1 && P.translate && part == 0)
                     for (int d = 0; d < 3; ++d) gcm[3 * g + d] = P.cm_new[d];
                 if (kCells) {
-                    for (int a = 0; a < m && !S.error; ++a) {
+                    auto cell_at = [&](const double4& A) {
+                        return (cell_of_coord(A.z, half, S.inv_cell, S.nc2) * S.nc1 +
+                                cell_of_coord(A.y, half, S.inv_cell, S.nc1)) * S.nc0 +
+                               cell_of_coord(A.x, half, S.inv_cell, S.nc0);
+                    };
+                    // every cell that gains atoms must hold them all (counting the atoms of
+                    // this move that enter it); otherwise the lists are left as they are and
+                    // the chain stops after this move
+                    bool fits = true;
+                    for (int a = 0; a < m && fits; ++a) {
+                        const int cnew = cell_at(s_new[a]);
+                        if (cnew == S.cell_of[P.mlo + a]) continue;
+                        if (S.cell_count[cnew] + m <= S.cap) continue;
+                        int entering = 0;
+                        for (int b = 0; b < m; ++b)
+                            if (cell_at(s_new[b]) == cnew && S.cell_of[P.mlo + b] != cnew) ++entering;
+                        if (S.cell_count[cnew] + entering > S.cap) fits = false;
+                    }
+                    if (!fits) S.error = 1;
+                    for (int a = 0; a < m && fits; ++a) {
                         const int i = P.mlo + a;
-                        const double4 A = s_new[a];
-                        const int cnew = (cell_of_coord(A.z, half, S.inv_cell, S.nc2) * S.nc1 +
-                                          cell_of_coord(A.y, half, S.inv_cell, S.nc1)) * S.nc0 +
-                                         cell_of_coord(A.x, half, S.inv_cell, S.nc0);
+                        const int cnew = cell_at(s_new[a]);
                         const int cold = S.cell_of[i];
                         if (cnew == cold) continue;
                         const int nold = S.cell_count[cold];
@@ -900,13 +917,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             }
                         S.cell_count[cold] = nold - 1;
                         const int nn = S.cell_count[cnew];
-                        if (nn >= S.cap) {
-                            S.error = 1; // cell capacity exceeded: the run stops with an error
-                        } else {
-                            S.cell_members[cnew * S.cap + nn] = i;
-                            S.cell_count[cnew] = nn + 1;
-                            S.cell_of[i] = cnew;
-                        }
+                        S.cell_members[cnew * S.cap + nn] = i;
+                        S.cell_count[cnew] = nn + 1;
+                        S.cell_of[i] = cnew;
                     }
                 }
             }
@@ -914,6 +927,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         __syncthreads();
         if (kSplit && s_abort) {
             if (tid == 0) atomicExch(&S.error, 2);
+            t_done = t;
             break;
         }
         const bool committed = s_accept && m > 0;
@@ -943,7 +957,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             prev_cm[1] = P.cm_new[1];
             prev_cm[2] = P.cm_new[2];
         }
-        if (!kSplit && S.error) break;
+        if (!kSplit && S.error) {
+            t_done = t + 1; // stopped after committing this move (cell capacity)
+            break;
+        }
         __syncthreads(); // moved coordinates and tables consumed before the next move
     }
 
@@ -951,7 +968,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     for (int s = 0; s < kMaxKPT; ++s)
         if (s < kpt && k0 + s < ke) S.q[k0 + s] = q[s];
     if (tid == 0 && part == 0) {
-        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.moves = move0 + static_cast<unsigned long long>(t_done);
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void mc_pipe_kernel(ReplicaState* __restrict__
     const int g = blockIdx.x;
     if (g >= G) return;
     ReplicaState& S = states[0];
+    if (S.error) return; // a stopped replica (read before any workgroup of this launch can set it)
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevCoulomb s_coul;
     __shared__ double s_dp[kMaxTypes];
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(256) void mc_pipe_kernel(ReplicaState* __restrict__
             }
             if (!ok) {
                 atomicExch(err, 1);
+                atomicMax(&S.error, 2);
                 s_abort = 1;
             }
         }
@@ -213,16 +215,20 @@ __global__ __launch_bounds__(256) void mc_pipe_kernel(ReplicaState* __restrict__
                                          pcell(pe.nx, half, S.inv_cell, S.nc0);
                         const int cold = S.cell_of[pe.idx];
                         if (cnew != cold) {
-                            const int nold = S.cell_count[cold];
-                            for (int kk = 0; kk < nold; ++kk)
-                                if (S.cell_members[cold * S.cap + kk] == pe.idx) {
-                                    S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
-                                    break;
-                                }
-                            S.cell_count[cold] = nold - 1;
                             const int nn = S.cell_count[cnew];
-                            if (nn >= S.cap) S.error = 1;
-                            else {
+                            if (nn >= S.cap) {
+                                // full cell: the lists are left as they are.  The moves in flight
+                                // were published by every workgroup, so the launch runs to its
+                                // end and the replica is marked inconsistent (refuses later runs).
+                                atomicMax(&S.error, 2);
+                            } else {
+                                const int nold = S.cell_count[cold];
+                                for (int kk = 0; kk < nold; ++kk)
+                                    if (S.cell_members[cold * S.cap + kk] == pe.idx) {
+                                        S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                        break;
+                                    }
+                                S.cell_count[cold] = nold - 1;
                                 S.cell_members[cnew * S.cap + nn] = pe.idx;
                                 S.cell_count[cnew] = nn + 1;
                                 S.cell_of[pe.idx] = cnew;

@@ -8,6 +8,14 @@
 // waits, its other waves load the next move's particle and stencil neighbours; the
 // next move uses them unless the accepted move changed one of the loaded cells.
 // The launch is cooperative, so all workgroups are resident.
+//
+// A replica whose cell list overflows stops after the move that overflowed: that move
+// is committed (position, Q(k), energy), the particle stays listed in its old cell,
+// S.error = 1, and workgroup 0 publishes an abort tag in its slot of the next move so
+// that the other workgroups stop there too.  A wait that exceeds its spin bound sets
+// S.error = 2 (the workgroups may then disagree on the last move).  Either way the
+// replica refuses later launches until its state is set again, and S.moves counts the
+// moves that were decided.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +33,8 @@ namespace {
 constexpr int kSplitMaxKPT = 8; // k-vectors per thread at 256 threads
 constexpr int kSplitMaxG = 64;
 constexpr int kGkMax = 1025;
+// set in a published tag: the publishing workgroup (0) stopped before this move
+constexpr unsigned long long kAbortTag = 1ull << 63;
 
 struct alignas(32) SplitSlot {
     double de, pn, po;
@@ -55,12 +65,13 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const int g = blockIdx.x / stride;
     if (g >= G) return;
     ReplicaState& S = states[0];
+    if (S.error) return; // a stopped replica (read before any workgroup of this launch can set it)
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevCoulomb s_coul;
     __shared__ double s_dp[kMaxTypes];
     __shared__ double s_gk[kGkMax];
     __shared__ double s_red[3][kThreads / 64];
-    __shared__ int s_accept, s_abort;
+    __shared__ int s_accept, s_abort, s_stop;
     __shared__ Proposal s_prop[64]; // proposals of the next 64 moves, one lane of wave 0 each
     __shared__ double2 s_phase[6 * 65];
     // Workgroup 0 at 512 threads with a cell list: waves 1-7 load the next move's
@@ -81,6 +92,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     if (tid == 0) {
         s_coul = *S.coul;
         s_abort = 0;
+        s_stop = 0;
         s_pf_ok = 0;
     }
     for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
@@ -120,6 +132,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     double f_px = 0.0, f_py = 0.0, f_pz = 0.0;
     int f_cell = -1, f_cnt = 0, f_j0 = -1, f_j1 = -1, f_t0 = 0, f_t1 = 0;
     double4 f_p0 = make_double4(0.0, 0.0, 0.0, 0.0), f_p1 = f_p0;
+    long long t_done = M; // moves decided by this launch
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -293,14 +306,19 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         // add them with the same butterfly in every workgroup
         double sde = 0.0, spn = 0.0, spo = 0.0; // totals, valid in thread 0
         if (tid < 64) {
-            int ok = 1;
+            int ok = 1, stopped = 0;
             double a = 0.0, b = 0.0, c = 0.0;
             if (tid < G) {
                 SplitSlot* sl = slots + parity * G + tid;
                 long long spins = 0;
                 // relaxed polling, then one acquire fence (an acquire load per poll would
                 // invalidate the caches on every iteration)
-                while (__hip_atomic_load(&sl->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+                unsigned long long v;
+                while ((v = __hip_atomic_load(&sl->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != tag) {
+                    if (v == (tag | kAbortTag)) {
+                        stopped = 1;
+                        break;
+                    }
                     if (++spins > spin_limit) {
                         ok = 0;
                         break;
@@ -308,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     __builtin_amdgcn_s_sleep(1);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                if (ok) {
+                if (ok && !stopped) {
                     a = sl->de;
                     b = sl->pn;
                     c = sl->po;
@@ -320,17 +338,21 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 c += __shfl_xor(c, off, 64);
             }
             const int all_ok = __all(ok);
+            const int any_stop = __any(stopped);
             if (tid == 0) {
                 sde = a;
                 spn = b;
                 spo = c;
-                if (!all_ok) {
+                if (any_stop) {
+                    s_stop = 2; // workgroup 0 stopped after the previous move: this one is not decided
+                } else if (!all_ok) {
                     atomicExch(err, 1);
+                    atomicMax(&S.error, 2);
                     s_abort = 1;
                 }
             }
         }
-        if (tid == 0 && !s_abort) {
+        if (tid == 0 && !s_abort && !s_stop) {
             const double drec = pref * sde;
             const double e_tot_new = spn + drec;
             double du;
@@ -380,17 +402,19 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                             s_mv_cell[1] = cnew;
                         }
                         if (cnew != cold) {
-                            const int nold = S.cell_count[cold];
-                            for (int kk = 0; kk < nold; ++kk)
-                                if (S.cell_members[cold * S.cap + kk] == i) {
-                                    S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
-                                    break;
-                                }
-                            S.cell_count[cold] = nold - 1;
                             const int nn = S.cell_count[cnew];
                             if (nn >= S.cap) {
-                                S.error = 1;
+                                // full cell: the lists are left as they are and the chain stops
+                                atomicMax(&S.error, 1);
+                                s_stop = 1;
                             } else {
+                                const int nold = S.cell_count[cold];
+                                for (int kk = 0; kk < nold; ++kk)
+                                    if (S.cell_members[cold * S.cap + kk] == i) {
+                                        S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                        break;
+                                    }
+                                S.cell_count[cold] = nold - 1;
                                 S.cell_members[cnew * S.cap + nn] = i;
                                 S.cell_count[cnew] = nn + 1;
                                 S.cell_of[i] = cnew;
@@ -453,7 +477,14 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             }
         }
         __syncthreads();
-        if (s_abort) break;
+        if (s_abort) {
+            t_done = t;
+            break;
+        }
+        if (s_stop == 2) {
+            t_done = t;
+            break;
+        }
         // the accepted move changed a loaded cell: load again at the next move
         if (kPre && g == 0 && tid >= 64 && f_cell >= 0 && (f_cell == s_mv_cell[0] || f_cell == s_mv_cell[1]))
             s_pf_ok = 0;
@@ -472,6 +503,16 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             h_pos[1][1] = ny_;
             h_pos[1][2] = nz_;
         }
+        if (s_stop == 1) {
+            // workgroup 0 committed this move and stopped on a full cell: the next move's
+            // tag carries the abort bit, so the other workgroups stop there as well
+            if (g == 0 && tid == 0 && t + 1 < M) {
+                SplitSlot* sl = slots + static_cast<int>((mid + 1ull) & 1ull) * G;
+                __hip_atomic_store(&sl->seq, (tag + 1ull) | kAbortTag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            t_done = t + 1;
+            break;
+        }
         __syncthreads(); // phase tables are rewritten by the next move
     }
 #pragma unroll
@@ -480,7 +521,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         if (k < ke) S.q[k] = q[s];
     }
     if (g == 0 && tid == 0) {
-        S.moves = move0 + static_cast<unsigned long long>(M);
+        S.moves = move0 + static_cast<unsigned long long>(t_done);
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
@@ -593,13 +634,15 @@ void Batch::run_split_general(long long moves, int groups, int stride, bool trac
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 
+// The replica's error (1: a cell overflowed, 2: a wait timed out or the state is
+// inconsistent), or the sync time-out flag of the last split or pipelined launch.
 int Batch::split_error() {
-    if (general_ == 1) return state(0).error;
-    if (!split_err_) return 0;
+    const int s = state(0).error;
+    if (general_ == 1 || !split_err_) return s;
     int e = 0;
     check_hip(hipStreamSynchronize(stream_), "sync");
     check_hip(hipMemcpy(&e, split_err_, sizeof(int), hipMemcpyDeviceToHost), "copy err");
-    return e;
+    return std::max(s, e);
 }
 
 } // namespace fmc

@@ -203,6 +203,30 @@ int fmc_batch_create(int fp32_pairs, fmc_batch** out);
  * loops over all pairs, src/energy.h:1155-1224).
  */
 int fmc_batch_set_cell_divisions(fmc_batch* batch, int divisions);
+/*
+ * Members per cell of the cell lists of the replicas added afterwards (0, the default:
+ * max(2 x the fullest initial cell, 4 x the mean + 32)).  A fixed capacity must hold the
+ * fullest initial cell.  A move that would overfill a cell is committed and the replica
+ * stops after it with error 1 (fmc_batch_replica_info); its cell list is rebuilt when its
+ * state is set again (fmc_batch_set_replica_state).  No reference counterpart (the
+ * reference has no cell list on this path).
+ */
+int fmc_batch_set_cell_capacity(fmc_batch* batch, int capacity);
+/*
+ * Checkpoint and resume of one replica, the device-resident counterpart of SaveState
+ * (src/analysis.cpp:645-680: positions, group mass centres and the move random-number
+ * state) and MetropolisMonteCarlo::restore followed by init (src/montecarlo.cpp:118-137).
+ * The random state is the replica's Philox move counter.  get: positions (3n), mass
+ * centres (3 per group; may be NULL), move counter and accepted count.  set: the same
+ * (NULL mass centres are recomputed from the positions); Q(k), the cell list (automatic
+ * capacity) and the running energy (Change::everything) are rebuilt, the error flag is
+ * cleared.  A chain run M moves, checkpointed, resumed and run M more is the chain of
+ * 2M moves.
+ */
+int fmc_batch_get_replica_state(fmc_batch* batch, int replica, double* xyz, double* mass_center,
+                                unsigned long long* moves, unsigned long long* accepted);
+int fmc_batch_set_replica_state(fmc_batch* batch, int replica, const double* xyz, const double* mass_center,
+                                unsigned long long moves, unsigned long long accepted);
 int fmc_batch_destroy(fmc_batch* batch);
 int fmc_batch_add_replica(fmc_batch* batch, const char* config_json, int n, const double* xyz, const double* charge,
                           const int* atom_id, int num_groups, const int* begin, const int* size, const int* molid,

@@ -136,30 +136,23 @@ def lattice_positions(n, box, sigma, rng):
     return (ijk + 0.5) * a - box / 2 + jit
 
 
-def polyelectrolyte_job(n_beads, n_total, seed, kmax=8, dp=10.0, monomer_molar=0.005):
-    """Polyelectrolyte serving job: n_beads monomers (q=-1), n_beads counterions (q=+1)
-    and 1:1 salt up to n_total particles, hard spheres sigma=4 A + Ewald alpha=5/L,
-    Rc=L/2, kmax=8, cubic box at the given monomer concentration.  Every particle is
-    translated with displacement dp (atomic groups; the chain bonds are not part of
-    this system)."""
-    n_salt = max(0, (n_total - 2 * n_beads) // 2)
-    n = 2 * n_beads + 2 * n_salt
-    box = (n_beads / (monomer_molar * AVOGADRO * 1e-27)) ** (1.0 / 3.0)
+def ions(energy, n_pairs=100, box=60.0, seed=1, dp=5.0, temperature=298.15):
+    """Na/Cl-like ions (q=+-1, sigma 3.3/4.4 A, eps 0.5/0.4 kJ/mol) in a cubic box with
+    the given `energy` list (reference input form), random sequential addition start."""
     cfg = {
-        "temperature": 298.15,
+        "temperature": temperature,
         "geometry": {"type": "cuboid", "length": box},
-        "atomlist": [{"P": {"q": -1.0, "sigma": 4.0, "dp": dp}}, {"C": {"q": 1.0, "sigma": 4.0, "dp": dp}},
-                     {"A": {"q": -1.0, "sigma": 4.0, "dp": dp}}],
-        "moleculelist": [{"ions": {"atoms": ["P", "C", "A"], "atomic": True}}],
-        "energy": [{"nonbonded": {"default": [
-            {"hardsphere": {}},
-            {"coulomb": {"type": "ewald", "epsr": 78.7, "cutoff": box / 2, "alpha": 5.0 / box, "ncutoff": kmax}}]}}],
+        "atomlist": [{"Na": {"q": 1.0, "sigma": 3.3, "eps": 0.5, "dp": dp}},
+                     {"Cl": {"q": -1.0, "sigma": 4.4, "eps": 0.4, "dp": dp}}],
+        "moleculelist": [{"salt": {"atoms": ["Na", "Cl"], "atomic": True}}],
+        "energy": energy,
     }
     rng = np.random.default_rng(seed)
-    ids = np.concatenate([np.zeros(n_beads, np.int32), np.ones(n_beads + n_salt, np.int32),
-                          np.full(n_salt, 2, np.int32)])
-    q = np.where(ids == 1, 1.0, -1.0)
-    xyz = lattice_positions(n, box, 4.0, rng)
+    n = 2 * n_pairs
+    ids = np.array([0, 1] * n_pairs, np.int32)
+    q = np.where(ids == 0, 1.0, -1.0)
+    sig = np.where(ids == 0, 3.3, 4.4)
+    xyz = rsa_positions(n, box, sig, rng)
     return cfg, xyz, q, ids, dict(begin=[0], size=[n], molid=[0])
 
 

@@ -9,7 +9,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import log
+from tests.gpu_helpers import compare_chain, log
 
 pytestmark = pytest.mark.gpu
 
@@ -21,15 +21,8 @@ def oracle_of(cfg, xyz, q, ids, g):
     return s, s.init()
 
 
-def compare(tr, rec, e_scale):
-    n = len(rec["du"])
-    mism = np.nonzero(tr["accepted"] != rec["accepted"])[0]
-    first = int(mism[0]) if len(mism) else n
-    fin = np.isfinite(rec["du"][:first])
-    assert np.array_equal(np.isinf(tr["du"][:first]), np.isinf(rec["du"][:first]))
-    err = np.abs(tr["du"][:first][fin] - rec["du"][:first][fin]) / np.maximum(np.abs(rec["du"][:first][fin]), e_scale)
-    return dict(first_mismatch=first, n=n, max_rel=float(err.max()) if len(err) else 0.0,
-                accepted=int(rec["accepted"][:first].sum()))
+def compare(tr, rec, e_scale, seed, replica=0, move0=0, rel=1e-6):
+    return compare_chain(tr, rec, e_scale, rel, seed, replica=replica, move0=move0)
 
 
 def run(sysd, kinds, nm, seed, fp32=False, cell_divisions=1):
@@ -41,7 +34,7 @@ def run(sysd, kinds, nm, seed, fp32=False, cell_divisions=1):
     b.sync()
     tr = b.trace(0, nm)
     rec = orc.run_philox_moves(nm, seed=seed, kinds=kinds, replica=0)
-    return b, orc, e0, compare(tr, rec, max(1.0, abs(e0)))
+    return b, orc, e0, compare(tr, rec, max(1.0, abs(e0)), seed=seed)
 
 
 @pytest.mark.parametrize("fp32,excl", [(False, 0), (True, 0), (False, 1)])
@@ -122,6 +115,6 @@ def test_batch_of_uneven_polyelectrolytes_and_ladder():
     for r, (sysd, kinds) in enumerate(sysds):
         orc, e0 = oracle_of(*sysd)
         rec = orc.run_philox_moves(nm, seed=500 + r, kinds=kinds, replica=r)
-        rep = compare(b.trace(r, nm), rec, max(1.0, abs(e0)))
+        rep = compare(b.trace(r, nm), rec, max(1.0, abs(e0)), seed=500 + r, replica=r)
         log("general batch replica", r, rep)
         assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10, rep

@@ -8,7 +8,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import log
+from tests.gpu_helpers import compare_chain, log
 
 pytestmark = pytest.mark.gpu
 
@@ -22,22 +22,8 @@ def oracle_of(system):
     return s, e0
 
 
-def compare_traces(tr, rec, e_scale, rel, beta=1.0):
-    n = len(rec["du"])
-    mism = np.nonzero(tr["accepted"] != rec["accepted"])[0]
-    first = int(mism[0]) if len(mism) else n
-    sl = slice(0, first)
-    assert np.array_equal(tr["index"][sl], rec["index"][sl])
-    fin = np.isfinite(rec["du"][sl])
-    dg, dc = tr["du"][sl][fin], rec["du"][sl][fin]
-    err = np.abs(dg - dc) / np.maximum(np.abs(dc), e_scale)
-    assert np.array_equal(np.isinf(tr["du"][sl]), np.isinf(rec["du"][sl]))
-    ties = []
-    if len(mism):
-        i = first
-        ties.append(float(abs(np.exp(-beta * rec["du"][i]))))
-    return dict(first_mismatch=first, n=n, max_rel=float(err.max()) if len(err) else 0.0,
-                accepted=int(tr["accepted"][sl].sum()), mismatches=len(mism))
+def compare_traces(tr, rec, e_scale, rel, seed, replica=0, move0=0, beta=1.0):
+    return compare_chain(tr, rec, e_scale, rel, seed, replica=replica, move0=move0, beta=beta)
 
 
 @pytest.mark.parametrize("fp32", [False, True])
@@ -53,7 +39,7 @@ def test_device_mc_primitive_model(fp32):
     rec = orc.run_philox_translate(nm, seed=12345, replica=0)
     e_scale = max(1.0, abs(e0))
     rel = 1e-6 if fp32 else 1e-10
-    rep = compare_traces(tr, rec, e_scale, rel)
+    rep = compare_traces(tr, rec, e_scale, rel, seed=12345)
     log("device MC primitive model fp32=%s" % fp32, rep)
     if not fp32:
         assert rep["first_mismatch"] == nm, rep
@@ -85,7 +71,7 @@ def test_device_mc_cell_list_nacl():
     b.sync()
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=99, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=99)
     log("device MC cell list", rep)
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
     tot, _ = b.total_energy(0)
@@ -103,7 +89,7 @@ def test_device_mc_full_size_nacl_short():
     b.sync()
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=7, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=7)
     log("device MC full size", rep)
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
     dq, drec = b.check_q(0)
@@ -124,7 +110,7 @@ def test_split_replica_matches_oracle(groups, stride):
     assert b.split_error() == 0
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=31, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=31)
     log("split", groups, stride, rep)
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
     # a second launch continues the same chain
@@ -132,7 +118,7 @@ def test_split_replica_matches_oracle(groups, stride):
     b.sync()
     tr2 = b.trace(0, 500)
     rec2 = orc.run_philox_translate(500, seed=31, replica=0, move0=nm)
-    rep2 = compare_traces(tr2, rec2, max(1.0, abs(e0)), 1e-10)
+    rep2 = compare_traces(tr2, rec2, max(1.0, abs(e0)), 1e-10, seed=31, move0=nm)
     assert rep2["first_mismatch"] == 500
     tot, _ = b.total_energy(0)
     assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
@@ -160,13 +146,13 @@ def test_pipelined_replica_matches_oracle(groups, depth, kind):
     assert b.split_error() == 0
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=77, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=77)
     log("pipe", groups, depth, kind, rep)
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
     b.run_pipe(700, groups=groups, depth=depth, trace=700)
     b.sync()
     rec2 = orc.run_philox_translate(700, seed=77, replica=0, move0=nm)
-    rep2 = compare_traces(b.trace(0, 700), rec2, max(1.0, abs(e0)), 1e-10)
+    rep2 = compare_traces(b.trace(0, 700), rec2, max(1.0, abs(e0)), 1e-10, seed=77, move0=nm)
     assert rep2["first_mismatch"] == 700
     tot, _ = b.total_energy(0)
     assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
@@ -186,7 +172,7 @@ def test_split_full_size():
     assert b.split_error() == 0
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=5, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=5)
     log("split full size", rep)
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
 
@@ -207,7 +193,7 @@ def test_batch_of_uneven_replicas_matches_oracle_per_replica():
         orc, e0 = oracle_of(sysd)
         rec = orc.run_philox_translate(nm, seed=1000 + r, replica=r)
         tr = b.trace(r, nm)
-        rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+        rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=1000 + r, replica=r)
         log("batch replica", r, rep)
         assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
 
@@ -246,7 +232,7 @@ def test_split_three_cell_grid_matches_oracle():
     assert b.split_error() == 0
     tr = b.trace(0, nm)
     rec = orc.run_philox_translate(nm, seed=41, replica=0)
-    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=41)
     log("split three-cell grid", rep, "accepted", int(rec["accepted"].sum()))
     assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
     tot, _ = b.total_energy(0)

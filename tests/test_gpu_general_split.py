@@ -11,7 +11,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import log
+from tests.gpu_helpers import compare_chain, log
 
 pytestmark = pytest.mark.gpu
 
@@ -23,15 +23,8 @@ def oracle_of(cfg, xyz, q, ids, g):
     return s, s.init()
 
 
-def compare(tr, rec, e_scale):
-    n = len(rec["du"])
-    mism = np.nonzero(tr["accepted"] != rec["accepted"])[0]
-    first = int(mism[0]) if len(mism) else n
-    fin = np.isfinite(rec["du"][:first])
-    assert np.array_equal(np.isinf(tr["du"][:first]), np.isinf(rec["du"][:first]))
-    err = np.abs(tr["du"][:first][fin] - rec["du"][:first][fin]) / np.maximum(np.abs(rec["du"][:first][fin]), e_scale)
-    return dict(first_mismatch=first, n=n, max_rel=float(err.max()) if len(err) else 0.0,
-                accepted=int(rec["accepted"][:first].sum()))
+def compare(tr, rec, e_scale, seed, replica=0, move0=0, rel=1e-6):
+    return compare_chain(tr, rec, e_scale, rel, seed, replica=replica, move0=move0)
 
 
 def check_fp64(b, orc, rep, label):
@@ -59,7 +52,7 @@ def test_split_polyelectrolyte_fp64(groups):
     b.sync()
     assert b.split_error() == 0
     rec = orc.run_philox_moves(nm, seed=33, kinds=kinds, replica=0)
-    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)))
+    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)), seed=33)
     assert b.info(0)["moves"] == nm
     check_fp64(b, orc, rep, "split polyelectrolyte groups=%d" % groups)
 
@@ -78,7 +71,7 @@ def test_split_polyelectrolyte_with_exclusions_fp64():
     b.sync()
     assert b.split_error() == 0
     rec = orc.run_philox_moves(nm, seed=34, kinds=kinds, replica=0)
-    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)))
+    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)), seed=34)
     check_fp64(b, orc, rep, "split polyelectrolyte exclusions")
 
 
@@ -94,7 +87,7 @@ def test_split_polyelectrolyte_fp32():
     b.sync()
     assert b.split_error() == 0
     rec = orc.run_philox_moves(nm, seed=33, kinds=kinds, replica=0)
-    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)))
+    rep = compare(b.trace(0, nm), rec, max(1.0, abs(e0)), seed=33)
     log("split polyelectrolyte fp32", rep)
     assert rep["max_rel"] <= 1e-6 and rep["first_mismatch"] > 500, rep
     tot, _ = b.total_energy(0)
