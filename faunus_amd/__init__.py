@@ -80,6 +80,7 @@ SIGNATURES = {
     "fmc_copy_probe": (C.c_int, [C.c_longlong, C.c_int, D]),
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
+    "fmc_splined_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, D, D, I, D, D, I]),
     "fmc_parse_state": (C.c_int, [C.c_char_p, I, I, D, D, D, I, I, I, I, D]),
     "fmc_load_state": (C.c_int, [C.c_void_p, C.c_char_p]),
     "fmc_server_create": (C.c_int, [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]),
@@ -357,6 +358,17 @@ class Engine:
         n = C.c_int()
         _check(lib().fmc_coulomb_spline(self.h, cap, _p(k), _p(c), C.byref(n)))
         return k[: n.value], c[: 6 * max(n.value - 1, 0)]
+
+    def splined_table(self, type_a, type_b, cap=256):
+        """The engine's nonbonded_splined table of a type pair."""
+        k = np.zeros(cap)
+        c = np.zeros(6 * cap)
+        n, hs = C.c_int(), C.c_int()
+        r0, r1 = C.c_double(), C.c_double()
+        _check(lib().fmc_splined_table(self.h, int(type_a), int(type_b), cap, _p(k), _p(c), C.byref(n), C.byref(r0),
+                                       C.byref(r1), C.byref(hs)))
+        m = n.value
+        return dict(knots=k[:m], coeff=c[:6 * max(m - 1, 0)], rmin2=r0.value, rmax2=r1.value, hardsphere=bool(hs.value))
 
     @property
     def num_kvectors(self):

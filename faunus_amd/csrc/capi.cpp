@@ -382,6 +382,23 @@ int fmc_coulomb_spline(fmc_engine* e, int capacity, double* knots, double* coeff
     });
 }
 
+int fmc_splined_table(fmc_engine* e, int type_a, int type_b, int capacity, double* knots, double* coeff,
+                      int* num_knots, double* rmin2, double* rmax2, int* hardsphere) {
+    return guarded([&] {
+        const fmc::Model& m = eng(e).model();
+        if (!m.splined) throw std::runtime_error("the nonbonded term is not nonbonded_splined");
+        if (type_a < 0 || type_b < 0 || type_a >= m.ntypes || type_b >= m.ntypes) throw std::runtime_error("bad atom type");
+        const fmc::SplinedPairCfg& s = m.spl[static_cast<size_t>(type_a * m.ntypes + type_b)];
+        *num_knots = static_cast<int>(s.table.knots.size());
+        *rmin2 = s.rmin2;
+        *rmax2 = s.rmax2;
+        *hardsphere = s.hardsphere ? 1 : 0;
+        for (int i = 0; i < capacity && i < *num_knots; ++i) knots[i] = s.table.knots[static_cast<size_t>(i)];
+        for (int i = 0; i < 6 * capacity && i < static_cast<int>(s.table.coeff.size()); ++i)
+            coeff[i] = s.table.coeff[static_cast<size_t>(i)];
+    });
+}
+
 int fmc_num_kvectors(fmc_engine* e, int* count) {
     return guarded([&] { *count = eng(e).num_kvectors(); });
 }

@@ -85,6 +85,47 @@ double Spline::eval(double x) const {
     return s + c[0];
 }
 
+double Model::pair_exact(int a, double qa, int b, double qb, double r2) const {
+    const PairParam& p = pairs[static_cast<size_t>(a * ntypes + b)];
+    double u = 0.0;
+    if (p.lj_e4 != 0.0) {
+        double x = p.lj_s2 / r2;
+        x = x * x * x;
+        u = u + p.lj_e4 * (x * x - x);
+    }
+    if (p.wca_e4 != 0.0 && !(r2 > p.wca_s2 * 1.2599210498948732)) {
+        double x = p.wca_s2 / r2;
+        x = x * x * x;
+        u = u + p.wca_e4 * (x * x - x + 0.25);
+    }
+    if (r2 < p.hs_s2) u = u + kInf;
+    if (p.coulomb != 0.0) {
+        const CoulombModel& c = coulomb;
+        double e = 0.0;
+        if (c.scheme == kCoulPM) e = c.lB * qa * qb / std::sqrt(r2);
+        else {
+            const double r = std::sqrt(r2) + 2.220446049250313e-16;
+            if (c.scheme == kCoulPlain) e = c.lB * (qa * qb / r);
+            else if (r < c.rc) {
+                const double q = r * c.rc_inv;
+                const double s = c.splined ? c.S.eval(q) : c.short_range_exact(q);
+                e = c.lB * (qa * qb / r * s);
+            }
+        }
+        u = u + e;
+    }
+    return u;
+}
+
+double Model::pair_energy(int a, double qa, int b, double qb, double r2) const {
+    if (!splined) return pair_exact(a, qa, b, qb, r2);
+    const SplinedPairCfg& s = spl[static_cast<size_t>(a * ntypes + b)];
+    if (r2 >= s.rmax2) return 0.0;
+    if (r2 > s.rmin2) return s.table.eval(r2);
+    if (s.hardsphere) return kInf;
+    return pair_exact(a, qa, b, qb, r2);
+}
+
 double CoulombModel::short_range_exact(double q) const {
     if (scheme == kCoulEwald) return std::erfc(alpha * rc * q);
     if (scheme == kCoulFanourgakis) {
@@ -190,6 +231,8 @@ CoulombModel parse_coulomb(const Model& m, const Value& j, bool pm) {
 }
 
 } // namespace
+
+static void build_splined(Model& m, const Value& js);
 
 Model Model::from_json(const std::string& text) {
     Model m;
@@ -311,12 +354,13 @@ Model Model::from_json(const std::string& text) {
                     PairParam& p = m.pairs[static_cast<size_t>(a * n + b)];
                     if (pname == "lennardjones") {
                         if (p.lj_e4 != 0 || p.lj_s2 != 0) throw std::runtime_error("duplicate lennardjones for a pair");
+                        // MixerPairPotentialBase::from_json (potentials.cpp:285-310) reads no
+                        // cutoff: a file with one runs untruncated LJ in the reference
+                        if (pcfg.contains("cutoff"))
+                            throw std::runtime_error("lennardjones has no cutoff in the reference; use nonbonded_splined "
+                                                     "with rmax");
                         p.lj_s2 = s2[static_cast<size_t>(a * n + b)];
                         p.lj_e4 = e4[static_cast<size_t>(a * n + b)];
-                        if (pcfg.contains("cutoff")) {
-                            const double rc = pcfg.at("cutoff").as_number();
-                            p.lj_cut2 = rc * rc;
-                        }
                     } else {
                         if (p.wca_e4 != 0 || p.wca_s2 != 0) throw std::runtime_error("duplicate wca for a pair");
                         p.wca_s2 = s2[static_cast<size_t>(a * n + b)];
@@ -341,8 +385,9 @@ Model Model::from_json(const std::string& text) {
         std::vector<std::pair<int, int>> all_cells;
         for (int a = 0; a < n; ++a)
             for (int b = 0; b < n; ++b) all_cells.emplace_back(a, b);
-        if (key == "nonbonded" || key == "nonbonded_exact") {
-            m.anisotropic_form = true;
+        if (key == "nonbonded" || key == "nonbonded_exact" || key == "nonbonded_splined") {
+            m.anisotropic_form = key != "nonbonded_splined";
+            m.splined = key == "nonbonded_splined";
             // cells overridden by "A B" lists do not take the default list
             std::vector<char> overridden(static_cast<size_t>(n * n), 0);
             for (auto& [pk, pv] : value.members()) {
@@ -418,12 +463,70 @@ Model Model::from_json(const std::string& text) {
     const bool coul_any = std::any_of(m.pairs.begin(), m.pairs.end(), [](const PairParam& p) { return p.coulomb != 0; });
     if (coul_any) range = std::max(range, m.coulomb.rc);
     for (const auto& p : m.pairs) {
-        if (p.lj_e4 != 0) range = std::max(range, std::sqrt(p.lj_cut2));
+        if (p.lj_e4 != 0) range = kInf;
         if (p.wca_e4 != 0) range = std::max(range, std::sqrt(p.wca_s2 * 1.2599210498948732));
         if (p.hs_s2 != 0) range = std::max(range, std::sqrt(p.hs_s2));
     }
+    if (m.splined) {
+        for (auto& rec : cfg.at("energy").items()) {
+            auto& [key, value] = jsonlite::single_item(rec);
+            if (key == "nonbonded_splined") build_splined(m, value);
+        }
+        range = 0.0; // every table is zero from its rmax on
+        for (const auto& s : m.spl) range = std::max(range, std::sqrt(s.rmax2));
+    }
     m.pair_range = range;
     return m;
+}
+
+// SplinedPotential::from_json (potentials.cpp:1513-1550): for every type pair, rmin from
+// half the summed sigmas and rmax from "rmax" (default 10 rmin) or "cutoff_g2g", moved in
+// steps of 0.01 A until |u(rmin)| is within 10 % of u_at_rmin (findLowerDistance,
+// 1452-1477) and |u(rmax)| <= u_at_rmax (findUpperDistance, 1489-1511); then an Andrea
+// table of u(r^2) with utol / ftol (createKnots, 1559-1600).  The pair's particles carry
+// the atom types' charges.
+static void build_splined(Model& m, const Value& js) {
+    const double utol = js.value("utol", 1e-3), ftol = js.value("ftol", 1e-2);
+    const bool hardsphere = js.value("hardsphere", false);
+    const double energy_at_rmin = js.value("u_at_rmin", 20.0), energy_at_rmax = js.value("u_at_rmax", 1e-6);
+    const double dr = 1e-2;
+    const long max_iterations = 1000000;
+    const int n = m.ntypes;
+    m.spl.assign(static_cast<size_t>(n * n), SplinedPairCfg{});
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double qi = m.atoms[static_cast<size_t>(i)].charge, qj = m.atoms[static_cast<size_t>(j)].charge;
+            auto u = [&](double r2) { return m.pair_exact(i, qi, j, qj, r2); };
+            double rmin = 0.5 * (m.atoms[static_cast<size_t>(i)].sigma + m.atoms[static_cast<size_t>(j)].sigma);
+            double rmax = js.value("rmax", rmin * 10);
+            if (js.contains("cutoff_g2g")) {
+                const Value& c = js.at("cutoff_g2g");
+                if (c.is_number()) rmax = c.as_number();
+                else if (c.is_object()) rmax = c.at("default").as_number();
+            }
+            long it = 0;
+            while (rmin >= dr) {
+                if (it++ == max_iterations) throw std::runtime_error("pair potential spline: cannot determine rmin");
+                const double e = std::fabs(u(rmin * rmin));
+                if (e > energy_at_rmin * 1.1) rmin += dr;
+                else if (e < energy_at_rmin / 1.1) rmin -= dr;
+                else break;
+            }
+            it = 0;
+            while (rmax >= dr) {
+                if (it++ == max_iterations) throw std::runtime_error("pair potential spline: cannot determine rmax");
+                if (std::fabs(u(rmax * rmax)) > energy_at_rmax) rmax += dr;
+                else break;
+            }
+            SplinedPairCfg s;
+            const double r0 = std::sqrt(rmin * rmin), r1 = std::sqrt(rmax * rmax);
+            s.table = Spline::build(u, rmin * rmin, rmax * rmax, utol, ftol);
+            s.rmin2 = r0 * r0;
+            s.rmax2 = r1 * r1;
+            s.hardsphere = hardsphere && !(s.table.eval(s.rmin2 + dr) < 0);
+            m.spl[static_cast<size_t>(i * n + j)] = s;
+            m.spl[static_cast<size_t>(j * n + i)] = s;
+        }
 }
 
 StateCfg StateCfg::from_json(const std::string& text) {

@@ -3,7 +3,8 @@
 // from the reference's JSON (the YAML input after yason.py conversion).
 //
 // Supported energy terms (energy.cpp:1280-1327): nonbonded / nonbonded_exact
-// (FunctorPotential with "default" and "A B" lists), nonbonded_coulomblj,
+// (FunctorPotential with "default" and "A B" lists), nonbonded_splined
+// (SplinedPotential: the same lists tabulated per type pair), nonbonded_coulomblj,
 // nonbonded_coulombwca, nonbonded_pm / nonbonded_coulombhs, nonbonded_pmwca, plus
 // `maxenergy` and `bonded` (intra-molecular harmonic bonds, energy.cpp:974-1124).  Pair potentials: lennardjones, wca, hardsphere, coulomb
 // {type: plain|ewald|fanourgakis}.  Ewald reciprocal space is added when the
@@ -57,10 +58,19 @@ struct CoulombModel {
 // Per type pair parameters of the additive pair form u = LJ + WCA + HS + Coulomb.
 // Terms absent for a pair have zero coefficients.
 struct PairParam {
-    double lj_s2 = 0, lj_e4 = 0, lj_cut2 = kInf;
+    double lj_s2 = 0, lj_e4 = 0;
     double wca_s2 = 0, wca_e4 = 0;
     double hs_s2 = 0;
     double coulomb = 0; // 1 if the pair carries the Coulomb term
+};
+
+// SplinedPotential's data for one type pair (potentials.h:770-824): Andrea table of the
+// exact pair energy in r^2 on [rmin^2, rmax^2], zero beyond rmax, exact (or +inf with
+// `hardsphere`) below rmin.
+struct SplinedPairCfg {
+    Spline table;
+    double rmin2 = 0, rmax2 = 0;
+    bool hardsphere = false;
 };
 
 struct EwaldModel {
@@ -105,6 +115,13 @@ struct Model {
     std::vector<std::string> term_names; // Hamiltonian order: energy list, with [selfenergy] before nonbonded, [ewald] after
     int bonded_term = -1;                // index of "bonded" in term_names, -1 if absent
     double pair_range = kInf; // distance beyond which every pair term vanishes
+    bool splined = false;               // nonbonded_splined
+    std::vector<SplinedPairCfg> spl;    // ntypes*ntypes (symmetric) when splined
+
+    // Exact pair energy u = LJ + WCA + HS + Coulomb of types a, b with charges qa, qb.
+    double pair_exact(int a, double qa, int b, double qb, double r2) const;
+    // The pair energy the nonbonded term uses (splined when the term is).
+    double pair_energy(int a, double qa, int b, double qb, double r2) const;
 
     static Model from_json(const std::string& text);
 };

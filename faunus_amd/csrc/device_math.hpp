@@ -74,10 +74,10 @@ __device__ __forceinline__ double coulomb_pair(const DevCoulomb& c, double qa, d
 }
 
 // Additive pair energy u = LJ + WCA + HS + Coulomb (absent terms have zero parameters).
-__device__ __forceinline__ double pair_energy(const DevPair& p, const DevCoulomb& c, double qa, double qb,
-                                              double r2) {
+__device__ __forceinline__ double pair_energy_exact(const DevPair& p, const DevCoulomb& c, double qa, double qb,
+                                                    double r2) {
     double u = 0.0;
-    if (p.lj_e4 != 0.0 && r2 < p.lj_cut2) {
+    if (p.lj_e4 != 0.0) {
         double x = p.lj_s2 / r2;
         x = x * x * x;
         u = u + p.lj_e4 * (x * x - x);
@@ -90,6 +90,64 @@ __device__ __forceinline__ double pair_energy(const DevPair& p, const DevCoulomb
     if (r2 < p.hs_s2) u = u + __builtin_huge_val();
     if (p.coul != 0.0) u = u + coulomb_pair(c, qa, qb, r2);
     return u;
+}
+
+// Andrea spline of a nonbonded_splined table at r2 in (rmin2, rmax2) (tabulate.h:184-202:
+// interval by lower_bound, Horner from the top coefficient).  The search starts one
+// knot before the first knot of r2's bucket and takes a fixed number of steps.
+__device__ __forceinline__ double spline_table_eval(const DevSplineTable& T, double r2) {
+    int b = static_cast<int>((r2 - T.x0) * T.inv_w);
+    b = b < 0 ? 0 : (b > kSplBuckets - 1 ? kSplBuckets - 1 : b);
+    int lo = static_cast<int>(T.bucket[b]) - 1;
+    lo = lo < 0 ? 0 : lo;
+    const int n = T.n;
+    for (int s = 0; s < T.steps; ++s) {
+        const double k = T.knots[lo < n ? lo : n - 1];
+        lo += (lo < n && k < r2) ? 1 : 0;
+    }
+    int pos = lo - 1;
+    pos = pos < 0 ? 0 : (pos > n - 2 ? n - 2 : pos);
+    const double* c = T.coeff + 6 * pos;
+    const double z = r2 - T.knots[pos];
+    double s = 0.0;
+    s = z * (s + c[5]);
+    s = z * (s + c[4]);
+    s = z * (s + c[3]);
+    s = z * (s + c[2]);
+    s = z * (s + c[1]);
+    return s + c[0];
+}
+
+// SplinedPotential::operator() (potentials.h:807-823): zero from rmax on, the spline
+// above rmin, +inf below rmin with hard-sphere repulsion, else the exact energy.
+// `below` is true when the exact energy is needed (r2 <= rmin2, no hard sphere).
+__device__ __forceinline__ double splined_pair(const DevSplineTable& T, double r2, bool& below) {
+    below = false;
+    if (r2 >= T.rmax2) return 0.0;
+    if (r2 > T.rmin2) return spline_table_eval(T, r2);
+    if (T.hs) return __builtin_huge_val();
+    below = true;
+    return 0.0;
+}
+
+// The nonbonded term's pair energy: splined when the term is nonbonded_splined.
+__device__ __forceinline__ double pair_energy(const DevPair& p, const DevCoulomb& c, double qa, double qb,
+                                              double r2) {
+    if (c.nspl > 0) {
+        bool below;
+        const double u = splined_pair(c.spl[p.spl], r2, below);
+        if (!below) return u;
+    }
+    return pair_energy_exact(p, c, qa, qb, r2);
+}
+
+// Copy of a struct into LDS by all threads of the block, 16 bytes per thread and step.
+template <class T>
+__device__ __forceinline__ void lds_copy(T& dst, const T* __restrict__ src, int tid, int nthreads) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "16-byte granules");
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(&dst);
+    for (int i = tid; i < static_cast<int>(sizeof(T) / 16); i += nthreads) d[i] = s[i];
 }
 
 // Wavefront (64 lanes) sum with cross-lane shuffles, then across the block's waves.

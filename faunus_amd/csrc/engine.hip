@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
     const bool is_pair_block = static_cast<int>(blockIdx.x) < pair_blocks;
     if (is_pair_block) {
         for (int i = tid; i < ntypes * ntypes; i += kBlock) s_pairs[i] = pairs_g[i];
-        if (tid == 0) s_coul = *coul_g;
+        lds_copy(s_coul, coul_g, tid, kBlock);
         __syncthreads();
         const bool single = mv.change_group >= 0;
         auto allowed_other = [&](int gi, int gj, bool new_state) -> bool {
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void all_pairs_kernel(const double4* __rest
     __shared__ uint32_t tmeta[kBlock];
     __shared__ double s_red[kBlock / 64];
     for (int i = threadIdx.x; i < ntypes * ntypes; i += kBlock) s_pairs[i] = pairs_g[i];
-    if (threadIdx.x == 0) s_coul = *coul_g;
+    lds_copy(s_coul, coul_g, static_cast<int>(threadIdx.x), kBlock);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double4 pi = make_double4(0, 0, 0, 0);
     uint32_t mi = 0;
@@ -576,9 +576,51 @@ void Engine::upload_potentials() {
     h_pairs_.assign(static_cast<size_t>(nt * nt), DevPair{});
     for (int i = 0; i < nt * nt; ++i) {
         const PairParam& p = model_.pairs[static_cast<size_t>(i)];
-        h_pairs_[static_cast<size_t>(i)] = DevPair{p.lj_s2, p.lj_e4, p.lj_cut2, p.wca_s2, p.wca_e4, p.hs_s2, p.coulomb, 0.0};
+        h_pairs_[static_cast<size_t>(i)] = DevPair{p.lj_s2, p.lj_e4, p.wca_s2, p.wca_e4, p.hs_s2, p.coulomb, 0, 0, 0.0};
     }
     std::memset(&h_coul_, 0, sizeof(h_coul_));
+    if (model_.splined) {
+        // one table per unordered type pair
+        std::vector<int> slot(static_cast<size_t>(nt * nt), -1);
+        int ns = 0;
+        for (int a = 0; a < nt; ++a)
+            for (int b = 0; b <= a; ++b) {
+                if (ns == kSplMaxTables)
+                    throw std::runtime_error("nonbonded_splined: at most " + std::to_string(kSplMaxTables) +
+                                             " type-pair tables on the device (four atom types)");
+                const SplinedPairCfg& s = model_.spl[static_cast<size_t>(a * nt + b)];
+                const int n = static_cast<int>(s.table.knots.size());
+                if (n < 2 || n > kSplMaxKnots)
+                    throw std::runtime_error("nonbonded_splined: a table has " + std::to_string(n) +
+                                             " knots; the device holds at most " + std::to_string(kSplMaxKnots));
+                DevSplineTable& T = h_coul_.spl[ns];
+                T.rmin2 = s.rmin2;
+                T.rmax2 = s.rmax2;
+                T.n = n;
+                T.hs = s.hardsphere ? 1 : 0;
+                std::copy(s.table.knots.begin(), s.table.knots.end(), T.knots);
+                std::copy(s.table.coeff.begin(), s.table.coeff.end(), T.coeff);
+                T.x0 = s.table.knots.front();
+                const double span = s.table.knots.back() - T.x0;
+                T.inv_w = kSplBuckets / span;
+                for (int k = 0; k <= kSplBuckets; ++k) {
+                    const double x = T.x0 + k / T.inv_w;
+                    T.bucket[k] = static_cast<unsigned char>(
+                        std::lower_bound(s.table.knots.begin(), s.table.knots.end(), x) - s.table.knots.begin());
+                }
+                // lower_bound(x) for x in bucket k lies in [bucket[k], bucket[k+1]] (one more
+                // for rounding of the bucket edges); the search starts one knot early
+                int steps = 1;
+                for (int k = 0; k < kSplBuckets; ++k) {
+                    const int start = std::max(0, static_cast<int>(T.bucket[k]) - 1);
+                    steps = std::max(steps, std::min(n, static_cast<int>(T.bucket[k + 1]) + 1) - start + 1);
+                }
+                T.steps = steps;
+                slot[static_cast<size_t>(a * nt + b)] = slot[static_cast<size_t>(b * nt + a)] = ns++;
+            }
+        h_coul_.nspl = ns;
+        for (int i = 0; i < nt * nt; ++i) h_pairs_[static_cast<size_t>(i)].spl = slot[static_cast<size_t>(i)];
+    }
     const CoulombModel& c = model_.coulomb;
     h_coul_.scheme = c.scheme;
     h_coul_.splined = c.splined ? 1 : 0;

@@ -19,14 +19,32 @@ constexpr int kSplineBuckets = 1024;   // uniform buckets of q in [0, 1) indexin
 constexpr int kMaxTypes = 16;          // atom types handled by the device kernels
 constexpr int kPhaseTableMoved = 8;    // moved particles with LDS phase tables (Ewald)
 constexpr int kBlock = 256;
+constexpr int kSplMaxKnots = 24;   // knots of one nonbonded_splined table (utol 1e-3 gives 7-10)
+constexpr int kSplMaxTables = 10;  // distinct type pairs with a table (four atom types)
+constexpr int kSplBuckets = 32;    // uniform buckets in r^2 indexing a table's knots
 
 // Device copy of one type pair's parameters (64 bytes).
 struct DevPair {
-    double lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
+    double lj_s2, lj_e4, wca_s2, wca_e4, hs_s2, coul;
+    int spl; // the pair's nonbonded_splined table (0 when the term is not splined)
+    int pad;
+    double pad2;
 };
 
-// Device copy of the Coulomb model (scheme, constants, spline of S(q)).
-struct DevCoulomb {
+// One nonbonded_splined table (SplinedPotential's KnotData, potentials.h:773-781): the
+// Andrea spline of the exact pair energy in r^2 on [rmin2, rmax2]; bucket[b] is the first
+// knot >= x0 + b / inv_w, where the interval search starts.
+struct DevSplineTable {
+    double rmin2, rmax2, x0, inv_w;
+    int n, steps, hs, pad;
+    double knots[kSplMaxKnots];
+    double coeff[6 * kSplMaxKnots];
+    unsigned char bucket[kSplBuckets + 8];
+};
+
+// Device copy of the nonbonded model's shared data: the Coulomb scheme, its constants
+// and spline of S(q), and the nonbonded_splined tables (nspl > 0 when the term is splined).
+struct alignas(16) DevCoulomb {
     int scheme;   // CoulombScheme
     int splined;
     int nknots;
@@ -36,6 +54,8 @@ struct DevCoulomb {
     double coeff[6 * kMaxKnots];
     // bucket[b]: index of the first knot >= b / kSplineBuckets (start of the interval search)
     unsigned short bucket[kSplineBuckets + 1];
+    int nspl;
+    DevSplineTable spl[kSplMaxTables];
 };
 
 // One moved particle: absolute index, type/group, new and old coordinates (x,y,z,q).

@@ -47,13 +47,10 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         s_pairs[i] = S.pairs[i];
         if (kF32) {
             const DevPair& p = S.pairs[i];
-            s_pairs_f[i] = DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4),
-                                    static_cast<float>(p.lj_cut2), static_cast<float>(p.wca_s2),
-                                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2),
-                                    static_cast<float>(p.coul), 0.0f};
+            s_pairs_f[i] = to_f32(p);
         }
     }
-    if (tid == 0) s_coul = *S.coul;
+    lds_copy(s_coul, S.coul, tid, kThreads);
     for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
     for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
     for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
@@ -62,8 +59,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         if (S.stab)
             for (int i = tid; i < kSTable; i += kThreads) s_stab[i] = S.stab[i];
     }
-    const CoulF cf{s_coul.scheme, static_cast<float>(s_coul.lB), static_cast<float>(s_coul.rc),
-                   static_cast<float>(s_coul.rc_inv)};
+    const CoulF cf = coul_f32(s_coul);
 
     // this thread's chunk of k-vectors and their running structure factors
     const int k0 = tid * kpt;
@@ -360,12 +356,17 @@ __global__ void ladder_swap_kernel(ReplicaState* states, const int* ladder_begin
 template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
 void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap) {
     auto fn = mc_advance_kernel<kThreads, kMaxKPT, kCells, kF32>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024),
+    // dynamic LDS: what the static LDS leaves of the CU's 160 KB, at most 96 KB (set once)
+    static size_t limit = 0;
+    if (limit == 0) {
+        hipFuncAttributes attr;
+        check_hip(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(fn)), "hipFuncGetAttributes");
+        limit = std::min<size_t>(96 * 1024, 160 * 1024 - attr.sharedSizeBytes);
+        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(limit)),
                   "hipFuncSetAttribute");
-        attr_set = true;
     }
+    if (lds > limit) throw std::runtime_error("k-vector table too large for LDS");
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, s, m, tr, cap);
 }
 

@@ -12,8 +12,14 @@ namespace fmc {
 constexpr int kMaxGk = 1025; // |n|^2 table (ncut <= 32)
 
 struct alignas(16) DevPairF {
-    float lj_s2, lj_e4, lj_cut2, wca_s2, wca_e4, hs_s2, coul, pad;
+    float lj_s2, lj_e4, wca_s2, wca_e4, hs_s2, coul;
+    int spl, pad;
 };
+
+__device__ __forceinline__ DevPairF to_f32(const DevPair& p) {
+    return DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4), static_cast<float>(p.wca_s2),
+                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2), static_cast<float>(p.coul), p.spl, 0};
+}
 
 // An empty asm statement that takes and returns the values: the loads that produce
 // them happen here, unconditionally, and are not moved into the branches the compiler
@@ -27,11 +33,16 @@ constexpr int kTermLJ = 1, kTermWCA = 2, kTermHS = 4, kTermAll = 7;
 // FP64 pair energy with the same term switch (exact arithmetic of pair_energy).
 __device__ __forceinline__ double pair_energy_t(const DevPair& p, const DevCoulomb& c, double qa, double qb, double r2,
                                                 int terms) {
+    if (c.nspl > 0) {
+        bool below;
+        const double us = splined_pair(c.spl[p.spl], r2, below);
+        if (!below) return us;
+    }
     double u = 0.0;
     if (terms & kTermLJ) {
         double x = p.lj_s2 / r2;
         x = x * x * x;
-        u = u + ((p.lj_e4 != 0.0 && r2 < p.lj_cut2) ? p.lj_e4 * (x * x - x) : 0.0);
+        u = u + ((p.lj_e4 != 0.0) ? p.lj_e4 * (x * x - x) : 0.0);
     }
     if (terms & kTermWCA) {
         double x = p.wca_s2 / r2;
@@ -55,7 +66,13 @@ constexpr int kSTable = 1024;
 struct CoulF {
     int scheme;
     float lB, rc, rc_inv;
+    const DevSplineTable* spl; // nonbonded_splined tables (in LDS), or null
 };
+
+__device__ __forceinline__ CoulF coul_f32(const DevCoulomb& c) {
+    return CoulF{c.scheme, static_cast<float>(c.lB), static_cast<float>(c.rc), static_cast<float>(c.rc_inv),
+                 c.nspl > 0 ? c.spl : nullptr};
+}
 
 // Minimum image by rounding (|dx| <= L for particles inside the box).
 __device__ __forceinline__ float dist2f_round(float L, float invL, float ax, float ay, float az, float bx, float by,
@@ -70,25 +87,31 @@ __device__ __forceinline__ float dist2f_round(float L, float invL, float ax, flo
 __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
                                                 int terms, float qa, float qb, float r2) {
     // both halves of the pair parameters in two vector loads, then register arithmetic
-    float4 pa = reinterpret_cast<const float4*>(&p)[0]; // lj_s2, lj_e4, lj_cut2, wca_s2
-    float4 pb = reinterpret_cast<const float4*>(&p)[1]; // wca_e4, hs_s2, coul, -
+    float4 pa = reinterpret_cast<const float4*>(&p)[0]; // lj_s2, lj_e4, wca_s2, wca_e4
+    float4 pb = reinterpret_cast<const float4*>(&p)[1]; // hs_s2, coul, spl (bits), -
     materialize(pa);
     materialize(pb);
+    if (c.spl != nullptr) {
+        // nonbonded_splined: the table is evaluated in FP64 from the FP32 distance
+        bool below;
+        const double us = splined_pair(c.spl[__float_as_int(pb.z)], static_cast<double>(r2), below);
+        if (!below) return static_cast<float>(us);
+    }
     float u = 0.0f;
     if (terms & (kTermLJ | kTermWCA)) {
         const float ir2 = __builtin_amdgcn_rcpf(r2);
         if (terms & kTermLJ) {
             float x = pa.x * ir2;
             x = x * x * x;
-            u += (r2 < pa.z) ? pa.y * (x * x - x) : 0.0f;
+            u += pa.y * (x * x - x);
         }
         if (terms & kTermWCA) {
-            float w = pa.w * ir2;
+            float w = pa.z * ir2;
             w = w * w * w;
-            u += (r2 > pa.w * 1.2599210498948732f) ? 0.0f : pb.x * (w * w - w + 0.25f);
+            u += (r2 > pa.z * 1.2599210498948732f) ? 0.0f : pa.w * (w * w - w + 0.25f);
         }
     }
-    if (terms & kTermHS) u += (r2 < pb.y) ? __builtin_huge_valf() : 0.0f;
+    if (terms & kTermHS) u += (r2 < pb.x) ? __builtin_huge_valf() : 0.0f;
     if (c.scheme != kCoulNone) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float e;
@@ -106,7 +129,7 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
             const float sv = ((k.w * f + k.z) * f + k.y) * f + k.x;
             e = (r < c.rc) ? c.lB * (qa * qb * ir * sv) : 0.0f;
         }
-        u += (pb.z != 0.0f) ? e : 0.0f;
+        u += (pb.y != 0.0f) ? e : 0.0f;
     }
     return u;
 }

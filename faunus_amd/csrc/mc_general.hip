@@ -349,16 +349,11 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         s_pairs[i] = S.pairs[i];
         if (kF32) {
             const DevPair& p = S.pairs[i];
-            s_pairs_f[i] = DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4),
-                                    static_cast<float>(p.lj_cut2), static_cast<float>(p.wca_s2),
-                                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2),
-                                    static_cast<float>(p.coul), 0.0f};
+            s_pairs_f[i] = to_f32(p);
         }
     }
-    if (tid == 0) {
-        s_coul = *S.coul;
-        s_abort = 0;
-    }
+    lds_copy(s_coul, S.coul, tid, kThreads);
+    if (tid == 0) s_abort = 0;
     for (int i = tid; i < nt; i += kThreads) s_dpa[i] = S.dp_atom[i];
     for (int i = tid; i < S.nkinds; i += kThreads) s_kinds[i] = S.kinds[i];
     for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
@@ -368,8 +363,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         if (S.stab)
             for (int i = tid; i < kSTable; i += kThreads) s_stab[i] = S.stab[i];
     }
-    const CoulF cf{s_coul.scheme, static_cast<float>(s_coul.lB), static_cast<float>(s_coul.rc),
-                   static_cast<float>(s_coul.rc_inv)};
+    const CoulF cf = coul_f32(s_coul);
 
     const int k0 = kb + tid * kpt;
     double2 q[kMaxKPT];

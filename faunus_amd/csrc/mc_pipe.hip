@@ -86,8 +86,8 @@ __global__ __launch_bounds__(256) void mc_pipe_kernel(ReplicaState* __restrict__
     const double L = S.L, half = S.half, inv = S.inv;
     const Box box{{L, L, L}, {half, half, half}};
     for (int i = tid; i < nt * nt; i += 256) s_pairs[i] = S.pairs[i];
+    lds_copy(s_coul, S.coul, tid, 256);
     if (tid == 0) {
-        s_coul = *S.coul;
         s_abort = 0;
         s_p = 0;
         s_hist_n = 0;

@@ -89,8 +89,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const int width = 2 * ncut + 1;
     const double L = S.L, half = S.half, inv = S.inv;
     for (int i = tid; i < nt * nt; i += kThreads) s_pairs[i] = S.pairs[i];
+    lds_copy(s_coul, S.coul, tid, kThreads);
     if (tid == 0) {
-        s_coul = *S.coul;
         s_abort = 0;
         s_stop = 0;
         s_pf_ok = 0;

@@ -357,6 +357,13 @@ int fmc_copy_probe(long long bytes, int repeats, double* ms_per_launch);
 /* Andrea spline of the Coulomb short-range function built by the engine (for tests). */
 int fmc_coulomb_spline(fmc_engine* engine, int capacity, double* knots, double* coeff, int* num_knots);
 int fmc_num_kvectors(fmc_engine* engine, int* count);
+/*
+ * nonbonded_splined table of a type pair built by the engine (SplinedPotential's
+ * KnotData, src/potentials.h:773-781; for tests): knots in r^2 (ascending), 6
+ * coefficients per interval, rmin^2, rmax^2 and the hard-sphere flag.
+ */
+int fmc_splined_table(fmc_engine* engine, int type_a, int type_b, int capacity, double* knots, double* coeff,
+                      int* num_knots, double* rmin2, double* rmax2, int* hardsphere);
 
 #ifdef __cplusplus
 }

@@ -128,8 +128,7 @@ double PairTerm::short_range(double q) const {
 double PairTerm::operator()(const Particle& a, const Particle& b, double r2) const {
     const size_t ij = static_cast<size_t>(a.id) * ntypes + b.id;
     switch (kind) {
-    case PotKind::LennardJones: { // potentials.h:42-49 (+ optional truncation extension)
-        if (r2 >= lj_cutoff2) return 0.0;
+    case PotKind::LennardJones: { // potentials.h:42-49
         double x = sigma2[ij] / r2;
         x = x * x * x;
         return eps4[ij] * (x * x - x);
@@ -267,10 +266,10 @@ static PairTerm make_mixed_term(PotKind kind, const Value& j, const std::vector<
         t.eps4 = pair_matrix(atoms, rule, false, eps, four_eps);
         apply_custom_pairs(j, atoms, t.eps4, "eps", [&u](double x) { return 4 * u.kJmol(x); });
     }
-    if (kind == PotKind::LennardJones && j.contains("cutoff")) {
-        const double rc = j.at("cutoff").as_number();
-        t.lj_cutoff2 = rc * rc;
-    }
+    // MixerPairPotentialBase::from_json (potentials.cpp:285-310) reads no cutoff: a file
+    // with one would run untruncated LJ in the reference, so it is refused here
+    if (kind == PotKind::LennardJones && j.contains("cutoff"))
+        throw std::runtime_error("lennardjones has no cutoff in the reference; use nonbonded_splined with rmax");
     return t;
 }
 
@@ -285,6 +284,63 @@ static PairTerm make_term(const std::string& name, const Value& cfg, const std::
     else throw std::runtime_error("potential '" + name + "': unknown or unsupported potential");
     t.ntypes = static_cast<int>(atoms.size());
     return t;
+}
+
+// ---------------------------------------------------------------- SplinedPotential
+void build_splined_tables(PairPotential& pp, const Value& js, const std::vector<AtomType>& atoms) {
+    AndreaSpline spline;
+    spline.set_tolerance(js.value("utol", 1e-3), js.value("ftol", 1e-2));
+    const bool hardsphere = js.value("hardsphere", false);
+    const double energy_at_rmin = js.value("u_at_rmin", 20.0);
+    const double energy_at_rmax = js.value("u_at_rmax", 1e-6);
+    const double dr = 1e-2;
+    const long max_iterations = 1000000;
+    const int nt = pp.ntypes;
+    pp.splined = true;
+    pp.tables.assign(static_cast<size_t>(nt * nt), SplinedPair{});
+    for (int i = 0; i < nt; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            if (atoms[static_cast<size_t>(i)].implicit || atoms[static_cast<size_t>(j)].implicit) continue;
+            // the pair's particles carry the atom types' charges (Particle(AtomData))
+            Particle a, b;
+            a.id = i;
+            a.charge = atoms[static_cast<size_t>(i)].charge;
+            b.id = j;
+            b.charge = atoms[static_cast<size_t>(j)].charge;
+            auto u = [&](double r2) { return pp.exact(a, b, r2); };
+            double rmin = 0.5 * (atoms[static_cast<size_t>(i)].sigma + atoms[static_cast<size_t>(j)].sigma);
+            double rmax = js.value("rmax", rmin * 10);
+            if (js.contains("cutoff_g2g")) {
+                const Value& c = js.at("cutoff_g2g");
+                if (c.is_number()) rmax = c.as_number();
+                else if (c.is_object()) rmax = c.at("default").as_number();
+            }
+            long iterations = 0;
+            while (rmin >= dr) { // findLowerDistance
+                if (iterations++ == max_iterations)
+                    throw std::runtime_error("Pair potential spline error: cannot determine minimum distance");
+                const double e = std::fabs(u(rmin * rmin));
+                if (e > energy_at_rmin * 1.1) rmin += dr;
+                else if (e < energy_at_rmin / 1.1) rmin -= dr;
+                else break;
+            }
+            iterations = 0;
+            while (rmax >= dr) { // findUpperDistance
+                if (iterations++ == max_iterations)
+                    throw std::runtime_error("Pair potential spline error: cannot determine maximum distance");
+                if (std::fabs(u(rmax * rmax)) > energy_at_rmax) rmax += dr;
+                else break;
+            }
+            SplinedPair s; // createKnots
+            s.table = spline.generate(u, rmin * rmin, rmax * rmax);
+            s.rmin2 = s.table.xmin;
+            s.rmax2 = s.table.xmax;
+            s.hardsphere = hardsphere;
+            if (AndreaSpline::eval(s.table, s.rmin2 + dr) < 0) s.hardsphere = false; // attractive below rmin
+            pp.tables[static_cast<size_t>(i * nt + j)] = s;
+            pp.tables[static_cast<size_t>(j * nt + i)] = s;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- Ewald
@@ -753,8 +809,9 @@ std::unique_ptr<Hamiltonian> System::build_hamiltonian() {
         pp.ntypes = nt;
         pp.umatrix.assign(static_cast<size_t>(nt * nt), {});
         const Value* coulomb_json = nullptr;
-        if (key == "nonbonded" || key == "nonbonded_exact") { // FunctorPotential, aniso form
-            pp.anisotropic_form = true;
+        if (key == "nonbonded" || key == "nonbonded_exact" || key == "nonbonded_splined") {
+            // FunctorPotential (aniso form); SplinedPotential takes sqdist (energy.cpp:1301-1305)
+            pp.anisotropic_form = key != "nonbonded_splined";
             auto add_list = [&](const Value& list) {
                 std::vector<int> idx;
                 for (auto& r : list.items()) {
@@ -781,6 +838,7 @@ std::unique_ptr<Hamiltonian> System::build_hamiltonian() {
             }
             for (auto& r : value.at("default").items())
                 if (r.is_object() && r.size() == 1 && r.contains("coulomb")) { coulomb_json = &r.at("coulomb"); break; }
+            if (key == "nonbonded_splined") build_splined_tables(pp, value, atoms);
         } else if (key == "nonbonded_coulomblj" || key == "nonbonded_coulombwca" || key == "nonbonded_pm" ||
                    key == "nonbonded_coulombhs" || key == "nonbonded_pmwca") { // CombinedPairPotential
             pp.anisotropic_form = false;

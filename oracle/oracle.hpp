@@ -15,6 +15,10 @@
 //    249-317, 665-762, plus a Madelung-constant check and partial==full update.
 //  * Nonbonded pair enumeration: brute force O(N^2) differences, the
 //    `examples/bulk` and `examples/minimal` golden energies.
+//  * nonbonded_splined tables: the Andrea generator above (doctest-pinned), the
+//    reference's search for rmin/rmax and its hard-sphere rule, checked against the
+//    exact pair energy to the spline tolerance over [rmin, rmax] and at the
+//    boundaries (tests/test_oracle_splined.py).
 //  * Ewald real-space S(q) spline inside CoulombGalore (absent third-party
 //    dependency, pinned commit 4055f58538d781acccb2937ab4580855fcba31f8):
 //    PARITY UNPINNED.  Two modes are offered: exact erfc and an Andrea spline of
@@ -193,7 +197,6 @@ struct PairTerm {
     PotKind kind = PotKind::LennardJones;
     int ntypes = 0;
     std::vector<double> sigma2, eps4; // per type pair (row-major ntypes x ntypes)
-    double lj_cutoff2 = infty;        // truncation extension (no reference counterpart)
     double lB = 0;
     Scheme scheme = Scheme::Plain;
     double rc = infty, rc_inv = 0, alpha = 0;
@@ -204,8 +207,17 @@ struct PairTerm {
     double operator()(const Particle& a, const Particle& b, double r2) const;
 };
 
+// SplinedPotential's per type pair data (potentials.h:770-824): an Andrea table of the
+// exact pair energy in r^2 on [rmin^2, rmax^2] and the hard-sphere flag below rmin.
+struct SplinedPair {
+    SplineTable table;
+    double rmin2 = 0, rmax2 = 0;
+    bool hardsphere = false;
+};
+
 // Per type pair list of PairTerms summed in order: FunctorPotential
-// (potentials.cpp:1202-1330) and CombinedPairPotential (potentials_base.h:226-240).
+// (potentials.cpp:1202-1330) and CombinedPairPotential (potentials_base.h:226-240);
+// with `splined`, SplinedPotential::operator() (potentials.h:807-823) over those sums.
 struct PairPotential {
     int ntypes = 0;
     std::vector<PairTerm> terms;
@@ -213,12 +225,27 @@ struct PairPotential {
     bool anisotropic_form = true;           // PairEnergy<.., true> uses vdist (energy.h:428-438)
     bool has_self_energy = false;
     double self_prefactor = 0;
-    double operator()(const Particle& a, const Particle& b, double r2) const {
+    bool splined = false;
+    std::vector<SplinedPair> tables; // ntypes*ntypes (symmetric) when splined
+    double exact(const Particle& a, const Particle& b, double r2) const {
         double u = 0.0;
         for (int t : umatrix[a.id * ntypes + b.id]) u = u + terms[t](a, b, r2);
         return u;
     }
+    double operator()(const Particle& a, const Particle& b, double r2) const {
+        if (!splined) return exact(a, b, r2);
+        const SplinedPair& s = tables[static_cast<size_t>(a.id * ntypes + b.id)];
+        if (r2 >= s.rmax2) return 0.0;
+        if (r2 > s.rmin2) return AndreaSpline::eval(s.table, r2);
+        if (s.hardsphere) return infty;
+        return exact(a, b, r2);
+    }
 };
+
+// SplinedPotential::from_json (potentials.cpp:1513-1550) with findLowerDistance
+// (1452-1477), findUpperDistance (1489-1511) and createKnots (1559-1600): the tables of
+// a parsed functor potential.
+void build_splined_tables(PairPotential& pp, const jsonlite::Value& js, const std::vector<AtomType>& atoms);
 
 // energy.h:92-122 and energy.cpp:28-59.
 struct EwaldData {

@@ -206,6 +206,24 @@ class System:
         _check(self.lib, self.lib.orc_mc_stats(self.h, C.byref(m), C.byref(a), C.byref(s)))
         return m.value, a.value, s.value
 
+    def pair_energy_exact(self, ta, qa, tb, qb, r2):
+        out = C.c_double()
+        _check(self.lib, self.lib.orc_pair_energy_exact(self.h, ta, C.c_double(qa), tb, C.c_double(qb),
+                                                        C.c_double(r2), C.byref(out)))
+        return out.value
+
+    def splined_table(self, ta, tb, cap=4096):
+        """nonbonded_splined table of a type pair: knots, coefficients, rmin2, rmax2, hardsphere."""
+        knots = np.zeros(cap)
+        coeff = np.zeros(6 * cap)
+        nk, hs = C.c_int(), C.c_int()
+        r0, r1 = C.c_double(), C.c_double()
+        _check(self.lib, self.lib.orc_splined_table(self.h, ta, tb, cap, _p(knots), _p(coeff), C.byref(nk),
+                                                    C.byref(r0), C.byref(r1), C.byref(hs)))
+        n = nk.value
+        return dict(knots=knots[:n], coeff=coeff[:6 * (n - 1)], rmin2=r0.value, rmax2=r1.value,
+                    hardsphere=bool(hs.value))
+
     def pair_energy(self, ta, qa, tb, qb, r2):
         out = C.c_double()
         _check(self.lib, self.lib.orc_pair_energy(self.h, ta, C.c_double(qa), tb, C.c_double(qb),

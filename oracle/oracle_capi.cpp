@@ -692,6 +692,52 @@ int orc_pair_energy(void* h, int type_a, double qa, int type_b, double qb, doubl
     });
 }
 
+// Exact (functor) pair energy of the nonbonded term, bypassing a spline table.
+int orc_pair_energy_exact(void* h, int type_a, double qa, int type_b, double qb, double r2, double* out) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        for (auto& t : s->accepted_pot->terms) {
+            if (auto* nb = dynamic_cast<NonbondedTerm*>(t.get())) {
+                Particle a, b;
+                a.id = type_a;
+                a.charge = qa;
+                b.id = type_b;
+                b.charge = qb;
+                *out = nb->pot.exact(a, b, r2);
+                return;
+            }
+        }
+        throw std::runtime_error("no nonbonded term");
+    });
+}
+
+// The nonbonded_splined table of a type pair: knots (ascending r^2), 6 coefficients per
+// interval, rmin^2, rmax^2 and the hard-sphere flag.
+int orc_splined_table(void* h, int type_a, int type_b, int cap, double* knots, double* coeff, int* num_knots,
+                      double* rmin2, double* rmax2, int* hardsphere) {
+    return guard([&] {
+        auto* s = static_cast<System*>(h);
+        for (auto& t : s->accepted_pot->terms) {
+            if (auto* nb = dynamic_cast<NonbondedTerm*>(t.get())) {
+                if (!nb->pot.splined) throw std::runtime_error("the nonbonded term is not splined");
+                const int nt = nb->pot.ntypes;
+                if (type_a < 0 || type_b < 0 || type_a >= nt || type_b >= nt) throw std::runtime_error("bad type");
+                const SplinedPair& p = nb->pot.tables[static_cast<size_t>(type_a * nt + type_b)];
+                const int n = static_cast<int>(p.table.knots.size());
+                *num_knots = n;
+                *rmin2 = p.rmin2;
+                *rmax2 = p.rmax2;
+                *hardsphere = p.hardsphere ? 1 : 0;
+                if (n > cap) throw std::runtime_error("knot buffer too small");
+                std::copy(p.table.knots.begin(), p.table.knots.end(), knots);
+                std::copy(p.table.coeff.begin(), p.table.coeff.end(), coeff);
+                return;
+            }
+        }
+        throw std::runtime_error("no nonbonded term");
+    });
+}
+
 double orc_bjerrum_length(double temperature, double epsr) {
     Units u;
     u.temperature = temperature;

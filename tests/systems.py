@@ -68,22 +68,31 @@ def primitive_model(n_pairs=100, box=118.0, seed=1, alpha=0.2, cutoff=30.0, kmax
     return cfg, xyz, q, ids, dict(begin=[0], size=[n], molid=[0])
 
 
-def nacl_lj(n_pairs=10000, box=255.0, seed=1, alpha=0.25, cutoff=12.0, kmax=20, lj_cutoff=12.0, splined=True,
+def pair_cut(default, rmax):
+    """The nonbonded term of a pair list cut at rmax: nonbonded_splined, whose tables are
+    zero from rmax on (SplinedPotential, the reference's only way to truncate LJ),
+    with u_at_rmax high enough that rmax stays where it is put; rmax None: the exact
+    nonbonded term."""
+    if rmax is None:
+        return {"nonbonded": {"default": default}}
+    return {"nonbonded_splined": {"default": default, "rmax": rmax, "u_at_rmax": 0.01}}
+
+
+def nacl_lj(n_pairs=10000, box=255.0, seed=1, alpha=0.25, cutoff=12.0, kmax=20, rmax=11.99, splined=True,
             dp=2.0):
-    """Na/Cl Lennard-Jones (Lorentz-Berthelot) ions + Ewald, LJ truncated at lj_cutoff."""
-    lj = {"mixing": "LB"}
-    if lj_cutoff is not None:
-        lj["cutoff"] = lj_cutoff
+    """Na/Cl Lennard-Jones (Lorentz-Berthelot) ions + Ewald (real-space cutoff `cutoff`),
+    the pair energy splined and cut at rmax (nonbonded_splined; 0.01 A inside the
+    Coulomb cutoff, where the tabulated energy is continuous)."""
     cfg = {
         "temperature": 298.15,
         "geometry": {"type": "cuboid", "length": box},
         "atomlist": [{"Na": {"q": 1.0, "sigma": 3.3, "eps": 0.5, "dp": dp}},
                      {"Cl": {"q": -1.0, "sigma": 4.4, "eps": 0.4, "dp": dp}}],
         "moleculelist": [{"salt": {"atoms": ["Na", "Cl"], "atomic": True}}],
-        "energy": [{"nonbonded": {"default": [
-            {"lennardjones": lj},
+        "energy": [pair_cut([
+            {"lennardjones": {"mixing": "LB"}},
             {"coulomb": {"type": "ewald", "epsr": 78.7, "cutoff": cutoff, "alpha": alpha, "ncutoff": kmax,
-                         "splined": splined}}]}}],
+                         "splined": splined}}], rmax)],
     }
     rng = np.random.default_rng(seed)
     n = 2 * n_pairs
@@ -156,27 +165,28 @@ def ions(energy, n_pairs=100, box=60.0, seed=1, dp=5.0, temperature=298.15):
     return cfg, xyz, q, ids, dict(begin=[0], size=[n], molid=[0])
 
 
-def water(n_mol=4096, box=None, seed=1, alpha=0.28, cutoff=10.0, kmax=9, lj_cutoff=10.0, dp=0.5, dprot=0.5,
+def water(n_mol=4096, box=None, seed=1, alpha=0.28, cutoff=10.0, kmax=9, rmax="auto", dp=0.5, dprot=0.5,
           splined=True):
     """Rigid 3-site water (O-H 1.0 A, H-O-H 109.47 deg, q_O=-0.8476, q_H=+0.4238, LJ
-    on O sigma=3.166 A eps=0.650 kJ/mol), molecules on a cubic lattice with random
-    orientations; Ewald alpha=0.28, Rc=10, kmax=9; LJ truncated at lj_cutoff.  The
-    move set is molecular translate+rotate with dp and dprot."""
+    on O sigma=3.166 A eps=0.650 kJ/mol; H sigma 2 A as in the reference's water
+    examples, with eps 0), molecules on a cubic lattice with random orientations; Ewald
+    alpha=0.28, Rc=10, kmax=9; the pair energy splined and cut at rmax (default 0.01 A
+    inside Rc; None: the exact nonbonded term).  The move set is molecular
+    translate+rotate with dp and dprot."""
     if box is None:
         box = 49.68 * (n_mol / 4096) ** (1.0 / 3.0)
-    lj = {"mixing": "LB"}
-    if lj_cutoff is not None:
-        lj["cutoff"] = lj_cutoff
+    if rmax == "auto":
+        rmax = cutoff - 0.01
     cfg = {
         "temperature": 298.15,
         "geometry": {"type": "cuboid", "length": box},
         "atomlist": [{"OW": {"q": -0.8476, "sigma": 3.166, "eps": 0.650, "mw": 15.999}},
-                     {"HW": {"q": 0.4238, "sigma": 0.0, "eps": 0.0, "mw": 1.008}}],
+                     {"HW": {"q": 0.4238, "sigma": 2.0, "eps": 0.0, "mw": 1.008}}],
         "moleculelist": [{"water": {"atoms": ["OW", "HW", "HW"], "rigid": True}}],
-        "energy": [{"nonbonded": {"default": [
-            {"lennardjones": lj},
+        "energy": [pair_cut([
+            {"lennardjones": {"mixing": "LB"}},
             {"coulomb": {"type": "ewald", "epsr": 1.0, "cutoff": cutoff, "alpha": alpha, "ncutoff": kmax,
-                         "splined": splined}}]}}],
+                         "splined": splined}}], rmax)],
     }
     rng = np.random.default_rng(seed)
     theta = math.radians(109.47)

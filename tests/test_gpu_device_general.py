@@ -61,7 +61,7 @@ def test_water_general_moves(n_mol, cells, rc):
     """Cell list at Rc = 10 A (cells of L/3 = 10.35 A: one stencil per moved atom) and at
     Rc = 8 A (Rc + the O-H distance fits in a cell: one stencil per state for the whole
     molecule)."""
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=3, kmax=6, lj_cutoff=rc if cells else None,
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=n_mol, seed=3, kmax=6, rmax="auto" if cells else None,
                                                 cutoff=rc)
     if cells:
         box = cfg["geometry"]["length"]
@@ -81,7 +81,7 @@ def test_water_on_the_finer_cell_grid(rc, molecule_stencil):
     at Rc = 9 A, Rc plus the O-H distance fits two cells and one stencil per state serves
     all three atoms; at Rc = 10 A each atom walks its own stencil, one (atom, state,
     cell) combination per lane."""
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=3, kmax=6, lj_cutoff=rc, cutoff=rc)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=3, kmax=6, cutoff=rc)
     box = cfg["geometry"]["length"]
     cell = box / np.floor(2 * box / rc)
     assert (rc + 1.0 <= 2 * cell) == molecule_stencil

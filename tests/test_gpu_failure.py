@@ -157,7 +157,7 @@ def test_checkpoint_resume_headline_split_kernel():
 
 @pytest.mark.parametrize("fp32", [False, True])
 def test_checkpoint_resume_water_replica(fp32):
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=3, kmax=6, lj_cutoff=10.0, cutoff=10.0)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=3, kmax=6, cutoff=10.0)
     checkpoint_roundtrip((cfg, xyz, q, ids, g), kinds, 800, seed=19, fp32=fp32)
 
 

@@ -97,7 +97,7 @@ def test_split_polyelectrolyte_fp32():
 def test_split_water_rigid_moves_continued_across_forms():
     """Rigid water (translate+rotate of molecules): the first half of the run split
     over 8 workgroups in two launches, the second half on one workgroup."""
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=216, seed=3, kmax=6, lj_cutoff=None, cutoff=9.0)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=216, seed=3, kmax=6, rmax=None, cutoff=9.0)
     sysd = (cfg, xyz, q, ids, g)
     orc, e0 = oracle_of(*sysd)
     b = F.Batch(fp32_pairs=False)

@@ -112,7 +112,7 @@ def test_polyelectrolyte_pivot_and_translations(excl):
 
 
 def test_rigid_water_transrot():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=216, seed=3, kmax=6, lj_cutoff=None, cutoff=9.0)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=216, seed=3, kmax=6, rmax=None, cutoff=9.0)
     rep, eng, orc, rec = run_case((cfg, xyz, q, ids, g), kinds, 800, seed=5)
     log("water transrot", rep)
     assert rep["first_mismatch"] == rep["n"], rep
@@ -121,7 +121,7 @@ def test_rigid_water_transrot():
 
 
 def test_water_with_cell_list_and_group_cutoff():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=4, kmax=7, lj_cutoff=10.0, cutoff=10.0)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=4, kmax=7, cutoff=10.0)
     cfg["energy"][0]["nonbonded"]["cutoff_g2g"] = 11.0
     rep, eng, orc, rec = run_case((cfg, xyz, q, ids, g), kinds, 500, seed=6)
     log("water cells + g2g", rep)
@@ -133,7 +133,7 @@ def test_water_with_cell_list_and_group_cutoff():
 
 
 def test_group_cutoff_needs_staged_mass_center():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=64, seed=5, kmax=4, lj_cutoff=None, cutoff=6.0)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=64, seed=5, kmax=4, rmax=None, cutoff=6.0)
     cfg["energy"][0]["nonbonded"]["cutoff_g2g"] = 8.0
     eng = F.Engine(cfg)
     eng.set_particles(xyz, q, ids)

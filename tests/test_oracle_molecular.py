@@ -100,7 +100,7 @@ def test_pivot_and_bead_moves_match_full_recompute():
 
 
 def test_transrot_water_matches_full_recompute():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=64, seed=2, kmax=5, lj_cutoff=None)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=64, seed=2, kmax=5, rmax=None)
     s, worst, n_acc = _per_move_consistency((cfg, xyz, q, ids, g), kinds, 200, seed=4)
     assert n_acc > 20
     assert worst < 1e-10, worst
@@ -111,7 +111,7 @@ def _min_image(d, box):
 
 
 def test_transrot_is_rigid_and_tracks_mass_center():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=27, seed=3, kmax=4, lj_cutoff=None)
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=27, seed=3, kmax=4, rmax=None)
     box = cfg["geometry"]["length"]
     s, e0 = _oracle(cfg, xyz, q, ids, g)
     rec = s.run_philox_moves(400, seed=9, kinds=kinds)
