@@ -55,6 +55,7 @@ SIGNATURES = {
                                               C.c_ulonglong, C.c_int, C.c_int, I, I, C.POINTER(C.c_uint), D, D, I]),
     "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
     "fmc_batch_run_split": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
+    "fmc_batch_run_split_static": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int]),
     "fmc_batch_run_pipe": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "fmc_batch_split_error": (C.c_int, [C.c_void_p, I]),
     "fmc_batch_sync": (C.c_int, [C.c_void_p, D]),
@@ -81,6 +82,7 @@ SIGNATURES = {
     "fmc_coulomb_spline": (C.c_int, [C.c_void_p, C.c_int, D, D, I]),
     "fmc_num_kvectors": (C.c_int, [C.c_void_p, I]),
     "fmc_splined_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, D, D, I, D, D, I]),
+    "fmc_pair_probe": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, D, D, D]),
     "fmc_parse_state": (C.c_int, [C.c_char_p, I, I, D, D, D, I, I, I, I, D]),
     "fmc_load_state": (C.c_int, [C.c_void_p, C.c_char_p]),
     "fmc_server_create": (C.c_int, [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]),
@@ -358,6 +360,14 @@ class Engine:
         n = C.c_int()
         _check(lib().fmc_coulomb_spline(self.h, cap, _p(k), _p(c), C.byref(n)))
         return k[: n.value], c[: 6 * max(n.value - 1, 0)]
+
+    def pair_probe(self, type_a, type_b, qa, qb, r2):
+        """Device pair energies at squared distances r2: (FP64 form, FP32 form)."""
+        r2 = np.ascontiguousarray(r2, np.float64)
+        u64, u32 = np.zeros(len(r2)), np.zeros(len(r2))
+        _check(lib().fmc_pair_probe(self.h, int(type_a), int(type_b), float(qa), float(qb), len(r2), _p(r2), _p(u64),
+                                    _p(u32)))
+        return u64, u32
 
     def splined_table(self, type_a, type_b, cap=256):
         """The engine's nonbonded_splined table of a type pair."""

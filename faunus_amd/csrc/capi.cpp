@@ -501,6 +501,15 @@ int fmc_batch_run_split(fmc_batch* b, long long moves, int groups, int stride, i
     return guarded([&] { bat(b).run_split(moves, groups, stride, trace_capacity > 0, trace_capacity); });
 }
 
+int fmc_pair_probe(fmc_engine* e, int type_a, int type_b, double qa, double qb, int n, const double* r2, double* u64,
+                   double* u32) {
+    return guarded([&] { fmc::pair_probe(eng(e), type_a, type_b, qa, qb, n, r2, u64, u32); });
+}
+
+int fmc_batch_run_split_static(fmc_batch* b, long long moves, int groups, int stride) {
+    return guarded([&] { bat(b).run_split_static(moves, groups, stride); });
+}
+
 int fmc_batch_run_pipe(fmc_batch* b, long long moves, int groups, int depth, int trace_capacity) {
     return guarded([&] { bat(b).run_pipe(moves, groups, depth, trace_capacity > 0, trace_capacity); });
 }

@@ -488,6 +488,57 @@ static std::vector<float4> short_range_table(const DevCoulomb& c) {
     return t;
 }
 
+namespace {
+// One pair energy per thread in both device forms: the FP64 one (pair_energy) and the
+// FP32 one of the device-resident kernels (pair_energy_ft, with its S(q) table).
+__global__ void pair_probe_kernel(const DevPair* __restrict__ pairs, const DevCoulomb* __restrict__ coul,
+                                  const float4* __restrict__ stab, int nt, int terms, int ta, int tb, double qa, double qb,
+                                  int n, const double* __restrict__ r2, double* __restrict__ u64,
+                                  double* __restrict__ u32) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const DevPair& p = pairs[ta * nt + tb];
+    u64[i] = pair_energy(p, *coul, qa, qb, r2[i]);
+    const DevPairF pf = to_f32(p);
+    const CoulF cf = coul_f32(*coul);
+    u32[i] = static_cast<double>(pair_energy_ft(pf, cf, stab, terms, static_cast<float>(qa), static_cast<float>(qb),
+                                                static_cast<float>(r2[i])));
+}
+} // namespace
+
+void pair_probe(Engine& e, int ta, int tb, double qa, double qb, int n, const double* r2, double* u64, double* u32) {
+    const int nt = e.model().ntypes;
+    if (ta < 0 || tb < 0 || ta >= nt || tb >= nt) throw std::runtime_error("pair probe: bad atom type");
+    if (n < 0 || n > (1 << 24)) throw std::runtime_error("pair probe: 0 to 2^24 distances");
+    if (n == 0) return;
+    std::vector<void*> owned;
+    try {
+        const DevCoulomb& dc = e.dev_coulomb();
+        const float4* stab = (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)
+                                 ? upload(short_range_table(dc), owned, "upload S(q) table")
+                                 : nullptr;
+        int terms = 0;
+        for (const DevPair& p : e.dev_pairs()) {
+            if (p.lj_e4 != 0.0) terms |= kTermLJ;
+            if (p.wca_e4 != 0.0) terms |= kTermWCA;
+            if (p.hs_s2 > 0.0) terms |= kTermHS;
+        }
+        const std::vector<double> x(r2, r2 + n);
+        const double* d_r2 = upload(x, owned, "upload r2");
+        double* d_out = upload(std::vector<double>(2 * static_cast<size_t>(n), 0.0), owned, "probe output");
+        hipLaunchKernelGGL(pair_probe_kernel, dim3((n + 255) / 256), dim3(256), 0, e.stream(), e.d_pairs(), e.d_coul(),
+                           stab, nt, terms, ta, tb, qa, qb, n, d_r2, d_out, d_out + n);
+        check_hip(hipGetLastError(), "pair_probe_kernel");
+        check_hip(hipStreamSynchronize(e.stream()), "pair probe");
+        check_hip(hipMemcpy(u64, d_out, sizeof(double) * n, hipMemcpyDeviceToHost), "copy probe");
+        check_hip(hipMemcpy(u32, d_out + n, sizeof(double) * n, hipMemcpyDeviceToHost), "copy probe");
+    } catch (...) {
+        for (void* p : owned) (void)hipFree(p);
+        throw;
+    }
+    for (void* p : owned) (void)hipFree(p);
+}
+
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
                                 std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds) {
     const Model& m = e.model();

@@ -107,6 +107,9 @@ struct TraceEntry {
 // are appended to `owned`, which the caller frees.
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
                                 std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds = nullptr);
+// Pair energies at the squared distances r2[n] of types ta, tb with charges qa, qb in the
+// two device forms: FP64 (u64) and FP32 (u32, the device-resident kernels' form).
+void pair_probe(Engine& e, int ta, int tb, double qa, double qb, int n, const double* r2, double* u64, double* u32);
 // Workgroup size of the translate-only kernel for a replica: 512 threads (two waves per
 // SIMD) without a cell list and with K <= 4096, else 256.
 int advance_threads(const ReplicaState& s);
@@ -150,6 +153,10 @@ class Batch {
     void run(const std::vector<long long>& moves, bool trace, int trace_cap);
     // one replica spread over `groups` workgroups (every `stride`-th block of the grid)
     void run_split(long long moves, int groups, int stride, bool trace, int trace_cap);
+    // run_split of a translate-only replica that allocates nothing: the exchange slots live
+    // in a static per-GPU workspace (at most 64 workgroups); the batch must hold its
+    // device state already
+    void run_split_static(long long moves, int groups, int stride);
     int split_error();
     // one replica over `groups` workgroups with up to `depth` moves in flight
     void run_pipe(long long moves, int groups, int depth, bool trace, int trace_cap);

@@ -11,6 +11,14 @@ namespace fmc {
 
 constexpr int kMaxGk = 1025; // |n|^2 table (ncut <= 32)
 
+namespace {
+// Zero fill of device words by one block (a kernel rather than a memset, so that a
+// captured launch sequence holds no memset node).
+__global__ void zero_words_kernel(unsigned long long* __restrict__ p, int words) {
+    for (int i = static_cast<int>(threadIdx.x); i < words; i += static_cast<int>(blockDim.x)) p[i] = 0ull;
+}
+} // namespace
+
 struct alignas(16) DevPairF {
     float lj_s2, lj_e4, wca_s2, wca_e4, hs_s2, coul;
     int spl, pad;

@@ -1100,6 +1100,21 @@ int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>&
     return static_cast<int>(map.size());
 }
 
+namespace {
+// Zero the exchange slots of every split replica of a launch before it starts: a slot
+// left by an earlier launch carries a tag that a resumed (rewound) chain would take for
+// its own.  Block r clears replica r's 2 x parts slots.
+__global__ void clear_general_slots_kernel(ReplicaState* __restrict__ states, int count) {
+    const int r = static_cast<int>(blockIdx.x);
+    if (r >= count) return;
+    const ReplicaState& s = states[r];
+    if (s.parts <= 1 || s.gslots == nullptr) return;
+    unsigned long long* w = static_cast<unsigned long long*>(s.gslots);
+    const int words = static_cast<int>(sizeof(GeneralSlot) / 8) * 2 * s.parts;
+    for (int i = static_cast<int>(threadIdx.x); i < words; i += static_cast<int>(blockDim.x)) w[i] = 0ull;
+}
+} // namespace
+
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap, int map_blocks) {
     const GeneralLaunch g = plan(h_states, count);
@@ -1118,6 +1133,9 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     const int grid = map_blocks > 0 ? map_blocks : g.grid;
     if (grid < g.grid) throw std::runtime_error("split general launch: block map shorter than the parts");
     if (grid > general_coresident_blocks(f32)) throw std::runtime_error("split general launch: the grid cannot be co-resident");
+    static_assert(sizeof(GeneralSlot) % 8 == 0, "whole words");
+    hipLaunchKernelGGL(clear_general_slots_kernel, dim3(count), dim3(64), 0, stream, d_states, count);
+    check_hip(hipGetLastError(), "clear_general_slots_kernel");
     void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin};
     check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWide), args, static_cast<unsigned>(g.lds), stream),
               "mc_general_kernel cooperative launch");

@@ -17,6 +17,7 @@
 
 #include "device_math.hpp"
 #include "mc_batch.hpp"
+#include "mc_common.hpp"
 #include "philox.hpp"
 
 namespace fmc {
@@ -485,11 +486,14 @@ void Batch::run_pipe(long long moves, int groups, int depth, bool trace, int tra
     if (!pipe_slots_) {
         check_hip(hipMalloc(&pipe_slots_, sizeof(PipeSlot) * R * kPipeMaxG), "hipMalloc slots");
         check_hip(hipMemset(pipe_slots_, 0, sizeof(PipeSlot) * R * kPipeMaxG), "memset slots");
-        if (!split_err_) {
-            check_hip(hipMalloc(&split_err_, sizeof(int)), "hipMalloc err");
-            check_hip(hipMemset(split_err_, 0, sizeof(int)), "memset err");
-        }
+        if (!split_err_) check_hip(hipMalloc(&split_err_, sizeof(int) * 2), "hipMalloc err");
     }
+    // every launch starts from zeroed slots (an earlier launch's tags would match a
+    // resumed, rewound chain) and a clear time-out flag
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(256), 0, stream_, static_cast<unsigned long long*>(pipe_slots_),
+                       static_cast<int>(sizeof(PipeSlot) * R * kPipeMaxG / 8));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream_, reinterpret_cast<unsigned long long*>(split_err_), 1);
+    check_hip(hipGetLastError(), "zero_words_kernel");
     if (trace && trace_cap > trace_cap_) {
         (void)hipFree(d_trace_);
         check_hip(hipMalloc(&d_trace_, sizeof(TraceEntry) * static_cast<size_t>(trace_cap)), "hipMalloc trace");

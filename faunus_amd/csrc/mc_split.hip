@@ -21,10 +21,13 @@
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 #include "device_math.hpp"
 #include "mc_batch.hpp"
+#include "mc_common.hpp"
 #include "philox.hpp"
 
 namespace fmc {
@@ -531,6 +534,66 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     }
 }
 
+// The split kernel for a replica over `groups` workgroups, every `stride`-th block of the
+// grid: 512 threads (two waves per SIMD, 8 lanes per stencil cell) when the slice fits 4
+// k-vectors per thread, else 256 threads with up to 8; the grid must be co-resident.
+struct SplitLaunch {
+    const void* fn;
+    int threads, grid;
+};
+
+SplitLaunch split_launch(const ReplicaState& s, int groups, int stride, const char* who) {
+    const std::string w(who);
+    if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error(w + ": 1..64 workgroups");
+    if (stride < 1) throw std::runtime_error(w + ": stride >= 1");
+    const int kw = (s.nk + groups - 1) / groups;
+    const int threads = kw <= 4 * 512 ? 512 : 256;
+    if ((kw + threads - 1) / threads > (threads == 512 ? 4 : kSplitMaxKPT))
+        throw std::runtime_error(w + ": too many k-vectors per workgroup");
+    if (2 * s.ncut + 1 > 65 || s.ncut * s.ncut >= kGkMax) throw std::runtime_error(w + ": ncutoff too large");
+    int dev = 0, ncu = 0, per_cu = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    const void* fn =
+        threads == 512
+            ? (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, true>)
+                       : reinterpret_cast<const void*>(mc_split_kernel<512, 4, false>))
+            : (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true>)
+                       : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false>));
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy");
+    const int grid = groups * stride;
+    if (per_cu * ncu < grid) throw std::runtime_error(w + ": the grid cannot be co-resident");
+    return SplitLaunch{fn, threads, grid};
+}
+
+// Workspace of the split kernel for callers that cannot allocate
+// (fmc_batch_run_split_static): the exchange slots of up to kStaticSlotsG workgroups for
+// both move parities and the time-out flag, in a file-scope device array of which each
+// GPU holds its own copy for the life of the process.  Host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the array's name.
+constexpr int kStaticSlotsG = 64;
+constexpr size_t kStaticSlotBytes = sizeof(SplitSlot) * 2 * kStaticSlotsG;
+constexpr size_t kScratchBytes = kStaticSlotBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per device.
+void* scratch_address() {
+    static std::mutex mu;
+    static void* ptr[kMaxDevices] = {};
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("run_split_static: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
 } // namespace
 
 // Advance replica 0 of the batch by `moves` moves with G workgroups (every
@@ -543,38 +606,20 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
         run_split_general(moves, groups, stride, trace, trace_cap);
         return;
     }
-    const ReplicaState& s = h_states_[0];
-    if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error("run_split: 1..64 workgroups");
-    if (stride < 1) throw std::runtime_error("run_split: stride >= 1");
-    const int kw = (s.nk + groups - 1) / groups;
-    // 512 threads (two waves per SIMD, 8 lanes per stencil cell) when the slice fits 4
-    // k-vectors per thread, else 256 threads with up to 8
-    const int threads = kw <= 4 * 512 ? 512 : 256;
-    if ((kw + threads - 1) / threads > (threads == 512 ? 4 : kSplitMaxKPT))
-        throw std::runtime_error("run_split: too many k-vectors per workgroup");
-    if (2 * s.ncut + 1 > 65) throw std::runtime_error("run_split: ncutoff too large");
-    if (s.ncut * s.ncut >= kGkMax) throw std::runtime_error("run_split: ncutoff too large");
-    hipDeviceProp_t prop;
-    int dev = 0;
-    check_hip(hipGetDevice(&dev), "hipGetDevice");
-    check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    const int grid = groups * stride;
-    int per_cu = 0;
-    const void* fn =
-        threads == 512
-            ? (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, true>)
-                       : reinterpret_cast<const void*>(mc_split_kernel<512, 4, false>))
-            : (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true>)
-                       : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false>));
-    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy");
-    if (per_cu * prop.multiProcessorCount < grid)
-        throw std::runtime_error("run_split: the grid cannot be co-resident");
+    const SplitLaunch sl = split_launch(h_states_[0], groups, stride, "run_split");
+    const void* fn = sl.fn;
+    const int threads = sl.threads, grid = sl.grid;
     if (!split_slots_) {
         check_hip(hipMalloc(&split_slots_, sizeof(SplitSlot) * 2 * kSplitMaxG), "hipMalloc slots");
-        check_hip(hipMemset(split_slots_, 0, sizeof(SplitSlot) * 2 * kSplitMaxG), "memset slots");
-        check_hip(hipMalloc(&split_err_, sizeof(int)), "hipMalloc err");
-        check_hip(hipMemset(split_err_, 0, sizeof(int)), "memset err");
+        check_hip(hipMalloc(&split_err_, sizeof(int) * 2), "hipMalloc err");
     }
+    // slots left by an earlier launch carry tags a resumed (rewound) chain would take for
+    // its own: every launch starts from zeroed slots and a clear time-out flag
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(256), 0, stream_, static_cast<unsigned long long*>(split_slots_),
+                       static_cast<int>(sizeof(SplitSlot) * 2 * kSplitMaxG / 8));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream_, reinterpret_cast<unsigned long long*>(split_err_),
+                       1);
+    check_hip(hipGetLastError(), "zero_words_kernel");
     if (trace && trace_cap > trace_cap_) {
         (void)hipFree(d_trace_);
         check_hip(hipMalloc(&d_trace_, sizeof(TraceEntry) * static_cast<size_t>(trace_cap)), "hipMalloc trace");
@@ -591,6 +636,41 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
     if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
     check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, stream_), "hipLaunchCooperativeKernel");
+    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
+// run_split without allocating: the exchange slots and the time-out flag live in the
+// file-scope workspace of the current GPU, cleared by a fill kernel on the batch's stream
+// before the launch; no trace.  Calls that overlap on one GPU share the workspace and race.
+void Batch::run_split_static(long long moves, int groups, int stride) {
+    if (!finalized_)
+        throw std::runtime_error("run_split_static: the batch has no device state yet (run it, or read a replica's "
+                                 "info, once first)");
+    if (size() != 1) throw std::runtime_error("run_split_static advances a batch of one replica");
+    if (general_ == 1) throw std::runtime_error("run_split_static: translate-only replicas");
+    if (groups > kStaticSlotsG)
+        throw std::runtime_error("run_split_static: at most " + std::to_string(kStaticSlotsG) +
+                                 " workgroups (the static workspace holds " + std::to_string(kStaticSlotsG) +
+                                 " exchange slots per move parity)");
+    if (moves < 0) throw std::runtime_error("run_split_static: moves >= 0");
+    const SplitLaunch sl = split_launch(h_states_[0], groups, stride, "run_split_static");
+    void* ws = scratch_address();
+    static_assert(kScratchBytes % 8 == 0, "whole words");
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(256), 0, stream_, static_cast<unsigned long long*>(ws),
+                       static_cast<int>(kScratchBytes / 8));
+    check_hip(hipGetLastError(), "zero_words_kernel");
+    ReplicaState* st = d_states_;
+    long long M = moves;
+    int G = groups, str = stride;
+    SplitSlot* slots = static_cast<SplitSlot*>(ws);
+    int* err = reinterpret_cast<int*>(static_cast<unsigned char*>(ws) + kStaticSlotBytes);
+    TraceEntry* tr = nullptr;
+    int cap = 0;
+    long long spin = 1LL << 24;
+    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    check_hip(hipLaunchCooperativeKernel(sl.fn, dim3(sl.grid), dim3(sl.threads), args, 0, stream_),
+              "hipLaunchCooperativeKernel");
     if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
 }
 

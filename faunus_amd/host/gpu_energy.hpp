@@ -104,6 +104,18 @@ class Engine {
     }
     double term(bool trial, int index) const { return trial ? new_terms_[static_cast<size_t>(index)] : old_terms_[static_cast<size_t>(index)]; }
     void invalidate() { cached_generation_ = ~0ul; }
+    // State::sync of the move `generation` (montecarlo.cpp:170-176): accept copies the trial
+    // state into the accepted one, reject restores the trial state.  Every term of both
+    // Hamiltonians calls it; the engine, which holds both states, commits once per move.
+    void commit(bool accept, unsigned long generation) {
+        if (generation == committed_generation_) {
+            if (accept != committed_accept_) throw std::runtime_error("sync error: terms disagree on the decision");
+            return;
+        }
+        check(fmc_sync(get(), accept ? 1 : 0));
+        committed_generation_ = generation;
+        committed_accept_ = accept;
+    }
 
   private:
     struct Deleter {
@@ -113,6 +125,8 @@ class Engine {
     std::vector<std::string> names_;
     std::vector<double> new_terms_, old_terms_;
     unsigned long cached_generation_ = ~0ul;
+    unsigned long committed_generation_ = ~0ul;
+    bool committed_accept_ = false;
 };
 
 // Energy::EnergyTerm shape (src/externalpotential.h:21-43).
@@ -128,7 +142,7 @@ class EnergyTerm {
     virtual void init() {}
 };
 
-// The GPU term `term_name` ("selfenergy", "nonbonded" or "ewald") of one state.
+// The GPU term `term_name` ("selfenergy", "nonbonded", "ewald" or "bonded") of one state.
 class GpuTerm : public EnergyTerm {
   public:
     GpuTerm(std::shared_ptr<Engine> engine, std::string term_name, const unsigned long* generation)
@@ -148,6 +162,15 @@ class GpuTerm : public EnergyTerm {
         engine_->evaluate(change, *generation_);
         return engine_->term(state == MonteCarloState::TRIAL, index_);
     }
+    // EnergyTerm::sync(other, change): the accepted term syncing from the trial one means
+    // the move was accepted, the trial term syncing from the accepted one that it was
+    // rejected (MetropolisMonteCarlo::performMove, src/montecarlo.cpp:170-176).
+    void sync(EnergyTerm* other, const Change&) override {
+        auto* o = dynamic_cast<GpuTerm*>(other);
+        if (!o || o->name != name || o->engine_ != engine_) throw std::runtime_error("sync error");
+        if (state == other->state) throw std::runtime_error("sync error: both terms in one state");
+        engine_->commit(state == MonteCarloState::ACCEPTED, *generation_);
+    }
 
   protected:
     std::shared_ptr<Engine> engine_;
@@ -155,35 +178,81 @@ class GpuTerm : public EnergyTerm {
     const unsigned long* generation_;
 };
 
-using GpuNonbonded = GpuTerm;
-using GpuEwald = GpuTerm;
+// Nonbonded<PairEnergy, GroupPairing> (src/energy.h:1502-1598): the affected-pair energy of
+// one state; updateState has nothing to do (the pair sum reads the Space).
+class GpuNonbonded : public GpuTerm {
+  public:
+    GpuNonbonded(std::shared_ptr<Engine> engine, const unsigned long* generation)
+        : GpuTerm(std::move(engine), "nonbonded", generation) {}
+};
 
-// Hamiltonian::energy with the reference's ordered sum and early break
-// (src/energy.cpp:1227-1241) over the GPU terms of one state.
+// Ewald (src/energy.h:266-285; src/energy.cpp:539-662): updateState applies the partial
+// update of the trial structure factors (energy.cpp:564-575); the engine fuses it with
+// both states' energies in one launch, which this call starts.
+class GpuEwald : public GpuTerm {
+  public:
+    GpuEwald(std::shared_ptr<Engine> engine, const unsigned long* generation)
+        : GpuTerm(std::move(engine), "ewald", generation) {}
+    void updateState(const Change& change) override {
+        if (change && !change.everything && state == MonteCarloState::TRIAL) engine_->evaluate(change, *generation_);
+    }
+};
+
+// ParticleSelfEnergy (src/externalpotential.h:194-198) and Bonded (src/energy.cpp:974-1124).
+class GpuSelfEnergy : public GpuTerm {
+  public:
+    GpuSelfEnergy(std::shared_ptr<Engine> engine, const unsigned long* generation)
+        : GpuTerm(std::move(engine), "selfenergy", generation) {}
+};
+class GpuBonded : public GpuTerm {
+  public:
+    GpuBonded(std::shared_ptr<Engine> engine, const unsigned long* generation)
+        : GpuTerm(std::move(engine), "bonded", generation) {}
+};
+
+// Hamiltonian (src/energy.h:1954-1977) over the GPU terms of one state, in the engine's
+// term order: energy with the ordered sum and early break (src/energy.cpp:1227-1241),
+// updateState (1248-1252) and sync (1254-1268).
 class GpuHamiltonian {
   public:
     GpuHamiltonian(std::shared_ptr<Engine> engine, EnergyTerm::MonteCarloState state, const unsigned long* generation,
                    double maximum_allowed_energy = INFINITY)
         : max_energy_(maximum_allowed_energy) {
         for (const auto& n : engine->term_names()) {
-            terms_.push_back(std::make_unique<GpuTerm>(engine, n, generation));
+            if (n == "nonbonded") terms_.push_back(std::make_unique<GpuNonbonded>(engine, generation));
+            else if (n == "ewald") terms_.push_back(std::make_unique<GpuEwald>(engine, generation));
+            else if (n == "selfenergy") terms_.push_back(std::make_unique<GpuSelfEnergy>(engine, generation));
+            else if (n == "bonded") terms_.push_back(std::make_unique<GpuBonded>(engine, generation));
+            else throw std::runtime_error("unknown engine term " + n);
             terms_.back()->state = state;
         }
     }
     double energy(const Change& change) {
+        latest_.clear();
         double sum = 0.0;
         for (auto& t : terms_) {
             const double e = t->energy(change);
+            latest_.push_back(e);
             sum += e;
             if (e >= max_energy_ || std::isnan(e)) break;
         }
         return sum;
     }
+    void updateState(const Change& change) {
+        for (auto& t : terms_) t->updateState(change);
+    }
+    void sync(GpuHamiltonian& other, const Change& change) {
+        if (other.terms_.size() != terms_.size()) throw std::runtime_error("hamiltonian mismatch");
+        latest_ = other.latest_;
+        for (size_t i = 0; i < terms_.size(); ++i) terms_[i]->sync(other.terms_[i].get(), change);
+    }
     size_t size() const { return terms_.size(); }
     EnergyTerm& operator[](size_t i) { return *terms_[i]; }
+    const std::vector<double>& latest_energies() const { return latest_; }
 
   private:
     std::vector<std::unique_ptr<GpuTerm>> terms_;
+    std::vector<double> latest_;
     double max_energy_;
 };
 
@@ -213,12 +282,15 @@ class Metropolis {
     bool perform(const Change& change, const std::vector<int>& moved, const std::vector<double>& xyz, double uniform,
                  double* du_out = nullptr) {
         ++generation_;
+        // Move::move changed the trial Space
         check(fmc_stage_trial(engine_->get(), static_cast<int>(moved.size()), moved.data(), xyz.data()));
-        const double new_energy = trial_.energy(change); // Ewald::updateState is fused into this call
+        trial_.updateState(change);
+        const double new_energy = trial_.energy(change);
         const double old_energy = accepted_.energy(change);
         const double du = energy_change(new_energy, old_energy);
         const bool accept = metropolis(du, uniform);
-        check(fmc_sync(engine_->get(), accept ? 1 : 0));
+        if (accept) accepted_.sync(trial_, change); // state->sync(*trial_state, change)
+        else trial_.sync(accepted_, change);        // trial_state->sync(*state, change)
         if (accept) sum_of_energy_changes_ += du;
         if (du_out) *du_out = du;
         return accept;

@@ -264,6 +264,19 @@ int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
  */
 int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int stride, int trace_capacity);
 /*
+ * fmc_batch_run_split for a translate-only replica without allocating device memory.
+ * No counterpart in the reference: for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The exchange
+ * slots and the time-out flag live in a static workspace of 4.3 KB that each GPU keeps
+ * for the life of the process (slots for at most 64 workgroups: more is an error); it is
+ * cleared by a fill kernel on the batch's stream before every launch.  The batch must
+ * already hold its device state (after any earlier fmc_batch_run* or
+ * fmc_batch_replica_info call), and no trace is recorded.  Calls that overlap on one GPU
+ * share the workspace and race.  The chain equals fmc_batch_run_split's at the same
+ * workgroup count; errors read through fmc_batch_split_error / fmc_batch_replica_info.
+ */
+int fmc_batch_run_split_static(fmc_batch* batch, long long moves, int groups, int stride);
+/*
  * As fmc_batch_run_split, with up to `depth` (1..4) moves evaluated ahead of their
  * predecessors' decisions: each move's partial sums carry the Ewald cross terms with
  * the pending moves' dQ(k) and the pair terms with each pending particle at its old
@@ -364,6 +377,14 @@ int fmc_num_kvectors(fmc_engine* engine, int* count);
  */
 int fmc_splined_table(fmc_engine* engine, int type_a, int type_b, int capacity, double* knots, double* coeff,
                       int* num_knots, double* rmin2, double* rmax2, int* hardsphere);
+/*
+ * The nonbonded term's pair energy (PairEnergy::potential, src/energy.h:428-438) of
+ * atom types a, b with charges qa, qb at n squared distances, evaluated on the GPU by
+ * the device functions of the FP64 forms (u64) and of the FP32-pair forms (u32).  For
+ * tests of the device pair arithmetic against the oracle.
+ */
+int fmc_pair_probe(fmc_engine* engine, int type_a, int type_b, double qa, double qb, int n, const double* r2,
+                   double* u64, double* u32);
 
 #ifdef __cplusplus
 }

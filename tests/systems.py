@@ -71,11 +71,11 @@ def primitive_model(n_pairs=100, box=118.0, seed=1, alpha=0.2, cutoff=30.0, kmax
 def pair_cut(default, rmax):
     """The nonbonded term of a pair list cut at rmax: nonbonded_splined, whose tables are
     zero from rmax on (SplinedPotential, the reference's only way to truncate LJ),
-    with u_at_rmax high enough that rmax stays where it is put; rmax None: the exact
-    nonbonded term."""
+    with u_at_rmax (0.1 kT) high enough that rmax stays where it is put rather than
+    walking out onto the Coulomb cutoff; rmax None: the exact nonbonded term."""
     if rmax is None:
         return {"nonbonded": {"default": default}}
-    return {"nonbonded_splined": {"default": default, "rmax": rmax, "u_at_rmax": 0.01}}
+    return {"nonbonded_splined": {"default": default, "rmax": rmax, "u_at_rmax": 0.1}}
 
 
 def nacl_lj(n_pairs=10000, box=255.0, seed=1, alpha=0.25, cutoff=12.0, kmax=20, rmax=11.99, splined=True,

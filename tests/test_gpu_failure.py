@@ -140,7 +140,9 @@ def checkpoint_roundtrip(sysd, kinds, m, seed, fp32=False, split_groups=0):
     log("checkpoint round trip", a, r)
     assert r["moves"] == a["moves"] == 2 * m and r["accepted"] == a["accepted"]
     assert np.array_equal(resumed.positions(0), whole.positions(0))
-    assert abs(r["energy"] - a["energy"]) <= 1e-9 * max(1.0, abs(a["energy"]))
+    # the resumed running energy starts from a full recompute; the FP32-pair form's running
+    # sum carries its own rounding
+    assert abs(r["energy"] - a["energy"]) <= (1e-6 if fp32 else 1e-9) * max(1.0, abs(a["energy"]))
     assert r["error"] == 0 and a["error"] == 0
     whole.close()
     resumed.close()

@@ -121,8 +121,13 @@ def test_rigid_water_transrot():
 
 
 def test_water_with_cell_list_and_group_cutoff():
-    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=1000, seed=4, kmax=7, cutoff=10.0)
-    cfg["energy"][0]["nonbonded"]["cutoff_g2g"] = 11.0
+    """nonbonded_splined with cutoff_g2g = 11 A: the tables end at 11 A (the spline takes
+    rmax from cutoff_g2g) inside the 11.5 A Coulomb cutoff, so the per-move path walks a
+    cell list and cuts molecule pairs on their mass centres."""
+    cfg, xyz, q, ids, g, kinds = systems.water(n_mol=2000, seed=4, kmax=7, cutoff=11.5)
+    term = cfg["energy"][0]["nonbonded_splined"]
+    del term["rmax"]
+    term["cutoff_g2g"] = 11.0
     rep, eng, orc, rec = run_case((cfg, xyz, q, ids, g), kinds, 500, seed=6)
     log("water cells + g2g", rep)
     assert rep["first_mismatch"] == rep["n"], rep
