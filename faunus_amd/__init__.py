@@ -56,6 +56,10 @@ SIGNATURES = {
     "fmc_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]),
     "fmc_batch_run_split": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "fmc_batch_run_split_static": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int]),
+    "fmc_batch_capture_interval": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_ulonglong, C.c_ulonglong]),
+    "fmc_batch_replay": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_batch_swap_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "fmc_batch_autotune": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, I, I, D, I]),
     "fmc_batch_run_pipe": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "fmc_batch_split_error": (C.c_int, [C.c_void_p, I]),
     "fmc_batch_sync": (C.c_int, [C.c_void_p, D]),
@@ -489,6 +493,32 @@ class Batch:
 
     def set_timing(self, on):
         _check(lib().fmc_batch_set_timing(self.h, int(bool(on))))
+
+    def capture_interval(self, moves, seed, round0=0):
+        """Capture one replica-exchange interval (moves per replica, then one round of
+        ladder swaps with a device round counter) as a HIP graph."""
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(moves, np.int64), (len(self.sizes),)), np.int64)
+        _check(lib().fmc_batch_capture_interval(self.h, m.ctypes.data_as(C.POINTER(C.c_longlong)), int(seed),
+                                                int(round0)))
+
+    def replay(self, count=1):
+        _check(lib().fmc_batch_replay(self.h, int(count)))
+
+    def swap_counts(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        _check(lib().fmc_batch_swap_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, trial_moves=200):
+        """Time the candidate launch shapes and keep the fastest: returns the chosen value
+        and {candidate: us per replica-move} (empty when there is no choice)."""
+        cap = 8
+        n, chosen = C.c_int(), C.c_int()
+        vals = np.zeros(cap, np.int32)
+        us = np.zeros(cap)
+        _check(lib().fmc_batch_autotune(self.h, int(trial_moves), cap, C.byref(n), _p(vals, C.c_int), _p(us),
+                                        C.byref(chosen)))
+        return chosen.value, {int(vals[i]): float(us[i]) for i in range(min(n.value, cap))}
 
     def swap(self, seed, rnd):
         a, b = C.c_longlong(), C.c_longlong()

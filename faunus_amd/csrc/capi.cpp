@@ -506,6 +506,34 @@ int fmc_pair_probe(fmc_engine* e, int type_a, int type_b, double qa, double qb, 
     return guarded([&] { fmc::pair_probe(eng(e), type_a, type_b, qa, qb, n, r2, u64, u32); });
 }
 
+int fmc_batch_capture_interval(fmc_batch* b, const long long* moves, unsigned long long seed, unsigned long long round0) {
+    return guarded([&] {
+        if (!moves) throw std::runtime_error("capture_interval: moves are null");
+        bat(b).capture_interval(std::vector<long long>(moves, moves + bat(b).size()), seed, round0);
+    });
+}
+
+int fmc_batch_replay(fmc_batch* b, int count) {
+    return guarded([&] { bat(b).replay(count); });
+}
+
+int fmc_batch_swap_counts(fmc_batch* b, long long* attempted, long long* accepted) {
+    return guarded([&] { bat(b).swap_counts(attempted, accepted); });
+}
+
+int fmc_batch_autotune(fmc_batch* b, long long trial_moves, int capacity, int* num_candidates, int* values,
+                       double* us_per_move, int* chosen) {
+    return guarded([&] {
+        const auto r = bat(b).autotune(trial_moves);
+        *num_candidates = static_cast<int>(r.size());
+        for (int i = 0; i < capacity && i < static_cast<int>(r.size()); ++i) {
+            values[i] = r[static_cast<size_t>(i)].value;
+            us_per_move[i] = r[static_cast<size_t>(i)].us_per_move;
+        }
+        *chosen = bat(b).tuned();
+    });
+}
+
 int fmc_batch_run_split_static(fmc_batch* b, long long moves, int groups, int stride) {
     return guarded([&] { bat(b).run_split_static(moves, groups, stride); });
 }

@@ -869,6 +869,11 @@ void Engine::set_cell_divisions(int d) {
     cell_divisions_ = d;
 }
 
+void Engine::regrid(int d) {
+    if (d < 1 || d > 2) throw std::runtime_error("cell divisions: 1 or 2");
+    cell_divisions_ = d; // the next reset_positions builds the grid
+}
+
 void Engine::build_cell_list() {
     cells_.enabled = false;
     const double range = model_.pair_range;

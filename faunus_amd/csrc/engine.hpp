@@ -147,6 +147,8 @@ class Engine {
     // Cells of at least pair range / d (d = 1 or 2) with a (2d+1)^3 stencil; before init().
     // Only the general device kernel walks d = 2; the other paths need d = 1.
     void set_cell_divisions(int d);
+    // Cell divisions of an initialised engine, applied by the next reset_positions.
+    void regrid(int d);
     int cell_radius() const { return cells_.radius; }
     // Members per cell of the cell list built by init(): 0 chooses max(2 x the fullest
     // cell, 4 x the mean + 32); a fixed capacity must hold the fullest initial cell.

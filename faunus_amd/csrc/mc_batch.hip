@@ -325,10 +325,18 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
 // Replica exchange along each ladder (odd/even neighbour pairs by round parity):
 // accept with min(1, exp((beta_a - beta_b)(U_a - U_b))), energies at T_ref; swaps the
 // beta labels (ParallelTempering semantics, move.cpp:844-965, without moving data).
+// With round_counter non-null (one block holding every ladder, as in a captured
+// interval), the round is read from that device counter, which the block then
+// advances, so that each replay of the interval takes the next round.
 __global__ void ladder_swap_kernel(ReplicaState* states, const int* ladder_begin, const int* ladder_size,
                                    int* rung_slot, int nladders, unsigned long long seed, unsigned long long round,
-                                   unsigned long long* counters) {
+                                   unsigned long long* counters, unsigned long long* round_counter) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (round_counter) {
+        round = *round_counter;
+        __syncthreads();
+        if (threadIdx.x == 0) *round_counter = round + 1ull;
+    }
     if (l >= nladders) return;
     const int b = ladder_begin[l], m = ladder_size[l];
     for (int k = static_cast<int>(round & 1); k + 1 < m; k += 2) {
@@ -426,6 +434,7 @@ Batch::Batch(int precision_fp32) : fp32_(precision_fp32) {
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (auto p : graph_owned_) (void)hipFree(p);
     for (auto p : owned_) (void)hipFree(p);
     (void)hipFree(d_states_);
     (void)hipFree(d_moves_);
@@ -626,7 +635,7 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.e_sum = s.e_sum2 = 0.0;
     s.pair_range = e.model().pair_range;
     s.cell_radius = e.cell_radius();
-    s.pad3 = 0;
+    s.threads = 0;
     const DevCoulomb& dc = e.dev_coulomb();
     if (fp32 && (dc.scheme == kCoulEwald || dc.scheme == kCoulFanourgakis)) s.stab = upload(short_range_table(dc), owned, "upload S(q) table");
     s.terms = 0;
@@ -714,7 +723,10 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     return s;
 }
 
-int advance_threads(const ReplicaState& s) { return (!s.cells && s.nk <= 8 * 512) ? 512 : 256; }
+int advance_threads(const ReplicaState& s) {
+    if (s.threads) return s.threads;
+    return (!s.cells && s.nk <= 8 * 512) ? 512 : 256;
+}
 
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream) {
@@ -922,7 +934,7 @@ void Batch::swap_ladders(unsigned long long seed, unsigned long long round) {
     check_hip(hipMemset(d_cnt, 0, sizeof(unsigned long long) * 2), "memset");
     const int nl = static_cast<int>(lb.size());
     hipLaunchKernelGGL(ladder_swap_kernel, dim3((nl + 63) / 64), dim3(64), 0, stream_, d_states_, d_lb, d_ls, d_slots, nl,
-                       seed, round, d_cnt);
+                       seed, round, d_cnt, static_cast<unsigned long long*>(nullptr));
     check_hip(hipGetLastError(), "ladder_swap_kernel");
     unsigned long long cnt[2] = {0, 0};
     check_hip(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, stream_), "copy");
@@ -959,6 +971,11 @@ void Batch::set_replica_state(int r, const double* xyz, const double* mass_cente
     for (ReplicaState* s : {&cur, &h}) {
         if (s->cells != (cl.enabled ? 1 : 0)) throw std::runtime_error("set_replica_state: cell list changed kind");
         if (cl.enabled) {
+            s->nc0 = cl.nc[0];
+            s->nc1 = cl.nc[1];
+            s->nc2 = cl.nc[2];
+            s->inv_cell = cl.inv_cell[0];
+            s->cell_radius = cl.radius;
             s->cap = cl.cap;
             s->cell_count = cl.count;
             s->cell_members = cl.members;
@@ -984,6 +1001,184 @@ void Batch::set_replica_state(int r, const double* xyz, const double* mass_cente
     cur.sum_du = 0.0;
     cur.error = 0;
     check_hip(hipMemcpy(d_states_ + r, &cur, sizeof(ReplicaState), hipMemcpyHostToDevice), "copy state");
+}
+
+// One replica-exchange interval as a HIP graph: the persistent launch that advances
+// replica r by moves[r] moves and one odd/even round of neighbour swaps on every ladder
+// (ParallelTempering, move.cpp:844-965), the round taken from a device counter so that
+// replays alternate the pairings.  Ladders, their rung order and the counters live on
+// the device from here on.
+void Batch::capture_interval(const std::vector<long long>& moves, unsigned long long seed, unsigned long long round0) {
+    if (!finalized_) finalize();
+    if (static_cast<int>(moves.size()) != size()) throw std::runtime_error("capture_interval: one move count per replica");
+    for (const auto& s : h_states_)
+        if (s.parts > 1) throw std::runtime_error("capture_interval: replicas shared by several workgroups are not captured");
+    sync();
+    // ladders: replicas with equal ladder id >= 0, rungs ordered by decreasing beta
+    std::vector<ReplicaState> cur(static_cast<size_t>(size()));
+    check_hip(hipMemcpy(cur.data(), d_states_, sizeof(ReplicaState) * size(), hipMemcpyDeviceToHost), "copy states");
+    std::vector<int> ids;
+    for (const auto& s : cur) ids.push_back(s.ladder);
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    std::vector<int> lb, ls, slots;
+    for (int lid : ids) {
+        if (lid < 0) continue;
+        std::vector<int> members;
+        for (int r = 0; r < size(); ++r)
+            if (cur[static_cast<size_t>(r)].ladder == lid) members.push_back(r);
+        std::sort(members.begin(), members.end(),
+                  [&](int a, int b) { return cur[static_cast<size_t>(a)].beta > cur[static_cast<size_t>(b)].beta; });
+        lb.push_back(static_cast<int>(slots.size()));
+        ls.push_back(static_cast<int>(members.size()));
+        slots.insert(slots.end(), members.begin(), members.end());
+    }
+    const int nl = static_cast<int>(lb.size());
+    if (nl > 1024) throw std::runtime_error("capture_interval: at most 1024 ladders");
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    for (void* p : graph_owned_) (void)hipFree(p);
+    graph_owned_.clear();
+    auto dev_copy = [&](const std::vector<int>& v) {
+        int* d = nullptr;
+        check_hip(hipMalloc(&d, sizeof(int) * std::max<size_t>(1, v.size())), "hipMalloc ladders");
+        graph_owned_.push_back(d);
+        if (!v.empty()) check_hip(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice), "copy ladders");
+        return d;
+    };
+    int* d_lb = dev_copy(lb);
+    int* d_ls = dev_copy(ls);
+    int* d_rung = dev_copy(slots);
+    unsigned long long* d_cnt = nullptr;
+    check_hip(hipMalloc(&d_cnt, sizeof(unsigned long long) * 3), "hipMalloc swap counters");
+    graph_owned_.push_back(d_cnt);
+    const unsigned long long init[3] = {0ull, 0ull, round0};
+    check_hip(hipMemcpy(d_cnt, init, sizeof(init), hipMemcpyHostToDevice), "copy swap counters");
+    graph_counters_ = d_cnt;
+    check_hip(hipMemcpy(d_moves_, moves.data(), sizeof(long long) * size(), hipMemcpyHostToDevice), "copy moves");
+    // one launch outside the capture sets the kernels' attributes (not a stream operation)
+    const std::vector<long long> zero(static_cast<size_t>(size()), 0);
+    check_hip(hipMemcpy(d_moves_, zero.data(), sizeof(long long) * size(), hipMemcpyHostToDevice), "copy moves");
+    if (general_ == 1) launch_general(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, nullptr, 0, stream_);
+    else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, nullptr, 0, stream_);
+    check_hip(hipStreamSynchronize(stream_), "attribute run");
+    check_hip(hipMemcpy(d_moves_, moves.data(), sizeof(long long) * size(), hipMemcpyHostToDevice), "copy moves");
+    hipGraph_t g = nullptr;
+    check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        if (general_ == 1) launch_general(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, nullptr, 0, stream_);
+        else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, nullptr, 0, stream_);
+        if (nl > 0) {
+            hipLaunchKernelGGL(ladder_swap_kernel, dim3(1), dim3(((nl + 63) / 64) * 64), 0, stream_, d_states_, d_lb, d_ls,
+                               d_rung, nl, seed, 0ull, d_cnt, d_cnt + 2);
+            check_hip(hipGetLastError(), "ladder_swap_kernel");
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check_hip(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+    const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check_hip(ie, "hipGraphInstantiate");
+}
+
+void Batch::replay(int count) {
+    if (!graph_) throw std::runtime_error("replay: no interval captured");
+    if (count < 0) throw std::runtime_error("replay: count >= 0");
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    for (int i = 0; i < count; ++i) check_hip(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
+    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+}
+
+// Tile autotuner: times each candidate launch shape of this batch on a short run and
+// keeps the fastest for the whole batch (one shape per batch keeps every replica in one
+// persistent launch).  Translate-only replicas: workgroups of 256 or 512 threads (K/256
+// or K/512 k-vectors per thread; 512 holds at most 8 per thread, so K <= 4096).
+// Move-set replicas with a cell list: cells of at least the pair range (27-cell stencils)
+// or half of it (125 smaller cells).  Every candidate runs `trial` moves per replica from
+// checkpoints that are restored afterwards, so the chains continue from their states
+// before tuning (Q(k) and the running energy rebuilt from the positions).
+std::vector<TuneCandidate> Batch::autotune(long long trial) {
+    if (trial < 1) throw std::runtime_error("autotune: trial moves >= 1");
+    if (!finalized_) finalize();
+    for (const auto& s : h_states_)
+        if (s.parts > 1) throw std::runtime_error("autotune: replicas shared by several workgroups");
+    sync();
+    struct Checkpoint {
+        std::vector<double> xyz, cm;
+        unsigned long long moves = 0, accepted = 0;
+    };
+    std::vector<Checkpoint> ck(static_cast<size_t>(size()));
+    for (int r = 0; r < size(); ++r) {
+        Checkpoint& c = ck[static_cast<size_t>(r)];
+        c.xyz.resize(3 * static_cast<size_t>(engines_[static_cast<size_t>(r)]->num_particles()));
+        c.cm.resize(3 * engines_[static_cast<size_t>(r)]->group_begin().size());
+        get_replica_state(r, c.xyz.data(), c.cm.data(), &c.moves, &c.accepted);
+    }
+    std::vector<int> cands;
+    if (general_ == 1) {
+        bool cells = false;
+        for (const auto& s : h_states_) cells = cells || s.cells;
+        if (!cells) return {};
+        cands = {1, 2};
+    } else {
+        cands = {256};
+        bool fits512 = true;
+        for (const auto& s : h_states_) fits512 = fits512 && (s.nk + 511) / 512 <= 8;
+        if (fits512) cands.push_back(512);
+    }
+    auto apply = [&](int c) {
+        for (int r = 0; r < size(); ++r) {
+            if (general_ == 1) engines_[static_cast<size_t>(r)]->regrid(c);
+            else h_states_[static_cast<size_t>(r)].threads = c;
+        }
+    };
+    auto restore = [&]() {
+        for (int r = 0; r < size(); ++r) {
+            const Checkpoint& c = ck[static_cast<size_t>(r)];
+            set_replica_state(r, c.xyz.data(), c.cm.data(), c.moves, c.accepted);
+        }
+    };
+    const bool timing = timing_;
+    timing_ = true;
+    std::vector<TuneCandidate> out;
+    try {
+        for (int c : cands) {
+            apply(c);
+            if (general_ == 1) restore(); // the new grid
+            run(std::vector<long long>(static_cast<size_t>(size()), trial), false, 0);
+            sync();
+            out.push_back(TuneCandidate{c, last_ms_ * 1e3 / (static_cast<double>(trial) * size())});
+            restore();
+        }
+    } catch (...) {
+        timing_ = timing;
+        throw;
+    }
+    timing_ = timing;
+    const auto best = std::min_element(out.begin(), out.end(),
+                                       [](const TuneCandidate& a, const TuneCandidate& b) { return a.us_per_move < b.us_per_move; });
+    apply(best->value);
+    restore();
+    tuned_ = best->value;
+    return out;
+}
+
+void Batch::swap_counts(long long* attempted, long long* accepted) {
+    sync();
+    long long a = swaps_att_, b = swaps_acc_;
+    if (graph_counters_) {
+        unsigned long long c[2] = {0, 0};
+        check_hip(hipMemcpy(c, graph_counters_, sizeof(c), hipMemcpyDeviceToHost), "copy swap counters");
+        a += static_cast<long long>(c[0]);
+        b += static_cast<long long>(c[1]);
+    }
+    *attempted = a;
+    *accepted = b;
 }
 
 ReplicaState Batch::state(int r) {

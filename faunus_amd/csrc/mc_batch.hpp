@@ -92,7 +92,7 @@ struct ReplicaState {
     double e_sum, e_sum2;
     double pair_range;        // distance beyond which every pair term vanishes (cell-list forms)
     int cell_radius;          // stencil half-width in cells (1: 27 cells, 2: 125 cells)
-    int pad3;
+    int threads;              // workgroup size of the translate-only kernel chosen by the tuner (0: the default rule)
 };
 
 // One recorded move of the trace (tests).
@@ -135,6 +135,13 @@ int general_blockmap(const ReplicaState* h_states, int count, std::vector<int2>&
 // Workgroups of the split general kernel that fit on the device at once.
 int general_coresident_blocks(bool f32);
 
+// One candidate launch shape of the tile autotuner and its measured kernel time per
+// replica-move (microseconds).
+struct TuneCandidate {
+    int value; // workgroup threads (translate-only replicas) or cell divisions (move sets)
+    double us_per_move;
+};
+
 // A set of replicas advanced together on one stream.
 class Batch {
   public:
@@ -161,6 +168,17 @@ class Batch {
     // one replica over `groups` workgroups with up to `depth` moves in flight
     void run_pipe(long long moves, int groups, int depth, bool trace, int trace_cap);
     void swap_ladders(unsigned long long seed, unsigned long long round);
+    // One replica-exchange interval captured as a HIP graph: the persistent launch
+    // advancing replica r by moves[r] and one round of ladder swaps (device round counter
+    // starting at round0); replay(count) launches it count times on the batch's stream.
+    void capture_interval(const std::vector<long long>& moves, unsigned long long seed, unsigned long long round0);
+    void replay(int count);
+    // swaps attempted and accepted so far (host rounds and captured rounds)
+    void swap_counts(long long* attempted, long long* accepted);
+    // time the batch's candidate launch shapes on `trial` moves per replica (restored
+    // afterwards) and keep the fastest; empty when the batch has no choice to make
+    std::vector<TuneCandidate> autotune(long long trial);
+    int tuned() const { return tuned_; }
     // energy samples every `every` moves (0: off) for all replicas, counters cleared
     void sample_energy(long long every);
     // pair-distance histogram of types a and b (a == b: pairs i < j) of replica r's current
@@ -188,9 +206,12 @@ class Batch {
     void set_threads(int threads) { threads_override_ = threads; }
     long long swaps_attempted() const { return swaps_att_; }
     long long swaps_accepted() const { return swaps_acc_; }
-    hipGraphExec_t graph_ = nullptr;
 
   private:
+    hipGraphExec_t graph_ = nullptr;
+    std::vector<void*> graph_owned_; // ladders, rung order and counters of the captured interval
+    unsigned long long* graph_counters_ = nullptr; // attempted, accepted, next round
+    int tuned_ = 0;
     int fp32_ = 0;
     int general_ = -1; // -1 unset, 0 translate-only replicas, 1 general move sets
     int cell_divisions_ = 1;

@@ -291,6 +291,31 @@ int fmc_batch_set_timing(fmc_batch* batch, int enable);
 /* One odd/even round of neighbour exchanges on every ladder. */
 int fmc_batch_swap(fmc_batch* batch, unsigned long long seed, unsigned long long round, long long* attempted,
                    long long* accepted);
+/*
+ * One replica-exchange interval as a HIP graph (ParallelTempering every interval,
+ * src/move.cpp:844-965): the persistent launch advancing replica r by moves[r] moves,
+ * then one odd/even round of neighbour swaps on every ladder, the round taken from a
+ * device counter that starts at round0, so that replays alternate the pairings.  The
+ * ladders' rung order and the swap counters stay on the device.  fmc_batch_replay
+ * launches the captured interval `count` times (asynchronously; fmc_batch_sync waits);
+ * fmc_batch_swap_counts gives the swaps attempted and accepted so far.  Replicas shared
+ * by several workgroups are not captured.  No reference counterpart as one call.
+ */
+int fmc_batch_capture_interval(fmc_batch* batch, const long long* moves, unsigned long long seed,
+                               unsigned long long round0);
+int fmc_batch_replay(fmc_batch* batch, int count);
+int fmc_batch_swap_counts(fmc_batch* batch, long long* attempted, long long* accepted);
+/*
+ * Tile autotuner: times each candidate launch shape of the batch on trial_moves moves
+ * per replica (from checkpoints restored afterwards) and keeps the fastest for the whole
+ * batch.  Candidates: 256 or 512 threads per workgroup for translate-only replicas
+ * (512 when every replica has K <= 4096); cells of at least the pair range or half of it
+ * (cell divisions 1 or 2) for move-set replicas with a cell list.  Writes up to
+ * `capacity` candidates with their kernel time per replica-move (microseconds) and the
+ * chosen value; num_candidates 0 means the batch has no choice.  No reference counterpart.
+ */
+int fmc_batch_autotune(fmc_batch* batch, long long trial_moves, int capacity, int* num_candidates, int* values,
+                       double* us_per_move, int* chosen);
 int fmc_batch_replica_info(fmc_batch* batch, int replica, double* energy, double* beta, unsigned long long* moves,
                            unsigned long long* accepted, int* error);
 int fmc_batch_positions(fmc_batch* batch, int replica, double* xyz);

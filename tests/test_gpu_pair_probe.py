@@ -1,8 +1,10 @@
 """The device pair functions against the oracle's pair energy, pair by pair: the FP64
-form (pair_energy) to 1e-12 relative and the FP32-pair form (pair_energy_ft: FP32
-arithmetic, FP32 S(q) table, splined tables evaluated from the FP32 distance) to 5e-6
-relative (floor 1e-6 kT), for every type pair of each potential form the parser takes,
-on a grid of distances from well inside contact to beyond the pair range."""
+form (pair_energy) within 1e-11 * max(|u|, 1e-3 kT) and the FP32-pair form
+(pair_energy_ft: FP32 arithmetic, FP32 S(q) table, splined tables evaluated from the
+FP32 distance) within 2e-6 * max(|u|, 1 kT) -- a few units of FP32 rounding of the
+larger of the terms that cancel near the LJ minimum -- for every type pair of each
+potential form the parser takes, on a grid of distances from well inside contact to
+beyond the pair range."""
 import json
 
 import numpy as np
@@ -44,8 +46,8 @@ def test_pair_functions_match_the_oracle(name):
             inf = np.isinf(uo)
             assert np.array_equal(inf, np.isinf(u64)) and np.array_equal(inf, np.isinf(u32)), (name, a, b)
             fin = ~inf
-            e64 = np.abs(u64[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-6)
-            e32 = np.abs(u32[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-6 / 5e-6)
+            e64 = np.abs(u64[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-3)
+            e32 = np.abs(u32[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1.0)
             k = int(np.argmax(e32)) if len(e32) else 0
             log(name, a, b, "fp64 max rel %.2e" % (e64.max() if len(e64) else 0),
                 "fp32 max rel %.2e at r=%.4f (u %.6g vs %.6g)" % (e32.max() if len(e32) else 0, r[fin][k] if len(e32) else 0,
@@ -53,5 +55,5 @@ def test_pair_functions_match_the_oracle(name):
             worst64 = max(worst64, float(e64.max()) if len(e64) else 0.0)
             worst32 = max(worst32, float(e32.max()) if len(e32) else 0.0)
     eng.close()
-    assert worst64 <= 1e-12, (name, worst64)
-    assert worst32 <= 5e-6, (name, worst32)
+    assert worst64 <= 1e-11, (name, worst64)
+    assert worst32 <= 2e-6, (name, worst32)
