@@ -68,8 +68,10 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < nk) ? S.q[k0 + s] : make_double2(0.0, 0.0);
 
     double4* __restrict__ pos = S.pos;
-    float4* __restrict__ pos32 = S.pos32;
+    FixPos* __restrict__ pos32 = S.pos32;
     const uint32_t* __restrict__ meta = S.meta;
+    const double h64 = L * kFixInv;
+    const float hf = static_cast<float>(h64);
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
@@ -118,13 +120,15 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
 
         // nonbonded pair sums of the moved particle in the new and old position
         double e_new = 0.0, e_old = 0.0;
+        FixPos fa{}, fb{};
+        if (kF32) {
+            fa = fixed_pos(nx_, ny_, nz_, po.w, inv);
+            fb = fixed_pos(po.x, po.y, po.z, po.w, inv);
+        }
         if (!kCells) {
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, 0, n, i, -1, true, s_pairs_f + ti * nt, cf, s_stab,
-                                       S.terms, static_cast<float>(L), static_cast<float>(inv), static_cast<float>(nx_),
-                                       static_cast<float>(ny_), static_cast<float>(nz_), static_cast<float>(po.x),
-                                       static_cast<float>(po.y), static_cast<float>(po.z), static_cast<float>(po.w),
-                                       e_new, e_old);
+                pair_row_f32<kThreads>(pos32, meta, 0, n, i, -1, true, s_pairs_f + ti * nt, s_pairs + ti * nt, cf, s_coul,
+                                       s_stab, S.terms, hf, h64, fa, fb, po.w, e_new, e_old);
             } else {
                 const Box b{{L, L, L}, {half, half, half}};
                 pair_row_f64<kThreads>(pos, meta, 0, n, i, -1, true, s_pairs + ti * nt, s_coul, S.terms, b, nx_, ny_, nz_,
@@ -149,12 +153,13 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                     const int tj = static_cast<int>(meta[j] & 0xffu);
                     double u;
                     if (kF32) {
-                        const float4 pj = pos32[j];
-                        const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                        u = static_cast<double>(pair_energy_ft(s_pairs_f[ti * nt + tj], cf, s_stab, S.terms,
-                                                              static_cast<float>(po.w), pj.w,
-                                                              dist2f_round(fL, fIL, static_cast<float>(px), static_cast<float>(py),
-                                                                     static_cast<float>(pz), pj.x, pj.y, pj.z)));
+                        const FixPos pj = pos32[j];
+                        const FixPos& A = st == 0 ? fa : fb;
+                        const float uf = pair_energy_ft(s_pairs_f[ti * nt + tj], cf, s_stab, S.terms, A.q, pj.q,
+                                                        dist2_fix(A, pj, hf));
+                        u = hot_pair(uf) ? pair_energy_t(s_pairs[ti * nt + tj], s_coul, po.w, static_cast<double>(pj.q),
+                                                         dist2_fix64(A, pj, h64), S.terms)
+                                         : static_cast<double>(uf);
                     } else {
                         const double4 pj = pos[j];
                         Box b{{L, L, L}, {half, half, half}};
@@ -237,8 +242,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                 sum_du += du;
                 ++accepted;
                 pos[i] = make_double4(nx_, ny_, nz_, po.w);
-                if (kF32) pos32[i] = make_float4(static_cast<float>(nx_), static_cast<float>(ny_), static_cast<float>(nz_),
-                                                 static_cast<float>(po.w));
+                if (kF32) pos32[i] = fa;
                 if (kCells) {
                     const int cxn = cell_of_coord(nx_, half, S.inv_cell, S.nc0);
                     const int cyn = cell_of_coord(ny_, half, S.inv_cell, S.nc1);
@@ -499,7 +503,8 @@ static std::vector<float4> short_range_table(const DevCoulomb& c) {
 
 namespace {
 // One pair energy per thread in both device forms: the FP64 one (pair_energy) and the
-// FP32 one of the device-resident kernels (pair_energy_ft, with its S(q) table).
+// FP32-pair form of the device-resident kernels (pair_energy_ft with its S(q) table, and
+// the FP64 re-evaluation of hot pairs), at the same distance.
 __global__ void pair_probe_kernel(const DevPair* __restrict__ pairs, const DevCoulomb* __restrict__ coul,
                                   const float4* __restrict__ stab, int nt, int terms, int ta, int tb, double qa, double qb,
                                   int n, const double* __restrict__ r2, double* __restrict__ u64,
@@ -510,8 +515,9 @@ __global__ void pair_probe_kernel(const DevPair* __restrict__ pairs, const DevCo
     u64[i] = pair_energy(p, *coul, qa, qb, r2[i]);
     const DevPairF pf = to_f32(p);
     const CoulF cf = coul_f32(*coul);
-    u32[i] = static_cast<double>(pair_energy_ft(pf, cf, stab, terms, static_cast<float>(qa), static_cast<float>(qb),
-                                                static_cast<float>(r2[i])));
+    const float uf = pair_energy_ft(pf, cf, stab, terms, static_cast<float>(qa), static_cast<float>(qb),
+                                    static_cast<float>(r2[i]));
+    u32[i] = hot_pair(uf) ? pair_energy_t(p, *coul, qa, qb, r2[i], terms) : static_cast<double>(uf);
 }
 } // namespace
 
@@ -614,18 +620,14 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     owned.push_back(d_dp);
     check_hip(hipMemcpy(d_dp, dpt.data(), sizeof(double) * m.ntypes, hipMemcpyHostToDevice), "copy dp");
     s.dp_type = d_dp;
-    float4* d_p32 = nullptr;
+    FixPos* d_p32 = nullptr;
     if (fp32) {
         std::vector<double> xyz(3 * static_cast<size_t>(n));
         e.get_positions(xyz.data());
-        const auto& q = e.host_charges();
-        std::vector<float4> p32(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i)
-            p32[static_cast<size_t>(i)] = make_float4(static_cast<float>(xyz[3 * i]), static_cast<float>(xyz[3 * i + 1]),
-                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(q[static_cast<size_t>(i)]));
-        check_hip(hipMalloc(&d_p32, sizeof(float4) * n), "hipMalloc pos32");
+        const std::vector<FixPos> p32 = fixed_positions(xyz.data(), e.host_charges(), n, s.inv);
+        check_hip(hipMalloc(&d_p32, sizeof(FixPos) * n), "hipMalloc pos32");
         owned.push_back(d_p32);
-        check_hip(hipMemcpy(d_p32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
+        check_hip(hipMemcpy(d_p32, p32.data(), sizeof(FixPos) * n, hipMemcpyHostToDevice), "copy pos32");
     }
     s.pos32 = d_p32;
     s.parts = 1;
@@ -834,7 +836,7 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
     if (threads_for(0) != 256) throw std::runtime_error("unsupported workgroup size");
     TraceEntry* tr = trace ? d_trace_ : nullptr;
     const int cap = trace ? trace_cap : 0;
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     if (general_ == 1) {
         // one workgroup per replica, also for replicas that earlier ran split
         std::vector<ReplicaState> one(h_states_);
@@ -842,7 +844,7 @@ void Batch::run(const std::vector<long long>& moves, bool trace, int trace_cap) 
         launch_general(one.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
     }
     else launch_advance(h_states_.data(), size(), d_states_, d_moves_, fp32_ != 0, tr, cap, stream_);
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 void Batch::sample_energy(long long every) {
@@ -890,10 +892,22 @@ void Batch::rdf(int r, int type_a, int type_b, double dr, int nbins, unsigned lo
     for (int b = 0; b < nbins; ++b) counts[b] += part[static_cast<size_t>(b)];
 }
 
+void Batch::time_begin() {
+    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+}
+
+void Batch::time_end() {
+    if (!timing_) return;
+    check_hip(hipEventRecord(ev1_, stream_), "event");
+    ev_pending_ = true;
+}
+
+// Waits for the batch's stream; the duration of the last timed launch is read once.
 void Batch::sync() {
     check_hip(hipStreamSynchronize(stream_), "batch sync");
-    if (timing_) {
+    if (ev_pending_) {
         float ms = 0;
+        ev_pending_ = false;
         check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
         last_ms_ = ms;
     }
@@ -983,13 +997,11 @@ void Batch::set_replica_state(int r, const double* xyz, const double* mass_cente
         }
     }
     if (cur.pos32) {
-        const auto& q = e.host_charges();
         const int n = e.num_particles();
-        std::vector<float4> p32(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i)
-            p32[static_cast<size_t>(i)] = make_float4(static_cast<float>(xyz[3 * i]), static_cast<float>(xyz[3 * i + 1]),
-                                                      static_cast<float>(xyz[3 * i + 2]), static_cast<float>(q[static_cast<size_t>(i)]));
-        check_hip(hipMemcpy(cur.pos32, p32.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "copy pos32");
+        std::vector<double> now(3 * static_cast<size_t>(n));
+        e.get_positions(now.data());
+        const std::vector<FixPos> p32 = fixed_positions(now.data(), e.host_charges(), n, cur.inv);
+        check_hip(hipMemcpy(cur.pos32, p32.data(), sizeof(FixPos) * n, hipMemcpyHostToDevice), "copy pos32");
     }
     std::vector<double> terms(static_cast<size_t>(e.num_terms()));
     e.total_energy(terms.data());
@@ -1089,9 +1101,9 @@ void Batch::capture_interval(const std::vector<long long>& moves, unsigned long 
 void Batch::replay(int count) {
     if (!graph_) throw std::runtime_error("replay: no interval captured");
     if (count < 0) throw std::runtime_error("replay: count >= 0");
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     for (int i = 0; i < count; ++i) check_hip(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 // Tile autotuner: times each candidate launch shape of this batch on a short run and

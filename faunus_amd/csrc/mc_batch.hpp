@@ -37,6 +37,8 @@ struct MoveKindSpec {
     double dp = 0.0, dprot = 0.0;
 };
 
+struct FixPos; // mc_common.hpp
+
 // Per-replica device state (one per workgroup).
 struct ReplicaState {
     int n, nk, ncut, ntypes;
@@ -46,7 +48,7 @@ struct ReplicaState {
     double inv_cell;
     double rec_prefactor; // 2 pi lB / V
     double4* pos;
-    float4* pos32;
+    FixPos* pos32;        // FP32-pair form: fixed-point coordinates and FP32 charge, or null
     const uint32_t* meta;
     double2* q;
     const int* kn;
@@ -230,6 +232,9 @@ class Batch {
     int threads_override_ = 0;
     long long swaps_att_ = 0, swaps_acc_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    bool ev_pending_ = false; // both events recorded around a launch not yet read by sync()
+    void time_begin();
+    void time_end();
     void* split_slots_ = nullptr;
     void* pipe_slots_ = nullptr;
     int* split_err_ = nullptr;

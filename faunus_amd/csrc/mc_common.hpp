@@ -4,6 +4,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <vector>
+
 #include "device_math.hpp"
 #include "engine.hpp"
 
@@ -82,15 +85,63 @@ __device__ __forceinline__ CoulF coul_f32(const DevCoulomb& c) {
                  c.nspl > 0 ? c.spl : nullptr};
 }
 
-// Minimum image by rounding (|dx| <= L for particles inside the box).
-__device__ __forceinline__ float dist2f_round(float L, float invL, float ax, float ay, float az, float bx, float by,
-                                              float bz) {
-    float dx = ax - bx, dy = ay - by, dz = az - bz;
-    dx -= L * rintf(dx * invL);
-    dy -= L * rintf(dy * invL);
-    dz -= L * rintf(dz * invL);
+// Coordinates of the FP32-pair form: each axis in 32-bit fixed point, the cubic box
+// [-L/2, L/2) mapped onto [0, 2^32) in steps of h = L / 2^32, and the charge as FP32
+// bits.  The difference of two fixed-point words taken modulo 2^32 and read as a signed
+// integer is the minimum-image displacement in units of h, with the same resolution
+// everywhere in the box (h = 6e-8 A at L = 255 A, where an FP32 coordinate near the box
+// edge resolves 8e-6 A and LJ's r^-12 would turn that into 1e-5 relative errors of
+// contact pairs).  Integer displacements also convert exactly to FP64 for the pairs
+// re-evaluated there.
+struct alignas(16) FixPos {
+    uint32_t x, y, z;
+    float q;
+};
+
+constexpr double kFix = 4294967296.0;        // 2^32
+constexpr double kFixInv = 2.3283064365386963e-10; // 2^-32
+
+// round(x 2^32 / L) + 2^31 modulo 2^32 for x in [-L/2, L/2]; `s` = 2^32 / L.  One
+// multiply and a round to nearest even: the same word on the host and the device.
+__host__ __device__ inline uint32_t fixed_coord(double x, double s) {
+    return static_cast<uint32_t>(static_cast<long long>(rint(x * s)) + 2147483648ll);
+}
+
+__host__ __device__ inline FixPos fixed_pos(double x, double y, double z, double q, double inv) {
+    const double s = kFix * inv;
+    return FixPos{fixed_coord(x, s), fixed_coord(y, s), fixed_coord(z, s), static_cast<float>(q)};
+}
+
+// The FP32-pair form's coordinates of n particles (xyz: 3 per particle, inside the box).
+inline std::vector<FixPos> fixed_positions(const double* xyz, const std::vector<double>& q, int n, double inv) {
+    std::vector<FixPos> out(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        out[static_cast<size_t>(i)] = fixed_pos(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q[static_cast<size_t>(i)], inv);
+    return out;
+}
+
+// Squared minimum-image distance in FP32 (h = L / 2^32 as FP32).
+__device__ __forceinline__ float dist2_fix(const FixPos& a, const FixPos& b, float h) {
+    const float dx = static_cast<float>(static_cast<int>(a.x - b.x)) * h;
+    const float dy = static_cast<float>(static_cast<int>(a.y - b.y)) * h;
+    const float dz = static_cast<float>(static_cast<int>(a.z - b.z)) * h;
     return dx * dx + dy * dy + dz * dz;
 }
+
+// The same distance in FP64 (exact displacement; h = L / 2^32).
+__device__ __forceinline__ double dist2_fix64(const FixPos& a, const FixPos& b, double h) {
+    const double dx = static_cast<double>(static_cast<int>(a.x - b.x)) * h;
+    const double dy = static_cast<double>(static_cast<int>(a.y - b.y)) * h;
+    const double dz = static_cast<double>(static_cast<int>(a.z - b.z)) * h;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Pairs whose FP32 energy exceeds this (kT; overlaps, +inf and NaN included) are
+// evaluated again in FP64: FP32 rounding of r^2 grows sixfold through r^-12, and one
+// overlapping pair would otherwise set the error of the move's dU.
+constexpr float kHotPair = 8.0f;
+
+__device__ __forceinline__ bool hot_pair(float u) { return !(__builtin_fabsf(u) <= kHotPair); }
 
 __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
                                                 int terms, float qa, float qb, float r2) {
@@ -144,38 +195,53 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
 
 // Pair sums of one moved particle, at its new (a) and old (b) position, against every
 // particle j in [jb, n) except `skip`; pairs inside group g count only if same_ok.
-// Lanes stride over j; each lane loads kU neighbours before evaluating them.
+// Lanes stride over j; each lane loads kU neighbours before evaluating them.  FP32
+// terms are summed in FP32 per block of kU and then in FP64; hot pairs (hot_pair) are
+// evaluated again in FP64 (row64, c64, qa) from the exact fixed-point displacement.
 template <int kThreads>
-__device__ __forceinline__ void pair_row_f32(const float4* __restrict__ pos32, const uint32_t* __restrict__ meta, int jb,
-                                             int n, int skip, int g, bool same_ok, const DevPairF* row, const CoulF& c,
-                                             const float4* tab, int terms, float L, float invL, float ax, float ay,
-                                             float az, float bx, float by, float bz, float qa, double& e_new,
+__device__ __forceinline__ void pair_row_f32(const FixPos* __restrict__ pos32, const uint32_t* __restrict__ meta, int jb,
+                                             int n, int skip, int g, bool same_ok, const DevPairF* row, const DevPair* row64,
+                                             const CoulF& c, const DevCoulomb& c64, const float4* tab, int terms, float h,
+                                             double h64, const FixPos& a, const FixPos& b, double qa, double& e_new,
                                              double& e_old) {
     constexpr int kU = 4;
     for (int j0 = jb + static_cast<int>(threadIdx.x); j0 < n; j0 += kThreads * kU) {
-        float4 pj[kU];
+        FixPos pj[kU];
         uint32_t mj[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int j = j0 + u * kThreads;
-            pj[u] = j < n ? pos32[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pj[u] = j < n ? pos32[j] : FixPos{0u, 0u, 0u, 0.0f};
             mj[u] = j < n ? meta[j] : 0u;
         }
         float sn = 0.0f, so = 0.0f;
+        unsigned hot = 0u; // bit 2u: new state of neighbour u, bit 2u+1: old state
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int j = j0 + u * kThreads;
             const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
             const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
-            const float un = pair_energy_ft(p, c, tab, terms, qa, pj[u].w,
-                                            dist2f_round(L, invL, ax, ay, az, pj[u].x, pj[u].y, pj[u].z));
-            const float uo = pair_energy_ft(p, c, tab, terms, qa, pj[u].w,
-                                            dist2f_round(L, invL, bx, by, bz, pj[u].x, pj[u].y, pj[u].z));
-            sn += ok ? un : 0.0f;
-            so += ok ? uo : 0.0f;
+            const float un = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(a, pj[u], h));
+            const float uo = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(b, pj[u], h));
+            const bool hn = ok && hot_pair(un), ho = ok && hot_pair(uo);
+            sn += (ok && !hn) ? un : 0.0f;
+            so += (ok && !ho) ? uo : 0.0f;
+            hot |= (hn ? 1u : 0u) << (2 * u);
+            hot |= (ho ? 2u : 0u) << (2 * u);
         }
         e_new += static_cast<double>(sn);
         e_old += static_cast<double>(so);
+        if (hot) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const unsigned m2 = (hot >> (2 * u)) & 3u;
+                if (m2 == 0u) continue;
+                const DevPair& p = row64[static_cast<int>(mj[u] & 0xffu)];
+                const double qb = static_cast<double>(pj[u].q);
+                if (m2 & 1u) e_new += pair_energy_t(p, c64, qa, qb, dist2_fix64(a, pj[u], h64), terms);
+                if (m2 & 2u) e_old += pair_energy_t(p, c64, qa, qb, dist2_fix64(b, pj[u], h64), terms);
+            }
+        }
     }
 }
 

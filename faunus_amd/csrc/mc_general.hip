@@ -212,6 +212,8 @@ struct MovedSet {
     bool same_ok;
     const double4* s_new;
     const double4* s_old;
+    const FixPos* s_fnew; // FP32 form: fixed-point copies of s_new and s_old
+    const FixPos* s_fold;
     const int* s_mt;
     const int* excl_off;
     const int* excl;
@@ -219,19 +221,21 @@ struct MovedSet {
 
 // Pair sums of the moved atoms (new and old coordinates) against the particles
 // base + tid + u * kThreads (u < kB) below je: the kB neighbours sit in registers and
-// the moved atoms stream from LDS, kB independent pair terms per atom and state.
+// the moved atoms stream from LDS, kB independent pair terms per atom and state.  In
+// the FP32 form, hot pairs (hot_pair) are evaluated again in FP64 from the exact
+// fixed-point displacement.
 // kExcl: the system has pair exclusions (looked up per pair inside the moved group);
 // without them the loop over the neighbours is straight-line code.
 template <int kThreads, int kB, bool kF32, bool kExcl>
 __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms, const uint32_t* __restrict__ meta,
-                                            const float4* __restrict__ pos32, const double4* __restrict__ pos,
+                                            const FixPos* __restrict__ pos32, const double4* __restrict__ pos,
                                             const DevPairF* s_pairs_f, const DevPair* s_pairs, int nt, const CoulF& cf,
                                             const float4* s_stab, const DevCoulomb& s_coul, int terms, const Box& box,
-                                            float fL, float fIL, double& e_new, double& e_old) {
+                                            float hf, double h64, double& e_new, double& e_old) {
     const int j0 = base + static_cast<int>(threadIdx.x);
     uint32_t mj[kB];
     double4 pj[kB];
-    float4 pf[kB];
+    FixPos pf[kB];
     int use[kB]; // 0 no pair, 1 pair, 2 pair inside the moved group (exclusions apply)
     int any = 0;
 #pragma unroll
@@ -239,7 +243,7 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
         const int j = j0 + u * kThreads;
         const bool in = j < je && !(j >= ms.mlo && j < ms.mhi);
         mj[u] = in ? meta[j] : 0u;
-        if (kF32) pf[u] = in ? pos32[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (kF32) pf[u] = in ? pos32[j] : FixPos{0u, 0u, 0u, 0.0f};
         else pj[u] = in ? pos[j] : make_double4(0.0, 0.0, 0.0, 0.0);
         const bool same = static_cast<int>(mj[u] >> 8) == ms.g;
         use[u] = !in ? 0 : (same ? (ms.same_ok ? 2 : 0) : 1);
@@ -247,30 +251,42 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
     }
     if (!any) return;
     for (int a = 0; a < ms.m; ++a) {
-        const double4 A = ms.s_new[a], B = ms.s_old[a];
         const int ta = ms.s_mt[a];
         if (kF32) {
             const DevPairF* row = s_pairs_f + ta * nt;
-            const float ax = static_cast<float>(A.x), ay = static_cast<float>(A.y), az = static_cast<float>(A.z),
-                        qa = static_cast<float>(A.w);
-            const float bx = static_cast<float>(B.x), by = static_cast<float>(B.y), bz = static_cast<float>(B.z);
+            const FixPos fa = ms.s_fnew[a], fb = ms.s_fold[a];
             float sn = 0.0f, so = 0.0f;
+            unsigned hot = 0u; // bit 2u: new state of neighbour u, bit 2u+1: old state
 #pragma unroll
             for (int u = 0; u < kB; ++u) {
                 const bool ok = kExcl ? (use[u] == 1 || (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
                                                                                         j0 + u * kThreads)))
                                       : use[u] != 0;
                 const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
-                const float un = pair_energy_ft(p, cf, s_stab, terms, qa, pf[u].w,
-                                                dist2f_round(fL, fIL, ax, ay, az, pf[u].x, pf[u].y, pf[u].z));
-                const float uo = pair_energy_ft(p, cf, s_stab, terms, qa, pf[u].w,
-                                                dist2f_round(fL, fIL, bx, by, bz, pf[u].x, pf[u].y, pf[u].z));
-                sn += ok ? un : 0.0f;
-                so += ok ? uo : 0.0f;
+                const float un = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf));
+                const float uo = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf));
+                const bool hn = ok && hot_pair(un), ho = ok && hot_pair(uo);
+                sn += (ok && !hn) ? un : 0.0f;
+                so += (ok && !ho) ? uo : 0.0f;
+                hot |= ((hn ? 1u : 0u) | (ho ? 2u : 0u)) << (2 * u);
             }
             e_new += static_cast<double>(sn);
             e_old += static_cast<double>(so);
+            if (hot) {
+                const DevPair* row64 = s_pairs + ta * nt;
+                const double qa = ms.s_new[a].w;
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    const unsigned m2 = (hot >> (2 * u)) & 3u;
+                    if (m2 == 0u) continue;
+                    const DevPair& p = row64[static_cast<int>(mj[u] & 0xffu)];
+                    const double qb = static_cast<double>(pf[u].q);
+                    if (m2 & 1u) e_new += pair_energy_t(p, s_coul, qa, qb, dist2_fix64(fa, pf[u], h64), terms);
+                    if (m2 & 2u) e_old += pair_energy_t(p, s_coul, qa, qb, dist2_fix64(fb, pf[u], h64), terms);
+                }
+            }
         } else {
+            const double4 A = ms.s_new[a], B = ms.s_old[a];
             const DevPair* row = s_pairs + ta * nt;
             double sn = 0.0, so = 0.0;
 #pragma unroll
@@ -321,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ int s_wsum[kThreads / 64];
     __shared__ MoveDraw s_draw[kDraw];
     // kn | phase chunk tables | new coordinates (two buffers in the split form) | old
-    // coordinates | moved types
+    // coordinates | (FP32 form) fixed-point new and old coordinates | moved types
     extern __shared__ int s_dyn[];
 
     const int tid = threadIdx.x;
@@ -343,7 +359,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int max_moved = S.max_moved;
     double4* s_nbuf = reinterpret_cast<double4*>(s_phase + kChunk * 6 * width);
     double4* s_old = s_nbuf + (kSplit ? 2 : 1) * max_moved;
-    int* s_mt = reinterpret_cast<int*>(s_old + max_moved);
+    FixPos* s_fnew = reinterpret_cast<FixPos*>(s_old + max_moved);
+    FixPos* s_fold = s_fnew + (kF32 ? max_moved : 0);
+    int* s_mt = reinterpret_cast<int*>(s_fold + (kF32 ? max_moved : 0));
 
     for (int i = tid; i < nt * nt; i += kThreads) {
         s_pairs[i] = S.pairs[i];
@@ -371,8 +389,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     for (int s = 0; s < kMaxKPT; ++s) q[s] = (s < kpt && k0 + s < ke) ? S.q[k0 + s] : make_double2(0.0, 0.0);
 
     double4* __restrict__ pos = S.pos;
-    float4* __restrict__ pos32 = S.pos32;
+    FixPos* __restrict__ pos32 = S.pos32;
     const uint32_t* __restrict__ meta = S.meta;
+    const double h64 = L * kFixInv;
+    const float hf = static_cast<float>(h64);
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
@@ -513,6 +533,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 z = wrap(rz + P.shift[2], L, half, inv);
             }
             s_new[a] = make_double4(x, y, z, p.w);
+            if (kF32) {
+                s_fnew[a] = fixed_pos(x, y, z, p.w, inv);
+                s_fold[a] = fixed_pos(p.x, p.y, p.z, p.w, inv);
+            }
         }
         __syncthreads();
 
@@ -525,11 +549,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             const double4 A = s_new[0], B = s_old[0];
             const int ta = s_mt[0];
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab,
-                                       terms, static_cast<float>(L), static_cast<float>(inv),
-                                       static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
-                                       static_cast<float>(B.x), static_cast<float>(B.y), static_cast<float>(B.z),
-                                       static_cast<float>(A.w), e_new, e_old);
+                pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, s_pairs + ta * nt, cf,
+                                       s_coul, s_stab, terms, hf, h64, s_fnew[0], s_fold[0], A.w, e_new, e_old);
             } else {
                 pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, terms, box, A.x,
                                        A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
@@ -538,8 +559,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (!kCells) {
                 // blocks of kB neighbours per lane (kB from 8 down to 1 as the slice
                 // runs out, so that few lanes sit idle on short slices)
-                const MovedSet ms{P.mlo, P.mhi, m, g, same_ok, s_new, s_old, s_mt, excl_off, excl};
-                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
+                const MovedSet ms{P.mlo, P.mhi, m, g, same_ok, s_new, s_old, s_fnew, s_fold, s_mt, excl_off, excl};
                 auto blocks = [&](auto excl_tag) {
                     constexpr bool kX = decltype(excl_tag)::value;
                     int base = jb;
@@ -547,19 +567,19 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int rem = je - base;
                         if (kF32 && rem >= 8 * kThreads) {
                             moved_block<kThreads, 8, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
                             base += 8 * kThreads;
                         } else if (rem >= 4 * kThreads) {
                             moved_block<kThreads, 4, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
                             base += 4 * kThreads;
                         } else if (rem > kThreads) {
                             moved_block<kThreads, 2, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
                             base += 2 * kThreads;
                         } else {
                             moved_block<kThreads, 1, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, fL, fIL, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
                             base += kThreads;
                         }
                     }
@@ -628,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                     }
                     uint32_t mj[kU];
                     double4 pj[kU];
-                    float4 pf[kU];
+                    FixPos pf[kU];
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         if (jj[u] >= 0) {
@@ -652,11 +672,13 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             const double4 A = st == 0 ? s_new[a] : s_old[a];
                             const int ta = s_mt[a];
                             if (kF32) {
-                                const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                                e += static_cast<double>(pair_energy_ft(
-                                    s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pf[u].w,
-                                    dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y),
-                                                 static_cast<float>(A.z), pf[u].x, pf[u].y, pf[u].z)));
+                                const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
+                                const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q,
+                                                                pf[u].q, dist2_fix(FA, pf[u], hf));
+                                e += hot_pair(uf) ? pair_energy_t(s_pairs[ta * nt + tj], s_coul, A.w,
+                                                                  static_cast<double>(pf[u].q),
+                                                                  dist2_fix64(FA, pf[u], h64), terms)
+                                                  : static_cast<double>(uf);
                             } else {
                                 e += pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
                                                  dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
@@ -687,12 +709,13 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int tj = static_cast<int>(mj & 0xffu);
                         double u;
                         if (kF32) {
-                            const float4 pj = pos32[j];
-                            const float fL = static_cast<float>(L), fIL = static_cast<float>(inv);
-                            u = static_cast<double>(pair_energy_ft(
-                                s_pairs_f[ta * nt + tj], cf, s_stab, terms, static_cast<float>(A.w), pj.w,
-                                dist2f_round(fL, fIL, static_cast<float>(A.x), static_cast<float>(A.y), static_cast<float>(A.z),
-                                       pj.x, pj.y, pj.z)));
+                            const FixPos pj = pos32[j];
+                            const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
+                            const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q, pj.q,
+                                                            dist2_fix(FA, pj, hf));
+                            u = hot_pair(uf) ? pair_energy_t(s_pairs[ta * nt + tj], s_coul, A.w, static_cast<double>(pj.q),
+                                                             dist2_fix64(FA, pj, h64), terms)
+                                             : static_cast<double>(uf);
                         } else {
                             const double4 pj = pos[j];
                             u = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj.w, dist2(box, A.x, A.y, A.z, pj.x, pj.y, pj.z));
@@ -933,9 +956,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 if (kSplit && (idx < jb || idx >= je)) continue;
                 const double4 A = s_new[a];
                 pos[idx] = A;
-                if (kF32)
-                    pos32[idx] = make_float4(static_cast<float>(A.x), static_cast<float>(A.y),
-                                             static_cast<float>(A.z), static_cast<float>(A.w));
+                if (kF32) pos32[idx] = s_fnew[a];
             }
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
@@ -1029,7 +1050,7 @@ struct GeneralLaunch {
     int grid;
 };
 
-GeneralLaunch plan(const ReplicaState* h_states, int count) {
+GeneralLaunch plan(const ReplicaState* h_states, int count, bool f32) {
     GeneralLaunch g{0, h_states[0].cells != 0, false, 0, 0};
     int maxnk = 0, maxw = 0, maxm = 0;
     for (int r = 0; r < count; ++r) {
@@ -1051,9 +1072,11 @@ GeneralLaunch plan(const ReplicaState* h_states, int count) {
     }
     if (g.maxkpt > kpt_cap(g.cells))
         throw std::runtime_error("too many k-vectors for the general device MC (max 4096 per workgroup)");
-    // the split form keeps the previous move's coordinates in a second buffer
+    // the split form keeps the previous move's coordinates in a second buffer; the FP32
+    // form adds fixed-point copies of the new and old coordinates
     g.lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) + sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
-            ((g.split ? 3 : 2) * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
+            ((g.split ? 3 : 2) * sizeof(double4) + (f32 ? 2 * sizeof(FixPos) : 0) + sizeof(int)) *
+                static_cast<size_t>(maxm);
     return g;
 }
 
@@ -1117,7 +1140,7 @@ __global__ void clear_general_slots_kernel(ReplicaState* __restrict__ states, in
 
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap, int map_blocks) {
-    const GeneralLaunch g = plan(h_states, count);
+    const GeneralLaunch g = plan(h_states, count, f32);
     const void* fn = select_kernel(g.maxkpt, g.cells, f32, g.split);
     if (g.lds > prepare(fn)) throw std::runtime_error("k-vector and molecule tables too large for LDS");
     // each workgroup lays out its own replica's tables inside the dynamic LDS

@@ -508,9 +508,9 @@ void Batch::run_pipe(long long moves, int groups, int depth, bool trace, int tra
     int cap = trace ? trace_cap : 0;
     long long spin = 1LL << 24;
     void* args[] = {&st, &M, &G, &slots, &RR, &err, &tr, &cap, &spin};
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     check_hip(hipLaunchCooperativeKernel(fn, dim3(groups), dim3(256), args, 0, stream_), "hipLaunchCooperativeKernel");
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 } // namespace fmc

@@ -634,9 +634,9 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     int cap = trace ? trace_cap : 0;
     long long spin = 1LL << 24;
     void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, stream_), "hipLaunchCooperativeKernel");
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 // run_split without allocating: the exchange slots and the time-out flag live in the
@@ -668,10 +668,10 @@ void Batch::run_split_static(long long moves, int groups, int stride) {
     int cap = 0;
     long long spin = 1LL << 24;
     void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     check_hip(hipLaunchCooperativeKernel(sl.fn, dim3(sl.grid), dim3(sl.threads), args, 0, stream_),
               "hipLaunchCooperativeKernel");
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 // The replica's general-move kernel over `groups` workgroups (cooperative launch).
@@ -708,10 +708,10 @@ void Batch::run_split_general(long long moves, int groups, int stride, bool trac
                              stream_),
               "copy block map");
     check_hip(hipMemcpyAsync(d_moves_, &h_split_moves_, sizeof(long long), hipMemcpyHostToDevice, stream_), "copy moves");
-    if (timing_) check_hip(hipEventRecord(ev0_, stream_), "event");
+    time_begin();
     launch_general(h_states_.data(), 1, d_states_, d_moves_, fp32_ != 0, trace ? d_trace_ : nullptr,
                    trace ? trace_cap : 0, stream_, groups > 1 ? d_map_ : nullptr, groups > 1 ? grid : 0);
-    if (timing_) check_hip(hipEventRecord(ev1_, stream_), "event");
+    time_end();
 }
 
 // The replica's error (1: a cell overflowed, 2: a wait timed out or the state is

@@ -1,10 +1,11 @@
 """The device pair functions against the oracle's pair energy, pair by pair: the FP64
-form (pair_energy) within 1e-11 * max(|u|, 1e-3 kT) and the FP32-pair form
-(pair_energy_ft: FP32 arithmetic, FP32 S(q) table, splined tables evaluated from the
-FP32 distance) within 2e-6 * max(|u|, 1 kT) -- a few units of FP32 rounding of the
-larger of the terms that cancel near the LJ minimum -- for every type pair of each
-potential form the parser takes, on a grid of distances from well inside contact to
-beyond the pair range."""
+form (pair_energy) within 1e-11 * max(|u|, 1e-3 kT), and the FP32-pair form at the same
+distance: pairs above the hot threshold (8 kT) are evaluated again in FP64 and must
+meet the FP64 bound; the FP32 terms below it (FP32 arithmetic, FP32 S(q) table,
+splined tables evaluated from the FP32 distance) must lie within 1e-6 * 8 kT -- a few
+units of FP32 rounding of the largest term that can cancel inside a pair below the
+threshold.  Every type pair of each potential form the parser takes, on a grid of
+distances from well inside contact to beyond the pair range."""
 import json
 
 import numpy as np
@@ -17,6 +18,8 @@ from tests.gpu_helpers import log
 from tests.test_gpu_potentials import VARIANTS
 
 pytestmark = pytest.mark.gpu
+
+HOT = 8.0  # kT: FP32 pair energies above it are evaluated again in FP64
 
 CASES = dict(VARIANTS)
 CASES["splined_nacl"] = systems.nacl_lj(n_pairs=10, box=60.0, seed=1, kmax=6)[0]["energy"]
@@ -47,13 +50,18 @@ def test_pair_functions_match_the_oracle(name):
             assert np.array_equal(inf, np.isinf(u64)) and np.array_equal(inf, np.isinf(u32)), (name, a, b)
             fin = ~inf
             e64 = np.abs(u64[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-3)
-            e32 = np.abs(u32[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1.0)
+            hot = np.abs(uo[fin]) > HOT * (1 + 1e-5)
+            # hot pairs: the FP64 bound; the rest: absolute, in units of the threshold
+            e32 = np.where(hot, np.abs(u32[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-3) / 1e-11,
+                           np.abs(u32[fin] - uo[fin]) / (1e-6 * HOT))
             k = int(np.argmax(e32)) if len(e32) else 0
             log(name, a, b, "fp64 max rel %.2e" % (e64.max() if len(e64) else 0),
-                "fp32 max rel %.2e at r=%.4f (u %.6g vs %.6g)" % (e32.max() if len(e32) else 0, r[fin][k] if len(e32) else 0,
-                                                                u32[fin][k] if len(e32) else 0, uo[fin][k] if len(e32) else 0))
+                "fp32 worst/bound %.3f at r=%.4f (u %.9g vs %.9g)" % (e32.max() if len(e32) else 0,
+                                                                     r[fin][k] if len(e32) else 0,
+                                                                     u32[fin][k] if len(e32) else 0,
+                                                                     uo[fin][k] if len(e32) else 0))
             worst64 = max(worst64, float(e64.max()) if len(e64) else 0.0)
             worst32 = max(worst32, float(e32.max()) if len(e32) else 0.0)
     eng.close()
     assert worst64 <= 1e-11, (name, worst64)
-    assert worst32 <= 2e-6, (name, worst32)
+    assert worst32 <= 1.0, (name, worst32)
