@@ -32,6 +32,8 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     __shared__ double s_gk[kMaxGk];
     __shared__ double s_red[3][kThreads / 64];
     __shared__ int s_accept;
+    __shared__ int s_nhot;                        // hot pairs queued this move (FP32 form)
+    __shared__ int2 s_hot[kF32 ? kHotCap : 1];
     __shared__ Proposal s_prop[64]; // the next 64 proposals, drawn one per lane of wave 0
     extern __shared__ int s_dyn[]; // kn[nk] followed by the phase tables (double2)
 
@@ -70,8 +72,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     double4* __restrict__ pos = S.pos;
     FixPos* __restrict__ pos32 = S.pos32;
     const uint32_t* __restrict__ meta = S.meta;
-    const double h64 = L * kFixInv;
-    const float hf = static_cast<float>(h64);
+    const float hf = static_cast<float>(L * kFixInv);
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
@@ -83,6 +84,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
     long long t_done = M; // moves decided by this launch
+    if (tid == 0) s_nhot = 0;
     __syncthreads();
 
     for (long long t = 0; t < M; ++t) {
@@ -127,8 +129,8 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         }
         if (!kCells) {
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, 0, n, i, -1, true, s_pairs_f + ti * nt, s_pairs + ti * nt, cf, s_coul,
-                                       s_stab, S.terms, hf, h64, fa, fb, po.w, e_new, e_old);
+                pair_row_f32<kThreads>(pos32, meta, 0, n, i, -1, true, s_pairs_f + ti * nt, cf, s_stab, S.terms, hf,
+                                       fa, fb, &s_nhot, s_hot, 0, e_new, e_old);
             } else {
                 const Box b{{L, L, L}, {half, half, half}};
                 pair_row_f64<kThreads>(pos, meta, 0, n, i, -1, true, s_pairs + ti * nt, s_coul, S.terms, b, nx_, ny_, nz_,
@@ -157,9 +159,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                         const FixPos& A = st == 0 ? fa : fb;
                         const float uf = pair_energy_ft(s_pairs_f[ti * nt + tj], cf, s_stab, S.terms, A.q, pj.q,
                                                         dist2_fix(A, pj, hf));
-                        u = hot_pair(uf) ? pair_energy_t(s_pairs[ti * nt + tj], s_coul, po.w, static_cast<double>(pj.q),
-                                                         dist2_fix64(A, pj, h64), S.terms)
-                                         : static_cast<double>(uf);
+                        u = (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, st)) ? 0.0 : static_cast<double>(uf);
                     } else {
                         const double4 pj = pos[j];
                         Box b{{L, L, L}, {half, half, half}};
@@ -170,7 +170,23 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                 }
             }
         }
-        __syncthreads(); // phase tables complete
+        __syncthreads(); // phase tables complete (and the hot-pair queue)
+        if (kF32) {
+            // hot pairs in FP64 from the FP64 coordinates
+            const Box b{{L, L, L}, {half, half, half}};
+            const int nh = min(s_nhot, kHotCap);
+            for (int h = tid; h < nh; h += kThreads) {
+                const int2 e = s_hot[h];
+                const double4 pj = pos[e.x];
+                const int tj = static_cast<int>(meta[e.x] & 0xffu);
+                const bool nw = (e.y & 1) == 0;
+                const double u = pair_energy_t(s_pairs[ti * nt + tj], s_coul, po.w, pj.w,
+                                               dist2(b, nw ? nx_ : po.x, nw ? ny_ : po.y, nw ? nz_ : po.z, pj.x, pj.y, pj.z),
+                                               S.terms);
+                if (nw) e_new += u;
+                else e_old += u;
+            }
+        }
 
         // Ewald: dE = sum_k A_k (2 Re(conj(Q) dQ) + |dQ|^2) over this thread's k-vectors
         double de = 0.0;
@@ -228,6 +244,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
             const double du = energy_change(a, b, pref * c);
             const bool acc = metropolis(beta * du, P.accept_u);
             s_accept = acc ? 1 : 0;
+            if (kF32) s_nhot = 0; // every thread has read the queue before the reduction
             if (trace && t < trace_cap) {
                 TraceEntry& T = trace[static_cast<size_t>(blockIdx.x) * trace_cap + t];
                 T.index = i;

@@ -128,20 +128,25 @@ __device__ __forceinline__ float dist2_fix(const FixPos& a, const FixPos& b, flo
     return dx * dx + dy * dy + dz * dz;
 }
 
-// The same distance in FP64 (exact displacement; h = L / 2^32).
-__device__ __forceinline__ double dist2_fix64(const FixPos& a, const FixPos& b, double h) {
-    const double dx = static_cast<double>(static_cast<int>(a.x - b.x)) * h;
-    const double dy = static_cast<double>(static_cast<int>(a.y - b.y)) * h;
-    const double dz = static_cast<double>(static_cast<int>(a.z - b.z)) * h;
-    return dx * dx + dy * dy + dz * dz;
-}
-
-// Pairs whose FP32 energy exceeds this (kT; overlaps, +inf and NaN included) are
+// Pairs whose FP32 energy exceeds kHotPair kT (overlaps, +inf and NaN included) are
 // evaluated again in FP64: FP32 rounding of r^2 grows sixfold through r^-12, and one
-// overlapping pair would otherwise set the error of the move's dU.
+// overlapping pair would otherwise set the error of the move's dU.  The pair loops
+// queue them in LDS (the neighbour, and moved atom << 1 | state with state 0 new and 1
+// old); after the loops each queued pair is evaluated from the FP64 coordinates, so the
+// FP64 pair code stays out of the loops' registers.  Beyond kHotCap entries a move's
+// further hot pairs keep their FP32 energy.
 constexpr float kHotPair = 8.0f;
+constexpr int kHotCap = 256;
 
 __device__ __forceinline__ bool hot_pair(float u) { return !(__builtin_fabsf(u) <= kHotPair); }
+
+// Queue one hot pair; false when the queue is full.
+__device__ __forceinline__ bool push_hot(int* n, int2* q, int j, int tag) {
+    const int s = atomicAdd(n, 1);
+    if (s >= kHotCap) return false;
+    q[s] = make_int2(j, tag);
+    return true;
+}
 
 __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
                                                 int terms, float qa, float qb, float r2) {
@@ -196,14 +201,13 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
 // Pair sums of one moved particle, at its new (a) and old (b) position, against every
 // particle j in [jb, n) except `skip`; pairs inside group g count only if same_ok.
 // Lanes stride over j; each lane loads kU neighbours before evaluating them.  FP32
-// terms are summed in FP32 per block of kU and then in FP64; hot pairs (hot_pair) are
-// evaluated again in FP64 (row64, c64, qa) from the exact fixed-point displacement.
+// terms are summed in FP32 per block of kU and then in FP64; hot pairs go to the queue
+// (hot_n, hot_q) with tag = moved atom << 1.
 template <int kThreads>
 __device__ __forceinline__ void pair_row_f32(const FixPos* __restrict__ pos32, const uint32_t* __restrict__ meta, int jb,
-                                             int n, int skip, int g, bool same_ok, const DevPairF* row, const DevPair* row64,
-                                             const CoulF& c, const DevCoulomb& c64, const float4* tab, int terms, float h,
-                                             double h64, const FixPos& a, const FixPos& b, double qa, double& e_new,
-                                             double& e_old) {
+                                             int n, int skip, int g, bool same_ok, const DevPairF* row, const CoulF& c,
+                                             const float4* tab, int terms, float h, const FixPos& a, const FixPos& b,
+                                             int* hot_n, int2* hot_q, int tag, double& e_new, double& e_old) {
     constexpr int kU = 4;
     for (int j0 = jb + static_cast<int>(threadIdx.x); j0 < n; j0 += kThreads * kU) {
         FixPos pj[kU];
@@ -226,8 +230,7 @@ __device__ __forceinline__ void pair_row_f32(const FixPos* __restrict__ pos32, c
             const bool hn = ok && hot_pair(un), ho = ok && hot_pair(uo);
             sn += (ok && !hn) ? un : 0.0f;
             so += (ok && !ho) ? uo : 0.0f;
-            hot |= (hn ? 1u : 0u) << (2 * u);
-            hot |= (ho ? 2u : 0u) << (2 * u);
+            hot |= ((hn ? 1u : 0u) | (ho ? 2u : 0u)) << (2 * u);
         }
         e_new += static_cast<double>(sn);
         e_old += static_cast<double>(so);
@@ -236,10 +239,12 @@ __device__ __forceinline__ void pair_row_f32(const FixPos* __restrict__ pos32, c
             for (int u = 0; u < kU; ++u) {
                 const unsigned m2 = (hot >> (2 * u)) & 3u;
                 if (m2 == 0u) continue;
-                const DevPair& p = row64[static_cast<int>(mj[u] & 0xffu)];
-                const double qb = static_cast<double>(pj[u].q);
-                if (m2 & 1u) e_new += pair_energy_t(p, c64, qa, qb, dist2_fix64(a, pj[u], h64), terms);
-                if (m2 & 2u) e_old += pair_energy_t(p, c64, qa, qb, dist2_fix64(b, pj[u], h64), terms);
+                const int j = j0 + u * kThreads;
+                const DevPairF& p = row[static_cast<int>(mj[u] & 0xffu)];
+                if ((m2 & 1u) && !push_hot(hot_n, hot_q, j, tag))
+                    e_new += pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(a, pj[u], h));
+                if ((m2 & 2u) && !push_hot(hot_n, hot_q, j, tag | 1))
+                    e_old += pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(b, pj[u], h));
             }
         }
     }

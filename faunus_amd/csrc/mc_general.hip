@@ -222,8 +222,7 @@ struct MovedSet {
 // Pair sums of the moved atoms (new and old coordinates) against the particles
 // base + tid + u * kThreads (u < kB) below je: the kB neighbours sit in registers and
 // the moved atoms stream from LDS, kB independent pair terms per atom and state.  In
-// the FP32 form, hot pairs (hot_pair) are evaluated again in FP64 from the exact
-// fixed-point displacement.
+// the FP32 form, hot pairs go to the queue (hot_n, hot_q).
 // kExcl: the system has pair exclusions (looked up per pair inside the moved group);
 // without them the loop over the neighbours is straight-line code.
 template <int kThreads, int kB, bool kF32, bool kExcl>
@@ -231,7 +230,7 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
                                             const FixPos* __restrict__ pos32, const double4* __restrict__ pos,
                                             const DevPairF* s_pairs_f, const DevPair* s_pairs, int nt, const CoulF& cf,
                                             const float4* s_stab, const DevCoulomb& s_coul, int terms, const Box& box,
-                                            float hf, double h64, double& e_new, double& e_old) {
+                                            float hf, int* hot_n, int2* hot_q, double& e_new, double& e_old) {
     const int j0 = base + static_cast<int>(threadIdx.x);
     uint32_t mj[kB];
     double4 pj[kB];
@@ -273,16 +272,16 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
             e_new += static_cast<double>(sn);
             e_old += static_cast<double>(so);
             if (hot) {
-                const DevPair* row64 = s_pairs + ta * nt;
-                const double qa = ms.s_new[a].w;
 #pragma unroll
                 for (int u = 0; u < kB; ++u) {
                     const unsigned m2 = (hot >> (2 * u)) & 3u;
                     if (m2 == 0u) continue;
-                    const DevPair& p = row64[static_cast<int>(mj[u] & 0xffu)];
-                    const double qb = static_cast<double>(pf[u].q);
-                    if (m2 & 1u) e_new += pair_energy_t(p, s_coul, qa, qb, dist2_fix64(fa, pf[u], h64), terms);
-                    if (m2 & 2u) e_old += pair_energy_t(p, s_coul, qa, qb, dist2_fix64(fb, pf[u], h64), terms);
+                    const int j = j0 + u * kThreads;
+                    const DevPairF& p = row[static_cast<int>(mj[u] & 0xffu)];
+                    if ((m2 & 1u) && !push_hot(hot_n, hot_q, j, a << 1))
+                        e_new += pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf));
+                    if ((m2 & 2u) && !push_hot(hot_n, hot_q, j, (a << 1) | 1))
+                        e_old += pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf));
                 }
             }
         } else {
@@ -332,6 +331,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
     __shared__ int s_accept, s_abort;
+    __shared__ int s_nhot;                        // hot pairs queued this move (FP32 form)
+    __shared__ int2 s_hot[kF32 ? kHotCap : 1];
     __shared__ int s_ccell[kCells ? kMaxComb : 1];
     __shared__ int s_cpre[kCells ? kMaxComb + 1 : 1];
     __shared__ int s_wsum[kThreads / 64];
@@ -371,7 +372,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         }
     }
     lds_copy(s_coul, S.coul, tid, kThreads);
-    if (tid == 0) s_abort = 0;
+    if (tid == 0) {
+        s_abort = 0;
+        s_nhot = 0;
+    }
     for (int i = tid; i < nt; i += kThreads) s_dpa[i] = S.dp_atom[i];
     for (int i = tid; i < S.nkinds; i += kThreads) s_kinds[i] = S.kinds[i];
     for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
@@ -391,8 +395,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     double4* __restrict__ pos = S.pos;
     FixPos* __restrict__ pos32 = S.pos32;
     const uint32_t* __restrict__ meta = S.meta;
-    const double h64 = L * kFixInv;
-    const float hf = static_cast<float>(h64);
+    const float hf = static_cast<float>(L * kFixInv);
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
@@ -549,8 +552,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             const double4 A = s_new[0], B = s_old[0];
             const int ta = s_mt[0];
             if (kF32) {
-                pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, s_pairs + ta * nt, cf,
-                                       s_coul, s_stab, terms, hf, h64, s_fnew[0], s_fold[0], A.w, e_new, e_old);
+                pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab, terms, hf,
+                                       s_fnew[0], s_fold[0], &s_nhot, s_hot, 0, e_new, e_old);
             } else {
                 pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, terms, box, A.x,
                                        A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
@@ -567,19 +570,19 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int rem = je - base;
                         if (kF32 && rem >= 8 * kThreads) {
                             moved_block<kThreads, 8, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 8 * kThreads;
                         } else if (rem >= 4 * kThreads) {
                             moved_block<kThreads, 4, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 4 * kThreads;
                         } else if (rem > kThreads) {
                             moved_block<kThreads, 2, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 2 * kThreads;
                         } else {
                             moved_block<kThreads, 1, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
-                                                               s_stab, s_coul, terms, box, hf, h64, e_new, e_old);
+                                                               s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += kThreads;
                         }
                     }
@@ -675,10 +678,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                                 const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
                                 const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q,
                                                                 pf[u].q, dist2_fix(FA, pf[u], hf));
-                                e += hot_pair(uf) ? pair_energy_t(s_pairs[ta * nt + tj], s_coul, A.w,
-                                                                  static_cast<double>(pf[u].q),
-                                                                  dist2_fix64(FA, pf[u], h64), terms)
-                                                  : static_cast<double>(uf);
+                                e += (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0
+                                                                                                  : static_cast<double>(uf);
                             } else {
                                 e += pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
                                                  dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
@@ -713,9 +714,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
                             const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q, pj.q,
                                                             dist2_fix(FA, pj, hf));
-                            u = hot_pair(uf) ? pair_energy_t(s_pairs[ta * nt + tj], s_coul, A.w, static_cast<double>(pj.q),
-                                                             dist2_fix64(FA, pj, h64), terms)
-                                             : static_cast<double>(uf);
+                            u = (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0
+                                                                                             : static_cast<double>(uf);
                         } else {
                             const double4 pj = pos[j];
                             u = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj.w, dist2(box, A.x, A.y, A.z, pj.x, pj.y, pj.z));
@@ -798,6 +798,23 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 }
             }
         }
+        if (kF32 && m > 0) {
+            // hot pairs in FP64 from the FP64 coordinates (the Ewald chunk loop's barrier
+            // follows every push)
+            const int nh = min(s_nhot, kHotCap);
+            for (int h = tid; h < nh; h += kThreads) {
+                const int2 e = s_hot[h];
+                const int a = e.y >> 1;
+                const bool nw = (e.y & 1) == 0;
+                const double4 A = nw ? s_new[a] : s_old[a];
+                const double4 pj = load_pos(e.x, s_prev);
+                const int tj = static_cast<int>(meta[e.x] & 0xffu);
+                const double u = pair_energy_t(s_pairs[s_mt[a] * nt + tj], s_coul, A.w, pj.w,
+                                               dist2(box, A.x, A.y, A.z, pj.x, pj.y, pj.z), terms);
+                if (nw) e_new += u;
+                else e_old += u;
+            }
+        }
         double de = 0.0;
 #pragma unroll
         for (int s = 0; s < kMaxKPT; ++s) {
@@ -877,6 +894,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 acc = metropolis(beta * du, P.accept_u);
             }
             s_accept = acc ? 1 : 0;
+            if (kF32) s_nhot = 0; // every thread has read the queue before the reduction
             if (trace && t < trace_cap && part == 0) {
                 TraceEntry& T = trace[static_cast<size_t>(rep) * trace_cap + t];
                 T.index = m > 0 ? P.mlo : -1;
