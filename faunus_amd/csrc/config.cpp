@@ -234,6 +234,37 @@ CoulombModel parse_coulomb(const Model& m, const Value& j, bool pm) {
 
 static void build_splined(Model& m, const Value& js);
 
+// cutoff_g2g (from_json(GroupCutoff), energy.cpp:1897-1933): a number is one cutoff for
+// every pair of molecule types; an object holds "default" (no cutoff if absent) and
+// "A B" entries naming two molecule types.
+static std::vector<double> parse_g2g(const Value& c, const std::vector<MoleculeCfg>& mols) {
+    const size_t nmol = mols.size();
+    const double big = std::numeric_limits<double>::max();
+    auto squared = [&](double rc) { return rc >= std::sqrt(big) ? big : rc * rc; };
+    std::vector<double> t(nmol * nmol, big);
+    if (c.is_number()) {
+        std::fill(t.begin(), t.end(), squared(c.as_number()));
+        return t;
+    }
+    if (!c.is_object()) throw std::runtime_error("cutoff_g2g: a number or an object");
+    if (c.contains("default")) std::fill(t.begin(), t.end(), squared(c.at("default").as_number()));
+    auto find = [&](const std::string& name) {
+        for (size_t k = 0; k < nmol; ++k)
+            if (mols[k].name == name) return k;
+        throw std::runtime_error("cutoff_g2g: unknown molecule '" + name + "'");
+    };
+    for (auto& [key, v] : c.members()) {
+        if (key == "default") continue;
+        const auto sp = key.find(' ');
+        if (sp == std::string::npos || key.find(' ', sp + 1) != std::string::npos)
+            throw std::runtime_error("cutoff_g2g: invalid molecules names '" + key + "'");
+        const size_t a = find(key.substr(0, sp)), b = find(key.substr(sp + 1));
+        t[a * nmol + b] = t[b * nmol + a] = squared(v.as_number());
+    }
+    return t;
+}
+
+
 Model Model::from_json(const std::string& text) {
     Model m;
     const Value cfg = jsonlite::parse(text);
@@ -427,11 +458,7 @@ Model Model::from_json(const std::string& text) {
         } else {
             throw std::runtime_error("energy term '" + key + "' is not supported by the GPU engine");
         }
-        if (value.contains("cutoff_g2g")) {
-            const Value& c = value.at("cutoff_g2g");
-            const double rc = c.is_number() ? c.as_number() : c.at("default").as_number();
-            m.g2g_cutoff2 = rc * rc;
-        }
+        if (value.contains("cutoff_g2g")) m.g2g2 = parse_g2g(value.at("cutoff_g2g"), m.molecules);
         const bool galore = m.coulomb.scheme == kCoulPlain || m.coulomb.scheme == kCoulEwald ||
                             m.coulomb.scheme == kCoulFanourgakis;
         if (galore) {

@@ -109,7 +109,11 @@ struct Model {
     CoulombModel coulomb;
     EwaldModel ewald;
     bool has_self_term = false;
-    double g2g_cutoff2 = std::numeric_limits<double>::max();
+    // GroupCutoff (energy.h:746-768; from_json, energy.cpp:1897-1933): squared cutoff of
+    // the mass-centre distance per pair of molecule types (nmol x nmol, row major); empty
+    // when the nonbonded term has no cutoff_g2g
+    std::vector<double> g2g2;
+    bool has_g2g() const { return !g2g2.empty(); }
     double maximum_allowed_energy = kInf;
     bool anisotropic_form = true;
     std::vector<std::string> term_names; // Hamiltonian order: energy list, with [selfenergy] before nonbonded, [ewald] after

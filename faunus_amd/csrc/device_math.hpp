@@ -180,6 +180,12 @@ __device__ __forceinline__ bool pair_excluded(const int* __restrict__ off, const
     return false;
 }
 
+// GroupCutoff::cut's squared cutoff (energy.h:761-768) for two groups' flag words
+// (molecule type from bit 8) from the per-pair table of molecule types.
+__device__ __forceinline__ double g2g_cutoff2(const double* __restrict__ table, int nmol, int flags_a, int flags_b) {
+    return table[(flags_a >> 8) * nmol + (flags_b >> 8)];
+}
+
 __device__ __forceinline__ int unpack_n(int packed, int which) {
     return static_cast<int>(static_cast<signed char>((packed >> (8 * which)) & 0xff));
 }

@@ -23,9 +23,10 @@ void check_hip(hipError_t e, const char* what) {
 namespace {
 
 struct GroupTable {
-    const int* flags;     // bit0 molecular (not atomic), bit1 rigid
+    const int* flags;     // bit0 molecular (not atomic), bit1 rigid, molecule type from bit 8
     const double* cm;     // 3 per group (accepted state)
-    double g2g2;          // group-to-group cutoff squared
+    const double* g2g2;   // squared cutoff per pair of molecule types (null: none)
+    int nmol;
     const int* excl_off;  // per-particle excluded partners (null: none)
     const int* excl;
 };
@@ -85,11 +86,11 @@ __global__ __launch_bounds__(kBlock) void delta_kernel(
             // pair between a moved particle in group gi and a static particle in group gj
             if (gi == gj) return single && mv.internal && (!kGroups || !(groups.flags[gi] & 2));
             if (!kGroups) return true;
-            if ((groups.flags[gi] & 1) && (groups.flags[gj] & 1)) { // GroupCutoff::cut (energy.h:761-768)
+            if (groups.g2g2 && (groups.flags[gi] & 1) && (groups.flags[gj] & 1)) { // GroupCutoff::cut (energy.h:761-768)
                 const double* cj = groups.cm + 3 * gj;
                 const double r2 = new_state ? dist2(box, cm_new_x, cm_new_y, cm_new_z, cj[0], cj[1], cj[2])
                                             : dist2(box, cm_old_x, cm_old_y, cm_old_z, cj[0], cj[1], cj[2]);
-                return r2 < groups.g2g2;
+                return r2 < g2g_cutoff2(groups.g2g2, groups.nmol, groups.flags[gi], groups.flags[gj]);
             }
             return true;
         };
@@ -453,10 +454,12 @@ __global__ __launch_bounds__(kBlock) void all_pairs_kernel(const double4* __rest
                     if (gi == gj) {
                         if (groups.flags[gi] & 2) continue; // rigid: no internal pairs
                         if (pair_excluded(groups.excl_off, groups.excl, i, j)) continue;
-                    } else if ((groups.flags[gi] & 1) && (groups.flags[gj] & 1)) {
+                    } else if (groups.g2g2 && (groups.flags[gi] & 1) && (groups.flags[gj] & 1)) {
                         const double* a = groups.cm + 3 * gi;
                         const double* b = groups.cm + 3 * gj;
-                        if (!(dist2(box, a[0], a[1], a[2], b[0], b[1], b[2]) < groups.g2g2)) continue;
+                        if (!(dist2(box, a[0], a[1], a[2], b[0], b[1], b[2]) <
+                              g2g_cutoff2(groups.g2g2, groups.nmol, groups.flags[gi], groups.flags[gj])))
+                            continue;
                     }
                 }
                 const double4 pj = tile[jj];
@@ -562,6 +565,7 @@ Engine::~Engine() {
     (void)hipFree(cells_.overflow);
     (void)hipFree(d_gflags_);
     (void)hipFree(d_gcm_);
+    (void)hipFree(d_g2g_);
     (void)hipFree(d_bond_off_);
     (void)hipFree(d_bond_);
     (void)hipFree(d_bond_all_);
@@ -734,7 +738,14 @@ void Engine::build_group_tables() {
     std::vector<int> flags(static_cast<size_t>(ng));
     for (int g = 0; g < ng; ++g) {
         const MoleculeCfg& mol = model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(g)])];
-        flags[static_cast<size_t>(g)] = (mol.atomic ? 0 : 1) | (mol.rigid ? 2 : 0);
+        flags[static_cast<size_t>(g)] = (mol.atomic ? 0 : 1) | (mol.rigid ? 2 : 0) | (g_molid_[static_cast<size_t>(g)] << 8);
+    }
+    (void)hipFree(d_g2g_);
+    d_g2g_ = nullptr;
+    if (model_.has_g2g()) {
+        check_hip(hipMalloc(&d_g2g_, sizeof(double) * model_.g2g2.size()), "hipMalloc g2g cutoffs");
+        check_hip(hipMemcpy(d_g2g_, model_.g2g2.data(), sizeof(double) * model_.g2g2.size(), hipMemcpyHostToDevice),
+                  "copy g2g cutoffs");
     }
     (void)hipFree(d_gflags_);
     (void)hipFree(d_gcm_);
@@ -874,9 +885,38 @@ void Engine::regrid(int d) {
     cell_divisions_ = d; // the next reset_positions builds the grid
 }
 
+// Distance beyond which no pair of atoms interacts: the pair potentials' range, or with
+// cutoff_g2g in a system made only of rigid molecules the largest group cutoff plus the
+// two largest distances of an atom from its group's mass centre (GroupCutoff::cut drops
+// every pair of groups whose centres are farther apart).
+double Engine::interaction_range() const {
+    double range = model_.pair_range;
+    if (!model_.has_g2g() || g_begin_.empty()) return range;
+    double rmax = 0.0;
+    for (size_t g = 0; g < g_begin_.size(); ++g) {
+        const MoleculeCfg& mol = model_.molecules[static_cast<size_t>(g_molid_[g])];
+        if (mol.atomic || !mol.rigid) return range;
+        for (int k = 0; k < g_size_[g]; ++k) {
+            const int i = g_begin_[g] + k;
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double x = h_xyz_[3 * static_cast<size_t>(i) + d] - g_cm_[3 * g + d];
+                x -= model_.box[d] * std::round(x / model_.box[d]);
+                r2 += x * x;
+            }
+            rmax = std::max(rmax, std::sqrt(r2));
+        }
+    }
+    double c2 = 0.0;
+    for (double v : model_.g2g2) c2 = std::max(c2, v);
+    // 1e-6 A above the bound: rigid rotations keep the atom distances up to rounding
+    return std::min(range, std::sqrt(c2) + 2.0 * rmax + 1e-6);
+}
+
 void Engine::build_cell_list() {
     cells_.enabled = false;
-    const double range = model_.pair_range;
+    const double range = interaction_range();
+    cell_range_ = range;
     if (!(range > 0) || !std::isfinite(range)) return;
     int nc[3];
     // cells of at least range / r with a (2r+1)^3 stencil; the finer grid needs 2r+1
@@ -1011,7 +1051,7 @@ void Engine::timing(double* total_ms, long* launches) {
 
 void Engine::launch_delta(const MoveArgs& args, int pair_blocks, int ewald_blocks, bool bond_block) {
     const Box box = make_box(model_);
-    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2, d_excl_off_, d_excl_};
+    GroupTable gt{d_gflags_, d_gcm_, d_g2g_, static_cast<int>(model_.molecules.size()), d_excl_off_, d_excl_};
     CellArgs ca{cells_.count, cells_.members, cells_.nc[0], cells_.nc[1], cells_.nc[2], cells_.cap,
                 cells_.inv_cell[0], cells_.inv_cell[1], cells_.inv_cell[2]};
     const int ewald_end = pair_blocks + ewald_blocks;
@@ -1091,7 +1131,7 @@ void Engine::delta(const std::vector<int>& groups, const std::vector<char>& all,
             }
         }
     }
-    if (!single && model_.g2g_cutoff2 < 1e300)
+    if (!single && model_.has_g2g())
         for (int gi : groups)
             if (!model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(gi)])].atomic)
                 throw std::runtime_error("delta: changes of several molecular groups with cutoff_g2g are not supported");
@@ -1130,7 +1170,7 @@ void Engine::delta(const std::vector<int>& groups, const std::vector<char>& all,
     else pair_blocks = std::max(1, std::min((n_ + kBlock * 4 - 1) / (kBlock * 4), 2 * num_cu_));
     int ewald_blocks = 0;
     if (nk_ > 0) ewald_blocks = std::max(1, std::min((nk_ + 2 * kBlock - 1) / (2 * kBlock), 2 * num_cu_));
-    if (single && any_molecular_ && model_.g2g_cutoff2 < 1e300 &&
+    if (single && any_molecular_ && model_.has_g2g() &&
         !model_.molecules[static_cast<size_t>(g_molid_[static_cast<size_t>(groups[0])])].atomic &&
         staged_cm_group_ != groups[0])
         throw std::runtime_error("delta: stage the trial mass centre of a moved molecular group when cutoff_g2g is set");
@@ -1243,7 +1283,7 @@ void Engine::total_energy(double* terms) {
     const int bond_blocks = nbonds_ > 0 ? std::min(64, (nbonds_ + kBlock - 1) / kBlock) : 0;
     double* d_part = nullptr;
     check_hip(hipMalloc(&d_part, sizeof(double) * (blocks + bond_blocks)), "hipMalloc partials");
-    GroupTable gt{d_gflags_, d_gcm_, model_.g2g_cutoff2, d_excl_off_, d_excl_};
+    GroupTable gt{d_gflags_, d_gcm_, d_g2g_, static_cast<int>(model_.molecules.size()), d_excl_off_, d_excl_};
     if (any_molecular_)
         hipLaunchKernelGGL((all_pairs_kernel<true>), dim3(blocks), dim3(kBlock), 0, stream_, d_pos_, d_meta_, n_,
                            d_pairs_, model_.ntypes, d_coul_, make_box(model_), gt, d_part);

@@ -126,6 +126,9 @@ class Engine {
     // Trial mass centre of a moved molecular group (needed with a group cutoff).
     void stage_mass_center(int group, const double* cm);
     const std::vector<double>& mass_centers() const { return g_cm_; }
+    // the range the cell list was built for (pair potentials, or cutoff_g2g over rigid molecules)
+    double cell_range() const { return cell_range_; }
+    double interaction_range() const;
     // Energies of the affected pairs/terms of `change` in both states, per term in
     // Hamiltonian order.  Writes the trial Q(k) (Ewald::updateState).
     void delta(const std::vector<int>& groups, const std::vector<char>& all, const std::vector<char>& internal,
@@ -169,6 +172,7 @@ class Engine {
     const DevCoulomb* d_coul() const { return d_coul_; }
     CellList& cells() { return cells_; }
     int* d_group_flags() { return d_gflags_; }
+    const double* d_g2g_cutoffs() { return d_g2g_; }
     double* d_mass_centers() { return d_gcm_; }
     const int* d_bond_offsets() const { return d_bond_off_; }
     const DevBond* d_bonds() const { return d_bond_; }
@@ -203,8 +207,10 @@ class Engine {
     // groups
     std::vector<int> g_begin_, g_size_, g_molid_;
     std::vector<double> g_cm_;
+    double cell_range_ = 0.0;
     bool any_molecular_ = false;
-    int* d_gflags_ = nullptr;    // bit0 molecular, bit1 rigid
+    int* d_gflags_ = nullptr;    // bit0 molecular, bit1 rigid, molecule type from bit 8
+    double* d_g2g_ = nullptr;    // cutoff_g2g: squared cutoff per pair of molecule types
     double* d_gcm_ = nullptr;    // accepted mass centres, 3 per group
     int staged_cm_group_ = -1, pending_cm_group_ = -1;
     double staged_cm_[3] = {0, 0, 0}, pending_cm_[3] = {0, 0, 0};

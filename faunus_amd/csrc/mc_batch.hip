@@ -652,7 +652,7 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.sample_every = 0;
     s.e_samples = 0;
     s.e_sum = s.e_sum2 = 0.0;
-    s.pair_range = e.model().pair_range;
+    s.pair_range = e.cells().enabled ? e.cell_range() : e.model().pair_range;
     s.cell_radius = e.cell_radius();
     s.threads = 0;
     const DevCoulomb& dc = e.dev_coulomb();
@@ -669,7 +669,6 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.beta = beta;
     if (kinds) {
         if (kinds->empty()) throw std::runtime_error("empty move set");
-        if (m.g2g_cutoff2 < 1e300) throw std::runtime_error("device MC with molecular moves needs cutoff_g2g unset");
         const auto& gb = e.group_begin();
         const auto& gs = e.group_size();
         const auto& gm = e.group_molid();
@@ -733,6 +732,11 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
         s.bonds = bonded ? e.d_bonds() : nullptr;
         s.excl_off = e.d_exclusion_offsets();
         s.excl = e.d_exclusions();
+        s.nmol = static_cast<int>(m.molecules.size());
+        s.g2g2 = e.d_g2g_cutoffs();
+        std::vector<double> mw(static_cast<size_t>(m.ntypes));
+        for (int t = 0; t < m.ntypes; ++t) mw[static_cast<size_t>(t)] = m.atoms[static_cast<size_t>(t)].mw;
+        s.mw = upload(mw, owned, "upload atom masses");
     }
     std::vector<double> terms(static_cast<size_t>(e.num_terms()));
     e.total_energy(terms.data());

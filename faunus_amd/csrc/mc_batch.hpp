@@ -76,8 +76,8 @@ struct ReplicaState {
     const int2* pivot_bonds;  // (i, j) relative to the molecule
     const int* gbegin;
     const int* gsize;
-    const int* gflags;        // bit0 molecular, bit1 rigid
-    double* gcm;              // mass centres (kept for translate+rotate targets)
+    const int* gflags;        // bit0 molecular, bit1 rigid, molecule type from bit 8
+    double* gcm;              // mass centres (translate+rotate targets; every molecular group with cutoff_g2g)
     const double* dp_atom;    // per type: the atom's dp, or -1 when the move's dp applies
     const int* bond_off;      // per-particle bond lists (bonded term), or null
     const DevBond* bonds;
@@ -85,7 +85,9 @@ struct ReplicaState {
     const int* excl;
     const float4* stab;       // FP32 forms: cubic table of S(q) (kSTable intervals), or null
     int parts;                // workgroups sharing this replica in a split general launch (1 otherwise)
-    int pad2;
+    int nmol;                 // molecule types (rows of g2g2)
+    const double* g2g2;       // cutoff_g2g: squared cutoff per pair of molecule types, or null
+    const double* mw;         // per type: the atom mass (mass centres of changed molecular groups)
     void* gslots;             // split general launch: 2 x parts exchange slots
     // energy samples (SystemEnergy, analysis.h:547-575): after every move whose index + 1 is a
     // multiple of sample_every (0: off), the running energy is added to e_sum and its square to e_sum2

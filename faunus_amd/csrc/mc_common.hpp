@@ -112,6 +112,26 @@ __host__ __device__ inline FixPos fixed_pos(double x, double y, double z, double
     return FixPos{fixed_coord(x, s), fixed_coord(y, s), fixed_coord(z, s), static_cast<float>(q)};
 }
 
+// GroupCutoff::cut (energy.h:761-768) between a moved molecular group and static group
+// gj: the groups' flag words (bit 0 molecular, molecule type from bit 8), the accepted
+// mass centres and the squared cutoff per pair of molecule types.
+struct GroupCut {
+    const int* flags;
+    const double* cm;
+    const double* table;
+    int nmol;
+    int fg; // flags of the moved group
+};
+
+// True when gj is molecular and its centre is at least the cutoff away from c (the moved
+// group's centre in one state).
+__device__ __forceinline__ bool group_cut(const GroupCut& gc, int gj, const double* c, const Box& b) {
+    const int fj = gc.flags[gj];
+    if (!(fj & 1)) return false;
+    const double* cj = gc.cm + 3 * gj;
+    return !(dist2(b, c[0], c[1], c[2], cj[0], cj[1], cj[2]) < g2g_cutoff2(gc.table, gc.nmol, gc.fg, fj));
+}
+
 // The FP32-pair form's coordinates of n particles (xyz: 3 per particle, inside the box).
 inline std::vector<FixPos> fixed_positions(const double* xyz, const std::vector<double>& q, int n, double inv) {
     std::vector<FixPos> out(static_cast<size_t>(n));

@@ -217,6 +217,9 @@ struct MovedSet {
     const int* s_mt;
     const int* excl_off;
     const int* excl;
+    const GroupCut* cut;  // cutoff_g2g against the moved molecular group, or null
+    const double* cm_new; // the moved group's mass centre (new, old state)
+    const double* cm_old;
 };
 
 // Pair sums of the moved atoms (new and old coordinates) against the particles
@@ -235,7 +238,9 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
     uint32_t mj[kB];
     double4 pj[kB];
     FixPos pf[kB];
-    int use[kB]; // 0 no pair, 1 pair, 2 pair inside the moved group (exclusions apply)
+    // bits 0-1: 0 no pair, 1 pair, 2 pair inside the moved group (exclusions apply);
+    // bit 2 / bit 3: the group pair is cut in the new / old state (cutoff_g2g)
+    int use[kB];
     int any = 0;
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
@@ -246,6 +251,10 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
         else pj[u] = in ? pos[j] : make_double4(0.0, 0.0, 0.0, 0.0);
         const bool same = static_cast<int>(mj[u] >> 8) == ms.g;
         use[u] = !in ? 0 : (same ? (ms.same_ok ? 2 : 0) : 1);
+        if (ms.cut != nullptr && use[u] == 1) {
+            const int gj = static_cast<int>(mj[u] >> 8);
+            use[u] |= (group_cut(*ms.cut, gj, ms.cm_new, box) ? 4 : 0) | (group_cut(*ms.cut, gj, ms.cm_old, box) ? 8 : 0);
+        }
         any |= use[u];
     }
     if (!any) return;
@@ -258,15 +267,17 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
             unsigned hot = 0u; // bit 2u: new state of neighbour u, bit 2u+1: old state
 #pragma unroll
             for (int u = 0; u < kB; ++u) {
-                const bool ok = kExcl ? (use[u] == 1 || (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
-                                                                                        j0 + u * kThreads)))
-                                      : use[u] != 0;
+                const int uu = use[u] & 3;
+                const bool ok = kExcl ? (uu == 1 || (uu == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
+                                                                                j0 + u * kThreads)))
+                                      : uu != 0;
+                const bool okn = ok && !(use[u] & 4), oko = ok && !(use[u] & 8);
                 const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
                 const float un = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf));
                 const float uo = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf));
-                const bool hn = ok && hot_pair(un), ho = ok && hot_pair(uo);
-                sn += (ok && !hn) ? un : 0.0f;
-                so += (ok && !ho) ? uo : 0.0f;
+                const bool hn = okn && hot_pair(un), ho = oko && hot_pair(uo);
+                sn += (okn && !hn) ? un : 0.0f;
+                so += (oko && !ho) ? uo : 0.0f;
                 hot |= ((hn ? 1u : 0u) | (ho ? 2u : 0u)) << (2 * u);
             }
             e_new += static_cast<double>(sn);
@@ -290,16 +301,17 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
             double sn = 0.0, so = 0.0;
 #pragma unroll
             for (int u = 0; u < kB; ++u) {
-                const bool ok = kExcl ? (use[u] == 1 || (use[u] == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
-                                                                                        j0 + u * kThreads)))
-                                      : use[u] != 0;
+                const int uu = use[u] & 3;
+                const bool ok = kExcl ? (uu == 1 || (uu == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
+                                                                                j0 + u * kThreads)))
+                                      : uu != 0;
                 const DevPair& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
                 const double un =
                     pair_energy_t(p, s_coul, A.w, pj[u].w, dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z), terms);
                 const double uo =
                     pair_energy_t(p, s_coul, B.w, pj[u].w, dist2(box, B.x, B.y, B.z, pj[u].x, pj[u].y, pj[u].z), terms);
-                sn += ok ? un : 0.0;
-                so += ok ? uo : 0.0;
+                sn += (ok && !(use[u] & 4)) ? un : 0.0;
+                so += (ok && !(use[u] & 8)) ? uo : 0.0;
             }
             e_new += sn;
             e_old += so;
@@ -328,6 +340,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ DevCoulomb s_coul;
     __shared__ double s_gk[kMaxGk];
     __shared__ double s_dpa[kMaxTypes];
+    __shared__ double s_mw[kMaxTypes];          // atom masses (cutoff_g2g mass centres)
+    __shared__ double s_cm[2][3];               // the moved group's mass centre, new and old state
+    __shared__ double s_cmred[4][kThreads / 64];
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
     __shared__ int s_accept, s_abort;
@@ -376,7 +391,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         s_abort = 0;
         s_nhot = 0;
     }
-    for (int i = tid; i < nt; i += kThreads) s_dpa[i] = S.dp_atom[i];
+    for (int i = tid; i < nt; i += kThreads) {
+        s_dpa[i] = S.dp_atom[i];
+        s_mw[i] = S.mw != nullptr ? S.mw[i] : 1.0;
+    }
     for (int i = tid; i < S.nkinds; i += kThreads) s_kinds[i] = S.kinds[i];
     for (int i = tid; i <= ncut * ncut && i < kMaxGk; i += kThreads) s_gk[i] = S.gk[i];
     for (int k = tid; k < nk; k += kThreads) s_kn[k] = S.kn[k];
@@ -417,6 +435,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int* __restrict__ excl_off = S.excl_off;
     const int* __restrict__ excl = S.excl;
     const int terms = S.terms;
+    const double* __restrict__ g2g2 = S.g2g2; // cutoff_g2g per pair of molecule types, or null
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
@@ -543,11 +562,76 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         }
         __syncthreads();
 
+        // ---- cutoff_g2g: the moved molecular group's mass centre in both states
+        // (GroupCutoff::cut, energy.h:761-768).  A translate+rotate carries it along; atom
+        // translations and pivots recompute it from the group's trial positions
+        // (Geometry::massCenter as translateParticle, move.cpp:225-240, and
+        // rotate_segment, chainmove.cpp:76-104, call it)
+        const int gflag = g >= 0 ? gflags[g] : 0;
+        const bool cutg = g2g2 != nullptr && m > 0 && (gflag & 1);
+        if (cutg) {
+            const double* co = (kSplit && g == prev_cg) ? prev_cm : gcm + 3 * g;
+            if (P.kind == 1) {
+                if (tid == 0)
+                    for (int d = 0; d < 3; ++d) {
+                        s_cm[0][d] = P.cm_new[d];
+                        s_cm[1][d] = co[d];
+                    }
+            } else {
+                // positions relative to the old centre (atom moves) or to the pivot's shift
+                // atom, the rotated segment in the unwrapped frame (pivots)
+                const double sx = P.kind == 0 ? co[0] : P.shift[0], sy = P.kind == 0 ? co[1] : P.shift[1],
+                             sz = P.kind == 0 ? co[2] : P.shift[2];
+                double v[4] = {0.0, 0.0, 0.0, 0.0};
+                const int gb0 = gbegin[g], gn = gsize[g];
+                for (int k = tid; k < gn; k += kThreads) {
+                    const int idx = gb0 + k;
+                    double fx, fy, fz;
+                    if (idx >= P.mlo && idx < P.mhi && P.kind == 2) {
+                        const double4 p = s_old[idx - P.mlo];
+                        const double ox = wrap(p.x - P.shift[0], L, half, inv) - P.origin[0];
+                        const double oy = wrap(p.y - P.shift[1], L, half, inv) - P.origin[1];
+                        const double oz = wrap(p.z - P.shift[2], L, half, inv) - P.origin[2];
+                        fx = P.R[0] * ox + P.R[1] * oy + P.R[2] * oz + P.origin[0];
+                        fy = P.R[3] * ox + P.R[4] * oy + P.R[5] * oz + P.origin[1];
+                        fz = P.R[6] * ox + P.R[7] * oy + P.R[8] * oz + P.origin[2];
+                    } else {
+                        const double4 p = (idx >= P.mlo && idx < P.mhi) ? s_new[idx - P.mlo] : load_pos(idx, s_prev);
+                        fx = wrap(p.x - sx, L, half, inv);
+                        fy = wrap(p.y - sy, L, half, inv);
+                        fz = wrap(p.z - sz, L, half, inv);
+                    }
+                    const double w = s_mw[meta[idx] & 0xffu];
+                    v[0] += w * fx;
+                    v[1] += w * fy;
+                    v[2] += w * fz;
+                    v[3] += w;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                if ((tid & 63) == 0)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) s_cmred[r][tid >> 6] = v[r];
+                __syncthreads();
+                if (tid == 0) {
+                    double c[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int w = 0; w < kThreads / 64; ++w)
+                        for (int r = 0; r < 4; ++r) c[r] += s_cmred[r][w];
+                    s_cm[0][0] = wrap(c[0] / c[3] + sx, L, half, inv);
+                    s_cm[0][1] = wrap(c[1] / c[3] + sy, L, half, inv);
+                    s_cm[0][2] = wrap(c[2] / c[3] + sz, L, half, inv);
+                    for (int d = 0; d < 3; ++d) s_cm[1][d] = co[d];
+                }
+            }
+            __syncthreads();
+        }
+        const GroupCut gcut{gflags, gcm, g2g2, S.nmol, gflag};
+
         // ---- nonbonded pairs: moved x static (GroupPairing, energy.h:885-1027, 1155-1224)
         double e_new = 0.0, e_old = 0.0;
-        const int gflag = g >= 0 ? gflags[g] : 0;
         const bool same_ok = P.internal && !(gflag & 2); // pairs inside the moved group
-        if (m == 1 && !kCells && excl_off == nullptr) {
+        if (m == 1 && !kCells && excl_off == nullptr && !cutg) {
             // one moved atom: the row sums with the moved coordinates in registers
             const double4 A = s_new[0], B = s_old[0];
             const int ta = s_mt[0];
@@ -562,7 +646,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (!kCells) {
                 // blocks of kB neighbours per lane (kB from 8 down to 1 as the slice
                 // runs out, so that few lanes sit idle on short slices)
-                const MovedSet ms{P.mlo, P.mhi, m, g, same_ok, s_new, s_old, s_fnew, s_fold, s_mt, excl_off, excl};
+                const MovedSet ms{P.mlo,   P.mhi,    m,     g,    same_ok,          s_new,    s_old,   s_fnew,
+                                  s_fold,  s_mt,     excl_off, excl, cutg ? &gcut : nullptr, s_cm[0], s_cm[1]};
                 auto blocks = [&](auto excl_tag) {
                     constexpr bool kX = decltype(excl_tag)::value;
                     int base = jb;
@@ -668,6 +753,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const int alo = mol ? 0 : cc[u] / (2 * ns), ahi = mol ? m : alo + 1;
                         const bool same = static_cast<int>(mj[u] >> 8) == g;
                         if (same && !same_ok) continue;
+                        if (cutg && !same && group_cut(gcut, static_cast<int>(mj[u] >> 8), s_cm[st], box)) continue;
                         const int tj = static_cast<int>(mj[u] & 0xffu);
                         double e = 0.0;
                         for (int a = alo; a < ahi; ++a) {
@@ -706,6 +792,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         const uint32_t mj = meta[j];
                         if (static_cast<int>(mj >> 8) == g &&
                             (!same_ok || pair_excluded(excl_off, excl, P.mlo + a, j)))
+                            continue;
+                        if (cutg && static_cast<int>(mj >> 8) != g && group_cut(gcut, static_cast<int>(mj >> 8), s_cm[st], box))
                             continue;
                         const int tj = static_cast<int>(mj & 0xffu);
                         double u;
@@ -917,7 +1005,11 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 ++e_samples;
             }
             if (acc && m > 0) {
-                if (P.kind == 1 && P.translate && part == 0)
+                // every part writes the (identical) centre, so each part's later reads see
+                // its own write; reads of this move's old centre precede every part's commit
+                if (cutg && P.kind != 1)
+                    for (int d = 0; d < 3; ++d) gcm[3 * g + d] = s_cm[0][d];
+                else if (P.kind == 1 && P.translate)
                     for (int d = 0; d < 3; ++d) gcm[3 * g + d] = P.cm_new[d];
                 if (kCells) {
                     auto cell_at = [&](const double4& A) {

@@ -559,8 +559,40 @@ double SelfEnergyTerm::energy(const Space& spc, const Space*, const Change& c) {
 // ---------------------------------------------------------------- Nonbonded
 // GroupCutoff::cut, energy.h:761-768.
 bool NonbondedTerm::cut(const Space& spc, const Group& g1, const Group& g2) const {
-    if (g1.atomic || g2.atomic) return false;
-    return spc.geometry.sqdist(g1.mass_center, g2.mass_center) >= default_g2g_cutoff2;
+    if (g1.atomic || g2.atomic || g2g_cutoff2.empty()) return false;
+    const size_t nmol = molecules->size();
+    return spc.geometry.sqdist(g1.mass_center, g2.mass_center) >=
+           g2g_cutoff2[static_cast<size_t>(g1.molid) * nmol + static_cast<size_t>(g2.molid)];
+}
+
+// from_json(GroupCutoff) (energy.cpp:1897-1933): a number is one cutoff for every pair of
+// molecule types; an object holds "default" (no cutoff if absent) and "A B" pairs of
+// molecule names.
+std::vector<double> parse_g2g(const Value& c, const std::vector<MoleculeType>& mols) {
+    const size_t nmol = mols.size();
+    const double big = std::numeric_limits<double>::max();
+    auto squared = [&](double rc) { return rc >= std::sqrt(big) ? big : rc * rc; };
+    std::vector<double> t(nmol * nmol, big);
+    if (c.is_number()) {
+        std::fill(t.begin(), t.end(), squared(c.as_number()));
+        return t;
+    }
+    if (!c.is_object()) throw std::runtime_error("cutoff_g2g: a number or an object");
+    if (c.contains("default")) std::fill(t.begin(), t.end(), squared(c.at("default").as_number()));
+    auto find = [&](const std::string& name) {
+        for (size_t m = 0; m < nmol; ++m)
+            if (mols[m].name == name) return m;
+        throw std::runtime_error("cutoff_g2g: unknown molecule '" + name + "'");
+    };
+    for (auto& [key, v] : c.members()) {
+        if (key == "default") continue;
+        const auto sp = key.find(' ');
+        if (sp == std::string::npos || key.find(' ', sp + 1) != std::string::npos)
+            throw std::runtime_error("cutoff_g2g: invalid molecules names '" + key + "'");
+        const size_t a = find(key.substr(0, sp)), b = find(key.substr(sp + 1));
+        t[a * nmol + b] = t[b * nmol + a] = squared(v.as_number());
+    }
+    return t;
 }
 
 double NonbondedTerm::energy(const Space& spc, const Space*, const Change& change) {
@@ -864,11 +896,7 @@ std::unique_ptr<Hamiltonian> System::build_hamiltonian() {
         } else {
             throw std::runtime_error("energy term '" + key + "' is not supported by this path");
         }
-        if (value.contains("cutoff_g2g")) {
-            const Value& c = value.at("cutoff_g2g");
-            const double rc = c.is_number() ? c.as_number() : c.at("default").as_number();
-            nb->default_g2g_cutoff2 = rc * rc;
-        }
+        if (value.contains("cutoff_g2g")) nb->g2g_cutoff2 = parse_g2g(value.at("cutoff_g2g"), h->molecules);
         if (pp.has_self_energy) { // PairEnergy::addPairPotentialSelfEnergy, energy.h:462-468
             auto self = std::make_unique<SelfEnergyTerm>();
             self->name = "selfenergy";

@@ -349,7 +349,9 @@ struct EwaldTerm : Term {
 
 struct NonbondedTerm : Term {
     PairPotential pot;
-    double default_g2g_cutoff2 = std::numeric_limits<double>::max();
+    // GroupCutoff::cutoff_squared (energy.h:746-768): squared cutoff per pair of molecule
+    // types (nmol x nmol); empty means no cutoff
+    std::vector<double> g2g_cutoff2;
     const std::vector<MoleculeType>* molecules = nullptr;
     double energy(const Space& spc, const Space* old_spc, const Change& c) override;
     double pair(const Space& spc, const Particle& a, const Particle& b) const {

@@ -284,3 +284,87 @@ def polyelectrolyte(n_beads, n_total, seed, kmax=8, dp=10.0, bead_dp=2.0, dprot=
     moves = [dict(kind=0, molid=1, weight=n_ions, dp=dp), dict(kind=0, molid=0, weight=n_beads, dp=bead_dp),
              dict(kind=2, molid=0, weight=max(1, n_beads - 1), dprot=dprot)]
     return cfg, xyz, q, ids, groups, moves
+
+
+def water_g2g(n_mol=1000, box=None, seed=1, alpha=0.2, cutoff=9.0, kmax=7, g2g=10.0, dp=0.5, dprot=0.5):
+    """Rigid water written as the reference's water example writes it
+    (examples/water/ewald.yml): nonbonded_coulomblj with LB-mixed LJ that only the
+    group-to-group cutoff truncates (cutoff_g2g on the O-H-H mass centres), Ewald
+    real space cut at `cutoff`; geometry, charges and moves as water()."""
+    cfg, xyz, q, ids, groups, moves = water(n_mol=n_mol, box=box, seed=seed, alpha=alpha, cutoff=cutoff, kmax=kmax,
+                                            dp=dp, dprot=dprot)
+    cfg["energy"] = [{"nonbonded_coulomblj": {
+        "lennardjones": {"mixing": "LB"},
+        "coulomb": {"type": "ewald", "epsr": 1.0, "cutoff": cutoff, "alpha": alpha, "ncutoff": kmax},
+        "cutoff_g2g": g2g}}]
+    return cfg, xyz, q, ids, groups, moves
+
+
+def chains_g2g(n_chains=6, n_beads=12, box=70.0, seed=1, kmax=6, g2g=25.0, dp=6.0, bead_dp=2.0, dprot=2.0,
+               bond_k=0.5, bond_req=4.0):
+    """Several bead-spring chains (q=-1, sigma=4 A, harmonic bonds) with their counterions
+    (q=+1), hard spheres + Ewald (alpha=5/L, Rc=L/2), and a group-to-group cutoff between
+    the chains (cutoff_g2g: {default: g2g}); the counterions form one atomic group, which
+    the cutoff never cuts.  Moves: ion and bead translations and pivots, weighted by the
+    number of ions, beads and bonds."""
+    bonds = [{"harmonic": {"index": [i, i + 1], "k": bond_k, "req": bond_req}} for i in range(n_beads - 1)]
+    cfg = {
+        "temperature": 298.15,
+        "geometry": {"type": "cuboid", "length": box},
+        "atomlist": [{"P": {"q": -1.0, "sigma": 4.0, "dp": bead_dp, "mw": 2.0}}, {"C": {"q": 1.0, "sigma": 4.0, "dp": dp}}],
+        "moleculelist": [{"chain": {"atoms": ["P"] * n_beads, "bondlist": bonds}},
+                         {"ions": {"atoms": ["C"], "atomic": True}}],
+        "energy": [{"bonded": {}},
+                   {"nonbonded": {"default": [
+                       {"hardsphere": {}},
+                       {"coulomb": {"type": "ewald", "epsr": 78.7, "cutoff": box / 2, "alpha": 5.0 / box,
+                                    "ncutoff": kmax}}],
+                       "cutoff_g2g": {"default": g2g}}}],
+    }
+    rng = np.random.default_rng(seed)
+    placed = []
+    step = bond_req * 1.05
+    for c in range(n_chains):
+        for _ in range(1000):
+            start = (rng.random(3) - 0.5) * box
+            chain = [start]
+            ok = True
+            for i in range(1, n_beads):
+                for _ in range(200):
+                    v = rng.normal(size=3)
+                    v /= np.linalg.norm(v)
+                    p = chain[-1] + step * v
+                    pw = p - box * np.round(p / box)
+                    near = [x for x in placed + chain[:-1]]
+                    if near:
+                        d = np.array(near) - pw
+                        d -= box * np.round(d / box)
+                        if (np.einsum("ij,ij->i", d, d) < 4.4 ** 2).any():
+                            continue
+                    chain.append(p)
+                    break
+                else:
+                    ok = False
+                    break
+            if ok:
+                placed.extend([p - box * np.round(p / box) for p in chain])
+                break
+        else:
+            raise RuntimeError("chain placement failed")
+    beads = np.array(placed)
+    n_ions = n_chains * n_beads
+    ions = []
+    while len(ions) < n_ions:
+        p = (rng.random(3) - 0.5) * box
+        d = np.concatenate([beads, np.array(ions).reshape(-1, 3)]) - p
+        d -= box * np.round(d / box)
+        if (np.einsum("ij,ij->i", d, d) >= 4.4 ** 2).all():
+            ions.append(p)
+    xyz = np.concatenate([beads, np.array(ions)])
+    ids = np.concatenate([np.zeros(n_ions, np.int32), np.ones(n_ions, np.int32)])
+    q = np.where(ids == 1, 1.0, -1.0)
+    begin = [c * n_beads for c in range(n_chains)] + [n_chains * n_beads]
+    groups = dict(begin=begin, size=[n_beads] * n_chains + [n_ions], molid=[0] * n_chains + [1])
+    moves = [dict(kind=0, molid=1, weight=n_ions, dp=dp), dict(kind=0, molid=0, weight=n_ions, dp=bead_dp),
+             dict(kind=2, molid=0, weight=n_chains * (n_beads - 1), dprot=dprot)]
+    return cfg, xyz, q, ids, groups, moves
