@@ -157,9 +157,10 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                     if (kF32) {
                         const FixPos pj = pos32[j];
                         const FixPos& A = st == 0 ? fa : fb;
+                        bool hot;
                         const float uf = pair_energy_ft(s_pairs_f[ti * nt + tj], cf, s_stab, S.terms, A.q, pj.q,
-                                                        dist2_fix(A, pj, hf));
-                        u = (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, st)) ? 0.0 : static_cast<double>(uf);
+                                                        dist2_fix(A, pj, hf), hot);
+                        u = (hot && push_hot(&s_nhot, s_hot, j, st)) ? 0.0 : static_cast<double>(uf);
                     } else {
                         const double4 pj = pos[j];
                         Box b{{L, L, L}, {half, half, half}};
@@ -532,9 +533,10 @@ __global__ void pair_probe_kernel(const DevPair* __restrict__ pairs, const DevCo
     u64[i] = pair_energy(p, *coul, qa, qb, r2[i]);
     const DevPairF pf = to_f32(p);
     const CoulF cf = coul_f32(*coul);
+    bool hot;
     const float uf = pair_energy_ft(pf, cf, stab, terms, static_cast<float>(qa), static_cast<float>(qb),
-                                    static_cast<float>(r2[i]));
-    u32[i] = hot_pair(uf) ? pair_energy_t(p, *coul, qa, qb, r2[i], terms) : static_cast<double>(uf);
+                                    static_cast<float>(r2[i]), hot);
+    u32[i] = hot ? pair_energy_t(p, *coul, qa, qb, r2[i], terms) : static_cast<double>(uf);
 }
 } // namespace
 

@@ -24,12 +24,25 @@ __global__ void zero_words_kernel(unsigned long long* __restrict__ p, int words)
 
 struct alignas(16) DevPairF {
     float lj_s2, lj_e4, wca_s2, wca_e4, hs_s2, coul;
-    int spl, pad;
+    int spl;
+    float hot_r2; // below this r^2 the pair is evaluated again in FP64 (hot_radius2)
 };
+
+constexpr double kHotRepulsion = 8.0; // kT
+
+// r^2 below which the pair's LJ or WCA repulsion e4 (s2/r2)^6 exceeds kHotRepulsion:
+// there FP32 rounding of r^2, grown sixfold by r^-12, would dominate the move's error.
+__device__ __forceinline__ double hot_radius2(const DevPair& p) {
+    double r2 = 0.0;
+    if (p.lj_e4 > 0.0) r2 = fmax(r2, p.lj_s2 * sqrt(cbrt(p.lj_e4 / kHotRepulsion)));
+    if (p.wca_e4 > 0.0) r2 = fmax(r2, p.wca_s2 * sqrt(cbrt(p.wca_e4 / kHotRepulsion)));
+    return r2;
+}
 
 __device__ __forceinline__ DevPairF to_f32(const DevPair& p) {
     return DevPairF{static_cast<float>(p.lj_s2), static_cast<float>(p.lj_e4), static_cast<float>(p.wca_s2),
-                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2), static_cast<float>(p.coul), p.spl, 0};
+                    static_cast<float>(p.wca_e4), static_cast<float>(p.hs_s2), static_cast<float>(p.coul), p.spl,
+                    static_cast<float>(hot_radius2(p))};
 }
 
 // An empty asm statement that takes and returns the values: the loads that produce
@@ -148,17 +161,19 @@ __device__ __forceinline__ float dist2_fix(const FixPos& a, const FixPos& b, flo
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Pairs whose FP32 energy exceeds kHotPair kT (overlaps, +inf and NaN included) are
-// evaluated again in FP64: FP32 rounding of r^2 grows sixfold through r^-12, and one
-// overlapping pair would otherwise set the error of the move's dU.  The pair loops
-// queue them in LDS (the neighbour, and moved atom << 1 | state with state 0 new and 1
-// old); after the loops each queued pair is evaluated from the FP64 coordinates, so the
-// FP64 pair code stays out of the loops' registers.  Beyond kHotCap entries a move's
-// further hot pairs keep their FP32 energy.
-constexpr float kHotPair = 8.0f;
+// Hot pairs are evaluated again in FP64: pairs inside their type pair's hot radius
+// (hot_radius2), and hard-sphere pairs within 2^-20 of contact (r^2 relative), where the
+// FP32 and FP64 distances may fall on different sides; their FP32 energy is the finite
+// kHsBand so that block sums stay finite (elsewhere an FP32 overlap is +inf in both
+// forms).  The pair loops queue hot pairs in LDS (the neighbour, and moved atom
+// << 1 | state with state 0 new and 1 old); after the loops each queued pair is
+// evaluated from the FP64 coordinates, so the FP64 pair code stays out of the loops'
+// registers.  Loops that sum blocks of neighbours per lane queue all pairs of a block
+// that holds a hot pair, or none; a pair or block that finds the queue full keeps its
+// FP32 energy.
 constexpr int kHotCap = 256;
-
-__device__ __forceinline__ bool hot_pair(float u) { return !(__builtin_fabsf(u) <= kHotPair); }
+constexpr float kHsSure = 0.99999904632568359375f; // 1 - 2^-20
+constexpr float kHsBand = 1e30f; // FP32 energy of a hard-sphere pair near contact (finite, hot)
 
 // Queue one hot pair; false when the queue is full.
 __device__ __forceinline__ bool push_hot(int* n, int2* q, int j, int tag) {
@@ -168,13 +183,28 @@ __device__ __forceinline__ bool push_hot(int* n, int2* q, int j, int tag) {
     return true;
 }
 
+// Reserve cnt consecutive queue entries, all or none (the count never passes the
+// capacity, so every entry below it has been written once the loops end); -1 when full.
+__device__ __forceinline__ int reserve_hot(int* n, int cnt) {
+    int cur = __hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (true) {
+        if (cur + cnt > kHotCap) return -1;
+        const int prev = atomicCAS(n, cur, cur + cnt);
+        if (prev == cur) return cur;
+        cur = prev;
+    }
+}
+
+// FP32 pair energy; `hot` tells whether the pair must be evaluated again in FP64.
 __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& c, const float4* __restrict__ tab,
-                                                int terms, float qa, float qb, float r2) {
+                                                int terms, float qa, float qb, float r2, bool& hot) {
     // both halves of the pair parameters in two vector loads, then register arithmetic
     float4 pa = reinterpret_cast<const float4*>(&p)[0]; // lj_s2, lj_e4, wca_s2, wca_e4
-    float4 pb = reinterpret_cast<const float4*>(&p)[1]; // hs_s2, coul, spl (bits), -
+    float4 pb = reinterpret_cast<const float4*>(&p)[1]; // hs_s2, coul, spl (bits), hot_r2
     materialize(pa);
     materialize(pb);
+    hot = r2 < pb.w;
+    if (terms & kTermHS) hot = hot || (r2 < pb.x && !(r2 < pb.x * kHsSure));
     if (c.spl != nullptr) {
         // nonbonded_splined: the table is evaluated in FP64 from the FP32 distance
         bool below;
@@ -195,7 +225,7 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
             u += (r2 > pa.z * 1.2599210498948732f) ? 0.0f : pa.w * (w * w - w + 0.25f);
         }
     }
-    if (terms & kTermHS) u += (r2 < pb.x) ? __builtin_huge_valf() : 0.0f;
+    if (terms & kTermHS) u += (r2 < pb.x) ? ((r2 < pb.x * kHsSure) ? __builtin_huge_valf() : kHsBand) : 0.0f;
     if (c.scheme != kCoulNone) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float e;
@@ -239,34 +269,46 @@ __device__ __forceinline__ void pair_row_f32(const FixPos* __restrict__ pos32, c
             mj[u] = j < n ? meta[j] : 0u;
         }
         float sn = 0.0f, so = 0.0f;
-        unsigned hot = 0u; // bit 2u: new state of neighbour u, bit 2u+1: old state
+        bool anyhot = false;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int j = j0 + u * kThreads;
             const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
             const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
-            const float un = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(a, pj[u], h));
-            const float uo = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(b, pj[u], h));
-            const bool hn = ok && hot_pair(un), ho = ok && hot_pair(uo);
-            sn += (ok && !hn) ? un : 0.0f;
-            so += (ok && !ho) ? uo : 0.0f;
-            hot |= ((hn ? 1u : 0u) | (ho ? 2u : 0u)) << (2 * u);
+            bool hn, ho;
+            const float un = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(a, pj[u], h), hn);
+            const float uo = pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(b, pj[u], h), ho);
+            sn += ok ? un : 0.0f;
+            so += ok ? uo : 0.0f;
+            anyhot = anyhot || (ok && (hn || ho));
+        }
+        if (anyhot) {
+            // a hot pair in this block: every pair of the block goes to the queue for
+            // FP64 (a state whose block sum is +inf, an overlap, keeps it)
+            const bool fin_n = sn != __builtin_huge_valf(), fin_o = so != __builtin_huge_valf();
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int j = j0 + u * kThreads;
+                const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
+                cnt += ok ? (fin_n ? 1 : 0) + (fin_o ? 1 : 0) : 0;
+            }
+            const int base = cnt > 0 ? reserve_hot(hot_n, cnt) : -1;
+            if (base >= 0) {
+                int k = base;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int j = j0 + u * kThreads;
+                    const bool ok = j < n && j != skip && (static_cast<int>(mj[u] >> 8) != g || same_ok);
+                    if (ok && fin_n) hot_q[k++] = make_int2(j, tag);
+                    if (ok && fin_o) hot_q[k++] = make_int2(j, tag | 1);
+                }
+                if (fin_n) sn = 0.0f;
+                if (fin_o) so = 0.0f;
+            }
         }
         e_new += static_cast<double>(sn);
         e_old += static_cast<double>(so);
-        if (hot) {
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const unsigned m2 = (hot >> (2 * u)) & 3u;
-                if (m2 == 0u) continue;
-                const int j = j0 + u * kThreads;
-                const DevPairF& p = row[static_cast<int>(mj[u] & 0xffu)];
-                if ((m2 & 1u) && !push_hot(hot_n, hot_q, j, tag))
-                    e_new += pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(a, pj[u], h));
-                if ((m2 & 2u) && !push_hot(hot_n, hot_q, j, tag | 1))
-                    e_old += pair_energy_ft(p, c, tab, terms, a.q, pj[u].q, dist2_fix(b, pj[u], h));
-            }
-        }
     }
 }
 

@@ -212,12 +212,13 @@ struct MovedSet {
     bool same_ok;
     const double4* s_new;
     const double4* s_old;
-    const FixPos* s_fnew; // FP32 form: fixed-point copies of s_new and s_old
-    const FixPos* s_fold;
+    const FixPos* s_fnew; // FP32 form: fixed-point copies of s_new and s_old in LDS, or null
+    const FixPos* s_fold; //   (then converted here from s_new and s_old with inv = 1 / L)
+    double inv;
     const int* s_mt;
     const int* excl_off;
     const int* excl;
-    const GroupCut* cut;  // cutoff_g2g against the moved molecular group, or null
+    GroupCut cut;         // cutoff_g2g against the moved molecular group (kCut)
     const double* cm_new; // the moved group's mass centre (new, old state)
     const double* cm_old;
 };
@@ -228,7 +229,7 @@ struct MovedSet {
 // the FP32 form, hot pairs go to the queue (hot_n, hot_q).
 // kExcl: the system has pair exclusions (looked up per pair inside the moved group);
 // without them the loop over the neighbours is straight-line code.
-template <int kThreads, int kB, bool kF32, bool kExcl>
+template <int kThreads, int kB, bool kF32, bool kExcl, bool kCut>
 __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms, const uint32_t* __restrict__ meta,
                                             const FixPos* __restrict__ pos32, const double4* __restrict__ pos,
                                             const DevPairF* s_pairs_f, const DevPair* s_pairs, int nt, const CoulF& cf,
@@ -251,9 +252,9 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
         else pj[u] = in ? pos[j] : make_double4(0.0, 0.0, 0.0, 0.0);
         const bool same = static_cast<int>(mj[u] >> 8) == ms.g;
         use[u] = !in ? 0 : (same ? (ms.same_ok ? 2 : 0) : 1);
-        if (ms.cut != nullptr && use[u] == 1) {
+        if (kCut && use[u] == 1) {
             const int gj = static_cast<int>(mj[u] >> 8);
-            use[u] |= (group_cut(*ms.cut, gj, ms.cm_new, box) ? 4 : 0) | (group_cut(*ms.cut, gj, ms.cm_old, box) ? 8 : 0);
+            use[u] |= (group_cut(ms.cut, gj, ms.cm_new, box) ? 4 : 0) | (group_cut(ms.cut, gj, ms.cm_old, box) ? 8 : 0);
         }
         any |= use[u];
     }
@@ -262,39 +263,66 @@ __device__ __forceinline__ void moved_block(int base, int je, const MovedSet& ms
         const int ta = ms.s_mt[a];
         if (kF32) {
             const DevPairF* row = s_pairs_f + ta * nt;
-            const FixPos fa = ms.s_fnew[a], fb = ms.s_fold[a];
-            float sn = 0.0f, so = 0.0f;
-            unsigned hot = 0u; // bit 2u: new state of neighbour u, bit 2u+1: old state
-#pragma unroll
-            for (int u = 0; u < kB; ++u) {
+            FixPos fa, fb;
+            if (ms.s_fnew != nullptr) {
+                fa = ms.s_fnew[a];
+                fb = ms.s_fold[a];
+            } else {
+                const double4 A = ms.s_new[a], B = ms.s_old[a];
+                fa = fixed_pos(A.x, A.y, A.z, A.w, ms.inv);
+                fb = fixed_pos(B.x, B.y, B.z, B.w, ms.inv);
+            }
+            auto pair_ok = [&](int u, bool& okn, bool& oko) {
                 const int uu = use[u] & 3;
                 const bool ok = kExcl ? (uu == 1 || (uu == 2 && !pair_excluded(ms.excl_off, ms.excl, ms.mlo + a,
                                                                                 j0 + u * kThreads)))
                                       : uu != 0;
-                const bool okn = ok && !(use[u] & 4), oko = ok && !(use[u] & 8);
+                okn = ok && !(use[u] & 4);
+                oko = ok && !(use[u] & 8);
+                return ok;
+            };
+            float sn = 0.0f, so = 0.0f;
+            bool anyhot = false;
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                bool okn, oko;
+                const bool ok = pair_ok(u, okn, oko);
                 const DevPairF& p = row[ok ? static_cast<int>(mj[u] & 0xffu) : 0];
-                const float un = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf));
-                const float uo = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf));
-                const bool hn = okn && hot_pair(un), ho = oko && hot_pair(uo);
-                sn += (okn && !hn) ? un : 0.0f;
-                so += (oko && !ho) ? uo : 0.0f;
-                hot |= ((hn ? 1u : 0u) | (ho ? 2u : 0u)) << (2 * u);
+                bool hn, ho;
+                const float un = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf), hn);
+                const float uo = pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf), ho);
+                sn += okn ? un : 0.0f;
+                so += oko ? uo : 0.0f;
+                anyhot = anyhot || (okn && hn) || (oko && ho);
+            }
+            if (anyhot) {
+                // a hot pair in this block: every pair of the block goes to the queue for
+                // FP64 (a state whose block sum is +inf, an overlap, keeps it)
+                const bool fin_n = sn != __builtin_huge_valf(), fin_o = so != __builtin_huge_valf();
+                int cnt = 0;
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    bool okn, oko;
+                    pair_ok(u, okn, oko);
+                    cnt += (okn && fin_n ? 1 : 0) + (oko && fin_o ? 1 : 0);
+                }
+                const int base = cnt > 0 ? reserve_hot(hot_n, cnt) : -1;
+                if (base >= 0) {
+                    int k = base;
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        bool okn, oko;
+                        pair_ok(u, okn, oko);
+                        const int j = j0 + u * kThreads;
+                        if (okn && fin_n) hot_q[k++] = make_int2(j, a << 1);
+                        if (oko && fin_o) hot_q[k++] = make_int2(j, (a << 1) | 1);
+                    }
+                    if (fin_n) sn = 0.0f;
+                    if (fin_o) so = 0.0f;
+                }
             }
             e_new += static_cast<double>(sn);
             e_old += static_cast<double>(so);
-            if (hot) {
-#pragma unroll
-                for (int u = 0; u < kB; ++u) {
-                    const unsigned m2 = (hot >> (2 * u)) & 3u;
-                    if (m2 == 0u) continue;
-                    const int j = j0 + u * kThreads;
-                    const DevPairF& p = row[static_cast<int>(mj[u] & 0xffu)];
-                    if ((m2 & 1u) && !push_hot(hot_n, hot_q, j, a << 1))
-                        e_new += pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fa, pf[u], hf));
-                    if ((m2 & 2u) && !push_hot(hot_n, hot_q, j, (a << 1) | 1))
-                        e_old += pair_energy_ft(p, cf, s_stab, terms, fa.q, pf[u].q, dist2_fix(fb, pf[u], hf));
-                }
-            }
         } else {
             const double4 A = ms.s_new[a], B = ms.s_old[a];
             const DevPair* row = s_pairs + ta * nt;
@@ -324,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                                                                const long long* __restrict__ moves,
                                                                TraceEntry* __restrict__ trace, int trace_cap,
                                                                const int2* __restrict__ blockmap,
-                                                               long long spin_limit) {
+                                                               long long spin_limit, int fix_lds) {
     static_assert(!(kSplit && kCells), "the split form walks all particles");
     // block -> (replica, part); without split every block is part 0 of its own replica
     const int2 bm = kSplit ? blockmap[blockIdx.x] : make_int2(static_cast<int>(blockIdx.x), 0);
@@ -343,6 +371,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     __shared__ double s_mw[kMaxTypes];          // atom masses (cutoff_g2g mass centres)
     __shared__ double s_cm[2][3];               // the moved group's mass centre, new and old state
     __shared__ double s_cmred[4][kThreads / 64];
+    __shared__ GeneralMove s_P;                 // the current move's parameters
     __shared__ MoveKindDev s_kinds[kMaxKinds];
     __shared__ double s_red[5][kThreads / 64];
     __shared__ int s_accept, s_abort;
@@ -375,9 +404,12 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const int max_moved = S.max_moved;
     double4* s_nbuf = reinterpret_cast<double4*>(s_phase + kChunk * 6 * width);
     double4* s_old = s_nbuf + (kSplit ? 2 : 1) * max_moved;
+    // fixed-point copies of the moved atoms for the FP32 pair paths when the launch's
+    // LDS holds them (fix_lds); otherwise they are converted where they are used
+    const bool fixlds = kF32 && fix_lds != 0;
     FixPos* s_fnew = reinterpret_cast<FixPos*>(s_old + max_moved);
-    FixPos* s_fold = s_fnew + (kF32 ? max_moved : 0);
-    int* s_mt = reinterpret_cast<int*>(s_fold + (kF32 ? max_moved : 0));
+    FixPos* s_fold = s_fnew + (fixlds ? max_moved : 0);
+    int* s_mt = reinterpret_cast<int*>(s_fold + (fixlds ? max_moved : 0));
 
     for (int i = tid; i < nt * nt; i += kThreads) {
         s_pairs[i] = S.pairs[i];
@@ -465,102 +497,107 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         double4* s_new = s_nbuf + (kSplit ? (t & 1) * max_moved : 0);
         const double4* s_prev = s_nbuf + (kSplit ? ((t + 1) & 1) * max_moved : 0);
         const MoveDraw& D = s_draw[slot];
-        GeneralMove P;
-        P.accept_u = D.accept_u;
-        P.kind = D.kind;
-        P.group = D.group;
-        P.mlo = D.mlo;
-        P.mhi = D.mhi;
-        P.internal = D.internal;
-        P.translate = (D.flags & 1) != 0;
-        P.rotate = (D.flags & 2) != 0;
-        P.scale = D.scale;
-        P.dir[0] = D.dir[0];
-        P.dir[1] = D.dir[1];
-        P.dir[2] = D.dir[2];
-        if (P.kind == 1) {
-            const int g = P.group;
+        GeneralMove Pm;
+        Pm.accept_u = D.accept_u;
+        Pm.kind = D.kind;
+        Pm.group = D.group;
+        Pm.mlo = D.mlo;
+        Pm.mhi = D.mhi;
+        Pm.internal = D.internal;
+        Pm.translate = (D.flags & 1) != 0;
+        Pm.rotate = (D.flags & 2) != 0;
+        Pm.scale = D.scale;
+        Pm.dir[0] = D.dir[0];
+        Pm.dir[1] = D.dir[1];
+        Pm.dir[2] = D.dir[2];
+        if (Pm.kind == 1) {
+            const int g = Pm.group;
             if (kSplit && g == prev_cg) {
-                P.cm_new[0] = prev_cm[0];
-                P.cm_new[1] = prev_cm[1];
-                P.cm_new[2] = prev_cm[2];
+                Pm.cm_new[0] = prev_cm[0];
+                Pm.cm_new[1] = prev_cm[1];
+                Pm.cm_new[2] = prev_cm[2];
             } else {
-                P.cm_new[0] = gcm[3 * g];
-                P.cm_new[1] = gcm[3 * g + 1];
-                P.cm_new[2] = gcm[3 * g + 2];
+                Pm.cm_new[0] = gcm[3 * g];
+                Pm.cm_new[1] = gcm[3 * g + 1];
+                Pm.cm_new[2] = gcm[3 * g + 2];
             }
-            if (P.translate) {
-                P.cm_new[0] = wrap(P.cm_new[0] + P.dir[0], L, half, inv);
-                P.cm_new[1] = wrap(P.cm_new[1] + P.dir[1], L, half, inv);
-                P.cm_new[2] = wrap(P.cm_new[2] + P.dir[2], L, half, inv);
+            if (Pm.translate) {
+                Pm.cm_new[0] = wrap(Pm.cm_new[0] + Pm.dir[0], L, half, inv);
+                Pm.cm_new[1] = wrap(Pm.cm_new[1] + Pm.dir[1], L, half, inv);
+                Pm.cm_new[2] = wrap(Pm.cm_new[2] + Pm.dir[2], L, half, inv);
             }
-            if (P.rotate)
-                for (int e = 0; e < 9; ++e) P.R[e] = D.R[e];
-        } else if (P.kind == 2) {
+            if (Pm.rotate)
+                for (int e = 0; e < 9; ++e) Pm.R[e] = D.R[e];
+        } else if (Pm.kind == 2) {
             const double4 pa = load_pos(D.atom0, s_prev), pb = load_pos(D.atom1, s_prev);
-            P.shift[0] = pa.x;
-            P.shift[1] = pa.y;
-            P.shift[2] = pa.z;
-            P.origin[0] = wrap(pa.x - P.shift[0], L, half, inv);
-            P.origin[1] = wrap(pa.y - P.shift[1], L, half, inv);
-            P.origin[2] = wrap(pa.z - P.shift[2], L, half, inv);
-            double v[3] = {P.origin[0] - wrap(pb.x - P.shift[0], L, half, inv),
-                           P.origin[1] - wrap(pb.y - P.shift[1], L, half, inv),
-                           P.origin[2] - wrap(pb.z - P.shift[2], L, half, inv)};
+            Pm.shift[0] = pa.x;
+            Pm.shift[1] = pa.y;
+            Pm.shift[2] = pa.z;
+            Pm.origin[0] = wrap(pa.x - Pm.shift[0], L, half, inv);
+            Pm.origin[1] = wrap(pa.y - Pm.shift[1], L, half, inv);
+            Pm.origin[2] = wrap(pa.z - Pm.shift[2], L, half, inv);
+            double v[3] = {Pm.origin[0] - wrap(pb.x - Pm.shift[0], L, half, inv),
+                           Pm.origin[1] - wrap(pb.y - Pm.shift[1], L, half, inv),
+                           Pm.origin[2] - wrap(pb.z - Pm.shift[2], L, half, inv)};
             for (int d = 0; d < 3; ++d) v[d] = min_image(v[d], L, half);
             const double norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
             const double u[3] = {v[0] / norm, v[1] / norm, v[2] / norm};
-            rotation_matrix_sc(u, D.R[0], D.R[1], P.R);
+            rotation_matrix_sc(u, D.R[0], D.R[1], Pm.R);
         }
-        const int m = P.mhi - P.mlo;
-        const int g = P.group;
+        const int m = Pm.mhi - Pm.mlo;
+        const int g = Pm.group;
+        // the move's parameters in LDS for the rest of the move (uniform values that
+        // would otherwise hold registers through the pair loops); read after the barrier
+        // that ends the trial coordinates
+        if (tid == 0) s_P = Pm;
 
         // ---- trial coordinates of the moved atoms
         for (int a = tid; a < m; a += kThreads) {
-            const int idx = P.mlo + a;
+            const int idx = Pm.mlo + a;
             const double4 p = load_pos(idx, s_prev);
             s_old[a] = p;
             s_mt[a] = static_cast<int>(meta[idx] & 0xffu);
             double x = p.x, y = p.y, z = p.z;
-            if (P.kind == 0) {
-                x = wrap(x + P.dir[0] * P.scale, L, half, inv);
-                y = wrap(y + P.dir[1] * P.scale, L, half, inv);
-                z = wrap(z + P.dir[2] * P.scale, L, half, inv);
-            } else if (P.kind == 1) {
-                if (P.translate) { // Group::translate (group.cpp:115-123)
-                    x = wrap(x + P.dir[0], L, half, inv);
-                    y = wrap(y + P.dir[1], L, half, inv);
-                    z = wrap(z + P.dir[2], L, half, inv);
+            if (Pm.kind == 0) {
+                x = wrap(x + Pm.dir[0] * Pm.scale, L, half, inv);
+                y = wrap(y + Pm.dir[1] * Pm.scale, L, half, inv);
+                z = wrap(z + Pm.dir[2] * Pm.scale, L, half, inv);
+            } else if (Pm.kind == 1) {
+                if (Pm.translate) { // Group::translate (group.cpp:115-123)
+                    x = wrap(x + Pm.dir[0], L, half, inv);
+                    y = wrap(y + Pm.dir[1], L, half, inv);
+                    z = wrap(z + Pm.dir[2], L, half, inv);
                 }
-                if (P.rotate) { // Geometry::rotate about the mass centre (geometry.h:613-626)
-                    x = wrap(x - P.cm_new[0], L, half, inv);
-                    y = wrap(y - P.cm_new[1], L, half, inv);
-                    z = wrap(z - P.cm_new[2], L, half, inv);
-                    const double rx = P.R[0] * x + P.R[1] * y + P.R[2] * z;
-                    const double ry = P.R[3] * x + P.R[4] * y + P.R[5] * z;
-                    const double rz = P.R[6] * x + P.R[7] * y + P.R[8] * z;
-                    x = wrap(rx + P.cm_new[0], L, half, inv);
-                    y = wrap(ry + P.cm_new[1], L, half, inv);
-                    z = wrap(rz + P.cm_new[2], L, half, inv);
+                if (Pm.rotate) { // Geometry::rotate about the mass centre (geometry.h:613-626)
+                    x = wrap(x - Pm.cm_new[0], L, half, inv);
+                    y = wrap(y - Pm.cm_new[1], L, half, inv);
+                    z = wrap(z - Pm.cm_new[2], L, half, inv);
+                    const double rx = Pm.R[0] * x + Pm.R[1] * y + Pm.R[2] * z;
+                    const double ry = Pm.R[3] * x + Pm.R[4] * y + Pm.R[5] * z;
+                    const double rz = Pm.R[6] * x + Pm.R[7] * y + Pm.R[8] * z;
+                    x = wrap(rx + Pm.cm_new[0], L, half, inv);
+                    y = wrap(ry + Pm.cm_new[1], L, half, inv);
+                    z = wrap(rz + Pm.cm_new[2], L, half, inv);
                 }
             } else { // ChainRotationMove::rotate_segment (chainmove.cpp:76-104)
-                const double fx = wrap(x - P.shift[0], L, half, inv) - P.origin[0];
-                const double fy = wrap(y - P.shift[1], L, half, inv) - P.origin[1];
-                const double fz = wrap(z - P.shift[2], L, half, inv) - P.origin[2];
-                const double rx = P.R[0] * fx + P.R[1] * fy + P.R[2] * fz + P.origin[0];
-                const double ry = P.R[3] * fx + P.R[4] * fy + P.R[5] * fz + P.origin[1];
-                const double rz = P.R[6] * fx + P.R[7] * fy + P.R[8] * fz + P.origin[2];
-                x = wrap(rx + P.shift[0], L, half, inv);
-                y = wrap(ry + P.shift[1], L, half, inv);
-                z = wrap(rz + P.shift[2], L, half, inv);
+                const double fx = wrap(x - Pm.shift[0], L, half, inv) - Pm.origin[0];
+                const double fy = wrap(y - Pm.shift[1], L, half, inv) - Pm.origin[1];
+                const double fz = wrap(z - Pm.shift[2], L, half, inv) - Pm.origin[2];
+                const double rx = Pm.R[0] * fx + Pm.R[1] * fy + Pm.R[2] * fz + Pm.origin[0];
+                const double ry = Pm.R[3] * fx + Pm.R[4] * fy + Pm.R[5] * fz + Pm.origin[1];
+                const double rz = Pm.R[6] * fx + Pm.R[7] * fy + Pm.R[8] * fz + Pm.origin[2];
+                x = wrap(rx + Pm.shift[0], L, half, inv);
+                y = wrap(ry + Pm.shift[1], L, half, inv);
+                z = wrap(rz + Pm.shift[2], L, half, inv);
             }
             s_new[a] = make_double4(x, y, z, p.w);
-            if (kF32) {
+            if (fixlds) {
                 s_fnew[a] = fixed_pos(x, y, z, p.w, inv);
                 s_fold[a] = fixed_pos(p.x, p.y, p.z, p.w, inv);
             }
         }
         __syncthreads();
+        const GeneralMove& P = s_P;
 
         // ---- cutoff_g2g: the moved molecular group's mass centre in both states
         // (GroupCutoff::cut, energy.h:761-768).  A translate+rotate carries it along; atom
@@ -637,7 +674,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             const int ta = s_mt[0];
             if (kF32) {
                 pair_row_f32<kThreads>(pos32, meta, jb, je, P.mlo, g, same_ok, s_pairs_f + ta * nt, cf, s_stab, terms, hf,
-                                       s_fnew[0], s_fold[0], &s_nhot, s_hot, 0, e_new, e_old);
+                                       fixed_pos(A.x, A.y, A.z, A.w, inv), fixed_pos(B.x, B.y, B.z, B.w, inv), &s_nhot,
+                                       s_hot, 0, e_new, e_old);
             } else {
                 pair_row_f64<kThreads>(pos, meta, jb, je, P.mlo, g, same_ok, s_pairs + ta * nt, s_coul, terms, box, A.x,
                                        A.y, A.z, B.x, B.y, B.z, A.w, e_new, e_old);
@@ -646,34 +684,40 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             if (!kCells) {
                 // blocks of kB neighbours per lane (kB from 8 down to 1 as the slice
                 // runs out, so that few lanes sit idle on short slices)
-                const MovedSet ms{P.mlo,   P.mhi,    m,     g,    same_ok,          s_new,    s_old,   s_fnew,
-                                  s_fold,  s_mt,     excl_off, excl, cutg ? &gcut : nullptr, s_cm[0], s_cm[1]};
-                auto blocks = [&](auto excl_tag) {
+                const MovedSet ms{P.mlo,    P.mhi, m,    g,    same_ok, s_new, s_old,   fixlds ? s_fnew : nullptr,
+                                  s_fold,   inv,   s_mt, excl_off, excl, gcut,  s_cm[0], s_cm[1]};
+                auto blocks = [&](auto excl_tag, auto cut_tag) {
                     constexpr bool kX = decltype(excl_tag)::value;
+                    constexpr bool kC = decltype(cut_tag)::value;
                     int base = jb;
                     while (base < je) {
                         const int rem = je - base;
                         if (kF32 && rem >= 8 * kThreads) {
-                            moved_block<kThreads, 8, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                            moved_block<kThreads, 8, kF32, kX, kC>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
                                                                s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 8 * kThreads;
                         } else if (rem >= 4 * kThreads) {
-                            moved_block<kThreads, 4, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                            moved_block<kThreads, 4, kF32, kX, kC>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
                                                                s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 4 * kThreads;
                         } else if (rem > kThreads) {
-                            moved_block<kThreads, 2, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                            moved_block<kThreads, 2, kF32, kX, kC>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
                                                                s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += 2 * kThreads;
                         } else {
-                            moved_block<kThreads, 1, kF32, kX>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
+                            moved_block<kThreads, 1, kF32, kX, kC>(base, je, ms, meta, pos32, pos, s_pairs_f, s_pairs, nt, cf,
                                                                s_stab, s_coul, terms, box, hf, &s_nhot, s_hot, e_new, e_old);
                             base += kThreads;
                         }
                     }
                 };
-                if (excl_off != nullptr) blocks(std::true_type{});
-                else blocks(std::false_type{});
+                if (excl_off != nullptr) {
+                    if (cutg) blocks(std::true_type{}, std::true_type{});
+                    else blocks(std::true_type{}, std::false_type{});
+                } else {
+                    if (cutg) blocks(std::false_type{}, std::true_type{});
+                    else blocks(std::false_type{}, std::false_type{});
+                }
             } else {
               // stencils of (2r+1)^3 cells (cells of at least range / r) around each moved
               // atom's new (st=0) and old (st=1) position.  When the pair range plus the
@@ -762,10 +806,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                             const int ta = s_mt[a];
                             if (kF32) {
                                 const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
+                                bool hot;
                                 const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q,
-                                                                pf[u].q, dist2_fix(FA, pf[u], hf));
-                                e += (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0
-                                                                                                  : static_cast<double>(uf);
+                                                                pf[u].q, dist2_fix(FA, pf[u], hf), hot);
+                                e += (hot && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0 : static_cast<double>(uf);
                             } else {
                                 e += pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj[u].w,
                                                  dist2(box, A.x, A.y, A.z, pj[u].x, pj[u].y, pj[u].z));
@@ -800,10 +844,10 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                         if (kF32) {
                             const FixPos pj = pos32[j];
                             const FixPos& FA = st == 0 ? s_fnew[a] : s_fold[a];
+                            bool hot;
                             const float uf = pair_energy_ft(s_pairs_f[ta * nt + tj], cf, s_stab, terms, FA.q, pj.q,
-                                                            dist2_fix(FA, pj, hf));
-                            u = (hot_pair(uf) && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0
-                                                                                             : static_cast<double>(uf);
+                                                            dist2_fix(FA, pj, hf), hot);
+                            u = (hot && push_hot(&s_nhot, s_hot, j, (a << 1) | st)) ? 0.0 : static_cast<double>(uf);
                         } else {
                             const double4 pj = pos[j];
                             u = pair_energy(s_pairs[ta * nt + tj], s_coul, A.w, pj.w, dist2(box, A.x, A.y, A.z, pj.x, pj.y, pj.z));
@@ -1066,7 +1110,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 if (kSplit && (idx < jb || idx >= je)) continue;
                 const double4 A = s_new[a];
                 pos[idx] = A;
-                if (kF32) pos32[idx] = s_fnew[a];
+                if (kF32) pos32[idx] = fixlds ? s_fnew[a] : fixed_pos(A.x, A.y, A.z, A.w, inv);
             }
 #pragma unroll
             for (int s = 0; s < kMaxKPT; ++s) {
@@ -1158,6 +1202,7 @@ struct GeneralLaunch {
     bool cells, split;
     size_t lds;
     int grid;
+    size_t fix_lds_bytes; // FP32 form: fixed-point copies of the moved atoms
 };
 
 GeneralLaunch plan(const ReplicaState* h_states, int count, bool f32) {
@@ -1182,11 +1227,10 @@ GeneralLaunch plan(const ReplicaState* h_states, int count, bool f32) {
     }
     if (g.maxkpt > kpt_cap(g.cells))
         throw std::runtime_error("too many k-vectors for the general device MC (max 4096 per workgroup)");
-    // the split form keeps the previous move's coordinates in a second buffer; the FP32
-    // form adds fixed-point copies of the new and old coordinates
+    // the split form keeps the previous move's coordinates in a second buffer
     g.lds = sizeof(int) * static_cast<size_t>((maxnk + 7) & ~7) + sizeof(double2) * kChunk * 6 * static_cast<size_t>(maxw) +
-            ((g.split ? 3 : 2) * sizeof(double4) + (f32 ? 2 * sizeof(FixPos) : 0) + sizeof(int)) *
-                static_cast<size_t>(maxm);
+            ((g.split ? 3 : 2) * sizeof(double4) + sizeof(int)) * static_cast<size_t>(maxm);
+    g.fix_lds_bytes = 2 * sizeof(FixPos) * static_cast<size_t>(maxm);
     return g;
 }
 
@@ -1250,14 +1294,19 @@ __global__ void clear_general_slots_kernel(ReplicaState* __restrict__ states, in
 
 void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
                     TraceEntry* trace, int trace_cap, hipStream_t stream, const int2* d_blockmap, int map_blocks) {
-    const GeneralLaunch g = plan(h_states, count, f32);
+    GeneralLaunch g = plan(h_states, count, f32);
     const void* fn = select_kernel(g.maxkpt, g.cells, f32, g.split);
-    if (g.lds > prepare(fn)) throw std::runtime_error("k-vector and molecule tables too large for LDS");
+    const size_t limit = prepare(fn);
+    // the FP32 form keeps fixed-point copies of the moved atoms in LDS where they fit
+    int fix_lds = f32 && g.lds + g.fix_lds_bytes <= limit ? 1 : 0;
+    if (fix_lds) g.lds += g.fix_lds_bytes;
+    if (f32 && g.cells && !fix_lds) throw std::runtime_error("molecules too large for the FP32 cell-list kernel's LDS");
+    if (g.lds > limit) throw std::runtime_error("k-vector and molecule tables too large for LDS");
     // each workgroup lays out its own replica's tables inside the dynamic LDS
     long long spin = 1LL << 24;
     if (!g.split) {
         const int2* none = nullptr;
-        void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &none, &spin};
+        void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &none, &spin, &fix_lds};
         check_hip(hipLaunchKernel(fn, dim3(count), dim3(threads_of(g.cells)), args, g.lds, stream),
                   "mc_general_kernel launch");
         return;
@@ -1269,7 +1318,7 @@ void launch_general(const ReplicaState* h_states, int count, ReplicaState* d_sta
     static_assert(sizeof(GeneralSlot) % 8 == 0, "whole words");
     hipLaunchKernelGGL(clear_general_slots_kernel, dim3(count), dim3(64), 0, stream, d_states, count);
     check_hip(hipGetLastError(), "clear_general_slots_kernel");
-    void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin};
+    void* args[] = {&d_states, &d_moves, &trace, &trace_cap, &d_blockmap, &spin, &fix_lds};
     check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWide), args, static_cast<unsigned>(g.lds), stream),
               "mc_general_kernel cooperative launch");
 }

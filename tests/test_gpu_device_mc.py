@@ -8,7 +8,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import compare_chain, log
+from tests.gpu_helpers import compare_chain, log, DRIFT_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -54,7 +54,7 @@ def test_device_mc_primitive_model(fp32):
     tot, terms = b.total_energy(0)
     log("running", info["energy"], "recomputed", tot)
     # FP32 pair terms accumulate ~1e-7-relative errors per move into the running sum
-    drift_tol = 1e-6 if fp32 else 1e-9
+    drift_tol = DRIFT_FP32 if fp32 else 1e-9
     assert abs(info["energy"] - tot) <= drift_tol * max(1.0, abs(tot))
     dq, drec = b.check_q(0)
     assert dq < 1e-9 and abs(drec) < 1e-11

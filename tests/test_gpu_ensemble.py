@@ -10,7 +10,7 @@ import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
 from tests.ensemble import block_stats, bulk_system, minimal_system, rdf_counts
-from tests.gpu_helpers import log
+from tests.gpu_helpers import log, DRIFT_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def test_minimal_mean_energy_and_acceptance(fp32):
     assert abs(mean - golden) <= 0.02 * abs(golden)
     assert abs(acc - 0.651) < 0.02
     tot, _ = b.total_energy(0)
-    assert abs(info["energy"] - tot) <= (1e-6 if fp32 else 1e-9) * abs(tot)
+    assert abs(info["energy"] - tot) <= (DRIFT_FP32 if fp32 else 1e-9) * abs(tot)
 
 
 def test_bulk_mean_energy_and_acceptance():

@@ -19,7 +19,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import compare_chain, log
+from tests.gpu_helpers import compare_chain, log, DRIFT_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -47,7 +47,7 @@ def check(b, orc, e0, rec, nm, seed, fp32, what, telescopes=True):
         tot = e0 + float(np.where(np.asarray(rec["accepted"]) == 1, rec["du"], 0.0).sum())
     if fp32:
         assert rep["max_rel"] <= 1e-6, rep
-        assert abs(info["energy"] - tot) <= 1e-6 * max(1.0, abs(tot))
+        assert abs(info["energy"] - tot) <= DRIFT_FP32 * max(1.0, abs(tot))
     else:
         assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10, rep
         np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)

@@ -1,11 +1,11 @@
 """The device pair functions against the oracle's pair energy, pair by pair: the FP64
 form (pair_energy) within 1e-11 * max(|u|, 1e-3 kT), and the FP32-pair form at the same
-distance: pairs above the hot threshold (8 kT) are evaluated again in FP64 and must
-meet the FP64 bound; the FP32 terms below it (FP32 arithmetic, FP32 S(q) table,
-splined tables evaluated from the FP32 distance) must lie within 1e-6 * 8 kT -- a few
-units of FP32 rounding of the largest term that can cancel inside a pair below the
-threshold.  Every type pair of each potential form the parser takes, on a grid of
-distances from well inside contact to beyond the pair range."""
+distance within 1e-6 * max(|u|, 8 kT) -- a few units of FP32 rounding of the largest
+term of a pair (terms that cancel near the LJ minimum included).  Pairs inside the hot
+radius (LJ or WCA repulsion above 8 kT) are evaluated again in FP64, so deep overlaps
+(|u| > 1000 kT) must meet the FP64 bound.  Every type pair of each potential form the
+parser takes, on a grid of distances from well inside contact to beyond the pair
+range."""
 import json
 
 import numpy as np
@@ -19,7 +19,7 @@ from tests.test_gpu_potentials import VARIANTS
 
 pytestmark = pytest.mark.gpu
 
-HOT = 8.0  # kT: FP32 pair energies above it are evaluated again in FP64
+HOT = 8.0  # kT: floor of the FP32 bound
 
 CASES = dict(VARIANTS)
 CASES["splined_nacl"] = systems.nacl_lj(n_pairs=10, box=60.0, seed=1, kmax=6)[0]["energy"]
@@ -50,10 +50,10 @@ def test_pair_functions_match_the_oracle(name):
             assert np.array_equal(inf, np.isinf(u64)) and np.array_equal(inf, np.isinf(u32)), (name, a, b)
             fin = ~inf
             e64 = np.abs(u64[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-3)
-            hot = np.abs(uo[fin]) > HOT * (1 + 1e-5)
-            # hot pairs: the FP64 bound; the rest: absolute, in units of the threshold
-            e32 = np.where(hot, np.abs(u32[fin] - uo[fin]) / np.maximum(np.abs(uo[fin]), 1e-3) / 1e-11,
-                           np.abs(u32[fin] - uo[fin]) / (1e-6 * HOT))
+            deep = np.abs(uo[fin]) > 1e3
+            # FP32 bound, and the FP64 bound (in units of the FP32 one) on deep overlaps
+            e32 = np.maximum(np.abs(u32[fin] - uo[fin]) / (1e-6 * np.maximum(np.abs(uo[fin]), HOT)),
+                             np.where(deep, np.abs(u32[fin] - uo[fin]) / (1e-11 * np.abs(uo[fin])), 0.0))
             k = int(np.argmax(e32)) if len(e32) else 0
             log(name, a, b, "fp64 max rel %.2e" % (e64.max() if len(e64) else 0),
                 "fp32 worst/bound %.3f at r=%.4f (u %.9g vs %.9g)" % (e32.max() if len(e32) else 0,

@@ -14,7 +14,7 @@ import pytest
 import faunus_amd as F
 from oracle import oracle as O
 from tests import systems
-from tests.gpu_helpers import build_pair, compare_chain, compare_replay, log
+from tests.gpu_helpers import build_pair, compare_chain, compare_replay, log, DRIFT_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -89,7 +89,7 @@ def test_device_resident(fp32):
         assert rep["first_mismatch"] == nm, rep
         np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
     tot, _ = b.total_energy(0)
-    assert abs(b.info(0)["energy"] - tot) <= (1e-6 if fp32 else 1e-9) * max(1.0, abs(tot))
+    assert abs(b.info(0)["energy"] - tot) <= (DRIFT_FP32 if fp32 else 1e-9) * max(1.0, abs(tot))
 
 
 def test_split_kernel():
