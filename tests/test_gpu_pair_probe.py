@@ -53,7 +53,7 @@ def test_pair_functions_match_the_oracle(name):
             deep = np.abs(uo[fin]) > 1e3
             # FP32 bound, and the FP64 bound (in units of the FP32 one) on deep overlaps
             e32 = np.maximum(np.abs(u32[fin] - uo[fin]) / (1e-6 * np.maximum(np.abs(uo[fin]), HOT)),
-                             np.where(deep, np.abs(u32[fin] - uo[fin]) / (1e-11 * np.abs(uo[fin])), 0.0))
+                             np.where(deep, np.abs(u32[fin] - uo[fin]) / (1e-11 * np.maximum(np.abs(uo[fin]), 1e3)), 0.0))
             k = int(np.argmax(e32)) if len(e32) else 0
             log(name, a, b, "fp64 max rel %.2e" % (e64.max() if len(e64) else 0),
                 "fp32 worst/bound %.3f at r=%.4f (u %.9g vs %.9g)" % (e32.max() if len(e32) else 0,
