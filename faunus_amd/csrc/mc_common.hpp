@@ -30,10 +30,13 @@ struct alignas(16) DevPairF {
 
 constexpr double kHotRepulsion = 8.0; // kT
 
-// r^2 below which the pair's LJ or WCA repulsion e4 (s2/r2)^6 exceeds kHotRepulsion:
-// there FP32 rounding of r^2, grown sixfold by r^-12, would dominate the move's error.
+// r^2 below which the pair is evaluated again in FP64: where its LJ or WCA repulsion
+// e4 (s2/r2)^6 exceeds kHotRepulsion (FP32 rounding of r^2, grown sixfold by r^-12,
+// would dominate the move's error), and inside hard-sphere contact (the FP32 form marks
+// an overlap only below 1 - 2^-20 of the contact r^2, so that the FP64 form decides the
+// pairs the two distances could place on different sides).
 __device__ __forceinline__ double hot_radius2(const DevPair& p) {
-    double r2 = 0.0;
+    double r2 = p.hs_s2;
     if (p.lj_e4 > 0.0) r2 = fmax(r2, p.lj_s2 * sqrt(cbrt(p.lj_e4 / kHotRepulsion)));
     if (p.wca_e4 > 0.0) r2 = fmax(r2, p.wca_s2 * sqrt(cbrt(p.wca_e4 / kHotRepulsion)));
     return r2;
@@ -161,11 +164,9 @@ __device__ __forceinline__ float dist2_fix(const FixPos& a, const FixPos& b, flo
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Hot pairs are evaluated again in FP64: pairs inside their type pair's hot radius
-// (hot_radius2), and hard-sphere pairs within 2^-20 of contact (r^2 relative), where the
-// FP32 and FP64 distances may fall on different sides; their FP32 energy is the finite
-// kHsBand so that block sums stay finite (elsewhere an FP32 overlap is +inf in both
-// forms).  The pair loops queue hot pairs in LDS (the neighbour, and moved atom
+// Hot pairs (inside their type pair's hot radius, hot_radius2) are evaluated again in
+// FP64; an FP32 hard-sphere overlap (+inf) stands, except within 2^-20 of contact where
+// the FP32 form adds nothing and the FP64 form decides.  The pair loops queue hot pairs in LDS (the neighbour, and moved atom
 // << 1 | state with state 0 new and 1 old); after the loops each queued pair is
 // evaluated from the FP64 coordinates, so the FP64 pair code stays out of the loops'
 // registers.  Loops that sum blocks of neighbours per lane queue all pairs of a block
@@ -173,7 +174,6 @@ __device__ __forceinline__ float dist2_fix(const FixPos& a, const FixPos& b, flo
 // FP32 energy.
 constexpr int kHotCap = 256;
 constexpr float kHsSure = 0.99999904632568359375f; // 1 - 2^-20
-constexpr float kHsBand = 1e30f; // FP32 energy of a hard-sphere pair near contact (finite, hot)
 
 // Queue one hot pair; false when the queue is full.
 __device__ __forceinline__ bool push_hot(int* n, int2* q, int j, int tag) {
@@ -204,7 +204,6 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
     materialize(pa);
     materialize(pb);
     hot = r2 < pb.w;
-    if (terms & kTermHS) hot = hot || (r2 < pb.x && !(r2 < pb.x * kHsSure));
     if (c.spl != nullptr) {
         // nonbonded_splined: the table is evaluated in FP64 from the FP32 distance
         bool below;
@@ -225,7 +224,7 @@ __device__ __forceinline__ float pair_energy_ft(const DevPairF& p, const CoulF& 
             u += (r2 > pa.z * 1.2599210498948732f) ? 0.0f : pa.w * (w * w - w + 0.25f);
         }
     }
-    if (terms & kTermHS) u += (r2 < pb.x) ? ((r2 < pb.x * kHsSure) ? __builtin_huge_valf() : kHsBand) : 0.0f;
+    if (terms & kTermHS) u += (r2 < pb.x * kHsSure) ? __builtin_huge_valf() : 0.0f;
     if (c.scheme != kCoulNone) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float e;
