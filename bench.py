@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 232.3e6
+HEADLINE_TRAFFIC_BYTES = 224.1e6
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 STEP_MOVES = 20000
@@ -712,7 +712,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
     ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve",
-                                       "gsplit", "water", "intervals", "fp32err"], default=None)
+                                       "gsplit", "water", "intervals", "fp32err", "batch32", "water32"], default=None,
+                    help="run one leg and print its results (batch32 / water32: the FP32 batched forms alone, "
+                         "for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="serving: the first jobs of the stream with N <= --serve-max-n, to 1000 sweeps")
     ap.add_argument("--serve-max-n", type=int, default=3000)
@@ -819,6 +821,13 @@ def main():
             out["cpu_serve"] = cpu_serve_baseline(stream, args.serve_sweeps)
             log("CPU baseline serving, one job per core: p50 %.1f s, p99 %.1f s" % (
                 out["cpu_serve"]["p50_s"], out["cpu_serve"]["p99_s"]))
+    if args.only == "batch32":
+        comps, temps = salt_mix()
+        out["batch32"] = bench_batch(comps, temps, True, args, 4000)
+        log("batched FP32-pair/FP64-Q, cost-balanced: %.3e replica-moves/s" % out["batch32"]["rate"])
+    if args.only == "water32":
+        out["water32"] = bench_water(args, fp32=True)
+        log("64 water replicas, FP32 pairs: %.3e replica-moves/s" % out["water32"]["rate"])
     if args.only in (None, "fp32err"):
         out["fp32err"] = fp32_errors(sysd)
         log("FP32-pair form vs FP64, max relative dU error per config: %s" % {
