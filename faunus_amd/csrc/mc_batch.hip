@@ -76,6 +76,8 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
+    const double surf = S.surf;
+    double dip0 = S.dip[0], dip1 = S.dip[1], dip2 = S.dip[2]; // thread 0's running dipole
     double energy = S.energy, sum_du = S.sum_du;
     unsigned long long accepted = S.accepted;
     const int replica = S.replica_id;
@@ -242,8 +244,21 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
                 b += s_red[1][w];
                 c += s_red[2][w];
             }
-            const double du = energy_change(a, b, pref * c);
+            // the surface term enters with the reciprocal one (Ewald::energy, energy.cpp:581-587)
+            double dsurf = 0.0, mx = dip0, my = dip1, mz = dip2;
+            if (surf != 0.0) {
+                mx += po.w * (nx_ - po.x);
+                my += po.w * (ny_ - po.y);
+                mz += po.w * (nz_ - po.z);
+                dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
+            }
+            const double du = energy_change(a, b, pref * c + dsurf);
             const bool acc = metropolis(beta * du, P.accept_u);
+            if (acc) {
+                dip0 = mx;
+                dip1 = my;
+                dip2 = mz;
+            }
             s_accept = acc ? 1 : 0;
             if (kF32) s_nhot = 0; // every thread has read the queue before the reduction
             if (trace && t < trace_cap) {
@@ -338,6 +353,9 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.dip[0] = dip0;
+        S.dip[1] = dip1;
+        S.dip[2] = dip2;
         S.e_samples = e_samples;
         S.e_sum = e_sum;
         S.e_sum2 = e_sum2;
@@ -573,6 +591,17 @@ void pair_probe(Engine& e, int ta, int tb, double qa, double qb, int n, const do
     for (void* p : owned) (void)hipFree(p);
 }
 
+// The dipole sum q r over the engine's particles (positions as stored).
+static void replica_dipole(Engine& e, double* dip) {
+    const int n = e.num_particles();
+    std::vector<double> xyz(3 * static_cast<size_t>(n));
+    e.get_positions(xyz.data());
+    const auto& q = e.host_charges();
+    dip[0] = dip[1] = dip[2] = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) dip[d] += xyz[3 * static_cast<size_t>(i) + d] * q[static_cast<size_t>(i)];
+}
+
 ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, int stream_id, double beta, int ladder,
                                 std::vector<void*>& owned, const std::vector<MoveKindSpec>* kinds) {
     const Model& m = e.model();
@@ -587,8 +616,6 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     if (!kinds)
         for (const auto& mol : m.molecules)
             if (!mol.atomic) throw std::runtime_error("device MC handles atomic groups (single-particle translations)");
-    if (m.ewald.present && m.ewald.const_inf > 0.5)
-        throw std::runtime_error("device MC needs tinfoil boundary conditions (epss = 0)");
     // Hamiltonian::energy stops summing a state's terms at the first one >= maxenergy
     // (energy.cpp:1227-1241); the device kernels form dU as one difference, so a finite
     // maxenergy is left to the per-move path
@@ -655,6 +682,15 @@ ReplicaState make_replica_state(Engine& e, bool fp32, unsigned long long seed, i
     s.e_samples = 0;
     s.e_sum = s.e_sum2 = 0.0;
     s.pair_range = e.cells().enabled ? e.cell_range() : e.model().pair_range;
+    // surface term (PolicyIonIon::surfaceEnergy, energy.cpp:466-482): the prefactor and
+    // the dipole sum, kept up to date by the kernels
+    s.surf = 0.0;
+    s.dip[0] = s.dip[1] = s.dip[2] = 0.0;
+    if (m.ewald.present && m.ewald.const_inf > 0.5) {
+        const double V = m.box[0] * m.box[1] * m.box[2];
+        s.surf = m.ewald.const_inf * 2.0 * kPi / ((2.0 * m.ewald.epss + 1.0) * V) * m.ewald.lB;
+        replica_dipole(e, s.dip);
+    }
     s.cell_radius = e.cell_radius();
     s.threads = 0;
     const DevCoulomb& dc = e.dev_coulomb();
@@ -1018,6 +1054,10 @@ void Batch::set_replica_state(int r, const double* xyz, const double* mass_cente
             s->cell_members = cl.members;
             s->cell_of = cl.cell_of;
         }
+    }
+    if (cur.surf != 0.0) {
+        replica_dipole(e, cur.dip);
+        for (int d = 0; d < 3; ++d) h.dip[d] = cur.dip[d];
     }
     if (cur.pos32) {
         const int n = e.num_particles();

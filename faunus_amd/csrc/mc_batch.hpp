@@ -95,6 +95,8 @@ struct ReplicaState {
     unsigned long long e_samples;
     double e_sum, e_sum2;
     double pair_range;        // distance beyond which every pair term vanishes (cell-list forms)
+    double dip[3];            // surface term: the replica's dipole sum q r (positions as stored)
+    double surf;              // surface term prefactor const_inf 2 pi lB / ((2 eps_s + 1) V), 0 without it
     int cell_radius;          // stencil half-width in cells (1: 27 cells, 2: 125 cells)
     int threads;              // workgroup size of the translate-only kernel chosen by the tuner (0: the default rule)
 };

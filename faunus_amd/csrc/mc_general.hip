@@ -449,6 +449,8 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
     const unsigned long long move0 = S.moves;
     const double beta = S.beta;
     const double pref = S.rec_prefactor;
+    const double surf = S.surf;
+    double dip0 = S.dip[0], dip1 = S.dip[1], dip2 = S.dip[2]; // thread 0's running dipole (every part)
     double energy = S.energy, sum_du = S.sum_du;
     unsigned long long accepted = S.accepted;
     const uint32_t replica = static_cast<uint32_t>(S.replica_id);
@@ -664,6 +666,31 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             __syncthreads();
         }
         const GroupCut gcut{gflags, gcm, g2g2, S.nmol, gflag};
+
+        // ---- surface term: the moved atoms' change of the dipole sum q r (thread 0)
+        double dmx = 0.0, dmy = 0.0, dmz = 0.0;
+        if (surf != 0.0 && m > 0) {
+            double v[3] = {0.0, 0.0, 0.0};
+            for (int a = tid; a < m; a += kThreads) {
+                const double4 An = s_new[a], Ao = s_old[a];
+                v[0] += An.w * (An.x - Ao.x);
+                v[1] += An.w * (An.y - Ao.y);
+                v[2] += An.w * (An.z - Ao.z);
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+            if ((tid & 63) == 0)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) s_cmred[r][tid >> 6] = v[r];
+            __syncthreads();
+            if (tid == 0)
+                for (int w = 0; w < kThreads / 64; ++w) {
+                    dmx += s_cmred[0][w];
+                    dmy += s_cmred[1][w];
+                    dmz += s_cmred[2][w];
+                }
+        }
 
         // ---- nonbonded pairs: moved x static (GroupPairing, energy.h:885-1027, 1155-1224)
         double e_new = 0.0, e_old = 0.0;
@@ -1022,8 +1049,19 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             double du = 0.0;
             bool acc = true;
             if (m > 0) {
-                du = energy_change(tot[0] + tot[3], tot[1] + tot[4], pref * tot[2]);
+                // the surface term enters with the reciprocal one (Ewald::energy, energy.cpp:581-587)
+                double dsurf = 0.0;
+                if (surf != 0.0) {
+                    const double mx = dip0 + dmx, my = dip1 + dmy, mz = dip2 + dmz;
+                    dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
+                }
+                du = energy_change(tot[0] + tot[3], tot[1] + tot[4], pref * tot[2] + dsurf);
                 acc = metropolis(beta * du, P.accept_u);
+                if (acc) {
+                    dip0 += dmx;
+                    dip1 += dmy;
+                    dip2 += dmz;
+                }
             }
             s_accept = acc ? 1 : 0;
             if (kF32) s_nhot = 0; // every thread has read the queue before the reduction
@@ -1141,6 +1179,9 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.dip[0] = dip0;
+        S.dip[1] = dip1;
+        S.dip[2] = dip2;
         S.e_samples = e_samples;
         S.e_sum = e_sum;
         S.e_sum2 = e_sum2;

@@ -461,6 +461,7 @@ void Batch::run_pipe(long long moves, int groups, int depth, bool trace, int tra
     if (!finalized_) finalize();
     if (size() != 1) throw std::runtime_error("run_pipe advances a batch of one replica");
     if (general_ == 1) throw std::runtime_error("run_pipe handles translate-only replicas");
+    if (h_states_[0].surf != 0.0) throw std::runtime_error("run_pipe: no surface term (epss = 0)");
     if (h_states_[0].sample_every > 0) throw std::runtime_error("run_pipe takes no energy samples");
     const ReplicaState& s = h_states_[0];
     if (groups < 1 || groups > kPipeMaxG) throw std::runtime_error("run_pipe: 1..64 workgroups");

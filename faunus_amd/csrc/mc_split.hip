@@ -119,6 +119,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const uint32_t* __restrict__ meta = S.meta;
     const unsigned long long move0 = S.moves;
     const double beta = S.beta, pref = S.rec_prefactor;
+    const double surf = S.surf;
+    double dip0 = S.dip[0], dip1 = S.dip[1], dip2 = S.dip[2]; // thread 0's running dipole (every workgroup)
     double energy = S.energy, sum_du = S.sum_du;
     unsigned long long accepted = S.accepted;
     const unsigned long long seed = S.seed;
@@ -356,7 +358,15 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             }
         }
         if (tid == 0 && !s_abort && !s_stop) {
-            const double drec = pref * sde;
+            // the surface term enters with the reciprocal one (Ewald::energy, energy.cpp:581-587)
+            double dsurf = 0.0, mx = dip0, my = dip1, mz = dip2;
+            if (surf != 0.0) {
+                mx += po.w * (nx_ - po.x);
+                my += po.w * (ny_ - po.y);
+                mz += po.w * (nz_ - po.z);
+                dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
+            }
+            const double drec = pref * sde + dsurf;
             const double e_tot_new = spn + drec;
             double du;
             if (isnan(spo) && !isnan(e_tot_new)) du = -__builtin_huge_val();
@@ -372,6 +382,11 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             else if (-bdu > 709.782712893384) acc = true;
             else acc = P.accept_u <= exp(-bdu);
             s_accept = acc ? 1 : 0;
+            if (acc) {
+                dip0 = mx;
+                dip1 = my;
+                dip2 = mz;
+            }
             if (g == 0) {
                 if (kPre) {
                     const long long tn = t + 1;
@@ -528,6 +543,9 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         S.accepted = accepted;
         S.energy = energy;
         S.sum_du = sum_du;
+        S.dip[0] = dip0;
+        S.dip[1] = dip1;
+        S.dip[2] = dip2;
         S.e_samples = e_samples;
         S.e_sum = e_sum;
         S.e_sum2 = e_sum2;

@@ -40,7 +40,7 @@ VARIANTS = {
     "surface_epss1": [{"nonbonded": {"default": [{"hardsphere": {}}, {"coulomb": dict(EWALD, epss=1.0)}]}}],
     "maxenergy": [{"nonbonded": {"default": [{"hardsphere": {}}, {"coulomb": EWALD}]}}, {"maxenergy": 0.0}],
 }
-DEVICE_OK = {"coulombwca", "pm", "pmwca", "lj_arithmetic", "lj_geometric", "lj_custom"}
+DEVICE_OK = {"coulombwca", "pm", "pmwca", "lj_arithmetic", "lj_geometric", "lj_custom", "surface_epss1"}
 
 
 def system(name, n_pairs=120, box=42.0, seed=5):
@@ -91,11 +91,62 @@ def test_device_resident(name, fp32):
     assert b.info(0)["error"] == 0
 
 
-@pytest.mark.parametrize("name", ["surface_epss1", "maxenergy"])
-def test_device_resident_refuses_per_move_only_policies(name):
+def test_device_resident_refuses_maxenergy_and_pipelined_surface():
     b = F.Batch()
-    with pytest.raises(RuntimeError, match="epss|maxenergy"):
-        b.add(system(name), seed=1)
+    with pytest.raises(RuntimeError, match="maxenergy"):
+        b.add(system("maxenergy"), seed=1)
+    b.close()
+    b = F.Batch()
+    b.add(system("surface_epss1"), seed=1)
+    with pytest.raises(RuntimeError, match="surface"):
+        b.run_pipe(10, groups=4, depth=2)
+    b.close()
+
+
+def test_surface_term_in_the_split_and_general_kernels():
+    """The surface term (epss = 1) with a running dipole on the split kernel (G = 4) and
+    on the general kernel (a polyelectrolyte with ion and bead translations and pivots,
+    one workgroup and split over four): FP64 decisions and du against the oracle."""
+    sysd = system("surface_epss1")
+    cfg, xyz, q, ids, g = sysd
+    orc = O.System(cfg)
+    orc.set_particles(xyz, q, ids)
+    orc.set_groups(g["begin"], g["size"], g["molid"])
+    e0 = orc.init()
+    b = F.Batch()
+    b.add(sysd, seed=62)
+    nm = 2000
+    b.run_split(nm, groups=4, trace=nm)
+    b.sync()
+    rec = orc.run_philox_translate(nm, seed=62, replica=0)
+    rep = compare_chain(b.trace(0, nm), rec, max(1.0, abs(e0)), 1e-10, seed=62)
+    log("surface split", rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10, rep
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+    b.close()
+    cfg, xyz, q, ids, g, kinds = systems.polyelectrolyte(30, 300, seed=9, kmax=6)
+    cfg["energy"][1]["nonbonded"]["default"][1]["coulomb"]["epss"] = 1.0
+    for groups in (1, 4):
+        orc = O.System(cfg)
+        orc.set_particles(xyz, q, ids)
+        orc.set_groups(g["begin"], g["size"], g["molid"])
+        e0 = orc.init()
+        b = F.Batch()
+        b.add((cfg, xyz, q, ids, g), seed=63, moves=kinds)
+        nm = 2000
+        if groups > 1:
+            b.run_split(nm, groups=groups, trace=nm)
+        else:
+            b.run([nm], trace=nm)
+        b.sync()
+        rec = orc.run_philox_moves(nm, seed=63, kinds=kinds, replica=0)
+        rep = compare_chain(b.trace(0, nm), rec, max(1.0, abs(e0)), 1e-10, seed=63)
+        log("surface general groups=%d" % groups, rep)
+        assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10, rep
+        np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+        tot, _ = b.total_energy(0)
+        assert abs(b.info(0)["energy"] - tot) <= 1e-9 * max(1.0, abs(tot))
+        b.close()
 
 
 def test_nan_old_state_gives_minus_infinite_du():
