@@ -9,8 +9,9 @@ is STEP_MOVES trial moves; the metric is trial moves per second.
 
 Also measured and reported beside it: the plain FP64 one-move-per-launch path on
 the same system, the FP32-pair form, the 256-replica 2:1-salt mix (8 compositions x
-32-rung temperature ladder: cost-balanced launches, and swap intervals of 100 sweeps
-replayed from a HIP graph, both after the tile autotuner) with the plain path on the
+32-rung temperature ladder: cost-balanced launches, swap intervals of 100 sweeps
+replayed from a HIP graph, and free-running ladders of 100-sweep intervals in one
+cooperative launch, all after the tile autotuner) with the plain path on the
 same per-replica move counts, 64 rigid-water replicas, serving under Poisson arrivals
 to 1000 sweeps per job, and CPU baselines (FP64 oracle: one core for the single chain,
 one replica or job per core for the batched and serving forms).
@@ -366,6 +367,37 @@ def bench_batch_intervals(comps, temps, fp32, intervals=1, sweeps=100):
                 roofline=roof)
 
 
+def bench_batch_free(comps, temps, fp32, sweeps=100, big_intervals=1):
+    """The configuration's schedule with free-running ladders: every replica advances
+    `sweeps` sweeps per interval and each composition's ladder swaps after each of its own
+    intervals at its own barrier, all in one cooperative launch.  A ladder of cost c
+    (interval moves 100 N x per-move cost N + 2K) runs about big_intervals x c_max / c
+    intervals, so every workgroup works for about as long as the largest composition's
+    intervals and none waits for another composition."""
+    b, sizes = batch_of_mix(comps, temps, fp32)
+    b.set_timing(True)
+    b.run(np.full(len(sizes), 100, np.int64))
+    b.sync()
+    tuned = b.autotune(300)
+    moves = (sweeps * sizes).astype(np.int64)
+    cost = moves * (sizes + 2 * 1152.0)
+    intervals = np.maximum(1, np.round(big_intervals * cost.max() / cost)).astype(np.int32)
+    t0 = time.perf_counter()
+    b.run_ladders(moves, intervals, seed=4242, round0=0)
+    kernel_ms = b.sync()
+    dt = time.perf_counter() - t0
+    check_replicas(b, len(sizes), "batched salt replicas (free-running ladders)")
+    swaps = b.swap_counts()
+    b.close()
+    total = int((moves * intervals).sum())
+    pf, ef = mix_flops(sizes, fp32=fp32)
+    roof = batch_roofline((pf * moves * intervals).sum(), (ef * moves * intervals).sum(), kernel_ms / 1e3, fp32)
+    per_comp = [int(intervals[ci * len(temps)]) for ci in range(len(comps))]
+    return dict(rate=total / dt, seconds=dt, kernel_ms=kernel_ms, moves=total, sweeps_per_interval=sweeps,
+                intervals_per_composition=per_comp, composition_sizes=[n for n, _, _ in comps], swaps=swaps,
+                tuned_threads=tuned[0], tuner_us_per_move=tuned[1], roofline=roof)
+
+
 def bench_plain_mix(comps, moves_per_replica, temps, per_replica_moves=300):
     """Plain FP64 one-move-per-launch path on the same mix and the same per-replica move
     counts: the per-move time of each composition (one engine at T_ref; the per-move cost
@@ -712,7 +744,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-replica mix")
     ap.add_argument("--only", choices=["device", "plain", "batch", "split", "pipe", "headline", "copy", "serve",
-                                       "gsplit", "water", "intervals", "fp32err", "batch32", "water32"], default=None,
+                                       "gsplit", "water", "intervals", "fp32err", "batch32", "water32", "free"],
+                    default=None,
                     help="run one leg and print its results (batch32 / water32: the FP32 batched forms alone, "
                          "for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=24,
@@ -784,6 +817,10 @@ def main():
         out["intervals32"] = bench_batch_intervals(comps, temps, True)
         log("batched FP32-pair/FP64-Q, 100-sweep swap intervals from a HIP graph: %.3e replica-moves/s (%.1f s)" % (
             out["intervals32"]["rate"], out["intervals32"]["seconds"]))
+        out["free32"] = bench_batch_free(comps, temps, True)
+        log("batched FP32-pair/FP64-Q, free-running ladders of 100-sweep intervals: %.3e replica-moves/s "
+            "(%.1f s, intervals per composition %s)" % (out["free32"]["rate"], out["free32"]["seconds"],
+                                                        out["free32"]["intervals_per_composition"]))
         if "plain_mix" not in out:
             out["plain_mix"] = bench_plain_mix(comps, np.array([100 * n for n, _, _ in comps for _ in temps]), temps)
         out["plain_mix_intervals"] = bench_plain_mix(comps, np.array([100 * n for n, _, _ in comps for _ in temps]), temps)
@@ -825,6 +862,11 @@ def main():
         comps, temps = salt_mix()
         out["batch32"] = bench_batch(comps, temps, True, args, 4000)
         log("batched FP32-pair/FP64-Q, cost-balanced: %.3e replica-moves/s" % out["batch32"]["rate"])
+    if args.only == "free":
+        comps, temps = salt_mix()
+        out["free32"] = bench_batch_free(comps, temps, True)
+        log("free-running ladders: %.3e replica-moves/s (%.1f s, intervals %s)" % (
+            out["free32"]["rate"], out["free32"]["seconds"], out["free32"]["intervals_per_composition"]))
     if args.only == "water32":
         out["water32"] = bench_water(args, fp32=True)
         log("64 water replicas, FP32 pairs: %.3e replica-moves/s" % out["water32"]["rate"])
@@ -893,6 +935,19 @@ def main():
             "speedup_fp32_batched_vs_plain": iv["rate"] / out["plain_mix_intervals"],
             "tuned_threads": iv["tuned_threads"], "tuner_us_per_replica_move": iv["tuner_us_per_move"],
             "swaps_attempted_accepted": iv["swaps"], "roofline": iv["roofline"]}
+        if "free32" in out:
+            fr = out["free32"]
+            line["forms"]["batched_256_replicas_free_running_ladders"] = {
+                "fp32_pairs_fp64_q_replica_moves_per_s": fr["rate"],
+                "schedule": "every replica 100 sweeps per interval; each composition's ladder swaps after each of "
+                            "its own intervals at a barrier of its own, all ladders in one cooperative launch; a "
+                            "composition runs intervals in inverse proportion to its interval cost",
+                "intervals_per_composition": fr["intervals_per_composition"],
+                "composition_sizes": fr["composition_sizes"], "seconds": fr["seconds"],
+                "plain_fp64_replica_moves_per_s_same_moves": out["plain_mix_intervals"],
+                "speedup_fp32_batched_vs_plain": fr["rate"] / out["plain_mix_intervals"],
+                "tuned_threads": fr["tuned_threads"], "swaps_attempted_accepted": fr["swaps"],
+                "roofline": fr["roofline"]}
         if bal:
             line["forms"]["batched_256_replicas_cost_balanced"] = {
                 "fp32_pairs_fp64_q_replica_moves_per_s": bal["rate"],

@@ -58,6 +58,7 @@ SIGNATURES = {
     "fmc_batch_run_split_static": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int]),
     "fmc_batch_capture_interval": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_ulonglong, C.c_ulonglong]),
     "fmc_batch_replay": (C.c_int, [C.c_void_p, C.c_int]),
+    "fmc_batch_run_ladders": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), I, C.c_ulonglong, C.c_ulonglong]),
     "fmc_batch_swap_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "fmc_batch_autotune": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, I, I, D, I]),
     "fmc_batch_run_pipe": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int]),
@@ -503,6 +504,16 @@ class Batch:
 
     def replay(self, count=1):
         _check(lib().fmc_batch_replay(self.h, int(count)))
+
+    def run_ladders(self, moves, intervals, seed, round0=0):
+        """Free-running ladders: replica r runs intervals[r] intervals of moves[r] moves in
+        one launch, each ladder swapping its rungs after each of its own intervals (round
+        round0 + interval).  Blocks until done."""
+        n = len(self.sizes)
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(moves, np.int64), (n,)), np.int64)
+        iv = np.ascontiguousarray(np.broadcast_to(np.asarray(intervals, np.int32), (n,)), np.int32)
+        _check(lib().fmc_batch_run_ladders(self.h, m.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                           iv.ctypes.data_as(C.POINTER(C.c_int)), int(seed), int(round0)))
 
     def swap_counts(self):
         a, b = C.c_longlong(), C.c_longlong()

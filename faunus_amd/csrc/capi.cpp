@@ -513,6 +513,16 @@ int fmc_batch_capture_interval(fmc_batch* b, const long long* moves, unsigned lo
     });
 }
 
+int fmc_batch_run_ladders(fmc_batch* b, const long long* moves, const int* intervals, unsigned long long seed,
+                          unsigned long long round0) {
+    return guarded([&] {
+        if (!moves || !intervals) throw std::runtime_error("run_ladders: moves or intervals are null");
+        const int n = bat(b).size();
+        bat(b).run_ladders(std::vector<long long>(moves, moves + n), std::vector<int>(intervals, intervals + n), seed,
+                           round0);
+    });
+}
+
 int fmc_batch_replay(fmc_batch* b, int count) {
     return guarded([&] { bat(b).replay(count); });
 }

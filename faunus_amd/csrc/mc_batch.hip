@@ -17,13 +17,45 @@
 namespace fmc {
 namespace {
 
+// One round of neighbour exchanges in one ladder (ParallelTempering, move.cpp:844-965):
+// rungs k, k+1 for k of the round's parity, accepted with min(1, exp((b_a - b_b)(E_a -
+// E_b))) on a Philox deviate of (round, ladder, k); an accepted swap exchanges the betas
+// and the rung slots.
+__device__ void swap_ladder_rungs(ReplicaState* states, int b, int m, int* rung_slot, int l, unsigned long long seed,
+                                  unsigned long long round, unsigned long long* counters) {
+    for (int k = static_cast<int>(round & 1); k + 1 < m; k += 2) {
+        const int sa = rung_slot[b + k], sb = rung_slot[b + k + 1];
+        ReplicaState& A = states[sa];
+        ReplicaState& B = states[sb];
+        const double x = (A.beta - B.beta) * (A.energy - B.energy);
+        const Philox4 r = philox4x32_10(static_cast<uint32_t>(round), static_cast<uint32_t>(round >> 32),
+                                        static_cast<uint32_t>(l), 0x5a5a0000u + static_cast<uint32_t>(k),
+                                        static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+        const double u = (static_cast<double>(r.v[0] >> 5) * 67108864.0 + static_cast<double>(r.v[1] >> 6)) *
+                         1.1102230246251565e-16;
+        atomicAdd(&counters[0], 1ull);
+        if (x >= 0.0 || u < exp(x)) {
+            const double tb = A.beta;
+            A.beta = B.beta;
+            B.beta = tb;
+            rung_slot[b + k] = sb;
+            rung_slot[b + k + 1] = sa;
+            atomicAdd(&counters[1], 1ull);
+        }
+    }
+}
+
 template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
 __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __restrict__ states,
                                                                const long long* __restrict__ moves,
-                                                               TraceEntry* __restrict__ trace, int trace_cap) {
+                                                               TraceEntry* __restrict__ trace, int trace_cap,
+                                                               const LadderRun* __restrict__ ladders) {
     ReplicaState& S = states[blockIdx.x];
     const long long M = moves[blockIdx.x];
-    if (M <= 0 || S.error) return;
+    // free-running ladders: every replica takes part in its ladder's barriers, also one
+    // with no moves or an error
+    const int n_iv = ladders != nullptr ? ladders->intervals[blockIdx.x] : 1;
+    if (ladders == nullptr && (M <= 0 || S.error)) return;
     __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
     __shared__ DevPairF s_pairs_f[kF32 ? kMaxTypes * kMaxTypes : 1];
     __shared__ float4 s_stab[kF32 ? kSTable : 1];
@@ -73,8 +105,8 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     FixPos* __restrict__ pos32 = S.pos32;
     const uint32_t* __restrict__ meta = S.meta;
     const float hf = static_cast<float>(L * kFixInv);
-    const unsigned long long move0 = S.moves;
-    const double beta = S.beta;
+    unsigned long long move0 = S.moves;
+    double beta = S.beta; // thread 0's (a ladder swap changes it between intervals)
     const double pref = S.rec_prefactor;
     const double surf = S.surf;
     double dip0 = S.dip[0], dip1 = S.dip[1], dip2 = S.dip[2]; // thread 0's running dipole
@@ -85,11 +117,13 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
-    long long t_done = M; // moves decided by this launch
     if (tid == 0) s_nhot = 0;
     __syncthreads();
 
-    for (long long t = 0; t < M; ++t) {
+    for (int iv = 0; iv < n_iv; ++iv) {
+    const long long Mi = (M > 0 && !S.error) ? M : 0;
+    long long t_done = Mi; // moves decided in this interval
+    for (long long t = 0; t < Mi; ++t) {
         if ((t & 63) == 0) {
             // the previous proposals were last read before this move's barriers; the
             // Philox blocks and sin/cos of 64 moves overlap instead of forming one serial
@@ -345,9 +379,6 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         __syncthreads(); // phase tables and the accepted position are consumed before the next move
     }
 
-#pragma unroll
-    for (int s = 0; s < kMaxKPT; ++s)
-        if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
     if (tid == 0) {
         S.moves = move0 + static_cast<unsigned long long>(t_done);
         S.accepted = accepted;
@@ -359,7 +390,42 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
         S.e_samples = e_samples;
         S.e_sum = e_sum;
         S.e_sum2 = e_sum2;
+        const int l = ladders != nullptr ? ladders->ladder_of[blockIdx.x] : -1;
+        if (l >= 0) {
+            const LadderRun& lr = *ladders;
+            // the ladder's barrier: the energy above is published before the arrival; the
+            // last replica to arrive swaps the rungs, the others wait for its release
+            __threadfence();
+            const unsigned target = static_cast<unsigned>(lr.size[l]) * static_cast<unsigned>(iv + 1);
+            if (atomicAdd(&lr.arrive[l], 1u) + 1u == target) {
+                __threadfence();
+                swap_ladder_rungs(states, lr.begin[l], lr.size[l], lr.rung_slot, l, lr.seed,
+                                  lr.round0 + static_cast<unsigned long long>(iv), lr.counters);
+                __threadfence();
+                __hip_atomic_store(&lr.gen[l], static_cast<unsigned>(iv + 1), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                long long spins = 0;
+                while (__hip_atomic_load(&lr.gen[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       static_cast<unsigned>(iv + 1)) {
+                    if (++spins > lr.spin_limit) {
+                        atomicMax(&S.error, 2); // a rung did not arrive: this replica stops
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(20);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            beta = __hip_atomic_load(&S.beta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
+    move0 += static_cast<unsigned long long>(t_done);
+    __syncthreads(); // the interval's state and error are written before the next interval reads them
+    }
+
+#pragma unroll
+    for (int s = 0; s < kMaxKPT; ++s)
+        if (s < kpt && k0 + s < nk) S.q[k0 + s] = q[s];
 }
 
 // Replica exchange along each ladder (odd/even neighbour pairs by round parity):
@@ -378,31 +444,12 @@ __global__ void ladder_swap_kernel(ReplicaState* states, const int* ladder_begin
         if (threadIdx.x == 0) *round_counter = round + 1ull;
     }
     if (l >= nladders) return;
-    const int b = ladder_begin[l], m = ladder_size[l];
-    for (int k = static_cast<int>(round & 1); k + 1 < m; k += 2) {
-        const int sa = rung_slot[b + k], sb = rung_slot[b + k + 1];
-        ReplicaState& A = states[sa];
-        ReplicaState& B = states[sb];
-        const double x = (A.beta - B.beta) * (A.energy - B.energy);
-        const Philox4 r = philox4x32_10(static_cast<uint32_t>(round), static_cast<uint32_t>(round >> 32),
-                                        static_cast<uint32_t>(l), 0x5a5a0000u + static_cast<uint32_t>(k),
-                                        static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
-        const double u = (static_cast<double>(r.v[0] >> 5) * 67108864.0 + static_cast<double>(r.v[1] >> 6)) *
-                         1.1102230246251565e-16;
-        atomicAdd(&counters[0], 1ull);
-        if (x >= 0.0 || u < exp(x)) {
-            const double tb = A.beta;
-            A.beta = B.beta;
-            B.beta = tb;
-            rung_slot[b + k] = sb;
-            rung_slot[b + k + 1] = sa;
-            atomicAdd(&counters[1], 1ull);
-        }
-    }
+    swap_ladder_rungs(states, ladder_begin[l], ladder_size[l], rung_slot, l, seed, round, counters);
 }
 
 template <int kThreads, int kMaxKPT, bool kCells, bool kF32>
-void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap) {
+void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m, TraceEntry* tr, int cap,
+                    const LadderRun* lr) {
     auto fn = mc_advance_kernel<kThreads, kMaxKPT, kCells, kF32>;
     // dynamic LDS: what the static LDS leaves of the CU's 160 KB, at most 96 KB (set once)
     static size_t limit = 0;
@@ -415,18 +462,34 @@ void launch_advance(int grid, size_t lds, hipStream_t st, ReplicaState* s, const
                   "hipFuncSetAttribute");
     }
     if (lds > limit) throw std::runtime_error("k-vector table too large for LDS");
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, s, m, tr, cap);
+    if (lr == nullptr) {
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, s, m, tr, cap, lr);
+        return;
+    }
+    // ladders wait for each other: every workgroup must be resident at once
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kThreads, lds),
+              "occupancy");
+    if (per_cu * prop.multiProcessorCount < grid)
+        throw std::runtime_error("run_ladders: the replicas cannot all be resident at once");
+    void* args[] = {&s, &m, &tr, &cap, &lr};
+    check_hip(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fn), dim3(grid), dim3(kThreads), args,
+                                         static_cast<unsigned>(lds), st),
+              "mc_advance_kernel cooperative launch");
 }
 
 template <int kThreads, int kMaxKPT>
 void dispatch_cells(bool cells, bool f32, int grid, size_t lds, hipStream_t st, ReplicaState* s, const long long* m,
-                    TraceEntry* tr, int cap) {
+                    TraceEntry* tr, int cap, const LadderRun* lr) {
     if (cells) {
-        if (f32) launch_advance<kThreads, kMaxKPT, true, true>(grid, lds, st, s, m, tr, cap);
-        else launch_advance<kThreads, kMaxKPT, true, false>(grid, lds, st, s, m, tr, cap);
+        if (f32) launch_advance<kThreads, kMaxKPT, true, true>(grid, lds, st, s, m, tr, cap, lr);
+        else launch_advance<kThreads, kMaxKPT, true, false>(grid, lds, st, s, m, tr, cap, lr);
     } else {
-        if (f32) launch_advance<kThreads, kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap);
-        else launch_advance<kThreads, kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap);
+        if (f32) launch_advance<kThreads, kMaxKPT, false, true>(grid, lds, st, s, m, tr, cap, lr);
+        else launch_advance<kThreads, kMaxKPT, false, false>(grid, lds, st, s, m, tr, cap, lr);
     }
 }
 
@@ -790,7 +853,8 @@ int advance_threads(const ReplicaState& s) {
 }
 
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream) {
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const LadderRun* ladders) {
+    const LadderRun* lr = ladders;
     // one launch per run of replicas with the same workgroup size (a function of each
     // replica alone, so that a replica's arithmetic does not depend on its batch)
     int first = 0;
@@ -815,10 +879,12 @@ void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_sta
         ReplicaState* ds = d_states + first;
         const long long* dm = d_moves + first;
         TraceEntry* tr = trace ? trace + static_cast<size_t>(first) * trace_cap : nullptr;
-        if (threads == 512) dispatch_cells<512, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
-        else if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
-        else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
-        else dispatch_cells<256, 68>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap);
+        if (lr != nullptr && (first != 0 || last != count))
+            throw std::runtime_error("run_ladders: every replica must take the same workgroup size");
+        if (threads == 512) dispatch_cells<512, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap, lr);
+        else if (maxkpt <= 8) dispatch_cells<256, 8>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap, lr);
+        else if (maxkpt <= 32) dispatch_cells<256, 32>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap, lr);
+        else dispatch_cells<256, 68>(cells, f32, grid, lds, stream, ds, dm, tr, trace_cap, lr);
         check_hip(hipGetLastError(), "mc_advance_kernel launch");
         first = last;
     }
@@ -1159,6 +1225,101 @@ void Batch::capture_interval(const std::vector<long long>& moves, unsigned long 
     const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     check_hip(ie, "hipGraphInstantiate");
+}
+
+// Free-running ladders: intervals[r] swap intervals of moves[r] moves per replica in one
+// cooperative launch of the translate-only kernel.  After each interval the replicas of a
+// ladder meet at a barrier in HBM and the last to arrive runs that ladder's round of swaps
+// (round0 + interval, the same rule and deviates as the captured interval), so a ladder
+// waits only for its own rungs: ladders of different sizes and interval counts share the
+// device without a device-wide barrier per interval.  The rungs of a ladder share its
+// interval count; replicas outside ladders just run their intervals.
+void Batch::run_ladders(const std::vector<long long>& moves, const std::vector<int>& intervals, unsigned long long seed,
+                        unsigned long long round0) {
+    if (!finalized_) finalize();
+    const int n = size();
+    if (static_cast<int>(moves.size()) != n || static_cast<int>(intervals.size()) != n)
+        throw std::runtime_error("run_ladders: one move count and one interval count per replica");
+    if (general_ == 1) throw std::runtime_error("run_ladders: translate-only replicas");
+    for (int r = 0; r < n; ++r) {
+        if (intervals[static_cast<size_t>(r)] < 0) throw std::runtime_error("run_ladders: interval counts >= 0");
+        if (h_states_[static_cast<size_t>(r)].parts > 1)
+            throw std::runtime_error("run_ladders: replicas shared by several workgroups");
+    }
+    sync();
+    std::vector<ReplicaState> cur(static_cast<size_t>(n));
+    check_hip(hipMemcpy(cur.data(), d_states_, sizeof(ReplicaState) * n, hipMemcpyDeviceToHost), "copy states");
+    // ladders as in capture_interval: ids >= 0 in increasing order, rungs by decreasing beta
+    std::vector<int> ids;
+    for (const auto& s : cur) ids.push_back(s.ladder);
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    std::vector<int> lb, ls, slots, ladder_of(static_cast<size_t>(n), -1);
+    for (int lid : ids) {
+        if (lid < 0) continue;
+        std::vector<int> members;
+        for (int r = 0; r < n; ++r)
+            if (cur[static_cast<size_t>(r)].ladder == lid) members.push_back(r);
+        std::sort(members.begin(), members.end(),
+                  [&](int a, int b) { return cur[static_cast<size_t>(a)].beta > cur[static_cast<size_t>(b)].beta; });
+        for (int r : members) {
+            if (intervals[static_cast<size_t>(r)] != intervals[static_cast<size_t>(members[0])])
+                throw std::runtime_error("run_ladders: the rungs of a ladder must share the interval count");
+            ladder_of[static_cast<size_t>(r)] = static_cast<int>(lb.size());
+        }
+        lb.push_back(static_cast<int>(slots.size()));
+        ls.push_back(static_cast<int>(members.size()));
+        slots.insert(slots.end(), members.begin(), members.end());
+    }
+    const size_t nl = lb.size();
+    // one allocation: counters (2 x u64), then intervals, ladder_of, begin, size, rung_slot,
+    // arrive, gen (ints); arrive and gen start at zero
+    std::vector<int> words;
+    words.insert(words.end(), intervals.begin(), intervals.end());
+    words.insert(words.end(), ladder_of.begin(), ladder_of.end());
+    words.insert(words.end(), lb.begin(), lb.end());
+    words.insert(words.end(), ls.begin(), ls.end());
+    words.insert(words.end(), slots.begin(), slots.end());
+    words.insert(words.end(), 2 * nl, 0);
+    // the descriptor first, then the counters (2 x u64), then the words
+    const size_t head = (sizeof(LadderRun) + 15) & ~static_cast<size_t>(15);
+    const size_t bytes = head + 16 + sizeof(int) * words.size();
+    std::vector<unsigned char> host(bytes, 0);
+    unsigned char* d = nullptr;
+    check_hip(hipMalloc(&d, bytes), "hipMalloc ladders");
+    try {
+        int* w = reinterpret_cast<int*>(d + head + 16);
+        LadderRun lr{};
+        lr.intervals = w;
+        lr.ladder_of = w + n;
+        lr.begin = w + 2 * n;
+        lr.size = w + 2 * n + nl;
+        lr.rung_slot = w + 2 * n + 2 * nl;
+        lr.arrive = reinterpret_cast<unsigned*>(w + 2 * n + 2 * nl + slots.size());
+        lr.gen = lr.arrive + nl;
+        lr.counters = reinterpret_cast<unsigned long long*>(d + head);
+        lr.seed = seed;
+        lr.round0 = round0;
+        lr.spin_limit = 1LL << 26;
+        std::memcpy(host.data(), &lr, sizeof(lr));
+        std::memcpy(host.data() + head + 16, words.data(), sizeof(int) * words.size());
+        check_hip(hipMemcpy(d_moves_, moves.data(), sizeof(long long) * n, hipMemcpyHostToDevice), "copy moves");
+        check_hip(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice), "copy ladders");
+        time_begin();
+        launch_advance(h_states_.data(), n, d_states_, d_moves_, fp32_ != 0, nullptr, 0, stream_,
+                       reinterpret_cast<const LadderRun*>(d));
+        time_end();
+        unsigned long long cnt[2] = {0, 0};
+        check_hip(hipMemcpyAsync(cnt, d + head, sizeof(cnt), hipMemcpyDeviceToHost, stream_), "copy swap counters");
+        sync();
+        swaps_att_ += static_cast<long long>(cnt[0]);
+        swaps_acc_ += static_cast<long long>(cnt[1]);
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipFree(d);
+        throw;
+    }
+    check_hip(hipFree(d), "hipFree ladders");
 }
 
 void Batch::replay(int count) {

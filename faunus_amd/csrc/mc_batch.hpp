@@ -101,6 +101,24 @@ struct ReplicaState {
     int threads;              // workgroup size of the translate-only kernel chosen by the tuner (0: the default rule)
 };
 
+// Free-running temperature ladders in one persistent launch (Batch::run_ladders): replica
+// r runs intervals[r] swap intervals of its moves[r] moves; after each, the replicas of
+// a ladder meet at a barrier in HBM and the last to arrive swaps the ladder's rungs, so
+// ladders of different sizes run their intervals at their own pace.  The kernel takes it
+// by a device pointer (null: one interval, no barrier, the ordinary launch).
+struct LadderRun {
+    const int* intervals;     // per replica
+    const int* ladder_of;     // per replica: its ladder's index into begin/size, -1 for none
+    const int* begin;         // per ladder: first entry in rung_slot
+    const int* size;          // per ladder: rungs
+    int* rung_slot;           // replica index per rung, ordered by decreasing beta
+    unsigned* arrive;         // per ladder: arrivals so far (zeroed before the launch)
+    unsigned* gen;            // per ladder: intervals whose swap is done (zeroed before the launch)
+    unsigned long long* counters; // swaps attempted, accepted
+    unsigned long long seed, round0;
+    long long spin_limit;     // polls of a waiting replica before it stops with error 2
+};
+
 // One recorded move of the trace (tests).
 struct TraceEntry {
     int index;
@@ -120,9 +138,11 @@ void pair_probe(Engine& e, int ta, int tb, double qa, double qb, int n, const do
 // SIMD) without a cell list and with K <= 4096, else 256.
 int advance_threads(const ReplicaState& s);
 // Persistent launches advancing `count` replicas (states already on the device), one
-// per run of replicas with the same workgroup size.
+// per run of replicas with the same workgroup size.  `ladders` (a device pointer, null
+// for one interval without swaps) makes it one cooperative launch of free-running
+// ladders.
 void launch_advance(const ReplicaState* h_states, int count, ReplicaState* d_states, const long long* d_moves, bool f32,
-                    TraceEntry* trace, int trace_cap, hipStream_t stream);
+                    TraceEntry* trace, int trace_cap, hipStream_t stream, const LadderRun* ladders = nullptr);
 // Same for replicas with a general move set (molecular moves, bonds).  Replicas with
 // parts > 1 are shared by that many workgroups: the launch is then cooperative, its
 // grid is the sum of the parts and block b advances part map[b].y of replica map[b].x
@@ -178,6 +198,10 @@ class Batch {
     // advancing replica r by moves[r] and one round of ladder swaps (device round counter
     // starting at round0); replay(count) launches it count times on the batch's stream.
     void capture_interval(const std::vector<long long>& moves, unsigned long long seed, unsigned long long round0);
+    // intervals[r] swap intervals of moves[r] moves for every replica in one cooperative
+    // launch, each ladder swapping after its own intervals (round round0 + interval)
+    void run_ladders(const std::vector<long long>& moves, const std::vector<int>& intervals, unsigned long long seed,
+                     unsigned long long round0);
     void replay(int count);
     // swaps attempted and accepted so far (host rounds and captured rounds)
     void swap_counts(long long* attempted, long long* accepted);

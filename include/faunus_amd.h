@@ -306,6 +306,19 @@ int fmc_batch_capture_interval(fmc_batch* batch, const long long* moves, unsigne
 int fmc_batch_replay(fmc_batch* batch, int count);
 int fmc_batch_swap_counts(fmc_batch* batch, long long* attempted, long long* accepted);
 /*
+ * Free-running ladders (ParallelTempering every interval, src/move.cpp:844-965): one
+ * cooperative launch in which replica r runs intervals[r] intervals of moves[r] moves;
+ * after each interval the rungs of a ladder meet at a barrier in device memory and the
+ * last to arrive runs that ladder's odd/even round of swaps (round round0 + interval,
+ * the rule and deviates of fmc_batch_capture_interval).  A ladder waits only for its own
+ * rungs, so ladders of different sizes run their intervals at their own pace; the rungs
+ * of one ladder must share the interval count.  Translate-only replicas of one workgroup
+ * size, all resident at once (an error otherwise).  Blocks until done; the swaps count
+ * in fmc_batch_swap_counts.  No reference counterpart as one call.
+ */
+int fmc_batch_run_ladders(fmc_batch* batch, const long long* moves, const int* intervals, unsigned long long seed,
+                          unsigned long long round0);
+/*
  * Tile autotuner: times each candidate launch shape of the batch on trial_moves moves
  * per replica (from checkpoints restored afterwards) and keeps the fastest for the whole
  * batch.  Candidates: 256 or 512 threads per workgroup for translate-only replicas
