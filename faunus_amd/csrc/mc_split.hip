@@ -1,13 +1,22 @@
 // One large replica advanced by many workgroups at once.  Workgroup g owns a
 // slice of the k-vectors (its Q(k) stays in registers for the whole launch);
 // workgroup 0 also evaluates the pair dU through the cell list and commits
-// accepted moves.  After each move every workgroup publishes its partial sums with
-// an agent-scope release store, waits for all others (relaxed polling with a bounded
-// spin, then one acquire fence), sums the partials with the same butterfly and so
+// accepted moves.  After each move every workgroup publishes its partial sums, waits
+// for all others (bounded spin), sums the partials with the same butterfly and so
 // takes the same decision as every other workgroup.  While wave 0 of workgroup 0
 // waits, its other waves load the next move's particle and stencil neighbours; the
 // next move uses them unless the accepted move changed one of the loaded cells.
 // The launch is cooperative, so all workgroups are resident.
+//
+// The exchange needs no fences.  Every published double travels as two words, its
+// high and its low 32 bits each beside the move's 32-bit tag, written and polled as
+// relaxed agent-scope atomics: a reader takes a value when both words carry the tag it
+// waits for, so no write has to be ordered before another and no cache has to be
+// written back or invalidated per move.  Only workgroup 0 reads or writes the
+// positions and the cell list after the launch starts: with its partials it publishes
+// the next move's particle position (as committed so far), and the other workgroups
+// take the next move's old position from there (or from the accepted move, when the
+// next move picks the same particle).
 //
 // A replica whose cell list overflows stops after the move that overflowed: that move
 // is committed (position, Q(k), energy), the particle stays listed in its old cell,
@@ -36,13 +45,29 @@ namespace {
 constexpr int kSplitMaxKPT = 8; // k-vectors per thread at 256 threads
 constexpr int kSplitMaxG = 64;
 constexpr int kGkMax = 1025;
-// set in a published tag: the publishing workgroup (0) stopped before this move
-constexpr unsigned long long kAbortTag = 1ull << 63;
+// published values per workgroup 0: de, pair sums at the new and the old position, and
+// the next move's particle x, y, z and charge (the others publish de only)
+constexpr int kSplitValues = 7;
+// tag bit set in a published word: the publishing workgroup (0) stopped before this move
+constexpr uint32_t kAbortBit = 0x80000000u;
 
-struct alignas(32) SplitSlot {
-    double de, pn, po;
-    unsigned long long seq;
+// One workgroup's exchange slot for one move parity: value v in words 2v (high half)
+// and 2v + 1 (low half), one 128-byte line.
+struct alignas(128) SplitSlot {
+    unsigned long long w[16];
 };
+
+// Tag of move `mid`: bit 30 set, so that zeroed slots never carry a valid tag; moves two
+// apart (the previous use of a slot) have different tags.
+__device__ __forceinline__ uint32_t split_tag(unsigned long long mid) {
+    return 0x40000000u | static_cast<uint32_t>((mid + 1ull) & 0x3fffffffull);
+}
+
+__device__ __forceinline__ void split_put(SplitSlot* sl, int v, double x, uint32_t tag) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    __hip_atomic_store(&sl->w[2 * v], (b & 0xffffffff00000000ull) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&sl->w[2 * v + 1], (b << 32) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ int cell_coord_s(double x, double half, double inv_cell, int nc) {
     int c = static_cast<int>(floor((x + half) * inv_cell));
@@ -75,13 +100,20 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     __shared__ double s_gk[kGkMax];
     __shared__ double s_red[3][kThreads / 64];
     __shared__ int s_accept, s_abort, s_stop;
-    __shared__ Proposal s_prop[64]; // proposals of the next 64 moves, one lane of wave 0 each
+    // proposals of the next 64 moves and the first of the following 64, one lane each
+    __shared__ Proposal s_prop[65];
     __shared__ double2 s_phase[6 * 65];
-    // Workgroup 0 at 512 threads with a cell list: waves 1-7 load the next move's
-    // particle and its stencil neighbours while wave 0 waits for the other workgroups'
-    // partials.  The loads are kept (s_pf_ok) unless the accepted move was the next
-    // particle or changed one of the 54 loaded cells (s_mv_cell).
+    __shared__ double s_npo[4]; // workgroups g > 0: the next move's particle (x, y, z, q)
+    __shared__ int s_nti;       // and its type
+    // Workgroup 0 at 512 threads with a cell list: waves 1-6 (7 lanes per stencil cell)
+    // load the next move's particle and its stencil neighbours while wave 0 waits for the
+    // other workgroups' partials.  The loads are kept (s_pf_ok) unless the accepted move
+    // was the next particle or changed one of the 54 loaded cells (s_mv_cell).
     constexpr bool kPre = kCells && kThreads >= 512;
+    constexpr int kPfLanes = 7;
+    // The last wave publishes the workgroup's partials, so that wave 0's polls do not
+    // wait for those stores to complete (the vector memory counter covers both).
+    constexpr int kPub = kThreads - 64;
     __shared__ double s_pf_pos[4];
     __shared__ int s_pf_ti, s_pf_ok;
     __shared__ int s_pf_cell_of[2]; // the next particle's cell_of entry, by move parity
@@ -101,10 +133,13 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
     for (int i = tid; i <= ncut * ncut && i < kGkMax; i += kThreads) s_gk[i] = S.gk[i];
 
-    // this workgroup's k-slice, interleaved over the threads
-    const int kw = (nk + G - 1) / G;
-    const int kb = g * kw;
-    const int ke = min(nk, kb + kw);
+    // this workgroup's k-slice, interleaved over the threads.  With G > 1 workgroup 0,
+    // whose pair walk and commit form the chain every move waits for, takes none: the
+    // k-vectors go to workgroups 1..G-1.
+    const int kparts = G > 1 ? G - 1 : 1;
+    const int kw = (nk + kparts - 1) / kparts;
+    const int kb = G > 1 ? (g == 0 ? 0 : (g - 1) * kw) : 0;
+    const int ke = G > 1 && g == 0 ? 0 : min(nk, kb + kw);
     int kn[kMaxKPT];
     double2 q[kMaxKPT], dq[kMaxKPT];
 #pragma unroll
@@ -128,11 +163,11 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
-    // the last two moves (identical in every workgroup): their writes to pos may not
-    // be visible yet, so reads of those particles take these values
-    int h_idx[2] = {-1, -1};
-    double h_pos[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    // this lane's loaded stencil cell of the next move (waves 1-7 of workgroup 0):
+    // thread 0: the next move's particle as committed so far (workgroup 0, published) or
+    // its type (the others)
+    double4 nxt = make_double4(0.0, 0.0, 0.0, 0.0);
+    int nxt_ti = 0;
+    // this lane's loaded stencil cell of the next move (waves 1-6 of workgroup 0):
     // the state's position, the cell, its count and its first two members
     double f_px = 0.0, f_py = 0.0, f_pz = 0.0;
     int f_cell = -1, f_cnt = 0, f_j0 = -1, f_j1 = -1, f_t0 = 0, f_t1 = 0;
@@ -144,29 +179,35 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
         if ((t & 63) == 0) {
             // the previous proposals were last read before this move's barriers
-            if (tid < 64) s_prop[tid] = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
+            if (tid < 65) s_prop[tid] = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
             __syncthreads();
         }
         const Proposal P = s_prop[t & 63];
         const int i = P.index;
+        const int i_next = s_prop[(t & 63) + 1].index;
+        if (t + 1 < M) {
+            // workgroup 0's commits so far are visible to its own threads
+            if (g == 0 && tid == kPub) nxt = pos[i_next];
+            if (g != 0 && tid == 0) nxt_ti = static_cast<int>(meta[i_next] & 0xffu);
+        }
         const bool pre = kPre && g == 0 && s_pf_ok;
         double4 po;
         int ti;
         if (pre) {
             po = make_double4(s_pf_pos[0], s_pf_pos[1], s_pf_pos[2], s_pf_pos[3]);
             ti = s_pf_ti;
+        } else if (g != 0 && t > 0) {
+            po = make_double4(s_npo[0], s_npo[1], s_npo[2], s_npo[3]);
+            ti = s_nti;
         } else {
-            po = pos[i];
-            // most recent accepted write of this particle wins
-            if (h_idx[1] == i) { po.x = h_pos[1][0]; po.y = h_pos[1][1]; po.z = h_pos[1][2]; }
-            else if (h_idx[0] == i) { po.x = h_pos[0][0]; po.y = h_pos[0][1]; po.z = h_pos[0][2]; }
+            po = pos[i]; // workgroup 0, or the launch's first move
             ti = static_cast<int>(meta[i] & 0xffu);
         }
         const double scale = s_dp[ti] * P.length_u;
         const double nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
         const double ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
         const double nz_ = wrap_s(po.z + P.dir[2] * scale, L, half, inv);
-        for (int e = tid; e < 6 * width; e += kThreads) {
+        for (int e = tid; e < (ke > kb ? 6 * width : 0); e += kThreads) {
             const int nn = e % width - ncut;
             const int axis = (e / width) % 3;
             const int st = e / (3 * width);
@@ -180,9 +221,9 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         if (g == 0) {
             const Box b{{L, L, L}, {half, half, half}};
             if (kPre) {
-                // 54 (state, cell) combos x 8 lanes on waves 1-7; with the loads of the
+                // 54 (state, cell) combos x 7 lanes on waves 1-6; with the loads of the
                 // previous move kept, the first two members of each lane come from registers
-                const int combo = (tid - 64) >> 3, slot0 = (tid - 64) & 7;
+                const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
                 if (tid >= 64 && combo < 54) {
                     double px, py, pz;
                     int cell, cnt;
@@ -204,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                         cnt = S.cell_count[cell];
                     }
                     double acc_u = 0.0;
-                    for (int k = slot0, u = 0; k < cnt; k += 8, ++u) {
+                    for (int k = slot0, u = 0; k < cnt; k += kPfLanes, ++u) {
                         int j, tj;
                         double4 pj;
                         if (pre && u == 0) {
@@ -279,12 +320,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             }
         }
         {
-            double a = de, b = e_new, c = e_old;
-            for (int off = 32; off > 0; off >>= 1) {
-                a += __shfl_xor(a, off, 64);
-                b += __shfl_xor(b, off, 64);
-                c += __shfl_xor(c, off, 64);
-            }
+            const double a = wave_sum(de), b = wave_sum(e_new), c = wave_sum(e_old);
             if ((tid & 63) == 0) {
                 s_red[0][tid >> 6] = a;
                 s_red[1][tid >> 6] = b;
@@ -293,34 +329,66 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         }
         __syncthreads();
         const int parity = static_cast<int>(mid & 1ull);
-        const unsigned long long tag = mid + 1ull;
-        if (tid == 0) {
-            SplitSlot* sl = slots + parity * G + g;
-            double a = 0.0, b = 0.0, c = 0.0;
+        const uint32_t tag = split_tag(mid);
+        // this workgroup's partials, summed in the same order by the publisher and by thread 0
+        double pa = 0.0, pb = 0.0, pc = 0.0;
+        if (tid == 0 || tid == kPub) {
             for (int w = 0; w < kThreads / 64; ++w) {
-                a += s_red[0][w];
-                b += s_red[1][w];
-                c += s_red[2][w];
+                pa += s_red[0][w];
+                pb += s_red[1][w];
+                pc += s_red[2][w];
             }
-            sl->de = a;
-            sl->pn = b;
-            sl->po = c;
-            __hip_atomic_store(&sl->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // gather every workgroup's partial (lane l of wave 0 waits for workgroup l) and
-        // add them with the same butterfly in every workgroup
+        if (tid == kPub) {
+            SplitSlot* sl = slots + parity * G + g;
+            if (g == 0) {
+                // the next particle's load (issued at the move's start) is waited for here,
+                // before the stores: a wait after them would wait for the stores too
+                asm volatile("" ::"v"(nxt.x), "v"(nxt.y), "v"(nxt.z), "v"(nxt.w));
+            }
+            split_put(sl, 0, pa, tag);
+            if (g == 0) {
+                split_put(sl, 1, pb, tag);
+                split_put(sl, 2, pc, tag);
+                split_put(sl, 3, nxt.x, tag);
+                split_put(sl, 4, nxt.y, tag);
+                split_put(sl, 5, nxt.z, tag);
+                split_put(sl, 6, nxt.w, tag);
+            }
+        }
+        // gather: job j < G is workgroup j's de, job G + v - 1 workgroup 0's value v
+        // (v = 1..6); lane l of wave 0 takes jobs l and l + 64, and a workgroup takes its
+        // own values from thread 0's registers, bit for bit what the others read.  The des
+        // are added with the same butterfly in every workgroup.
         double sde = 0.0, spn = 0.0, spo = 0.0; // totals, valid in thread 0
+        double4 npub = make_double4(0.0, 0.0, 0.0, 0.0); // published next particle, thread 0
         if (tid < 64) {
             int ok = 1, stopped = 0;
-            double a = 0.0, b = 0.0, c = 0.0;
-            if (tid < G) {
-                SplitSlot* sl = slots + parity * G + tid;
+            double a = 0.0, extra = 0.0;
+            // own values: workgroup 0 does not use the next particle it publishes
+            const double own[kSplitValues] = {readlane_f64(pa, 0), readlane_f64(pb, 0), readlane_f64(pc, 0), 0.0, 0.0,
+                                              0.0, 0.0};
+            for (int j = tid; j < G + kSplitValues - 1; j += 64) {
+                const int src = j < G ? j : 0;
+                const int v = j < G ? 0 : j - G + 1;
+                if (src == g) {
+                    double x = own[0];
+#pragma unroll
+                    for (int u = 1; u < kSplitValues; ++u)
+                        if (v == u) x = own[u];
+                    if (j < G) a = x;
+                    else extra = x;
+                    continue;
+                }
+                SplitSlot* sl = slots + parity * G + src;
                 long long spins = 0;
-                // relaxed polling, then one acquire fence (an acquire load per poll would
-                // invalidate the caches on every iteration)
-                unsigned long long v;
-                while ((v = __hip_atomic_load(&sl->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != tag) {
-                    if (v == (tag | kAbortTag)) {
+                unsigned long long hi, lo;
+                for (;;) {
+                    hi = __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo = __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t th = static_cast<uint32_t>(hi), tl = static_cast<uint32_t>(lo);
+                    if (th == tag && tl == tag) break;
+                    if (th == (tag | kAbortBit) || tl == (tag | kAbortBit)) {
                         stopped = 1;
                         break;
                     }
@@ -330,24 +398,25 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                if (ok && !stopped) {
-                    a = sl->de;
-                    b = sl->pn;
-                    c = sl->po;
-                }
+                const double x = (ok && !stopped)
+                                     ? __longlong_as_double(static_cast<long long>((hi & 0xffffffff00000000ull) | (lo >> 32)))
+                                     : 0.0;
+                if (j < G) a = x;
+                else extra = x;
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                a += __shfl_xor(a, off, 64);
-                b += __shfl_xor(b, off, 64);
-                c += __shfl_xor(c, off, 64);
-            }
+            a = wave_sum(a);
+            const double b = readlane_f64(extra, G & 63), c = readlane_f64(extra, (G + 1) & 63);
+            const double ex = readlane_f64(extra, (G + 2) & 63);
+            const double ey = readlane_f64(extra, (G + 3) & 63);
+            const double ez = readlane_f64(extra, (G + 4) & 63);
+            const double eq = readlane_f64(extra, (G + 5) & 63);
             const int all_ok = __all(ok);
             const int any_stop = __any(stopped);
             if (tid == 0) {
                 sde = a;
                 spn = b;
                 spo = c;
+                npub = make_double4(ex, ey, ez, eq);
                 if (any_stop) {
                     s_stop = 2; // workgroup 0 stopped after the previous move: this one is not decided
                 } else if (!all_ok) {
@@ -387,10 +456,19 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 dip1 = my;
                 dip2 = mz;
             }
+            if (g != 0) {
+                // the next move's particle: moved by this move, or as workgroup 0 published it
+                const bool same = acc && i_next == i;
+                s_npo[0] = same ? nx_ : npub.x;
+                s_npo[1] = same ? ny_ : npub.y;
+                s_npo[2] = same ? nz_ : npub.z;
+                s_npo[3] = npub.w;
+                s_nti = nxt_ti;
+            }
             if (g == 0) {
                 if (kPre) {
                     const long long tn = t + 1;
-                    s_pf_ok = (tn < M && (tn & 63) != 0 && !(acc && s_prop[tn & 63].index == i)) ? 1 : 0;
+                    s_pf_ok = (tn < M && !(acc && i_next == i)) ? 1 : 0;
                     s_mv_cell[0] = -1;
                     s_mv_cell[1] = -1;
                 }
@@ -451,10 +529,10 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         if (kPre && g == 0 && tid >= 64) {
             // load the next move's particle and stencil while wave 0 waits
             const long long tn = t + 1;
-            const int combo = (tid - 64) >> 3, slot0 = (tid - 64) & 7;
+            const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
             f_cell = -1;
-            if (tn < M && (tn & 63) != 0 && combo < 54) {
-                const Proposal& Q = s_prop[tn & 63];
+            if (tn < M && combo < 54) {
+                const Proposal& Q = s_prop[(t & 63) + 1];
                 const int in = Q.index;
                 const double4 qo = pos[in];
                 const int tq = static_cast<int>(meta[in] & 0xffu);
@@ -483,7 +561,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 f_cell = (cz * S.nc1 + cy) * S.nc0 + cx;
                 f_cnt = S.cell_count[f_cell];
                 f_j0 = slot0 < f_cnt ? S.cell_members[f_cell * S.cap + slot0] : -1;
-                f_j1 = slot0 + 8 < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + 8] : -1;
+                f_j1 = slot0 + kPfLanes < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + kPfLanes] : -1;
                 if (f_j0 >= 0) {
                     f_p0 = pos[f_j0];
                     f_t0 = static_cast<int>(meta[f_j0] & 0xffu);
@@ -512,21 +590,13 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 q[s].x += dq[s].x;
                 q[s].y += dq[s].y;
             }
-            h_idx[0] = h_idx[1];
-            h_pos[0][0] = h_pos[1][0];
-            h_pos[0][1] = h_pos[1][1];
-            h_pos[0][2] = h_pos[1][2];
-            h_idx[1] = i;
-            h_pos[1][0] = nx_;
-            h_pos[1][1] = ny_;
-            h_pos[1][2] = nz_;
         }
         if (s_stop == 1) {
-            // workgroup 0 committed this move and stopped on a full cell: the next move's
-            // tag carries the abort bit, so the other workgroups stop there as well
+            // workgroup 0 committed this move and stopped on a full cell: its words of the
+            // next move carry the abort bit, so the other workgroups stop there as well
             if (g == 0 && tid == 0 && t + 1 < M) {
                 SplitSlot* sl = slots + static_cast<int>((mid + 1ull) & 1ull) * G;
-                __hip_atomic_store(&sl->seq, (tag + 1ull) | kAbortTag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                for (int v = 0; v < kSplitValues; ++v) split_put(sl, v, 0.0, split_tag(mid + 1ull) | kAbortBit);
             }
             t_done = t + 1;
             break;
@@ -553,7 +623,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
 }
 
 // The split kernel for a replica over `groups` workgroups, every `stride`-th block of the
-// grid: 512 threads (two waves per SIMD, 8 lanes per stencil cell) when the slice fits 4
+// grid: 512 threads (two waves per SIMD, 7 lanes per stencil cell) when the slice fits 4
 // k-vectors per thread, else 256 threads with up to 8; the grid must be co-resident.
 struct SplitLaunch {
     const void* fn;
@@ -564,7 +634,8 @@ SplitLaunch split_launch(const ReplicaState& s, int groups, int stride, const ch
     const std::string w(who);
     if (groups < 1 || groups > kSplitMaxG) throw std::runtime_error(w + ": 1..64 workgroups");
     if (stride < 1) throw std::runtime_error(w + ": stride >= 1");
-    const int kw = (s.nk + groups - 1) / groups;
+    const int kparts = groups > 1 ? groups - 1 : 1; // workgroup 0 takes no k-vectors when G > 1
+    const int kw = (s.nk + kparts - 1) / kparts;
     const int threads = kw <= 4 * 512 ? 512 : 256;
     if ((kw + threads - 1) / threads > (threads == 512 ? 4 : kSplitMaxKPT))
         throw std::runtime_error(w + ": too many k-vectors per workgroup");

@@ -267,7 +267,7 @@ int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int strid
  * fmc_batch_run_split for a translate-only replica without allocating device memory.
  * No counterpart in the reference: for callers that cannot allocate (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The exchange
- * slots and the time-out flag live in a static workspace of 4.3 KB that each GPU keeps
+ * slots and the time-out flag live in a static workspace of 16.3 KB that each GPU keeps
  * for the life of the process (slots for at most 64 workgroups: more is an error); it is
  * cleared by a fill kernel on the batch's stream before every launch.  The batch must
  * already hold its device state (after any earlier fmc_batch_run* or
