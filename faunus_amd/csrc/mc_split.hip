@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     __shared__ int s_accept, s_abort, s_stop;
     // proposals of the next 64 moves and the first of the following 64, one lane each
     __shared__ Proposal s_prop[65];
+    __shared__ double s_thr[65]; // -log(u) of each proposal's acceptance deviate
     __shared__ double2 s_phase[6 * 65];
     __shared__ double s_npo[4]; // workgroups g > 0: the next move's particle (x, y, z, q)
     __shared__ int s_nti;       // and its type
@@ -167,11 +168,10 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     // its type (the others)
     double4 nxt = make_double4(0.0, 0.0, 0.0, 0.0);
     int nxt_ti = 0;
-    // this lane's loaded stencil cell of the next move (waves 1-6 of workgroup 0):
-    // the state's position, the cell, its count and its first two members
-    double f_px = 0.0, f_py = 0.0, f_pz = 0.0;
-    int f_cell = -1, f_cnt = 0, f_j0 = -1, f_j1 = -1, f_t0 = 0, f_t1 = 0;
-    double4 f_p0 = make_double4(0.0, 0.0, 0.0, 0.0), f_p1 = f_p0;
+    // this lane's stencil cell of the next move (waves 1-6 of workgroup 0) and its pair
+    // sum there, both from the look-ahead
+    int f_cell = -1;
+    double f_sum = 0.0;
     long long t_done = M; // moves decided by this launch
     __syncthreads();
 
@@ -179,7 +179,11 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
         if ((t & 63) == 0) {
             // the previous proposals were last read before this move's barriers
-            if (tid < 65) s_prop[tid] = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
+            if (tid < 65) {
+                const Proposal Q = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
+                s_prop[tid] = Q;
+                s_thr[tid] = -log(Q.accept_u);
+            }
             __syncthreads();
         }
         const Proposal P = s_prop[t & 63];
@@ -221,49 +225,29 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         if (g == 0) {
             const Box b{{L, L, L}, {half, half, half}};
             if (kPre) {
-                // 54 (state, cell) combos x 7 lanes on waves 1-6; with the loads of the
-                // previous move kept, the first two members of each lane come from registers
+                // 54 (state, cell) combos x 7 lanes on waves 1-6; a lane whose sum was
+                // computed during the previous move takes it from a register
                 const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
                 if (tid >= 64 && combo < 54) {
-                    double px, py, pz;
-                    int cell, cnt;
+                    double acc_u = 0.0;
                     if (pre) {
-                        px = f_px;
-                        py = f_py;
-                        pz = f_pz;
-                        cell = f_cell;
-                        cnt = f_cnt;
+                        acc_u = f_sum;
                     } else {
                         const int st = combo / 27, c = combo % 27;
-                        px = st == 0 ? nx_ : po.x;
-                        py = st == 0 ? ny_ : po.y;
-                        pz = st == 0 ? nz_ : po.z;
+                        const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
                         const int cx = (cell_coord_s(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
                         const int cy = (cell_coord_s(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
                         const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
-                        cell = (cz * S.nc1 + cy) * S.nc0 + cx;
-                        cnt = S.cell_count[cell];
-                    }
-                    double acc_u = 0.0;
-                    for (int k = slot0, u = 0; k < cnt; k += kPfLanes, ++u) {
-                        int j, tj;
-                        double4 pj;
-                        if (pre && u == 0) {
-                            j = f_j0;
-                            pj = f_p0;
-                            tj = f_t0;
-                        } else if (pre && u == 1) {
-                            j = f_j1;
-                            pj = f_p1;
-                            tj = f_t1;
-                        } else {
-                            j = S.cell_members[cell * S.cap + k];
+                        const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
+                        const int cnt = S.cell_count[cell];
+                        for (int k = slot0; k < cnt; k += kPfLanes) {
+                            const int j = S.cell_members[cell * S.cap + k];
                             if (j == i) continue;
-                            pj = pos[j];
-                            tj = static_cast<int>(meta[j] & 0xffu);
+                            const double4 pj = pos[j];
+                            const int tj = static_cast<int>(meta[j] & 0xffu);
+                            acc_u += pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w,
+                                                 dist2(b, px, py, pz, pj.x, pj.y, pj.z));
                         }
-                        if (j == i) continue;
-                        acc_u += pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w, dist2(b, px, py, pz, pj.x, pj.y, pj.z));
                     }
                     if (combo < 27) e_new = acc_u;
                     else e_old = acc_u;
@@ -382,10 +366,17 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 }
                 SplitSlot* sl = slots + parity * G + src;
                 long long spins = 0;
-                unsigned long long hi, lo;
+                // two polls in flight: the next pair of loads is issued before the
+                // previous pair is examined, so a value is seen about one memory round
+                // trip after it lands rather than up to two
+                unsigned long long hi = __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned long long lo = __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 for (;;) {
-                    hi = __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    lo = __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __builtin_amdgcn_s_sleep(1);
+                    const unsigned long long hi2 =
+                        __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long lo2 =
+                        __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const uint32_t th = static_cast<uint32_t>(hi), tl = static_cast<uint32_t>(lo);
                     if (th == tag && tl == tag) break;
                     if (th == (tag | kAbortBit) || tl == (tag | kAbortBit)) {
@@ -396,7 +387,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                         ok = 0;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    hi = hi2;
+                    lo = lo2;
                 }
                 const double x = (ok && !stopped)
                                      ? __longlong_as_double(static_cast<long long>((hi & 0xffffffff00000000ull) | (lo >> 32)))
@@ -449,7 +441,16 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             bool acc;
             if (isinf(bdu) && bdu < 0.0) acc = true;
             else if (-bdu > 709.782712893384) acc = true;
-            else acc = P.accept_u <= exp(-bdu);
+            else {
+                // u <= exp(-bdu) decided against the table's -log(u) away from the
+                // boundary (the margin exceeds the rounding of exp and log many times),
+                // with exp itself only within the margin: the same decision, without exp
+                // on the chain
+                const double thr = s_thr[t & 63], margin = 1e-12 * (1.0 + fabs(thr));
+                if (bdu < thr - margin) acc = true;
+                else if (bdu > thr + margin) acc = false;
+                else acc = P.accept_u <= exp(-bdu);
+            }
             s_accept = acc ? 1 : 0;
             if (acc) {
                 dip0 = mx;
@@ -527,17 +528,21 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             }
         }
         if (kPre && g == 0 && tid >= 64) {
-            // load the next move's particle and stencil while wave 0 waits
+            // while wave 0 waits: load the next move's particle and stencil and sum its
+            // pair terms there, as of the commits so far; the sums are kept unless this
+            // move's commit changes one of the loaded cells or moves that particle
             const long long tn = t + 1;
             const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
             f_cell = -1;
             if (tn < M && combo < 54) {
+                const Box b{{L, L, L}, {half, half, half}};
                 const Proposal& Q = s_prop[(t & 63) + 1];
                 const int in = Q.index;
                 const double4 qo = pos[in];
                 const int tq = static_cast<int>(meta[in] & 0xffu);
                 const double sc = s_dp[tq] * Q.length_u;
                 const int st = combo / 27, c = combo % 27;
+                double f_px, f_py, f_pz;
                 if (st == 0) {
                     f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
                     f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
@@ -559,17 +564,42 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 const int cy = (cell_coord_s(f_py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
                 const int cz = (cell_coord_s(f_pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
                 f_cell = (cz * S.nc1 + cy) * S.nc0 + cx;
-                f_cnt = S.cell_count[f_cell];
-                f_j0 = slot0 < f_cnt ? S.cell_members[f_cell * S.cap + slot0] : -1;
-                f_j1 = slot0 + kPfLanes < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + kPfLanes] : -1;
-                if (f_j0 >= 0) {
-                    f_p0 = pos[f_j0];
-                    f_t0 = static_cast<int>(meta[f_j0] & 0xffu);
+                const int f_cnt = S.cell_count[f_cell];
+                // the first two members are loaded at once, further ones in the loop
+                const int j0 = slot0 < f_cnt ? S.cell_members[f_cell * S.cap + slot0] : -1;
+                const int j1 = slot0 + kPfLanes < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + kPfLanes] : -1;
+                double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
+                int t0 = 0, t1 = 0;
+                if (j0 >= 0) {
+                    p0 = pos[j0];
+                    t0 = static_cast<int>(meta[j0] & 0xffu);
                 }
-                if (f_j1 >= 0) {
-                    f_p1 = pos[f_j1];
-                    f_t1 = static_cast<int>(meta[f_j1] & 0xffu);
+                if (j1 >= 0) {
+                    p1 = pos[j1];
+                    t1 = static_cast<int>(meta[j1] & 0xffu);
                 }
+                double fs = 0.0;
+                for (int k = slot0, u = 0; k < f_cnt; k += kPfLanes, ++u) {
+                    int j, tj;
+                    double4 pj;
+                    if (u == 0) {
+                        j = j0;
+                        pj = p0;
+                        tj = t0;
+                    } else if (u == 1) {
+                        j = j1;
+                        pj = p1;
+                        tj = t1;
+                    } else {
+                        j = S.cell_members[f_cell * S.cap + k];
+                        if (j == in) continue;
+                        pj = pos[j];
+                        tj = static_cast<int>(meta[j] & 0xffu);
+                    }
+                    if (j == in) continue;
+                    fs += pair_energy(s_pairs[tq * nt + tj], s_coul, qo.w, pj.w, dist2(b, f_px, f_py, f_pz, pj.x, pj.y, pj.z));
+                }
+                f_sum = fs;
             }
         }
         __syncthreads();
