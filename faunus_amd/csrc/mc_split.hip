@@ -118,6 +118,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     __shared__ double s_pf_pos[4];
     __shared__ int s_pf_ti, s_pf_ok;
     __shared__ int s_pf_cell_of[2]; // the next particle's cell_of entry, by move parity
+    __shared__ int s_cnew[2];       // the cell of a move's new position, by move parity
     __shared__ int s_mv_cell[2];
 
     const int tid = threadIdx.x;
@@ -153,6 +154,12 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     }
     double4* __restrict__ pos = S.pos;
     const uint32_t* __restrict__ meta = S.meta;
+    // the cell list, read once
+    const double inv_cell = S.inv_cell;
+    const int nc0 = S.nc0, nc1 = S.nc1, nc2 = S.nc2, ccap = S.cap;
+    int* __restrict__ cell_count = S.cell_count;
+    int* __restrict__ cell_members = S.cell_members;
+    int* __restrict__ cell_of = S.cell_of;
     const unsigned long long move0 = S.moves;
     const double beta = S.beta, pref = S.rec_prefactor;
     const double surf = S.surf;
@@ -235,13 +242,14 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     } else {
                         const int st = combo / 27, c = combo % 27;
                         const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
-                        const int cx = (cell_coord_s(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
-                        const int cy = (cell_coord_s(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
-                        const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
-                        const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
-                        const int cnt = S.cell_count[cell];
+                        const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                        const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                        const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                        const int cell = (cz * nc1 + cy) * nc0 + cx;
+                        if (combo == 13 && slot0 == 0) s_cnew[t & 1] = cell; // the new position's own cell
+                        const int cnt = cell_count[cell];
                         for (int k = slot0; k < cnt; k += kPfLanes) {
-                            const int j = S.cell_members[cell * S.cap + k];
+                            const int j = cell_members[cell * ccap + k];
                             if (j == i) continue;
                             const double4 pj = pos[j];
                             const int tj = static_cast<int>(meta[j] & 0xffu);
@@ -259,13 +267,13 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 if (combo < 54) {
                     const int st = combo / 27, c = combo % 27;
                     const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
-                    const int cx = (cell_coord_s(px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
-                    const int cy = (cell_coord_s(py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
-                    const int cz = (cell_coord_s(pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
-                    const int cell = (cz * S.nc1 + cy) * S.nc0 + cx;
-                    const int cnt = S.cell_count[cell];
+                    const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                    const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                    const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                    const int cell = (cz * nc1 + cy) * nc0 + cx;
+                    const int cnt = cell_count[cell];
                     for (int k = slot0; k < cnt; k += kLanes) {
-                        const int j = S.cell_members[cell * S.cap + k];
+                        const int j = cell_members[cell * ccap + k];
                         if (j == i) continue;
                         const double4 pj = pos[j];
                         const double u = pair_energy(s_pairs[ti * nt + static_cast<int>(meta[j] & 0xffu)], s_coul, po.w,
@@ -488,33 +496,36 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     ++accepted;
                     pos[i] = make_double4(nx_, ny_, nz_, po.w);
                     if (kCells) {
-                        const int cnew = (cell_coord_s(nz_, half, S.inv_cell, S.nc2) * S.nc1 +
-                                          cell_coord_s(ny_, half, S.inv_cell, S.nc1)) * S.nc0 +
-                                         cell_coord_s(nx_, half, S.inv_cell, S.nc0);
+                        // workgroup 0 with the look-ahead: the stencil lane of the new
+                        // position's own cell found it already
+                        const int cnew = kPre ? s_cnew[t & 1]
+                                              : (cell_coord_s(nz_, half, inv_cell, nc2) * nc1 +
+                                                 cell_coord_s(ny_, half, inv_cell, nc1)) * nc0 +
+                                                    cell_coord_s(nx_, half, inv_cell, nc0);
                         // cell_of changes only when its particle moves, so the loaded entry is
                         // current whenever the other loads of this move were kept
-                        const int cold = pre ? s_pf_cell_of[t & 1] : S.cell_of[i];
+                        const int cold = pre ? s_pf_cell_of[t & 1] : cell_of[i];
                         if (kPre) {
                             s_mv_cell[0] = cold;
                             s_mv_cell[1] = cnew;
                         }
                         if (cnew != cold) {
-                            const int nn = S.cell_count[cnew];
-                            if (nn >= S.cap) {
+                            const int nn = cell_count[cnew];
+                            if (nn >= ccap) {
                                 // full cell: the lists are left as they are and the chain stops
                                 atomicMax(&S.error, 1);
                                 s_stop = 1;
                             } else {
-                                const int nold = S.cell_count[cold];
+                                const int nold = cell_count[cold];
                                 for (int kk = 0; kk < nold; ++kk)
-                                    if (S.cell_members[cold * S.cap + kk] == i) {
-                                        S.cell_members[cold * S.cap + kk] = S.cell_members[cold * S.cap + nold - 1];
+                                    if (cell_members[cold * ccap + kk] == i) {
+                                        cell_members[cold * ccap + kk] = cell_members[cold * ccap + nold - 1];
                                         break;
                                     }
-                                S.cell_count[cold] = nold - 1;
-                                S.cell_members[cnew * S.cap + nn] = i;
-                                S.cell_count[cnew] = nn + 1;
-                                S.cell_of[i] = cnew;
+                                cell_count[cold] = nold - 1;
+                                cell_members[cnew * ccap + nn] = i;
+                                cell_count[cnew] = nn + 1;
+                                cell_of[i] = cnew;
                             }
                         }
                     }
@@ -558,16 +569,17 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     s_pf_pos[2] = qo.z;
                     s_pf_pos[3] = qo.w;
                     s_pf_ti = tq;
-                    s_pf_cell_of[tn & 1] = S.cell_of[in];
+                    s_pf_cell_of[tn & 1] = cell_of[in];
                 }
-                const int cx = (cell_coord_s(f_px, half, S.inv_cell, S.nc0) + (c % 3) - 1 + S.nc0) % S.nc0;
-                const int cy = (cell_coord_s(f_py, half, S.inv_cell, S.nc1) + ((c / 3) % 3) - 1 + S.nc1) % S.nc1;
-                const int cz = (cell_coord_s(f_pz, half, S.inv_cell, S.nc2) + (c / 9) - 1 + S.nc2) % S.nc2;
-                f_cell = (cz * S.nc1 + cy) * S.nc0 + cx;
-                const int f_cnt = S.cell_count[f_cell];
+                const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                f_cell = (cz * nc1 + cy) * nc0 + cx;
+                if (combo == 13 && slot0 == 0) s_cnew[tn & 1] = f_cell;
+                const int f_cnt = cell_count[f_cell];
                 // the first two members are loaded at once, further ones in the loop
-                const int j0 = slot0 < f_cnt ? S.cell_members[f_cell * S.cap + slot0] : -1;
-                const int j1 = slot0 + kPfLanes < f_cnt ? S.cell_members[f_cell * S.cap + slot0 + kPfLanes] : -1;
+                const int j0 = slot0 < f_cnt ? cell_members[f_cell * ccap + slot0] : -1;
+                const int j1 = slot0 + kPfLanes < f_cnt ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
                 double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
                 int t0 = 0, t1 = 0;
                 if (j0 >= 0) {
@@ -591,7 +603,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                         pj = p1;
                         tj = t1;
                     } else {
-                        j = S.cell_members[f_cell * S.cap + k];
+                        j = cell_members[f_cell * ccap + k];
                         if (j == in) continue;
                         pj = pos[j];
                         tj = static_cast<int>(meta[j] & 0xffu);
