@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 224.1e6
+HEADLINE_TRAFFIC_BYTES = 381.2e6
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 STEP_MOVES = 20000
@@ -899,8 +899,8 @@ def main():
         "data": "synthetic (random sequential addition, seed 1; device Philox proposals)",
         "config": {"workload": "N=20000 Na/Cl LJ(LB)+Ewald(alpha 0.25, Rc 12, kmax 20, K=17328), pair energy "
                                "nonbonded_splined (rmax 11.99 A, utol 1e-3), L=255 A, translate dp=2 A, "
-                               "device-resident Metropolis, %d cooperating workgroups of 512 threads (k-vector "
-                               "slices)" % HEADLINE_GROUPS,
+                               "device-resident Metropolis, %d cooperating workgroups of 512 threads (workgroup 0: "
+                               "pair walk and commit; the others: k-vector slices)" % HEADLINE_GROUPS,
                    "step_moves": STEP_MOVES, "acceptance": d["acceptance"], "energy_drift": d["drift"]},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": HEADLINE_TRAFFIC_BYTES,
@@ -908,12 +908,14 @@ def main():
                      "algorithmic_bytes_per_move": algo_bytes, "kernel_us_per_move": kernel_s_per_move * 1e6,
                      "fp64_tflops": achieved_tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                      "algorithmic_flops_per_move": flops,
-                     "limiter": "per-move latency chain (one Metropolis chain: proposal -> pair walk on workgroup 0 "
-                                "-> Ewald slice and block reduction -> cross-CU exchange -> decision and commit)",
+                     "limiter": "per-move latency chain across CUs (one Metropolis chain: decision and commit on "
+                                "workgroup 0 -> one-way exchange of about 1.5 us -> phase tables and Ewald slices "
+                                "on workgroups 1-15 -> exchange back; the pair walk runs one move ahead)",
                      "note": "one launch runs STEP_MOVES sequential moves; achieved = algorithmic bytes per move "
                              "/ kernel time per move (HIP events); Q(k) stays in registers, so the measured HBM "
-                             "traffic (about 12 KB per move) is far below the algorithmic bytes; neither HBM nor "
-                             "the FP64 VALU binds: the move time is a chain of dependent latencies"},
+                             "traffic (about 19 KB per move, mostly the exchange's polls) is far below the "
+                             "algorithmic bytes; neither HBM nor the FP64 VALU binds: the move time is a chain of "
+                             "dependent latencies"},
         "cpu_baseline": cpu,
         "forms": {
             "plain_fp64_one_move_per_launch": {"moves_per_s": out["plain"]["rate"],
