@@ -150,10 +150,32 @@ __device__ __forceinline__ void lds_copy(T& dst, const T* __restrict__ src, int 
     for (int i = tid; i < static_cast<int>(sizeof(T) / 16); i += nthreads) d[i] = s[i];
 }
 
-// Wavefront (64 lanes) sum with cross-lane shuffles, then across the block's waves.
+// Lane `lane` (uniform) of x, in every lane.
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Sum of x over the 64 lanes of a wave, the same value in every lane and the same order
+// of additions on every call: DPP moves within rows of 16 lanes (pair and quad swaps,
+// half-row and row mirrors; no LDS round trip), then the four row sums read from lanes
+// 0, 16, 32 and 48.  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum(double x) {
+    x += __builtin_amdgcn_update_dpp(0.0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    x += __builtin_amdgcn_update_dpp(0.0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    x += __builtin_amdgcn_update_dpp(0.0, x, 0x141, 0xF, 0xF, false); // row_half_mirror
+    x += __builtin_amdgcn_update_dpp(0.0, x, 0x140, 0xF, 0xF, false); // row_mirror
+    const double r0 = readlane_f64(x, 0), r1 = readlane_f64(x, 16);
+    const double r2 = readlane_f64(x, 32), r3 = readlane_f64(x, 48);
+    return (r0 + r1) + (r2 + r3);
+}
+
+// Wavefront (64 lanes) sum, then across the block's waves.
 template <int kThreads>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = wave_sum(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) lds[wave] = v;

@@ -258,12 +258,7 @@ __global__ __launch_bounds__(kThreads) void mc_advance_kernel(ReplicaState* __re
 
         // block reduction (fixed order) and the acceptance test on thread 0
         {
-            double a = e_new, b = e_old, c = de;
-            for (int off = 32; off > 0; off >>= 1) {
-                a += __shfl_xor(a, off, 64);
-                b += __shfl_xor(b, off, 64);
-                c += __shfl_xor(c, off, 64);
-            }
+            const double a = wave_sum(e_new), b = wave_sum(e_old), c = wave_sum(de);
             if ((tid & 63) == 0) {
                 s_red[0][tid >> 6] = a;
                 s_red[1][tid >> 6] = b;

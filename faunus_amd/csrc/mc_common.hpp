@@ -22,28 +22,6 @@ __global__ void zero_words_kernel(unsigned long long* __restrict__ p, int words)
 }
 } // namespace
 
-// Lane `lane` (uniform) of x, in every lane.
-__device__ __forceinline__ double readlane_f64(double x, int lane) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-
-// Sum of x over the 64 lanes of a wave, the same value in every lane and the same order
-// of additions on every call: DPP moves within rows of 16 lanes (pair and quad swaps,
-// half-row and row mirrors; no LDS round trip), then the four row sums read from lanes
-// 0, 16, 32 and 48.  Every lane of the wave must be active.
-__device__ __forceinline__ double wave_sum(double x) {
-    x += __builtin_amdgcn_update_dpp(0.0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-    x += __builtin_amdgcn_update_dpp(0.0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-    x += __builtin_amdgcn_update_dpp(0.0, x, 0x141, 0xF, 0xF, false); // row_half_mirror
-    x += __builtin_amdgcn_update_dpp(0.0, x, 0x140, 0xF, 0xF, false); // row_mirror
-    const double r0 = readlane_f64(x, 0), r1 = readlane_f64(x, 16);
-    const double r2 = readlane_f64(x, 32), r3 = readlane_f64(x, 48);
-    return (r0 + r1) + (r2 + r3);
-}
-
 struct alignas(16) DevPairF {
     float lj_s2, lj_e4, wca_s2, wca_e4, hs_s2, coul;
     int spl;

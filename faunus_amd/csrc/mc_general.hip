@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                    v[r] = wave_sum(v[r]);
                 if ((tid & 63) == 0)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) s_cmred[r][tid >> 6] = v[r];
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             }
 #pragma unroll
             for (int r = 0; r < 3; ++r)
-                for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                v[r] = wave_sum(v[r]);
             if ((tid & 63) == 0)
 #pragma unroll
                 for (int r = 0; r < 3; ++r) s_cmred[r][tid >> 6] = v[r];
@@ -991,7 +991,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
             double v[5] = {e_new, e_old, de, b_new, b_old};
 #pragma unroll
             for (int r = 0; r < 5; ++r)
-                for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                v[r] = wave_sum(v[r]);
             if ((tid & 63) == 0)
 #pragma unroll
                 for (int r = 0; r < 5; ++r) s_red[r][tid >> 6] = v[r];
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(kThreads) void mc_general_kernel(ReplicaState* __re
                 }
 #pragma unroll
                 for (int r = 0; r < 5; ++r)
-                    for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off, 64);
+                    v[r] = wave_sum(v[r]);
                 const int all_ok = __all(ok);
                 if (tid == 0) {
                     for (int r = 0; r < 5; ++r) tot[r] = v[r];

@@ -380,10 +380,6 @@ __global__ __launch_bounds__(256) void mc_pipe_kernel(ReplicaState* __restrict__
         }
         // block reduction: [a, pn, po, c(position 0..D-1)]
         {
-            auto wave_sum = [](double x) {
-                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-                return x;
-            };
             const double ra = wave_sum(acc_a), rpn = wave_sum(acc_pn), rpo = wave_sum(acc_po);
             double rc[D];
 #pragma unroll
