@@ -538,6 +538,7 @@ Batch::~Batch() {
     (void)hipFree(d_moves_);
     (void)hipFree(d_trace_);
     (void)hipFree(split_slots_);
+    (void)hipFree(split_priv_);
     (void)hipFree(pipe_slots_);
     (void)hipFree(split_err_);
     (void)hipFree(d_map_);

@@ -264,6 +264,8 @@ class Batch {
     void time_begin();
     void time_end();
     void* split_slots_ = nullptr;
+    void* split_priv_ = nullptr; // private position copies of the split kernel's workgroups 1..G-1
+    size_t split_priv_bytes_ = 0;
     void* pipe_slots_ = nullptr;
     int* split_err_ = nullptr;
     int2* d_map_ = nullptr; // block map of a split general launch

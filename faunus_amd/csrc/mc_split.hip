@@ -12,11 +12,12 @@
 // high and its low 32 bits each beside the move's 32-bit tag, written and polled as
 // relaxed agent-scope atomics: a reader takes a value when both words carry the tag it
 // waits for, so no write has to be ordered before another and no cache has to be
-// written back or invalidated per move.  Only workgroup 0 reads or writes the
-// positions and the cell list after the launch starts: with its partials it publishes
-// the next move's particle position (as committed so far), and the other workgroups
-// take the next move's old position from there (or from the accepted move, when the
-// next move picks the same particle).
+// written back or invalidated per move.  Only workgroup 0 reads or writes the shared
+// positions and the cell list after the launch starts.  The other workgroups keep
+// private copies of the positions (copied at the launch's start, each accepted move
+// applied by every workgroup), so workgroup 0 publishes a single value, the pair energy
+// change; without the copies (the no-allocation entry point) it also publishes the next
+// move's particle position as committed so far.
 //
 // A replica whose cell list overflows stops after the move that overflowed: that move
 // is committed (position, Q(k), energy), the particle stays listed in its old cell,
@@ -45,9 +46,9 @@ namespace {
 constexpr int kSplitMaxKPT = 8; // k-vectors per thread at 256 threads
 constexpr int kSplitMaxG = 64;
 constexpr int kGkMax = 1025;
-// published values per workgroup 0: de, pair sums at the new and the old position, and
-// the next move's particle x, y, z and charge (the others publish de only)
-constexpr int kSplitValues = 7;
+// published values: workgroup 0 the pair energy change and, without private position
+// copies, the next move's particle x, y, z and charge; the others their de
+constexpr int kSplitValues = 5;
 // tag bit set in a published word: the publishing workgroup (0) stopped before this move
 constexpr uint32_t kAbortBit = 0x80000000u;
 
@@ -88,7 +89,7 @@ template <int kThreads, int kMaxKPT, bool kCells>
 __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
                                                        int stride, SplitSlot* __restrict__ slots,
                                                        int* __restrict__ err, TraceEntry* __restrict__ trace,
-                                                       int trace_cap, long long spin_limit) {
+                                                       int trace_cap, long long spin_limit, double4* __restrict__ priv) {
     if (blockIdx.x % stride != 0) return;
     const int g = blockIdx.x / stride;
     if (g >= G) return;
@@ -154,6 +155,12 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     }
     double4* __restrict__ pos = S.pos;
     const uint32_t* __restrict__ meta = S.meta;
+    // workgroups g > 0: this workgroup's private copy of the positions (null: none)
+    double4* __restrict__ mine = (priv != nullptr && G > 1 && g > 0)
+                                     ? priv + static_cast<size_t>(g - 1) * static_cast<size_t>(S.n)
+                                     : nullptr;
+    if (mine != nullptr)
+        for (int j = tid; j < S.n; j += kThreads) mine[j] = pos[j];
     // the cell list, read once
     const double inv_cell = S.inv_cell;
     const int nc0 = S.nc0, nc1 = S.nc1, nc2 = S.nc2, ccap = S.cap;
@@ -171,8 +178,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
-    // thread 0: the next move's particle as committed so far (workgroup 0, published) or
-    // its type (the others)
+    // the next move's particle as committed so far: workgroup 0's publisher (published
+    // without private copies) or thread 0 of the others (from the private copy); its type
     double4 nxt = make_double4(0.0, 0.0, 0.0, 0.0);
     int nxt_ti = 0;
     // this lane's stencil cell of the next move (waves 1-6 of workgroup 0) and its pair
@@ -197,9 +204,13 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         const int i = P.index;
         const int i_next = s_prop[(t & 63) + 1].index;
         if (t + 1 < M) {
-            // workgroup 0's commits so far are visible to its own threads
-            if (g == 0 && tid == kPub) nxt = pos[i_next];
-            if (g != 0 && tid == 0) nxt_ti = static_cast<int>(meta[i_next] & 0xffu);
+            // workgroup 0's commits so far are visible to its own threads, and each
+            // workgroup's updates of its private copy to itself
+            if (g == 0 && tid == kPub && mine == nullptr && priv == nullptr) nxt = pos[i_next];
+            if (g != 0 && tid == 0) {
+                nxt_ti = static_cast<int>(meta[i_next] & 0xffu);
+                if (mine != nullptr) nxt = mine[i_next];
+            }
         }
         const bool pre = kPre && g == 0 && s_pf_ok;
         double4 po;
@@ -331,91 +342,90 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 pc += s_red[2][w];
             }
         }
-        if (tid == kPub) {
+        // workgroup 0's pair energy change as one value: -inf for a NaN old sum (and a new
+        // one that is not NaN), +inf for a NaN or +inf new sum, else new - old.  The
+        // decision adds the reciprocal change to a finite one; for a finite reciprocal
+        // change this is getEnergyChange's handling of the two sums (montecarlo.cpp:193-209),
+        // and a NaN one gives +inf as there
+        const double pd = (isnan(pc) && !isnan(pb)) ? -__builtin_huge_val()
+                          : (isnan(pb) || (pb > 0.0 && isinf(pb))) ? __builtin_huge_val()
+                                                                    : pb - pc;
+        const bool particle_msg = priv == nullptr && G > 1; // workgroup 0 also sends the next particle
+        if (tid == kPub && G > 1) {
             SplitSlot* sl = slots + parity * G + g;
             if (g == 0) {
-                // the next particle's load (issued at the move's start) is waited for here,
-                // before the stores: a wait after them would wait for the stores too
-                asm volatile("" ::"v"(nxt.x), "v"(nxt.y), "v"(nxt.z), "v"(nxt.w));
-            }
-            split_put(sl, 0, pa, tag);
-            if (g == 0) {
-                split_put(sl, 1, pb, tag);
-                split_put(sl, 2, pc, tag);
-                split_put(sl, 3, nxt.x, tag);
-                split_put(sl, 4, nxt.y, tag);
-                split_put(sl, 5, nxt.z, tag);
-                split_put(sl, 6, nxt.w, tag);
+                if (particle_msg) {
+                    // the next particle's load (issued at the move's start) is waited for
+                    // here, before the stores: a wait after them would wait for the stores too
+                    asm volatile("" ::"v"(nxt.x), "v"(nxt.y), "v"(nxt.z), "v"(nxt.w));
+                }
+                split_put(sl, 0, pd, tag);
+                if (particle_msg) {
+                    split_put(sl, 1, nxt.x, tag);
+                    split_put(sl, 2, nxt.y, tag);
+                    split_put(sl, 3, nxt.z, tag);
+                    split_put(sl, 4, nxt.w, tag);
+                }
+            } else {
+                split_put(sl, 0, pa, tag);
             }
         }
-        // gather: job j < G is workgroup j's de, job G + v - 1 workgroup 0's value v
-        // (v = 1..6); lane l of wave 0 takes jobs l and l + 64, and a workgroup takes its
-        // own values from thread 0's registers, bit for bit what the others read.  The des
-        // are added with the same butterfly in every workgroup.
-        double sde = 0.0, spn = 0.0, spo = 0.0; // totals, valid in thread 0
+        // gather: lane j < G takes workgroup j's value (workgroup 0's pair change, the
+        // others' de); lanes 58-61 take workgroup 0's next particle when it is sent.  A
+        // workgroup takes its own value from registers, bit for bit what the others read.
+        // The des are added with the same butterfly in every workgroup.
+        double sde = 0.0, spd = 0.0;                      // totals, valid in thread 0
         double4 npub = make_double4(0.0, 0.0, 0.0, 0.0); // published next particle, thread 0
         if (tid < 64) {
             int ok = 1, stopped = 0;
-            double a = 0.0, extra = 0.0;
-            // own values: workgroup 0 does not use the next particle it publishes
-            const double own[kSplitValues] = {readlane_f64(pa, 0), readlane_f64(pb, 0), readlane_f64(pc, 0), 0.0, 0.0,
-                                              0.0, 0.0};
-            for (int j = tid; j < G + kSplitValues - 1; j += 64) {
-                const int src = j < G ? j : 0;
-                const int v = j < G ? 0 : j - G + 1;
-                if (src == g) {
-                    double x = own[0];
-#pragma unroll
-                    for (int u = 1; u < kSplitValues; ++u)
-                        if (v == u) x = own[u];
-                    if (j < G) a = x;
-                    else extra = x;
-                    continue;
-                }
+            double a = 0.0, pdv = 0.0, extra = 0.0;
+            const double own_de = readlane_f64(pa, 0), own_pd = readlane_f64(pd, 0);
+            auto poll = [&](int src, int v) -> double {
                 SplitSlot* sl = slots + parity * G + src;
                 long long spins = 0;
-                // two polls in flight: the next pair of loads is issued before the
-                // previous pair is examined, so a value is seen about one memory round
-                // trip after it lands rather than up to two
-                unsigned long long hi = __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                unsigned long long lo = __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned long long hi, lo;
                 for (;;) {
-                    __builtin_amdgcn_s_sleep(1);
-                    const unsigned long long hi2 =
-                        __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const unsigned long long lo2 =
-                        __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    hi = __hip_atomic_load(&sl->w[2 * v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo = __hip_atomic_load(&sl->w[2 * v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const uint32_t th = static_cast<uint32_t>(hi), tl = static_cast<uint32_t>(lo);
                     if (th == tag && tl == tag) break;
                     if (th == (tag | kAbortBit) || tl == (tag | kAbortBit)) {
                         stopped = 1;
-                        break;
+                        return 0.0;
                     }
                     if (++spins > spin_limit) {
                         ok = 0;
-                        break;
+                        return 0.0;
                     }
-                    hi = hi2;
-                    lo = lo2;
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                const double x = (ok && !stopped)
-                                     ? __longlong_as_double(static_cast<long long>((hi & 0xffffffff00000000ull) | (lo >> 32)))
-                                     : 0.0;
-                if (j < G) a = x;
-                else extra = x;
+                return __longlong_as_double(static_cast<long long>((hi & 0xffffffff00000000ull) | (lo >> 32)));
+            };
+            if (tid < G) {
+                if (tid == g) {
+                    if (g == 0) {
+                        pdv = own_pd;
+                        a = G == 1 ? own_de : 0.0; // with G > 1 workgroup 0 holds no k-vectors
+                    } else {
+                        a = own_de;
+                    }
+                } else {
+                    const double x = poll(tid, 0);
+                    if (tid == 0) pdv = x;
+                    else a = x;
+                }
             }
+            const int e = tid - 58; // 0..3 on lanes 58-61
+            if (particle_msg && g != 0 && e >= 0 && e < 4) extra = poll(0, 1 + e);
             a = wave_sum(a);
-            const double b = readlane_f64(extra, G & 63), c = readlane_f64(extra, (G + 1) & 63);
-            const double ex = readlane_f64(extra, (G + 2) & 63);
-            const double ey = readlane_f64(extra, (G + 3) & 63);
-            const double ez = readlane_f64(extra, (G + 4) & 63);
-            const double eq = readlane_f64(extra, (G + 5) & 63);
+            const double pdt = readlane_f64(pdv, 0);
+            const double ex = readlane_f64(extra, 58), ey = readlane_f64(extra, 59);
+            const double ez = readlane_f64(extra, 60), eq = readlane_f64(extra, 61);
             const int all_ok = __all(ok);
             const int any_stop = __any(stopped);
             if (tid == 0) {
                 sde = a;
-                spn = b;
-                spo = c;
+                spd = pdt;
                 npub = make_double4(ex, ey, ez, eq);
                 if (any_stop) {
                     s_stop = 2; // workgroup 0 stopped after the previous move: this one is not decided
@@ -436,15 +446,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
             }
             const double drec = pref * sde + dsurf;
-            const double e_tot_new = spn + drec;
-            double du;
-            if (isnan(spo) && !isnan(e_tot_new)) du = -__builtin_huge_val();
-            else if (isnan(e_tot_new)) du = __builtin_huge_val();
-            else if (e_tot_new > 0.0 && isinf(e_tot_new)) du = __builtin_huge_val();
-            else {
-                du = (spn - spo) + drec;
-                if (isnan(du)) du = 0.0;
-            }
+            double du = isnan(drec) ? __builtin_huge_val() : (isinf(spd) ? spd : spd + drec);
+            if (isnan(du)) du = 0.0;
             const double bdu = beta * du;
             bool acc;
             if (isinf(bdu) && bdu < 0.0) acc = true;
@@ -466,13 +469,16 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 dip2 = mz;
             }
             if (g != 0) {
-                // the next move's particle: moved by this move, or as workgroup 0 published it
+                // the next move's particle: moved by this move, or as committed so far (the
+                // private copy, or workgroup 0's message)
                 const bool same = acc && i_next == i;
-                s_npo[0] = same ? nx_ : npub.x;
-                s_npo[1] = same ? ny_ : npub.y;
-                s_npo[2] = same ? nz_ : npub.z;
-                s_npo[3] = npub.w;
+                const double4 nb = mine != nullptr ? nxt : npub;
+                s_npo[0] = same ? nx_ : nb.x;
+                s_npo[1] = same ? ny_ : nb.y;
+                s_npo[2] = same ? nz_ : nb.z;
+                s_npo[3] = nb.w;
                 s_nti = nxt_ti;
+                if (acc && mine != nullptr) mine[i] = make_double4(nx_, ny_, nz_, po.w);
             }
             if (g == 0) {
                 if (kPre) {
@@ -639,6 +645,7 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             if (g == 0 && tid == 0 && t + 1 < M) {
                 SplitSlot* sl = slots + static_cast<int>((mid + 1ull) & 1ull) * G;
                 for (int v = 0; v < kSplitValues; ++v) split_put(sl, v, 0.0, split_tag(mid + 1ull) | kAbortBit);
+                asm volatile("" ::: "memory"); // the abort words are stored before the loop ends
             }
             t_done = t + 1;
             break;
@@ -764,7 +771,19 @@ void Batch::run_split(long long moves, int groups, int stride, bool trace, int t
     TraceEntry* tr = trace ? d_trace_ : nullptr;
     int cap = trace ? trace_cap : 0;
     long long spin = 1LL << 24;
-    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
+    // private position copies of workgroups 1..G-1
+    const size_t priv_bytes = sizeof(double4) * static_cast<size_t>(std::max(0, G - 1)) *
+                              static_cast<size_t>(h_states_[0].n);
+    if (priv_bytes > split_priv_bytes_) {
+        check_hip(hipStreamSynchronize(stream_), "batch sync");
+        (void)hipFree(split_priv_);
+        split_priv_ = nullptr;
+        split_priv_bytes_ = 0;
+        check_hip(hipMalloc(&split_priv_, priv_bytes), "hipMalloc private positions");
+        split_priv_bytes_ = priv_bytes;
+    }
+    double4* priv = static_cast<double4*>(split_priv_);
+    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin, &priv};
     time_begin();
     check_hip(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, stream_), "hipLaunchCooperativeKernel");
     time_end();
@@ -798,7 +817,8 @@ void Batch::run_split_static(long long moves, int groups, int stride) {
     TraceEntry* tr = nullptr;
     int cap = 0;
     long long spin = 1LL << 24;
-    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin};
+    double4* priv = nullptr; // no private position copies: workgroup 0 sends the next particle
+    void* args[] = {&st, &M, &G, &str, &slots, &err, &tr, &cap, &spin, &priv};
     time_begin();
     check_hip(hipLaunchCooperativeKernel(sl.fn, dim3(sl.grid), dim3(sl.threads), args, 0, stream_),
               "hipLaunchCooperativeKernel");
