@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 381.2e6
+HEADLINE_TRAFFIC_BYTES = 394.7e6
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 STEP_MOVES = 20000
@@ -909,11 +909,11 @@ def main():
                      "fp64_tflops": achieved_tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                      "algorithmic_flops_per_move": flops,
                      "limiter": "per-move latency chain across CUs (one Metropolis chain: decision and commit on "
-                                "workgroup 0 -> one-way exchange of about 1.5 us -> phase tables and Ewald slices "
-                                "on workgroups 1-15 -> exchange back; the pair walk runs one move ahead)",
+                                "workgroup 0 -> one-way exchange -> phase tables and Ewald slices on workgroups 1-15 "
+                                "-> exchange back; the pair walk runs one move ahead)",
                      "note": "one launch runs STEP_MOVES sequential moves; achieved = algorithmic bytes per move "
                              "/ kernel time per move (HIP events); Q(k) stays in registers, so the measured HBM "
-                             "traffic (about 19 KB per move, mostly the exchange's polls) is far below the "
+                             "traffic (about 20 KB per move, mostly the exchange's polls) is far below the "
                              "algorithmic bytes; neither HBM nor the FP64 VALU binds: the move time is a chain of "
                              "dependent latencies"},
         "cpu_baseline": cpu,

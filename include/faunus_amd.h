@@ -273,7 +273,9 @@ int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int strid
  * already hold its device state (after any earlier fmc_batch_run* or
  * fmc_batch_replica_info call), and no trace is recorded.  Calls that overlap on one GPU
  * share the workspace and race.  The chain equals fmc_batch_run_split's at the same
- * workgroup count; errors read through fmc_batch_split_error / fmc_batch_replica_info.
+ * workgroup count, bit for bit; without the private position copies that
+ * fmc_batch_run_split allocates, workgroup 0 also sends each next particle, so it runs a
+ * little slower.  Errors read through fmc_batch_split_error / fmc_batch_replica_info.
  */
 int fmc_batch_run_split_static(fmc_batch* batch, long long moves, int groups, int stride);
 /*
