@@ -96,7 +96,7 @@ def test_device_mc_full_size_nacl_short():
     assert dq < 1e-9
 
 
-@pytest.mark.parametrize("groups,stride", [(4, 1), (16, 8), (64, 1)])
+@pytest.mark.parametrize("groups,stride", [(3, 1), (4, 1), (16, 8), (64, 1)])
 def test_split_replica_matches_oracle(groups, stride):
     """One replica spread over several cooperating workgroups (k-vector slices)."""
     box = 255.0 * (2000 / 20000) ** (1 / 3)
@@ -124,6 +124,26 @@ def test_split_replica_matches_oracle(groups, stride):
     assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
     dq, _ = b.check_q(0)
     assert dq < 1e-9
+
+
+@pytest.mark.parametrize("groups", [1, 3, 8])
+def test_split_without_cell_list_matches_oracle(groups):
+    """The split kernel on a system without a cell list (2:1 salt with Rc = L/2):
+    workgroup 0 sums all pairs, the others the k-vector slices."""
+    sysd = systems.salt21(n=600, conc=0.1, seed=4, kmax=8)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=77)
+    nm = 2000
+    b.run_split(nm, groups=groups, stride=1, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=77, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=77)
+    log("split without cells", groups, rep)
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
 
 
 @pytest.mark.parametrize("groups,depth,kind", [(4, 1, "cells"), (16, 2, "cells"), (16, 4, "cells"),
