@@ -94,17 +94,27 @@ __device__ __forceinline__ double pair_energy_exact(const DevPair& p, const DevC
 
 // Andrea spline of a nonbonded_splined table at r2 in (rmin2, rmax2) (tabulate.h:184-202:
 // interval by lower_bound, Horner from the top coefficient).  The search starts one
-// knot before the first knot of r2's bucket and takes a fixed number of steps.
+// knot before the first knot of r2's bucket; within the table's `steps` knots from there
+// lower_bound's index advances once per knot below r2, and the knots are increasing, so
+// those knots are read independently and counted (one LDS round, not `steps` dependent ones).
 __device__ __forceinline__ double spline_table_eval(const DevSplineTable& T, double r2) {
     int b = static_cast<int>((r2 - T.x0) * T.inv_w);
     b = b < 0 ? 0 : (b > kSplBuckets - 1 ? kSplBuckets - 1 : b);
     int lo = static_cast<int>(T.bucket[b]) - 1;
     lo = lo < 0 ? 0 : lo;
-    const int n = T.n;
-    for (int s = 0; s < T.steps; ++s) {
-        const double k = T.knots[lo < n ? lo : n - 1];
-        lo += (lo < n && k < r2) ? 1 : 0;
+    const int n = T.n, steps = T.steps;
+    int below = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int k = lo + s;
+        const double v = T.knots[k < n ? k : n - 1];
+        below += (s < steps && k < n && v < r2) ? 1 : 0;
     }
+    for (int s = 8; s < steps; ++s) {
+        const int k = lo + s;
+        below += (k < n && T.knots[k < n ? k : n - 1] < r2) ? 1 : 0;
+    }
+    lo += below;
     int pos = lo - 1;
     pos = pos < 0 ? 0 : (pos > n - 2 ? n - 2 : pos);
     const double* c = T.coeff + 6 * pos;

@@ -43,7 +43,7 @@
 namespace fmc {
 namespace {
 
-constexpr int kSplitMaxKPT = 8; // k-vectors per thread at 256 threads
+constexpr int kSplitMaxKPT = 12; // k-vectors per thread at 256 threads
 constexpr int kSplitMaxG = 64;
 constexpr int kGkMax = 1025;
 // published values: workgroup 0 the pair energy change and, without private position
@@ -85,44 +85,77 @@ __device__ __forceinline__ double wrap_s(double x, double L, double half, double
     return x;
 }
 
-template <int kThreads, int kMaxKPT, bool kCells>
-__global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
-                                                       int stride, SplitSlot* __restrict__ slots,
-                                                       int* __restrict__ err, TraceEntry* __restrict__ trace,
-                                                       int trace_cap, long long spin_limit, double4* __restrict__ priv) {
-    if (blockIdx.x % stride != 0) return;
-    const int g = blockIdx.x / stride;
-    if (g >= G) return;
+// LDS of the split kernel, shared by its three workgroup roles
+template <int kThreads>
+struct SplitLds {
+    DevPair pairs[kMaxTypes * kMaxTypes];
+    DevCoulomb coul;
+    double dp[kMaxTypes];
+    double gk[kGkMax];
+    // per-wave partials: 0/1 the next move's de if the current move is rejected/accepted,
+    // 2/3 a pair walk's new/old sums, 4/5 the look-ahead's new/old sums
+    double red[6][kThreads / 64];
+    int cf[kThreads / 64]; // look-ahead waves: a loaded cell is one the current move changes
+    int accept, abort_, stop, ovf;
+    // proposals of the next 64 moves and the three after them, one lane each
+    Proposal prop[67];
+    double thr[67]; // -log(u) of each proposal's acceptance deviate
+    double2 phase[6 * 65];
+    double kb[4]; // k-slice holders: the particle of the move after next, as of the current move
+    int kti;
+    double msg[4];       // workgroups g > 0 without private copies: workgroup 0's particle message
+    double pf_pos[2][4]; // the look-ahead's particle, by move parity
+    int pf_ti[2], pf_ok;
+    int cold[2], cnew[2]; // a move's old and new cell, by move parity
+    // the commit's reads, made while wave 0 waits: the new cell's count, the old cell's
+    // count, the moved particle's slot there and the old cell's last member
+    int cm[4];
+};
+
+// Roles: 0 workgroup 0 of several (pair walk, commit), 1 a k-slice holder (g > 0),
+// 2 a single workgroup that does both.  Each role is its own instantiation, so that
+// registers live in one role do not count against the others.
+template <int kThreads, int kMaxKPT, bool kCells, int kRole>
+__device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThreads>& sh, ReplicaState* __restrict__ states,
+                                                             long long M, int G, int g, SplitSlot* __restrict__ slots,
+                                                             int* __restrict__ err, TraceEntry* __restrict__ trace,
+                                                             int trace_cap, long long spin_limit,
+                                                             double4* __restrict__ priv) {
     ReplicaState& S = states[0];
-    if (S.error) return; // a stopped replica (read before any workgroup of this launch can set it)
-    __shared__ DevPair s_pairs[kMaxTypes * kMaxTypes];
-    __shared__ DevCoulomb s_coul;
-    __shared__ double s_dp[kMaxTypes];
-    __shared__ double s_gk[kGkMax];
-    __shared__ double s_red[3][kThreads / 64];
-    __shared__ int s_accept, s_abort, s_stop;
-    // proposals of the next 64 moves and the first of the following 64, one lane each
-    __shared__ Proposal s_prop[65];
-    __shared__ double s_thr[65]; // -log(u) of each proposal's acceptance deviate
-    __shared__ double2 s_phase[6 * 65];
-    __shared__ double s_npo[4]; // workgroups g > 0: the next move's particle (x, y, z, q)
-    __shared__ int s_nti;       // and its type
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kKT = kThreads - 64; // threads of the k-slice: waves 1.. (wave 0 gathers and decides)
     // Workgroup 0 at 512 threads with a cell list: waves 1-6 (7 lanes per stencil cell)
-    // load the next move's particle and its stencil neighbours while wave 0 waits for the
-    // other workgroups' partials.  The loads are kept (s_pf_ok) unless the accepted move
-    // was the next particle or changed one of the 54 loaded cells (s_mv_cell).
+    // walk the next move's stencils while wave 0 waits for the partials of the current one.
     constexpr bool kPre = kCells && kThreads >= 512;
     constexpr int kPfLanes = 7;
-    // The last wave publishes the workgroup's partials, so that wave 0's polls do not
-    // wait for those stores to complete (the vector memory counter covers both).
+    // The last wave publishes, so that wave 0's polls do not wait for those stores to
+    // complete (the vector memory counter covers loads and stores).
     constexpr int kPub = kThreads - 64;
-    __shared__ double s_pf_pos[4];
-    __shared__ int s_pf_ti, s_pf_ok;
-    __shared__ int s_pf_cell_of[2]; // the next particle's cell_of entry, by move parity
-    __shared__ int s_cnew[2];       // the cell of a move's new position, by move parity
-    __shared__ int s_mv_cell[2];
+    auto& s_pairs = sh.pairs;
+    auto& s_coul = sh.coul;
+    auto& s_dp = sh.dp;
+    auto& s_gk = sh.gk;
+    auto& s_red = sh.red;
+    auto& s_cf = sh.cf;
+    int& s_accept = sh.accept;
+    int& s_abort = sh.abort_;
+    int& s_stop = sh.stop;
+    int& s_ovf = sh.ovf;
+    auto& s_prop = sh.prop;
+    auto& s_thr = sh.thr;
+    auto& s_phase = sh.phase;
+    auto& s_kb = sh.kb;
+    int& s_kti = sh.kti;
+    auto& s_msg = sh.msg;
+    auto& s_pf_pos = sh.pf_pos;
+    auto& s_pf_ti = sh.pf_ti;
+    int& s_pf_ok = sh.pf_ok;
+    auto& s_cold = sh.cold;
+    auto& s_cnew = sh.cnew;
+    auto& s_cm = sh.cm;
 
     const int tid = threadIdx.x;
+    const int wid = tid >> 6;
     const int n = S.n, nk = S.nk, ncut = S.ncut, nt = S.ntypes;
     const int width = 2 * ncut + 1;
     const double L = S.L, half = S.half, inv = S.inv;
@@ -136,22 +169,25 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     for (int i = tid; i < nt; i += kThreads) s_dp[i] = S.dp_type[i];
     for (int i = tid; i <= ncut * ncut && i < kGkMax; i += kThreads) s_gk[i] = S.gk[i];
 
-    // this workgroup's k-slice, interleaved over the threads.  With G > 1 workgroup 0,
-    // whose pair walk and commit form the chain every move waits for, takes none: the
-    // k-vectors go to workgroups 1..G-1.
+    // Roles.  Workgroup 0 walks the pair stencils and commits accepted moves.  The
+    // k-vectors are sliced over workgroups 1..G-1 (over workgroup 0 when G = 1), each
+    // slice interleaved over the threads of waves 1.. of its workgroup.
+    constexpr bool pairs = kRole != 1;
+    constexpr bool hasK = kRole != 0; // also a holder whose slice is empty: it publishes zero
     const int kparts = G > 1 ? G - 1 : 1;
     const int kw = (nk + kparts - 1) / kparts;
-    const int kb = G > 1 ? (g == 0 ? 0 : (g - 1) * kw) : 0;
-    const int ke = G > 1 && g == 0 ? 0 : min(nk, kb + kw);
+    const int kb = kRole == 1 ? (g - 1) * kw : 0;
+    const int ke = kRole == 0 ? 0 : min(nk, kb + kw);
     int kn[kMaxKPT];
-    double2 q[kMaxKPT], dq[kMaxKPT];
+    double2 q[kMaxKPT], dq[kMaxKPT], dq1[kMaxKPT];
 #pragma unroll
     for (int s = 0; s < kMaxKPT; ++s) {
-        const int k = kb + s * kThreads + tid;
-        const bool on = k < ke;
+        const int k = kb + s * kKT + tid - 64;
+        const bool on = tid >= 64 && k < ke;
         kn[s] = on ? S.kn[k] : 0;
         q[s] = on ? S.q[k] : make_double2(0.0, 0.0);
         dq[s] = make_double2(0.0, 0.0);
+        dq1[s] = make_double2(0.0, 0.0);
     }
     double4* __restrict__ pos = S.pos;
     const uint32_t* __restrict__ meta = S.meta;
@@ -161,6 +197,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                                      : nullptr;
     if (mine != nullptr)
         for (int j = tid; j < S.n; j += kThreads) mine[j] = pos[j];
+    // workgroup 0 also publishes particles when the others keep no copies
+    const bool particle_msg = priv == nullptr && G > 1;
     // the cell list, read once
     const double inv_cell = S.inv_cell;
     const int nc0 = S.nc0, nc1 = S.nc1, nc2 = S.nc2, ccap = S.cap;
@@ -178,97 +216,205 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
     const long long sample_every = S.sample_every;
     unsigned long long e_samples = S.e_samples;
     double e_sum = S.e_sum, e_sum2 = S.e_sum2;
-    // the next move's particle as committed so far: workgroup 0's publisher (published
-    // without private copies) or thread 0 of the others (from the private copy); its type
-    double4 nxt = make_double4(0.0, 0.0, 0.0, 0.0);
-    int nxt_ti = 0;
-    // this lane's stencil cell of the next move (waves 1-6 of workgroup 0) and its pair
-    // sum there, both from the look-ahead
-    int f_cell = -1;
-    double f_sum = 0.0;
     long long t_done = M; // moves decided by this launch
-    __syncthreads();
+    // thread 0's values of the current move (this workgroup's share of the decision):
+    // workgroup 0 the pair energy change, the others their de
+    double own_pd = 0.0, own_de = 0.0;
+    // k-slice holders: the current move's particle (old position, charge; new position)
+    // and the next move's, as the phase tables of the next move were built
+    double4 cur_po = make_double4(0.0, 0.0, 0.0, 0.0), nb_po = cur_po;
+    double cur_nx = 0.0, cur_ny = 0.0, cur_nz = 0.0, nb_nx = 0.0, nb_ny = 0.0, nb_nz = 0.0;
+    int cur_ti = 0, nb_ti = 0;
+    // the publisher's early loads: the particle of the move after next (k-slice holders
+    // with private copies, or workgroup 0 when G = 1 or when it sends particles) and of
+    // the move after that (workgroup 0 when it sends particles), as committed so far
+    double4 ahead2 = make_double4(0.0, 0.0, 0.0, 0.0), ahead3 = ahead2;
+    int ahead2_ti = 0;
 
-    for (long long t = 0; t < M; ++t) {
-        const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
-        if ((t & 63) == 0) {
-            // the previous proposals were last read before this move's barriers
-            if (tid < 65) {
-                const Proposal Q = make_proposal(seed, replica, mid + static_cast<unsigned long long>(tid), n);
-                s_prop[tid] = Q;
-                s_thr[tid] = -log(Q.accept_u);
-            }
-            __syncthreads();
-        }
-        const Proposal P = s_prop[t & 63];
-        const int i = P.index;
-        const int i_next = s_prop[(t & 63) + 1].index;
-        if (t + 1 < M) {
-            // workgroup 0's commits so far are visible to its own threads, and each
-            // workgroup's updates of its private copy to itself
-            if (g == 0 && tid == kPub && mine == nullptr && priv == nullptr) nxt = pos[i_next];
-            if (g != 0 && tid == 0) {
-                nxt_ti = static_cast<int>(meta[i_next] & 0xffu);
-                if (mine != nullptr) nxt = mine[i_next];
-            }
-        }
-        const bool pre = kPre && g == 0 && s_pf_ok;
-        double4 po;
-        int ti;
-        if (pre) {
-            po = make_double4(s_pf_pos[0], s_pf_pos[1], s_pf_pos[2], s_pf_pos[3]);
-            ti = s_pf_ti;
-        } else if (g != 0 && t > 0) {
-            po = make_double4(s_npo[0], s_npo[1], s_npo[2], s_npo[3]);
-            ti = s_nti;
-        } else {
-            po = pos[i]; // workgroup 0, or the launch's first move
-            ti = static_cast<int>(meta[i] & 0xffu);
-        }
-        const double scale = s_dp[ti] * P.length_u;
-        const double nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
-        const double ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
-        const double nz_ = wrap_s(po.z + P.dir[2] * scale, L, half, inv);
-        for (int e = tid; e < (ke > kb ? 6 * width : 0); e += kThreads) {
+    auto build_phases = [&](double px, double py, double pz, double ox, double oy, double oz)
+                            __attribute__((always_inline)) {
+        for (int e = tid; e < 6 * width; e += kThreads) {
             const int nn = e % width - ncut;
             const int axis = (e / width) % 3;
             const int st = e / (3 * width);
-            const double x = st == 0 ? (axis == 0 ? nx_ : (axis == 1 ? ny_ : nz_))
-                                     : (axis == 0 ? po.x : (axis == 1 ? po.y : po.z));
+            const double x = st == 0 ? (axis == 0 ? px : (axis == 1 ? py : pz)) : (axis == 0 ? ox : (axis == 1 ? oy : oz));
             double sn, cs;
             sincos((6.283185307179586 * (static_cast<double>(nn) / L)) * x, &sn, &cs);
             s_phase[e] = make_double2(cs, sn);
         }
-        double e_new = 0.0, e_old = 0.0;
-        if (g == 0) {
+    };
+    // dQ of the move whose phase tables are in LDS (charge w), into dq1, and the thread's
+    // share of its de against Q (x0) and against Q + dq (x1): the next move's de if the
+    // current move is rejected, and if it is accepted
+    auto kslice = [&](double w, double& x0, double& x1) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < kMaxKPT; ++s) {
+            const int k = kb + s * kKT + tid - 64;
+            if (k < ke) {
+                const int nx = unpack_n(kn[s], 0), ny = unpack_n(kn[s], 1), nz = unpack_n(kn[s], 2);
+                const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut], zn = s_phase[2 * width + nz + ncut];
+                const double2 xo = s_phase[3 * width + nx + ncut], yo = s_phase[4 * width + ny + ncut],
+                              zo = s_phase[5 * width + nz + ncut];
+                const double an = xn.x * yn.x - xn.y * yn.y, bn = xn.x * yn.y + xn.y * yn.x;
+                const double ao = xo.x * yo.x - xo.y * yo.y, bo = xo.x * yo.y + xo.y * yo.x;
+                const double dre = w * ((an * zn.x - bn * zn.y) - (ao * zo.x - bo * zo.y));
+                const double dim = w * ((an * zn.y + bn * zn.x) - (ao * zo.y + bo * zo.x));
+                dq1[s] = make_double2(dre, dim);
+                const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[nx * nx + ny * ny + nz * nz];
+                const double sq = dre * dre + dim * dim;
+                x0 += A * (2.0 * (q[s].x * dre + q[s].y * dim) + sq);
+                const double ax = q[s].x + dq[s].x, ay = q[s].y + dq[s].y;
+                x1 += A * (2.0 * (ax * dre + ay * dim) + sq);
+            } else {
+                dq1[s] = make_double2(0.0, 0.0);
+            }
+        }
+    };
+    // the sum of a row of per-wave partials over waves w0..kWaves-1, in wave order
+    auto row_sum = [&](int r, int w0, int w1) __attribute__((always_inline)) {
+        double a = 0.0;
+        for (int w = w0; w < w1; ++w) a += s_red[r][w];
+        return a;
+    };
+    // workgroup 0's pair energy change as one value: -inf for a NaN old sum (and a new one
+    // that is not NaN), +inf for a NaN or +inf new sum, else new - old.  The decision adds
+    // the reciprocal change to a finite one; for a finite reciprocal change this is
+    // getEnergyChange's handling of the two sums (montecarlo.cpp:193-209), and a NaN one
+    // gives +inf as there
+    auto pair_change = [](double pb, double pc) __attribute__((always_inline)) {
+        return (isnan(pc) && !isnan(pb)) ? -__builtin_huge_val()
+               : (isnan(pb) || (pb > 0.0 && isinf(pb))) ? __builtin_huge_val()
+                                                         : pb - pc;
+    };
+    auto refill = [&](unsigned long long m0) __attribute__((always_inline)) {
+        if (tid < 67) {
+            const Proposal Q = make_proposal(seed, replica, m0 + static_cast<unsigned long long>(tid), n);
+            s_prop[tid] = Q;
+            s_thr[tid] = -log(Q.accept_u);
+        }
+    };
+
+    refill(move0);
+    __syncthreads();
+    if (hasK && M > 0) {
+        // the launch's first move, against the state the launch starts from: no workgroup
+        // commits before every k-slice holder has published this move's de
+        const Proposal& P0 = s_prop[0];
+        const int i0 = P0.index;
+        cur_po = pos[i0];
+        cur_ti = static_cast<int>(meta[i0] & 0xffu);
+        const double sc = s_dp[cur_ti] * P0.length_u;
+        cur_nx = wrap_s(cur_po.x + P0.dir[0] * sc, L, half, inv);
+        cur_ny = wrap_s(cur_po.y + P0.dir[1] * sc, L, half, inv);
+        cur_nz = wrap_s(cur_po.z + P0.dir[2] * sc, L, half, inv);
+        if (tid == kPub && M > 1) {
+            const int i1 = s_prop[1].index;
+            ahead2 = pos[i1];
+            ahead2_ti = static_cast<int>(meta[i1] & 0xffu);
+        }
+        build_phases(cur_nx, cur_ny, cur_nz, cur_po.x, cur_po.y, cur_po.z);
+        __syncthreads();
+        if (tid >= 64) {
+            double x0 = 0.0, x1 = 0.0;
+            kslice(cur_po.w, x0, x1);
+            const double a = wave_sum(x0);
+            if ((tid & 63) == 0) s_red[0][wid] = a;
+        }
+#pragma unroll
+        for (int s = 0; s < kMaxKPT; ++s) dq[s] = dq1[s];
+        __syncthreads();
+        if (tid == 0 || tid == kPub) {
+            const double de0 = row_sum(0, 1, kWaves);
+            if (tid == 0) own_de = de0;
+            if (tid == kPub && G > 1) split_put(slots + static_cast<int>(move0 & 1ull) * G + g, 0, de0, split_tag(move0));
+            if (tid == kPub && M > 1) {
+                s_kb[0] = ahead2.x;
+                s_kb[1] = ahead2.y;
+                s_kb[2] = ahead2.z;
+                s_kb[3] = ahead2.w;
+                s_kti = ahead2_ti;
+            }
+        }
+        __syncthreads();
+    }
+
+    for (long long t = 0; t < M; ++t) {
+        const unsigned long long mid = move0 + static_cast<unsigned long long>(t);
+        if (t > 0 && (t & 63) == 0) {
+            // the previous proposals were last read before the previous move's last barrier
+            refill(mid);
+            __syncthreads();
+        }
+        const int slot = static_cast<int>(t & 63);
+        const Proposal P = s_prop[slot];
+        const int i = P.index;
+        const int i1 = s_prop[slot + 1].index, i2 = s_prop[slot + 2].index, i3 = s_prop[slot + 3].index;
+        const int parity = static_cast<int>(mid & 1ull);
+        const uint32_t tag = split_tag(mid);
+        if (tid == kPub) {
+            // committed so far: this workgroup's commits and private-copy updates are visible
+            // to its own threads
+            if (hasK && t + 2 < M && (mine != nullptr || pairs)) {
+                ahead2 = mine != nullptr ? mine[i2] : pos[i2];
+                ahead2_ti = static_cast<int>(meta[i2] & 0xffu);
+            }
+            if (pairs && particle_msg) {
+                if (!hasK) ahead2 = pos[i2];
+                ahead3 = pos[i3];
+            }
+            if (hasK && mine == nullptr && !pairs) ahead2_ti = static_cast<int>(meta[i2] & 0xffu);
+        }
+        // the current move's particle: workgroup 0 from the look-ahead or the positions,
+        // the k-slice holders from their registers
+        const bool pre = kPre && pairs && s_pf_ok;
+        double4 po;
+        int ti;
+        double nx_, ny_, nz_;
+        if (pairs) {
+            if (pre) {
+                po = make_double4(s_pf_pos[t & 1][0], s_pf_pos[t & 1][1], s_pf_pos[t & 1][2], s_pf_pos[t & 1][3]);
+                ti = s_pf_ti[t & 1];
+            } else {
+                po = pos[i];
+                ti = static_cast<int>(meta[i] & 0xffu);
+            }
+            const double scale = s_dp[ti] * P.length_u;
+            nx_ = wrap_s(po.x + P.dir[0] * scale, L, half, inv);
+            ny_ = wrap_s(po.y + P.dir[1] * scale, L, half, inv);
+            nz_ = wrap_s(po.z + P.dir[2] * scale, L, half, inv);
+        } else {
+            po = cur_po;
+            ti = cur_ti;
+            nx_ = cur_nx;
+            ny_ = cur_ny;
+            nz_ = cur_nz;
+        }
+
+        // (1) workgroup 0 without a valid look-ahead: walk this move's stencils now and publish
+        if (pairs && !pre) {
+            double e_new = 0.0, e_old = 0.0;
             const Box b{{L, L, L}, {half, half, half}};
             if (kPre) {
-                // 54 (state, cell) combos x 7 lanes on waves 1-6; a lane whose sum was
-                // computed during the previous move takes it from a register
                 const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+                if (tid == 64) s_cold[t & 1] = cell_of[i];
                 if (tid >= 64 && combo < 54) {
+                    const int st = combo / 27, c = combo % 27;
+                    const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
+                    const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                    const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                    const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                    const int cell = (cz * nc1 + cy) * nc0 + cx;
+                    if (combo == 13 && slot0 == 0) s_cnew[t & 1] = cell; // the new position's own cell
+                    const int cnt = cell_count[cell];
                     double acc_u = 0.0;
-                    if (pre) {
-                        acc_u = f_sum;
-                    } else {
-                        const int st = combo / 27, c = combo % 27;
-                        const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
-                        const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
-                        const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
-                        const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
-                        const int cell = (cz * nc1 + cy) * nc0 + cx;
-                        if (combo == 13 && slot0 == 0) s_cnew[t & 1] = cell; // the new position's own cell
-                        const int cnt = cell_count[cell];
-                        for (int k = slot0; k < cnt; k += kPfLanes) {
-                            const int j = cell_members[cell * ccap + k];
-                            if (j == i) continue;
-                            const double4 pj = pos[j];
-                            const int tj = static_cast<int>(meta[j] & 0xffu);
-                            acc_u += pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w,
-                                                 dist2(b, px, py, pz, pj.x, pj.y, pj.z));
-                        }
+                    for (int k = slot0; k < cnt; k += kPfLanes) {
+                        const int j = cell_members[cell * ccap + k];
+                        if (j == i) continue;
+                        const double4 pj = pos[j];
+                        const int tj = static_cast<int>(meta[j] & 0xffu);
+                        acc_u += pair_energy(s_pairs[ti * nt + tj], s_coul, po.w, pj.w, dist2(b, px, py, pz, pj.x, pj.y, pj.z));
                     }
-                    if (combo < 27) e_new = acc_u;
+                    if (st == 0) e_new = acc_u;
                     else e_old = acc_u;
                 }
             } else if (kCells) {
@@ -302,84 +448,182 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                     e_old += pair_energy(p, s_coul, po.w, pj.w, dist2(b, po.x, po.y, po.z, pj.x, pj.y, pj.z));
                 }
             }
-        }
-        __syncthreads(); // phase tables
-        double de = 0.0;
-#pragma unroll
-        for (int s = 0; s < kMaxKPT; ++s) {
-            const int k = kb + s * kThreads + tid;
-            if (k < ke) {
-                const int nx = unpack_n(kn[s], 0), ny = unpack_n(kn[s], 1), nz = unpack_n(kn[s], 2);
-                const double2 xn = s_phase[nx + ncut], yn = s_phase[width + ny + ncut], zn = s_phase[2 * width + nz + ncut];
-                const double2 xo = s_phase[3 * width + nx + ncut], yo = s_phase[4 * width + ny + ncut],
-                              zo = s_phase[5 * width + nz + ncut];
-                const double an = xn.x * yn.x - xn.y * yn.y, bn = xn.x * yn.y + xn.y * yn.x;
-                const double ao = xo.x * yo.x - xo.y * yo.y, bo = xo.x * yo.y + xo.y * yo.x;
-                const double dre = po.w * ((an * zn.x - bn * zn.y) - (ao * zo.x - bo * zo.y));
-                const double dim = po.w * ((an * zn.y + bn * zn.x) - (ao * zo.y + bo * zo.x));
-                dq[s] = make_double2(dre, dim);
-                const double A = (nx > 0 ? 2.0 : 1.0) * s_gk[nx * nx + ny * ny + nz * nz];
-                de += A * (2.0 * (q[s].x * dre + q[s].y * dim) + (dre * dre + dim * dim));
+            {
+                const double a = wave_sum(e_new), c = wave_sum(e_old);
+                if ((tid & 63) == 0) {
+                    s_red[2][wid] = a;
+                    s_red[3][wid] = c;
+                }
+            }
+            __syncthreads();
+            if (tid == 0 || tid == kPub) {
+                const double pd = pair_change(row_sum(2, 0, kWaves), row_sum(3, 0, kWaves));
+                if (tid == 0) own_pd = pd;
+                if (tid == kPub && G > 1) {
+                    SplitSlot* sl = slots + parity * G;
+                    if (particle_msg) {
+                        // the particle's load is waited for here, before the stores: a wait
+                        // after them would wait for the stores too
+                        asm volatile("" ::"v"(ahead2.x), "v"(ahead2.y), "v"(ahead2.z), "v"(ahead2.w));
+                    }
+                    split_put(sl, 0, pd, tag);
+                    if (particle_msg) {
+                        // the particle of the move after next, as committed before this move
+                        split_put(sl, 1, ahead2.x, tag);
+                        split_put(sl, 2, ahead2.y, tag);
+                        split_put(sl, 3, ahead2.z, tag);
+                        split_put(sl, 4, ahead2.w, tag);
+                    }
+                }
             }
         }
-        {
-            const double a = wave_sum(de), b = wave_sum(e_new), c = wave_sum(e_old);
+
+        // (2) k-slice holders: the next move's phase tables.  Its particle is taken as
+        // committed before this move; if this move is accepted and moves the same particle,
+        // the tables are built again after the decision.
+        if (hasK) {
+            if (t + 1 < M) {
+                nb_po = make_double4(s_kb[0], s_kb[1], s_kb[2], s_kb[3]);
+                nb_ti = s_kti;
+                const Proposal& P1 = s_prop[slot + 1];
+                const double sc1 = s_dp[nb_ti] * P1.length_u;
+                nb_nx = wrap_s(nb_po.x + P1.dir[0] * sc1, L, half, inv);
+                nb_ny = wrap_s(nb_po.y + P1.dir[1] * sc1, L, half, inv);
+                nb_nz = wrap_s(nb_po.z + P1.dir[2] * sc1, L, half, inv);
+                build_phases(nb_nx, nb_ny, nb_nz, nb_po.x, nb_po.y, nb_po.z);
+            }
+            __syncthreads(); // phase tables
+        }
+
+        // (3) while wave 0 gathers this move's partials and decides: the next move's de for
+        // both outcomes (k-slice holders), and the next move's stencil sums as committed so
+        // far (workgroup 0's waves 1-6)
+        if (kPre && pairs && tid >= 64 && tid < 448) {
+            const long long tn = t + 1;
+            const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+            double fs = 0.0;
+            int cf = 0;
+            if (tn < M && combo < 54) {
+                const Box b{{L, L, L}, {half, half, half}};
+                const Proposal& Q = s_prop[slot + 1];
+                const int in = Q.index;
+                const double4 qo = pos[in];
+                const int tq = static_cast<int>(meta[in] & 0xffu);
+                const double sc = s_dp[tq] * Q.length_u;
+                const int st = combo / 27, c = combo % 27;
+                double f_px, f_py, f_pz;
+                if (st == 0) {
+                    f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
+                    f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
+                    f_pz = wrap_s(qo.z + Q.dir[2] * sc, L, half, inv);
+                } else {
+                    f_px = qo.x;
+                    f_py = qo.y;
+                    f_pz = qo.z;
+                }
+                if (tid == 64) {
+                    s_pf_pos[tn & 1][0] = qo.x;
+                    s_pf_pos[tn & 1][1] = qo.y;
+                    s_pf_pos[tn & 1][2] = qo.z;
+                    s_pf_pos[tn & 1][3] = qo.w;
+                    s_pf_ti[tn & 1] = tq;
+                    s_cold[tn & 1] = cell_of[in];
+                }
+                const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                const int f_cell = (cz * nc1 + cy) * nc0 + cx;
+                if (combo == 13 && slot0 == 0) s_cnew[tn & 1] = f_cell;
+                cf = (f_cell == s_cold[t & 1] || f_cell == s_cnew[t & 1]) ? 1 : 0;
+                // the count and the first two member slots are loaded at once, further
+                // members in the loop
+                const int f_cnt = cell_count[f_cell];
+                const int r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
+                const int r1 = slot0 + kPfLanes < ccap ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
+                const int j0 = slot0 < f_cnt ? r0 : -1;
+                const int j1 = slot0 + kPfLanes < f_cnt ? r1 : -1;
+                double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
+                int t0 = 0, t1 = 0;
+                if (j0 >= 0) {
+                    p0 = pos[j0];
+                    t0 = static_cast<int>(meta[j0] & 0xffu);
+                }
+                if (j1 >= 0) {
+                    p1 = pos[j1];
+                    t1 = static_cast<int>(meta[j1] & 0xffu);
+                }
+                for (int k = slot0, u = 0; k < f_cnt; k += kPfLanes, ++u) {
+                    int j, tj;
+                    double4 pj;
+                    if (u == 0) {
+                        j = j0;
+                        pj = p0;
+                        tj = t0;
+                    } else if (u == 1) {
+                        j = j1;
+                        pj = p1;
+                        tj = t1;
+                    } else {
+                        j = cell_members[f_cell * ccap + k];
+                        if (j == in) continue;
+                        pj = pos[j];
+                        tj = static_cast<int>(meta[j] & 0xffu);
+                    }
+                    if (j == in) continue;
+                    fs += pair_energy(s_pairs[tq * nt + tj], s_coul, qo.w, pj.w, dist2(b, f_px, f_py, f_pz, pj.x, pj.y, pj.z));
+                }
+            }
+            const double a = wave_sum(combo < 27 ? fs : 0.0), c = wave_sum(combo < 27 ? 0.0 : fs);
+            const int any_cf = __any(cf);
             if ((tid & 63) == 0) {
-                s_red[0][tid >> 6] = a;
-                s_red[1][tid >> 6] = b;
-                s_red[2][tid >> 6] = c;
+                s_red[4][wid] = a;
+                s_red[5][wid] = c;
+                s_cf[wid] = any_cf;
             }
         }
-        __syncthreads();
-        const int parity = static_cast<int>(mid & 1ull);
-        const uint32_t tag = split_tag(mid);
-        // this workgroup's partials, summed in the same order by the publisher and by thread 0
-        double pa = 0.0, pb = 0.0, pc = 0.0;
-        if (tid == 0 || tid == kPub) {
-            for (int w = 0; w < kThreads / 64; ++w) {
-                pa += s_red[0][w];
-                pb += s_red[1][w];
-                pc += s_red[2][w];
+        if (kPre && pairs && wid == kWaves - 1) {
+            // the commit's reads (wave 7): the cell counts, the moved particle's slot in its
+            // old cell (one member per lane) and that cell's last member; and whether the
+            // current move, if accepted, overflows its new cell
+            const int cn = s_cnew[t & 1], co = s_cold[t & 1];
+            const int lane = tid & 63;
+            int nn = 0, nold = 0, kk = -1;
+            if (cn != co) {
+                nn = cell_count[cn];
+                nold = cell_count[co];
+                for (int b0 = 0; b0 < nold; b0 += 64) {
+                    const int l = b0 + lane;
+                    const int mj = l < nold ? cell_members[co * ccap + l] : -1;
+                    const unsigned long long hit = __ballot(mj == i);
+                    if (hit != 0ull && kk < 0) kk = b0 + static_cast<int>(__builtin_ctzll(hit));
+                    if (l == nold - 1) s_cm[3] = mj;
+                }
+            }
+            if (lane == 0) {
+                s_ovf = (cn != co && nn >= ccap) ? 1 : 0;
+                s_cm[0] = nn;
+                s_cm[1] = nold;
+                s_cm[2] = kk;
             }
         }
-        // workgroup 0's pair energy change as one value: -inf for a NaN old sum (and a new
-        // one that is not NaN), +inf for a NaN or +inf new sum, else new - old.  The
-        // decision adds the reciprocal change to a finite one; for a finite reciprocal
-        // change this is getEnergyChange's handling of the two sums (montecarlo.cpp:193-209),
-        // and a NaN one gives +inf as there
-        const double pd = (isnan(pc) && !isnan(pb)) ? -__builtin_huge_val()
-                          : (isnan(pb) || (pb > 0.0 && isinf(pb))) ? __builtin_huge_val()
-                                                                    : pb - pc;
-        const bool particle_msg = priv == nullptr && G > 1; // workgroup 0 also sends the next particle
-        if (tid == kPub && G > 1) {
-            SplitSlot* sl = slots + parity * G + g;
-            if (g == 0) {
-                if (particle_msg) {
-                    // the next particle's load (issued at the move's start) is waited for
-                    // here, before the stores: a wait after them would wait for the stores too
-                    asm volatile("" ::"v"(nxt.x), "v"(nxt.y), "v"(nxt.z), "v"(nxt.w));
-                }
-                split_put(sl, 0, pd, tag);
-                if (particle_msg) {
-                    split_put(sl, 1, nxt.x, tag);
-                    split_put(sl, 2, nxt.y, tag);
-                    split_put(sl, 3, nxt.z, tag);
-                    split_put(sl, 4, nxt.w, tag);
-                }
-            } else {
-                split_put(sl, 0, pa, tag);
+        if (hasK && tid >= 64) {
+            double x0 = 0.0, x1 = 0.0;
+            if (t + 1 < M) kslice(nb_po.w, x0, x1);
+            const double a = wave_sum(x0), c = wave_sum(x1);
+            if ((tid & 63) == 0) {
+                s_red[0][wid] = a;
+                s_red[1][wid] = c;
             }
         }
         // gather: lane j < G takes workgroup j's value (workgroup 0's pair change, the
-        // others' de); lanes 58-61 take workgroup 0's next particle when it is sent.  A
+        // others' de); lanes 58-61 take workgroup 0's particle when it is sent.  A
         // workgroup takes its own value from registers, bit for bit what the others read.
         // The des are added with the same butterfly in every workgroup.
-        double sde = 0.0, spd = 0.0;                      // totals, valid in thread 0
-        double4 npub = make_double4(0.0, 0.0, 0.0, 0.0); // published next particle, thread 0
+        double du = 0.0;
         if (tid < 64) {
             int ok = 1, stopped = 0;
             double a = 0.0, pdv = 0.0, extra = 0.0;
-            const double own_de = readlane_f64(pa, 0), own_pd = readlane_f64(pd, 0);
+            const double own_d = readlane_f64(own_de, 0), own_p = readlane_f64(own_pd, 0);
             auto poll = [&](int src, int v) -> double {
                 SplitSlot* sl = slots + parity * G + src;
                 long long spins = 0;
@@ -401,14 +645,15 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                 }
                 return __longlong_as_double(static_cast<long long>((hi & 0xffffffff00000000ull) | (lo >> 32)));
             };
-            if (tid < G) {
+            if (G == 1) {
+                if (tid == 0) {
+                    pdv = own_p;
+                    a = own_d;
+                }
+            } else if (tid < G) {
                 if (tid == g) {
-                    if (g == 0) {
-                        pdv = own_pd;
-                        a = G == 1 ? own_de : 0.0; // with G > 1 workgroup 0 holds no k-vectors
-                    } else {
-                        a = own_de;
-                    }
+                    if (g == 0) pdv = own_p;
+                    else a = own_d;
                 } else {
                     const double x = poll(tid, 0);
                     if (tid == 0) pdv = x;
@@ -424,103 +669,177 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
             const int all_ok = __all(ok);
             const int any_stop = __any(stopped);
             if (tid == 0) {
-                sde = a;
-                spd = pdt;
-                npub = make_double4(ex, ey, ez, eq);
                 if (any_stop) {
                     s_stop = 2; // workgroup 0 stopped after the previous move: this one is not decided
                 } else if (!all_ok) {
                     atomicExch(err, 1);
                     atomicMax(&S.error, 2);
                     s_abort = 1;
+                } else {
+                    if (particle_msg && g != 0) {
+                        s_msg[0] = ex;
+                        s_msg[1] = ey;
+                        s_msg[2] = ez;
+                        s_msg[3] = eq;
+                    }
+                    // the surface term enters with the reciprocal one (Ewald::energy, energy.cpp:581-587)
+                    double dsurf = 0.0, mx = dip0, my = dip1, mz = dip2;
+                    if (surf != 0.0) {
+                        mx += po.w * (nx_ - po.x);
+                        my += po.w * (ny_ - po.y);
+                        mz += po.w * (nz_ - po.z);
+                        dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
+                    }
+                    const double drec = pref * a + dsurf;
+                    du = isnan(drec) ? __builtin_huge_val() : (isinf(pdt) ? pdt : pdt + drec);
+                    if (isnan(du)) du = 0.0;
+                    const double bdu = beta * du;
+                    bool acc;
+                    if (isinf(bdu) && bdu < 0.0) acc = true;
+                    else if (-bdu > 709.782712893384) acc = true;
+                    else {
+                        // u <= exp(-bdu) decided against the table's -log(u) away from the
+                        // boundary (the margin exceeds the rounding of exp and log many times),
+                        // with exp itself only within the margin: the same decision, without
+                        // exp on the chain
+                        const double thr = s_thr[slot], margin = 1e-12 * (1.0 + fabs(thr));
+                        if (bdu < thr - margin) acc = true;
+                        else if (bdu > thr + margin) acc = false;
+                        else acc = P.accept_u <= exp(-bdu);
+                    }
+                    s_accept = acc ? 1 : 0;
+                    if (acc) {
+                        dip0 = mx;
+                        dip1 = my;
+                        dip2 = mz;
+                    }
                 }
             }
         }
-        if (tid == 0 && !s_abort && !s_stop) {
-            // the surface term enters with the reciprocal one (Ewald::energy, energy.cpp:581-587)
-            double dsurf = 0.0, mx = dip0, my = dip1, mz = dip2;
-            if (surf != 0.0) {
-                mx += po.w * (nx_ - po.x);
-                my += po.w * (ny_ - po.y);
-                mz += po.w * (nz_ - po.z);
-                dsurf = surf * ((mx * mx + my * my + mz * mz) - (dip0 * dip0 + dip1 * dip1 + dip2 * dip2));
-            }
-            const double drec = pref * sde + dsurf;
-            double du = isnan(drec) ? __builtin_huge_val() : (isinf(spd) ? spd : spd + drec);
-            if (isnan(du)) du = 0.0;
-            const double bdu = beta * du;
-            bool acc;
-            if (isinf(bdu) && bdu < 0.0) acc = true;
-            else if (-bdu > 709.782712893384) acc = true;
-            else {
-                // u <= exp(-bdu) decided against the table's -log(u) away from the
-                // boundary (the margin exceeds the rounding of exp and log many times),
-                // with exp itself only within the margin: the same decision, without exp
-                // on the chain
-                const double thr = s_thr[t & 63], margin = 1e-12 * (1.0 + fabs(thr));
-                if (bdu < thr - margin) acc = true;
-                else if (bdu > thr + margin) acc = false;
-                else acc = P.accept_u <= exp(-bdu);
-            }
-            s_accept = acc ? 1 : 0;
-            if (acc) {
-                dip0 = mx;
-                dip1 = my;
-                dip2 = mz;
-            }
-            if (g != 0) {
-                // the next move's particle: moved by this move, or as committed so far (the
-                // private copy, or workgroup 0's message)
-                const bool same = acc && i_next == i;
-                const double4 nb = mine != nullptr ? nxt : npub;
-                s_npo[0] = same ? nx_ : nb.x;
-                s_npo[1] = same ? ny_ : nb.y;
-                s_npo[2] = same ? nz_ : nb.z;
-                s_npo[3] = nb.w;
-                s_nti = nxt_ti;
-                if (acc && mine != nullptr) mine[i] = make_double4(nx_, ny_, nz_, po.w);
-            }
-            if (g == 0) {
-                if (kPre) {
-                    const long long tn = t + 1;
-                    s_pf_ok = (tn < M && !(acc && i_next == i)) ? 1 : 0;
-                    s_mv_cell[0] = -1;
-                    s_mv_cell[1] = -1;
-                }
-                if (trace && t < trace_cap) {
-                    TraceEntry& T = trace[t];
-                    T.index = i;
-                    T.accepted = acc ? 1 : 0;
-                    T.du = du;
-                    T.x = nx_;
-                    T.y = ny_;
-                    T.z = nz_;
-                }
+        __syncthreads(); // the decision, the partials of the next move
+        if (s_abort || s_stop == 2) {
+            t_done = t;
+            break;
+        }
+        const bool acc = s_accept != 0;
+
+        // (4) publish the next move's values at once, then apply this move
+        const bool refresh = hasK && acc && t + 1 < M && i1 == i; // the next move's particle just moved
+        if (hasK) {
+#pragma unroll
+            for (int s = 0; s < kMaxKPT; ++s) {
                 if (acc) {
-                    energy += du;
-                    sum_du += du;
-                    ++accepted;
+                    q[s].x += dq[s].x;
+                    q[s].y += dq[s].y;
+                }
+                dq[s] = dq1[s];
+            }
+        }
+        if (tid == 0 || tid == kPub) {
+            const unsigned long long mn = mid + 1ull;
+            SplitSlot* sl = slots + static_cast<int>(mn & 1ull) * G + g;
+            const uint32_t tag1 = split_tag(mn);
+            if (tid == kPub && hasK && t + 2 < M) {
+                // the particle of the move after next as of this move: from the private copy
+                // or the positions (loaded at this move's start), or from workgroup 0's
+                // message, with this move applied.  First, so that the wait for the early
+                // load does not also wait for the stores below.
+                double4 a2 = ahead2;
+                if (mine == nullptr && !pairs) a2 = make_double4(s_msg[0], s_msg[1], s_msg[2], s_msg[3]);
+                const bool moved = acc && i2 == i;
+                s_kb[0] = moved ? nx_ : a2.x;
+                s_kb[1] = moved ? ny_ : a2.y;
+                s_kb[2] = moved ? nz_ : a2.z;
+                s_kb[3] = a2.w;
+                s_kti = ahead2_ti;
+            }
+            if (kPre && pairs && t + 1 < M) {
+                int cf = 0;
+                for (int w = 1; w < 7; ++w) cf |= s_cf[w];
+                const bool stop = acc && s_ovf;
+                const bool valid = !(acc && (i1 == i || s_ovf || cf));
+                const double pd1 = pair_change(row_sum(4, 1, 7), row_sum(5, 1, 7));
+                if (tid == 0 && valid) own_pd = pd1;
+                if (tid == kPub) {
+                    s_pf_ok = valid ? 1 : 0;
+                    if (G > 1 && stop) {
+                        // the cell overflows: workgroup 0 stops after this move, and its words
+                        // of the next move carry the abort bit so that the others stop there
+                        for (int v = 0; v < kSplitValues; ++v) split_put(sl, v, 0.0, tag1 | kAbortBit);
+                    } else if (G > 1 && valid) {
+                        if (particle_msg) asm volatile("" ::"v"(ahead3.x), "v"(ahead3.y), "v"(ahead3.z), "v"(ahead3.w));
+                        split_put(sl, 0, pd1, tag1);
+                        if (particle_msg) {
+                            // the particle of the move after next as of this move
+                            const bool moved = acc && i3 == i;
+                            split_put(sl, 1, moved ? nx_ : ahead3.x, tag1);
+                            split_put(sl, 2, moved ? ny_ : ahead3.y, tag1);
+                            split_put(sl, 3, moved ? nz_ : ahead3.z, tag1);
+                            split_put(sl, 4, ahead3.w, tag1);
+                        }
+                    }
+                }
+            }
+            if (hasK && t + 1 < M && !refresh) {
+                const double de1 = acc ? row_sum(1, 1, kWaves) : row_sum(0, 1, kWaves);
+                if (tid == 0) own_de = de1;
+                if (tid == kPub && G > 1) split_put(sl, 0, de1, tag1);
+            }
+            if (tid == kPub && hasK && acc && mine != nullptr) mine[i] = make_double4(nx_, ny_, nz_, po.w);
+            if (kPre && pairs && tid == kPub && acc) {
+                // the commit, after the publish: stores only, from the reads made during the wait
+                pos[i] = make_double4(nx_, ny_, nz_, po.w);
+                const int cn = s_cnew[t & 1], co = s_cold[t & 1];
+                if (cn != co && !s_ovf) {
+                    const int nn = s_cm[0], nold = s_cm[1], kk = s_cm[2];
+                    if (kk >= 0) cell_members[co * ccap + kk] = s_cm[3];
+                    cell_count[co] = nold - 1;
+                    cell_members[cn * ccap + nn] = i;
+                    cell_count[cn] = nn + 1;
+                    cell_of[i] = cn;
+                }
+            }
+        }
+        if (pairs && tid == 0) {
+            if (trace && t < trace_cap) {
+                TraceEntry& T = trace[t];
+                T.index = i;
+                T.accepted = acc ? 1 : 0;
+                T.du = du;
+                T.x = nx_;
+                T.y = ny_;
+                T.z = nz_;
+            }
+            if (acc) {
+                energy += du;
+                sum_du += du;
+                ++accepted;
+                if (kPre) {
+                    // the publisher commits; a full new cell stops the chain (the lists are
+                    // left as they are)
+                    if (s_ovf) {
+                        atomicMax(&S.error, 1);
+                        s_stop = 1;
+                    }
+                } else {
                     pos[i] = make_double4(nx_, ny_, nz_, po.w);
                     if (kCells) {
-                        // workgroup 0 with the look-ahead: the stencil lane of the new
-                        // position's own cell found it already
-                        const int cnew = kPre ? s_cnew[t & 1]
-                                              : (cell_coord_s(nz_, half, inv_cell, nc2) * nc1 +
-                                                 cell_coord_s(ny_, half, inv_cell, nc1)) * nc0 +
-                                                    cell_coord_s(nx_, half, inv_cell, nc0);
-                        // cell_of changes only when its particle moves, so the loaded entry is
-                        // current whenever the other loads of this move were kept
-                        const int cold = pre ? s_pf_cell_of[t & 1] : cell_of[i];
-                        if (kPre) {
-                            s_mv_cell[0] = cold;
-                            s_mv_cell[1] = cnew;
-                        }
+                        const int cnew = (cell_coord_s(nz_, half, inv_cell, nc2) * nc1 +
+                                          cell_coord_s(ny_, half, inv_cell, nc1)) * nc0 +
+                                         cell_coord_s(nx_, half, inv_cell, nc0);
+                        const int cold = cell_of[i];
                         if (cnew != cold) {
                             const int nn = cell_count[cnew];
                             if (nn >= ccap) {
-                                // full cell: the lists are left as they are and the chain stops
+                                // full cell: the lists are left as they are, the chain stops,
+                                // and the next move's words carry the abort bit
                                 atomicMax(&S.error, 1);
                                 s_stop = 1;
+                                if (G > 1 && t + 1 < M) {
+                                    SplitSlot* sl = slots + static_cast<int>((mid + 1ull) & 1ull) * G;
+                                    for (int v = 0; v < kSplitValues; ++v)
+                                        split_put(sl, v, 0.0, split_tag(mid + 1ull) | kAbortBit);
+                                }
                             } else {
                                 const int nold = cell_count[cold];
                                 for (int kk = 0; kk < nold; ++kk)
@@ -536,126 +855,59 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
                         }
                     }
                 }
-                if (sample_every > 0 && isfinite(energy) &&
-                    (mid + 1ull) % static_cast<unsigned long long>(sample_every) == 0ull) {
-                    e_sum += energy;
-                    e_sum2 += energy * energy;
-                    ++e_samples;
-                }
+            }
+            if (sample_every > 0 && isfinite(energy) &&
+                (mid + 1ull) % static_cast<unsigned long long>(sample_every) == 0ull) {
+                e_sum += energy;
+                e_sum2 += energy * energy;
+                ++e_samples;
             }
         }
-        if (kPre && g == 0 && tid >= 64) {
-            // while wave 0 waits: load the next move's particle and stencil and sum its
-            // pair terms there, as of the commits so far; the sums are kept unless this
-            // move's commit changes one of the loaded cells or moves that particle
-            const long long tn = t + 1;
-            const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
-            f_cell = -1;
-            if (tn < M && combo < 54) {
-                const Box b{{L, L, L}, {half, half, half}};
-                const Proposal& Q = s_prop[(t & 63) + 1];
-                const int in = Q.index;
-                const double4 qo = pos[in];
-                const int tq = static_cast<int>(meta[in] & 0xffu);
-                const double sc = s_dp[tq] * Q.length_u;
-                const int st = combo / 27, c = combo % 27;
-                double f_px, f_py, f_pz;
-                if (st == 0) {
-                    f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
-                    f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
-                    f_pz = wrap_s(qo.z + Q.dir[2] * sc, L, half, inv);
-                } else {
-                    f_px = qo.x;
-                    f_py = qo.y;
-                    f_pz = qo.z;
-                }
-                if (tid == 64) {
-                    s_pf_pos[0] = qo.x;
-                    s_pf_pos[1] = qo.y;
-                    s_pf_pos[2] = qo.z;
-                    s_pf_pos[3] = qo.w;
-                    s_pf_ti = tq;
-                    s_pf_cell_of[tn & 1] = cell_of[in];
-                }
-                const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
-                const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
-                const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
-                f_cell = (cz * nc1 + cy) * nc0 + cx;
-                if (combo == 13 && slot0 == 0) s_cnew[tn & 1] = f_cell;
-                const int f_cnt = cell_count[f_cell];
-                // the first two members are loaded at once, further ones in the loop
-                const int j0 = slot0 < f_cnt ? cell_members[f_cell * ccap + slot0] : -1;
-                const int j1 = slot0 + kPfLanes < f_cnt ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
-                double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
-                int t0 = 0, t1 = 0;
-                if (j0 >= 0) {
-                    p0 = pos[j0];
-                    t0 = static_cast<int>(meta[j0] & 0xffu);
-                }
-                if (j1 >= 0) {
-                    p1 = pos[j1];
-                    t1 = static_cast<int>(meta[j1] & 0xffu);
-                }
-                double fs = 0.0;
-                for (int k = slot0, u = 0; k < f_cnt; k += kPfLanes, ++u) {
-                    int j, tj;
-                    double4 pj;
-                    if (u == 0) {
-                        j = j0;
-                        pj = p0;
-                        tj = t0;
-                    } else if (u == 1) {
-                        j = j1;
-                        pj = p1;
-                        tj = t1;
-                    } else {
-                        j = cell_members[f_cell * ccap + k];
-                        if (j == in) continue;
-                        pj = pos[j];
-                        tj = static_cast<int>(meta[j] & 0xffu);
-                    }
-                    if (j == in) continue;
-                    fs += pair_energy(s_pairs[tq * nt + tj], s_coul, qo.w, pj.w, dist2(b, f_px, f_py, f_pz, pj.x, pj.y, pj.z));
-                }
-                f_sum = fs;
+        if (refresh) {
+            // the next move's particle is this move's: its phase tables and de again, against
+            // Q with this move applied (the tables were last read before the barrier above)
+            nb_po = make_double4(nx_, ny_, nz_, po.w);
+            nb_ti = ti;
+            const Proposal& P1 = s_prop[slot + 1];
+            const double sc1 = s_dp[nb_ti] * P1.length_u;
+            nb_nx = wrap_s(nb_po.x + P1.dir[0] * sc1, L, half, inv);
+            nb_ny = wrap_s(nb_po.y + P1.dir[1] * sc1, L, half, inv);
+            nb_nz = wrap_s(nb_po.z + P1.dir[2] * sc1, L, half, inv);
+            build_phases(nb_nx, nb_ny, nb_nz, nb_po.x, nb_po.y, nb_po.z);
+            __syncthreads();
+            if (tid >= 64) {
+                double x0 = 0.0, x1 = 0.0;
+                kslice(nb_po.w, x0, x1);
+                const double a = wave_sum(x0);
+                if ((tid & 63) == 0) s_red[0][wid] = a;
             }
-        }
-        __syncthreads();
-        if (s_abort) {
-            t_done = t;
-            break;
-        }
-        if (s_stop == 2) {
-            t_done = t;
-            break;
-        }
-        // the accepted move changed a loaded cell: load again at the next move
-        if (kPre && g == 0 && tid >= 64 && f_cell >= 0 && (f_cell == s_mv_cell[0] || f_cell == s_mv_cell[1]))
-            s_pf_ok = 0;
-        if (s_accept) {
 #pragma unroll
-            for (int s = 0; s < kMaxKPT; ++s) {
-                q[s].x += dq[s].x;
-                q[s].y += dq[s].y;
+            for (int s = 0; s < kMaxKPT; ++s) dq[s] = dq1[s];
+            __syncthreads();
+            if (tid == 0 || tid == kPub) {
+                const double de1 = row_sum(0, 1, kWaves);
+                if (tid == 0) own_de = de1;
+                if (tid == kPub && G > 1)
+                    split_put(slots + static_cast<int>((mid + 1ull) & 1ull) * G + g, 0, de1, split_tag(mid + 1ull));
             }
         }
+        if (hasK) {
+            cur_po = nb_po;
+            cur_ti = nb_ti;
+            cur_nx = nb_nx;
+            cur_ny = nb_ny;
+            cur_nz = nb_nz;
+        }
+        __syncthreads(); // commits, the next move's look-ahead flag and particle
         if (s_stop == 1) {
-            // workgroup 0 committed this move and stopped on a full cell: its words of the
-            // next move carry the abort bit, so the other workgroups stop there as well
-            if (g == 0 && tid == 0 && t + 1 < M) {
-                SplitSlot* sl = slots + static_cast<int>((mid + 1ull) & 1ull) * G;
-                for (int v = 0; v < kSplitValues; ++v) split_put(sl, v, 0.0, split_tag(mid + 1ull) | kAbortBit);
-                asm volatile("" ::: "memory"); // the abort words are stored before the loop ends
-            }
             t_done = t + 1;
             break;
         }
-        __syncthreads(); // phase tables are rewritten by the next move
     }
 #pragma unroll
     for (int s = 0; s < kMaxKPT; ++s) {
-        const int k = kb + s * kThreads + tid;
-        if (k < ke) S.q[k] = q[s];
+        const int k = kb + s * kKT + tid - 64;
+        if (tid >= 64 && k < ke) S.q[k] = q[s];
     }
     if (g == 0 && tid == 0) {
         S.moves = move0 + static_cast<unsigned long long>(t_done);
@@ -669,6 +921,23 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
         S.e_sum = e_sum;
         S.e_sum2 = e_sum2;
     }
+}
+
+template <int kThreads, int kMaxKPT, bool kCells, bool kSingle>
+__global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __restrict__ states, long long M, int G,
+                                                       int stride, SplitSlot* __restrict__ slots,
+                                                       int* __restrict__ err, TraceEntry* __restrict__ trace,
+                                                       int trace_cap, long long spin_limit, double4* __restrict__ priv) {
+    if (blockIdx.x % stride != 0) return;
+    const int g = blockIdx.x / stride;
+    if (g >= G) return;
+    if (states[0].error) return; // a stopped replica (read before any workgroup of this launch can set it)
+    __shared__ SplitLds<kThreads> sh;
+    // one workgroup (kSingle, G = 1) is its own instantiation: it holds both roles' registers
+    if (kSingle) split_body<kThreads, kMaxKPT, kCells, 2>(sh, states, M, G, g, slots, err, trace, trace_cap, spin_limit, priv);
+    else if (g == 0)
+        split_body<kThreads, kMaxKPT, kCells, 0>(sh, states, M, G, g, slots, err, trace, trace_cap, spin_limit, priv);
+    else split_body<kThreads, kMaxKPT, kCells, 1>(sh, states, M, G, g, slots, err, trace, trace_cap, spin_limit, priv);
 }
 
 // The split kernel for a replica over `groups` workgroups, every `stride`-th block of the
@@ -685,19 +954,25 @@ SplitLaunch split_launch(const ReplicaState& s, int groups, int stride, const ch
     if (stride < 1) throw std::runtime_error(w + ": stride >= 1");
     const int kparts = groups > 1 ? groups - 1 : 1; // workgroup 0 takes no k-vectors when G > 1
     const int kw = (s.nk + kparts - 1) / kparts;
-    const int threads = kw <= 4 * 512 ? 512 : 256;
-    if ((kw + threads - 1) / threads > (threads == 512 ? 4 : kSplitMaxKPT))
+    // the k-slice is spread over waves 1.. of a workgroup (wave 0 gathers and decides)
+    const int threads = kw <= 4 * (512 - 64) ? 512 : 256;
+    if ((kw + threads - 65) / (threads - 64) > (threads == 512 ? 4 : kSplitMaxKPT))
         throw std::runtime_error(w + ": too many k-vectors per workgroup");
     if (2 * s.ncut + 1 > 65 || s.ncut * s.ncut >= kGkMax) throw std::runtime_error(w + ": ncutoff too large");
     int dev = 0, ncu = 0, per_cu = 0;
     check_hip(hipGetDevice(&dev), "hipGetDevice");
     check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    const bool one = groups == 1;
     const void* fn =
         threads == 512
-            ? (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, true>)
-                       : reinterpret_cast<const void*>(mc_split_kernel<512, 4, false>))
-            : (s.cells ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true>)
-                       : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false>));
+            ? (s.cells ? (one ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, true, true>)
+                              : reinterpret_cast<const void*>(mc_split_kernel<512, 4, true, false>))
+                       : (one ? reinterpret_cast<const void*>(mc_split_kernel<512, 4, false, true>)
+                              : reinterpret_cast<const void*>(mc_split_kernel<512, 4, false, false>)))
+            : (s.cells ? (one ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true, true>)
+                              : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, true, false>))
+                       : (one ? reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false, true>)
+                              : reinterpret_cast<const void*>(mc_split_kernel<256, kSplitMaxKPT, false, false>)));
     check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0), "occupancy");
     const int grid = groups * stride;
     if (per_cu * ncu < grid) throw std::runtime_error(w + ": the grid cannot be co-resident");
