@@ -104,9 +104,9 @@ struct SplitLds {
     double kb[4]; // k-slice holders: the particle of the move after next, as of the current move
     int kti;
     double msg[4];       // workgroups g > 0 without private copies: workgroup 0's particle message
-    double pf_pos[2][4]; // the look-ahead's particle, by move parity
-    int pf_ti[2], pf_ok;
-    int cold[2], cnew[2]; // a move's old and new cell, by move parity
+    double pf_pos[4][4]; // the look-ahead's particle, by move mod 4
+    int pf_ti[4], pf_ok;
+    int cold[4], cnew[4]; // a move's old and new cell, by move mod 4
     // the commit's reads, made while wave 0 waits: the new cell's count, the old cell's
     // count, the moved particle's slot there and the old cell's last member
     int cm[4];
@@ -294,6 +294,69 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         }
     };
 
+    // Workgroup 0's look-ahead (waves 1-6).  Its first part, for the move after next, runs
+    // while the publisher commits the current move: the particle and the stencil cell of
+    // the lane, and that cell's count and first two member slots.  A value the concurrent
+    // commit changes is taken from the move itself (the moved particle, its new cell) or
+    // read again before use (the two cells the commit changed).
+    int la_tq = 0, la_in = -1, la_cell = -1, la_cnt = 0, la_r0 = -1, la_r1 = -1;
+    double la_px = 0.0, la_py = 0.0, la_pz = 0.0, la_w = 0.0;
+    auto la_prep = [&](long long tn, int ps, int ci, bool cacc, double cx, double cy, double cz, int ccell)
+                       __attribute__((always_inline)) {
+        const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+        la_cell = -1;
+        if (tn < M && combo < 54) {
+            const Proposal& Q = s_prop[ps];
+            const int in = Q.index;
+            double4 qo = pos[in];
+            const int tq = static_cast<int>(meta[in] & 0xffu);
+            int co = tid == 64 ? cell_of[in] : 0;
+            if (cacc && in == ci) {
+                qo.x = cx;
+                qo.y = cy;
+                qo.z = cz;
+                co = ccell;
+            }
+            const double sc = s_dp[tq] * Q.length_u;
+            const int st = combo / 27, c = combo % 27;
+            double f_px, f_py, f_pz;
+            if (st == 0) {
+                f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
+                f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
+                f_pz = wrap_s(qo.z + Q.dir[2] * sc, L, half, inv);
+            } else {
+                f_px = qo.x;
+                f_py = qo.y;
+                f_pz = qo.z;
+            }
+            if (tid == 64) {
+                s_pf_pos[tn & 3][0] = qo.x;
+                s_pf_pos[tn & 3][1] = qo.y;
+                s_pf_pos[tn & 3][2] = qo.z;
+                s_pf_pos[tn & 3][3] = qo.w;
+                s_pf_ti[tn & 3] = tq;
+                s_cold[tn & 3] = co;
+            }
+            const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+            const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+            const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+            const int f_cell = (cz * nc1 + cy) * nc0 + cx;
+            if (combo == 13 && slot0 == 0) s_cnew[tn & 3] = f_cell; // the new position's own cell
+            // the count and the first two member slots at once, further members later
+            la_cnt = cell_count[f_cell];
+            la_r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
+            la_r1 = slot0 + kPfLanes < ccap ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
+            la_in = in;
+            la_tq = tq;
+            la_w = qo.w;
+            la_px = f_px;
+            la_py = f_py;
+            la_pz = f_pz;
+            la_cell = f_cell;
+        }
+    };
+    bool prev_acc = false; // workgroup 0: the previous move was accepted (its commit overlapped the loads)
+
     refill(move0);
     __syncthreads();
     if (hasK && M > 0) {
@@ -372,8 +435,8 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         double nx_, ny_, nz_;
         if (pairs) {
             if (pre) {
-                po = make_double4(s_pf_pos[t & 1][0], s_pf_pos[t & 1][1], s_pf_pos[t & 1][2], s_pf_pos[t & 1][3]);
-                ti = s_pf_ti[t & 1];
+                po = make_double4(s_pf_pos[t & 3][0], s_pf_pos[t & 3][1], s_pf_pos[t & 3][2], s_pf_pos[t & 3][3]);
+                ti = s_pf_ti[t & 3];
             } else {
                 po = pos[i];
                 ti = static_cast<int>(meta[i] & 0xffu);
@@ -396,7 +459,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
             const Box b{{L, L, L}, {half, half, half}};
             if (kPre) {
                 const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
-                if (tid == 64) s_cold[t & 1] = cell_of[i];
+                if (tid == 64) s_cold[t & 3] = cell_of[i];
                 if (tid >= 64 && combo < 54) {
                     const int st = combo / 27, c = combo % 27;
                     const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
@@ -404,7 +467,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                     const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
                     const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
                     const int cell = (cz * nc1 + cy) * nc0 + cx;
-                    if (combo == 13 && slot0 == 0) s_cnew[t & 1] = cell; // the new position's own cell
+                    if (combo == 13 && slot0 == 0) s_cnew[t & 3] = cell; // the new position's own cell
                     const int cnt = cell_count[cell];
                     double acc_u = 0.0;
                     for (int k = slot0; k < cnt; k += kPfLanes) {
@@ -499,49 +562,23 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         // both outcomes (k-slice holders), and the next move's stencil sums as committed so
         // far (workgroup 0's waves 1-6)
         if (kPre && pairs && tid >= 64 && tid < 448) {
-            const long long tn = t + 1;
+            if (t == 0) la_prep(1, 1, -1, false, 0.0, 0.0, 0.0, 0); // the launch's first look-ahead
             const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
             double fs = 0.0;
             int cf = 0;
-            if (tn < M && combo < 54) {
+            if (t + 1 < M && la_cell >= 0) {
                 const Box b{{L, L, L}, {half, half, half}};
-                const Proposal& Q = s_prop[slot + 1];
-                const int in = Q.index;
-                const double4 qo = pos[in];
-                const int tq = static_cast<int>(meta[in] & 0xffu);
-                const double sc = s_dp[tq] * Q.length_u;
-                const int st = combo / 27, c = combo % 27;
-                double f_px, f_py, f_pz;
-                if (st == 0) {
-                    f_px = wrap_s(qo.x + Q.dir[0] * sc, L, half, inv);
-                    f_py = wrap_s(qo.y + Q.dir[1] * sc, L, half, inv);
-                    f_pz = wrap_s(qo.z + Q.dir[2] * sc, L, half, inv);
-                } else {
-                    f_px = qo.x;
-                    f_py = qo.y;
-                    f_pz = qo.z;
+                const int in = la_in, tq = la_tq, f_cell = la_cell;
+                if (t > 0 && prev_acc && (f_cell == s_cold[(t - 1) & 3] || f_cell == s_cnew[(t - 1) & 3])) {
+                    // the previous move's commit changed this cell while it was read
+                    la_cnt = cell_count[f_cell];
+                    la_r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
+                    la_r1 = slot0 + kPfLanes < ccap ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
                 }
-                if (tid == 64) {
-                    s_pf_pos[tn & 1][0] = qo.x;
-                    s_pf_pos[tn & 1][1] = qo.y;
-                    s_pf_pos[tn & 1][2] = qo.z;
-                    s_pf_pos[tn & 1][3] = qo.w;
-                    s_pf_ti[tn & 1] = tq;
-                    s_cold[tn & 1] = cell_of[in];
-                }
-                const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
-                const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
-                const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
-                const int f_cell = (cz * nc1 + cy) * nc0 + cx;
-                if (combo == 13 && slot0 == 0) s_cnew[tn & 1] = f_cell;
-                cf = (f_cell == s_cold[t & 1] || f_cell == s_cnew[t & 1]) ? 1 : 0;
-                // the count and the first two member slots are loaded at once, further
-                // members in the loop
-                const int f_cnt = cell_count[f_cell];
-                const int r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
-                const int r1 = slot0 + kPfLanes < ccap ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
-                const int j0 = slot0 < f_cnt ? r0 : -1;
-                const int j1 = slot0 + kPfLanes < f_cnt ? r1 : -1;
+                cf = (f_cell == s_cold[t & 3] || f_cell == s_cnew[t & 3]) ? 1 : 0;
+                const int f_cnt = la_cnt;
+                const int j0 = slot0 < f_cnt ? la_r0 : -1;
+                const int j1 = slot0 + kPfLanes < f_cnt ? la_r1 : -1;
                 double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
                 int t0 = 0, t1 = 0;
                 if (j0 >= 0) {
@@ -570,7 +607,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                         tj = static_cast<int>(meta[j] & 0xffu);
                     }
                     if (j == in) continue;
-                    fs += pair_energy(s_pairs[tq * nt + tj], s_coul, qo.w, pj.w, dist2(b, f_px, f_py, f_pz, pj.x, pj.y, pj.z));
+                    fs += pair_energy(s_pairs[tq * nt + tj], s_coul, la_w, pj.w, dist2(b, la_px, la_py, la_pz, pj.x, pj.y, pj.z));
                 }
             }
             const double a = wave_sum(combo < 27 ? fs : 0.0), c = wave_sum(combo < 27 ? 0.0 : fs);
@@ -585,7 +622,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
             // the commit's reads (wave 7): the cell counts, the moved particle's slot in its
             // old cell (one member per lane) and that cell's last member; and whether the
             // current move, if accepted, overflows its new cell
-            const int cn = s_cnew[t & 1], co = s_cold[t & 1];
+            const int cn = s_cnew[t & 3], co = s_cold[t & 3];
             const int lane = tid & 63;
             int nn = 0, nold = 0, kk = -1;
             if (cn != co) {
@@ -789,7 +826,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
             if (kPre && pairs && tid == kPub && acc) {
                 // the commit, after the publish: stores only, from the reads made during the wait
                 pos[i] = make_double4(nx_, ny_, nz_, po.w);
-                const int cn = s_cnew[t & 1], co = s_cold[t & 1];
+                const int cn = s_cnew[t & 3], co = s_cold[t & 3];
                 if (cn != co && !s_ovf) {
                     const int nn = s_cm[0], nold = s_cm[1], kk = s_cm[2];
                     if (kk >= 0) cell_members[co * ccap + kk] = s_cm[3];
@@ -890,6 +927,11 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                 if (tid == kPub && G > 1)
                     split_put(slots + static_cast<int>((mid + 1ull) & 1ull) * G + g, 0, de1, split_tag(mid + 1ull));
             }
+        }
+        if (kPre && pairs) {
+            // the look-ahead of the move after next starts during the commit
+            if (tid >= 64 && tid < 448) la_prep(t + 2, slot + 2, i, acc, nx_, ny_, nz_, s_cnew[t & 3]);
+            prev_acc = acc;
         }
         if (hasK) {
             cur_po = nb_po;
