@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 394.7e6
+HEADLINE_TRAFFIC_BYTES = 378.2e6
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 STEP_MOVES = 20000
@@ -908,12 +908,13 @@ def main():
                      "algorithmic_bytes_per_move": algo_bytes, "kernel_us_per_move": kernel_s_per_move * 1e6,
                      "fp64_tflops": achieved_tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                      "algorithmic_flops_per_move": flops,
-                     "limiter": "per-move latency chain across CUs (one Metropolis chain: decision and commit on "
-                                "workgroup 0 -> one-way exchange -> phase tables and Ewald slices on workgroups 1-15 "
-                                "-> exchange back; the pair walk runs one move ahead)",
+                     "limiter": "per-move latency chain of workgroup 0 (one Metropolis chain: every workgroup "
+                                "publishes its share of the next move right after each decision, so the exchange "
+                                "and the k-vector slices of workgroups 1-15 run in the shadow of workgroup 0's "
+                                "stencil walk of the next move and its commit)",
                      "note": "one launch runs STEP_MOVES sequential moves; achieved = algorithmic bytes per move "
                              "/ kernel time per move (HIP events); Q(k) stays in registers, so the measured HBM "
-                             "traffic (about 20 KB per move, mostly the exchange's polls) is far below the "
+                             "traffic (about 19 KB per move, mostly the exchange's polls) is far below the "
                              "algorithmic bytes; neither HBM nor the FP64 VALU binds: the move time is a chain of "
                              "dependent latencies"},
         "cpu_baseline": cpu,
