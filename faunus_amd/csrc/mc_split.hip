@@ -1,11 +1,14 @@
-// One large replica advanced by many workgroups at once.  Workgroup g owns a
-// slice of the k-vectors (its Q(k) stays in registers for the whole launch);
-// workgroup 0 also evaluates the pair dU through the cell list and commits
-// accepted moves.  After each move every workgroup publishes its partial sums, waits
-// for all others (bounded spin), sums the partials with the same butterfly and so
-// takes the same decision as every other workgroup.  While wave 0 of workgroup 0
-// waits, its other waves load the next move's particle and stencil neighbours; the
-// next move uses them unless the accepted move changed one of the loaded cells.
+// One large replica advanced by many workgroups at once.  Workgroups 1..G-1 own slices
+// of the k-vectors (Q(k) stays in registers for the whole launch); workgroup 0
+// evaluates the pair dU through the cell list and commits accepted moves (with G = 1
+// the one workgroup does both).  Every
+// workgroup gathers the shares of each move (bounded spin), sums them with the same
+// wave sum and so takes the same decision as every other workgroup.  Each workgroup
+// has its share of the next move ready before that decision and publishes it right
+// after: a k-slice holder sums the next move's de against Q and against Q + dQ of the
+// pending move while its wave 0 gathers, and sends the one the decision selects;
+// workgroup 0 walks the next move's stencils while its wave 0 gathers and sends the
+// result unless the accepted move changed a loaded cell.
 // The launch is cooperative, so all workgroups are resident.
 //
 // The exchange needs no fences.  Every published double travels as two words, its
@@ -16,8 +19,8 @@
 // positions and the cell list after the launch starts.  The other workgroups keep
 // private copies of the positions (copied at the launch's start, each accepted move
 // applied by every workgroup), so workgroup 0 publishes a single value, the pair energy
-// change; without the copies (the no-allocation entry point) it also publishes the next
-// move's particle position as committed so far.
+// change; without the copies (the no-allocation entry point) it also publishes, with each
+// move's pair change, the particle of the move after next as committed before that move.
 //
 // A replica whose cell list overflows stops after the move that overflowed: that move
 // is committed (position, Q(k), energy), the particle stays listed in its old cell,
@@ -984,7 +987,8 @@ __global__ __launch_bounds__(kThreads) void mc_split_kernel(ReplicaState* __rest
 
 // The split kernel for a replica over `groups` workgroups, every `stride`-th block of the
 // grid: 512 threads (two waves per SIMD, 7 lanes per stencil cell) when the slice fits 4
-// k-vectors per thread, else 256 threads with up to 8; the grid must be co-resident.
+// k-vectors per thread of waves 1..7, else 256 threads with up to 12 per thread of waves
+// 1..3; the grid must be co-resident.
 struct SplitLaunch {
     const void* fn;
     int threads, grid;
