@@ -258,3 +258,32 @@ def test_split_three_cell_grid_matches_oracle():
     tot, _ = b.total_energy(0)
     assert abs(b.info(0)["energy"] - tot) <= 1e-9 * abs(tot)
     np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("groups", [3, 8])
+def test_split_small_system_repeats_particles(groups):
+    """N = 24 on a 3x3x3 cell grid: the next move's particle is the current one about
+    once in 24 moves.  After such a move is accepted, the k-vector workgroups build the
+    next move's share again and workgroup 0 loads that particle again; decisions stay
+    exact against the oracle, and the final positions and energy agree."""
+    sysd = systems.nacl_lj(n_pairs=12, box=40.0, seed=5, kmax=6)
+    orc, e0 = oracle_of(sysd)
+    b = F.Batch()
+    b.add(sysd, seed=43)
+    nm = 3000
+    print("split small system, groups", groups, flush=True)
+    b.run_split(nm, groups=groups, stride=1, trace=nm)
+    b.sync()
+    assert b.split_error() == 0
+    tr = b.trace(0, nm)
+    rec = orc.run_philox_translate(nm, seed=43, replica=0)
+    rep = compare_traces(tr, rec, max(1.0, abs(e0)), 1e-10, seed=43)
+    idx, acc = np.asarray(rec["index"]), np.asarray(rec["accepted"]).astype(bool)
+    repeats = int(np.sum((idx[1:] == idx[:-1]) & acc[:-1]))
+    log("split small system", groups, rep, "accepted repeats", repeats)
+    assert repeats >= 20
+    assert rep["first_mismatch"] == nm and rep["max_rel"] <= 1e-10
+    tot, _ = b.total_energy(0)
+    assert abs(b.info(0)["energy"] - tot) <= 1e-9 * max(1.0, abs(tot))
+    np.testing.assert_allclose(b.positions(0), orc.positions(), rtol=0, atol=1e-9)
+    b.close()

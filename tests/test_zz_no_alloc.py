@@ -48,6 +48,15 @@ def test_static_workspace_matches_run_split_on_two_calls():
     compare(small, 700, 4, seed=92)
 
 
+@pytest.mark.parametrize("groups", [3, 8])
+def test_static_workspace_matches_run_split_when_particles_repeat(groups):
+    """N = 24: the move after next often moves the particle of the pending move, so the
+    particle message of the call that allocates nothing is corrected by that move's
+    decision about once in 24 moves."""
+    tiny = systems.nacl_lj(n_pairs=12, box=40.0, seed=5, kmax=6)
+    compare(tiny, 3000, groups, seed=94)
+
+
 def test_static_workspace_refuses_what_it_cannot_hold():
     sysd = systems.nacl_lj(n_pairs=300, box=40.0, seed=3, kmax=6)
     b = F.Batch()
