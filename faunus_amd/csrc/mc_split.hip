@@ -580,17 +580,14 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                 }
                 cf = (f_cell == s_cold[t & 3] || f_cell == s_cnew[t & 3]) ? 1 : 0;
                 const int f_cnt = la_cnt;
+                // the first member's position is loaded at once; a second member slot of
+                // the lane (cells of more than 7) in the loop
                 const int j0 = slot0 < f_cnt ? la_r0 : -1;
-                const int j1 = slot0 + kPfLanes < f_cnt ? la_r1 : -1;
-                double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0), p1 = p0;
-                int t0 = 0, t1 = 0;
+                double4 p0 = make_double4(0.0, 0.0, 0.0, 0.0);
+                int t0 = 0;
                 if (j0 >= 0) {
                     p0 = pos[j0];
                     t0 = static_cast<int>(meta[j0] & 0xffu);
-                }
-                if (j1 >= 0) {
-                    p1 = pos[j1];
-                    t1 = static_cast<int>(meta[j1] & 0xffu);
                 }
                 for (int k = slot0, u = 0; k < f_cnt; k += kPfLanes, ++u) {
                     int j, tj;
@@ -599,12 +596,8 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                         j = j0;
                         pj = p0;
                         tj = t0;
-                    } else if (u == 1) {
-                        j = j1;
-                        pj = p1;
-                        tj = t1;
                     } else {
-                        j = cell_members[f_cell * ccap + k];
+                        j = u == 1 ? la_r1 : cell_members[f_cell * ccap + k];
                         if (j == in) continue;
                         pj = pos[j];
                         tj = static_cast<int>(meta[j] & 0xffu);
