@@ -254,9 +254,11 @@ int fmc_batch_add_replica_moves(fmc_batch* batch, const char* config_json, int n
 int fmc_batch_run(fmc_batch* batch, const long long* moves, int trace_capacity);
 /*
  * Advance a batch of ONE replica by `moves` moves with `groups` cooperating
- * workgroups (each owns a slice of the k-vectors; every `stride`-th block of the grid
- * takes part, so with stride = number of XCDs the participants share one XCD).
- * The workgroups exchange per-move partial sums through device memory; a wait that
+ * workgroups (workgroup 0 walks the pair stencils and commits, the others own slices of
+ * the k-vectors; every `stride`-th block of the grid takes part, so with stride = number
+ * of XCDs the participants share one XCD).  The workgroups exchange per-move partial
+ * sums through device memory, each publishing its share of the next move right after
+ * the previous decision; a wait that
  * exceeds its spin bound stops the run and sets the error read by
  * fmc_batch_split_error instead of hanging.  A replica with a general move set
  * (fmc_batch_add_replica_moves, no cell list, 1..64 groups) runs the split form of the
@@ -274,8 +276,8 @@ int fmc_batch_run_split(fmc_batch* batch, long long moves, int groups, int strid
  * fmc_batch_replica_info call), and no trace is recorded.  Calls that overlap on one GPU
  * share the workspace and race.  The chain equals fmc_batch_run_split's at the same
  * workgroup count, bit for bit; without the private position copies that
- * fmc_batch_run_split allocates, workgroup 0 also sends each next particle, so it runs a
- * little slower.  Errors read through fmc_batch_split_error / fmc_batch_replica_info.
+ * fmc_batch_run_split allocates, workgroup 0 also sends, with each move's pair change,
+ * the particle of the move after next, so it runs a little slower.  Errors read through fmc_batch_split_error / fmc_batch_replica_info.
  */
 int fmc_batch_run_split_static(fmc_batch* batch, long long moves, int groups, int stride);
 /*
