@@ -306,9 +306,9 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
     double la_px = 0.0, la_py = 0.0, la_pz = 0.0, la_w = 0.0;
     auto la_prep = [&](long long tn, int ps, int ci, bool cacc, double cx, double cy, double cz, int ccell)
                        __attribute__((always_inline)) {
-        const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+        const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
         la_cell = -1;
-        if (tn < M && combo < 54) {
+        if (tn < M && slot0 < kPfLanes) {
             const Proposal& Q = s_prop[ps];
             const int in = Q.index;
             double4 qo = pos[in];
@@ -461,9 +461,9 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
             double e_new = 0.0, e_old = 0.0;
             const Box b{{L, L, L}, {half, half, half}};
             if (kPre) {
-                const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+                const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
                 if (tid == 64) s_cold[t & 3] = cell_of[i];
-                if (tid >= 64 && combo < 54) {
+                if (tid >= 64 && slot0 < kPfLanes) {
                     const int st = combo / 27, c = combo % 27;
                     const double px = st == 0 ? nx_ : po.x, py = st == 0 ? ny_ : po.y, pz = st == 0 ? nz_ : po.z;
                     const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
@@ -566,7 +566,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         // far (workgroup 0's waves 1-6)
         if (kPre && pairs && tid >= 64 && tid < 448) {
             if (t == 0) la_prep(1, 1, -1, false, 0.0, 0.0, 0.0, 0); // the launch's first look-ahead
-            const int combo = (tid - 64) / kPfLanes, slot0 = (tid - 64) % kPfLanes;
+            const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
             double fs = 0.0;
             int cf = 0;
             if (t + 1 < M && la_cell >= 0) {
