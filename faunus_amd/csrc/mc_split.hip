@@ -562,8 +562,9 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         }
 
         // (3) while wave 0 gathers this move's partials and decides: the next move's de for
-        // both outcomes (k-slice holders), and the next move's stencil sums as committed so
-        // far (workgroup 0's waves 1-6)
+        // both outcomes (k-slice holders), and the next move's stencil sums as committed
+        // before this move, from the loads started during the previous commit (workgroup
+        // 0's waves 1-6)
         if (kPre && pairs && tid >= 64 && tid < 448) {
             if (t == 0) la_prep(1, 1, -1, false, 0.0, 0.0, 0.0, 0); // the launch's first look-ahead
             const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
@@ -651,7 +652,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         // gather: lane j < G takes workgroup j's value (workgroup 0's pair change, the
         // others' de); lanes 58-61 take workgroup 0's particle when it is sent.  A
         // workgroup takes its own value from registers, bit for bit what the others read.
-        // The des are added with the same butterfly in every workgroup.
+        // The des are added with the same wave sum in every workgroup.
         double du = 0.0;
         if (tid < 64) {
             int ok = 1, stopped = 0;
