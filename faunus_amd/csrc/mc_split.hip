@@ -113,6 +113,10 @@ struct SplitLds {
     // the commit's reads, made while wave 0 waits: the new cell's count, the old cell's
     // count, the moved particle's slot there and the old cell's last member
     int cm[4];
+    // the particle of the move after next as committed before the current move (position,
+    // charge; type and cell), loaded by wave 7 while wave 0 waits
+    double la_pos[4];
+    int la_ti, la_co;
 };
 
 // Roles: 0 workgroup 0 of several (pair walk, commit), 1 a k-slice holder (g > 0),
@@ -156,6 +160,9 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
     auto& s_cold = sh.cold;
     auto& s_cnew = sh.cnew;
     auto& s_cm = sh.cm;
+    auto& s_la_pos = sh.la_pos;
+    int& s_la_ti = sh.la_ti;
+    int& s_la_co = sh.la_co;
 
     const int tid = threadIdx.x;
     const int wid = tid >> 6;
@@ -304,16 +311,24 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
     // read again before use (the two cells the commit changed).
     int la_tq = 0, la_in = -1, la_cell = -1, la_cnt = 0, la_r0 = -1, la_r1 = -1;
     double la_px = 0.0, la_py = 0.0, la_pz = 0.0, la_w = 0.0;
-    auto la_prep = [&](long long tn, int ps, int ci, bool cacc, double cx, double cy, double cz, int ccell)
-                       __attribute__((always_inline)) {
+    auto la_prep = [&](long long tn, int ps, int ci, bool cacc, double cx, double cy, double cz, int ccell,
+                       bool staged) __attribute__((always_inline)) {
         const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
         la_cell = -1;
         if (tn < M && slot0 < kPfLanes) {
             const Proposal& Q = s_prop[ps];
             const int in = Q.index;
-            double4 qo = pos[in];
-            const int tq = static_cast<int>(meta[in] & 0xffu);
-            int co = tid == 64 ? cell_of[in] : 0;
+            double4 qo;
+            int tq, co;
+            if (staged) {
+                qo = make_double4(s_la_pos[0], s_la_pos[1], s_la_pos[2], s_la_pos[3]);
+                tq = s_la_ti;
+                co = s_la_co;
+            } else {
+                qo = pos[in];
+                tq = static_cast<int>(meta[in] & 0xffu);
+                co = tid == 64 ? cell_of[in] : 0;
+            }
             if (cacc && in == ci) {
                 qo.x = cx;
                 qo.y = cy;
@@ -566,7 +581,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         // before this move, from the loads started during the previous commit (workgroup
         // 0's waves 1-6)
         if (kPre && pairs && tid >= 64 && tid < 448) {
-            if (t == 0) la_prep(1, 1, -1, false, 0.0, 0.0, 0.0, 0); // the launch's first look-ahead
+            if (t == 0) la_prep(1, 1, -1, false, 0.0, 0.0, 0.0, 0, false); // the launch's first look-ahead
             const int combo = (tid - 64) % 54, slot0 = (tid - 64) / 54;
             double fs = 0.0;
             int cf = 0;
@@ -618,9 +633,20 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         if (kPre && pairs && wid == kWaves - 1) {
             // the commit's reads (wave 7): the cell counts, the moved particle's slot in its
             // old cell (one member per lane) and that cell's last member; and whether the
-            // current move, if accepted, overflows its new cell
+            // current move, if accepted, overflows its new cell.  Lane 1 also stages the
+            // particle of the move after next for the look-ahead that starts at the commit.
             const int cn = s_cnew[t & 3], co = s_cold[t & 3];
             const int lane = tid & 63;
+            if (lane == 1 && t + 2 < M) {
+                const int in2 = s_prop[slot + 2].index;
+                const double4 p2 = pos[in2];
+                s_la_pos[0] = p2.x;
+                s_la_pos[1] = p2.y;
+                s_la_pos[2] = p2.z;
+                s_la_pos[3] = p2.w;
+                s_la_ti = static_cast<int>(meta[in2] & 0xffu);
+                s_la_co = cell_of[in2];
+            }
             int nn = 0, nold = 0, kk = -1;
             if (cn != co) {
                 nn = cell_count[cn];
@@ -927,7 +953,7 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
         }
         if (kPre && pairs) {
             // the look-ahead of the move after next starts during the commit
-            if (tid >= 64 && tid < 448) la_prep(t + 2, slot + 2, i, acc, nx_, ny_, nz_, s_cnew[t & 3]);
+            if (tid >= 64 && tid < 448) la_prep(t + 2, slot + 2, i, acc, nx_, ny_, nz_, s_cnew[t & 3], true);
             prev_acc = acc;
         }
         if (hasK) {
