@@ -117,6 +117,8 @@ struct SplitLds {
     // charge; type and cell), loaded by wave 7 while wave 0 waits
     double la_pos[4];
     int la_ti, la_co;
+    // and that move's 54 stencil cells (combo order), each cell's count and first two members
+    int la_cell[54], la_cnt[54], la_m0[54], la_m1[54];
 };
 
 // Roles: 0 workgroup 0 of several (pair walk, commit), 1 a k-slice holder (g > 0),
@@ -163,6 +165,10 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
     auto& s_la_pos = sh.la_pos;
     int& s_la_ti = sh.la_ti;
     int& s_la_co = sh.la_co;
+    auto& s_la_cell = sh.la_cell;
+    auto& s_la_cnt = sh.la_cnt;
+    auto& s_la_m0 = sh.la_m0;
+    auto& s_la_m1 = sh.la_m1;
 
     const int tid = threadIdx.x;
     const int wid = tid >> 6;
@@ -355,14 +361,25 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
                 s_pf_ti[tn & 3] = tq;
                 s_cold[tn & 3] = co;
             }
-            const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
-            const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
-            const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
-            const int f_cell = (cz * nc1 + cy) * nc0 + cx;
+            // the cells, counts and first two members staged by wave 7, unless this move
+            // moved the particle
+            const bool fresh = !staged || (cacc && in == ci);
+            int f_cell;
+            if (fresh) {
+                const int cx = (cell_coord_s(f_px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                const int cy = (cell_coord_s(f_py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                const int cz = (cell_coord_s(f_pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                f_cell = (cz * nc1 + cy) * nc0 + cx;
+                la_cnt = cell_count[f_cell];
+                la_r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
+            } else {
+                f_cell = s_la_cell[combo];
+                la_cnt = s_la_cnt[combo];
+                la_r0 = slot0 == 0 ? s_la_m0[combo]
+                                   : (slot0 == 1 ? s_la_m1[combo]
+                                                 : (slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1));
+            }
             if (combo == 13 && slot0 == 0) s_cnew[tn & 3] = f_cell; // the new position's own cell
-            // the count and the first two member slots at once, further members later
-            la_cnt = cell_count[f_cell];
-            la_r0 = slot0 < ccap ? cell_members[f_cell * ccap + slot0] : -1;
             la_r1 = slot0 + kPfLanes < ccap ? cell_members[f_cell * ccap + slot0 + kPfLanes] : -1;
             la_in = in;
             la_tq = tq;
@@ -637,15 +654,39 @@ __device__ __attribute__((always_inline)) inline void split_body(SplitLds<kThrea
             // particle of the move after next for the look-ahead that starts at the commit.
             const int cn = s_cnew[t & 3], co = s_cold[t & 3];
             const int lane = tid & 63;
-            if (lane == 1 && t + 2 < M) {
-                const int in2 = s_prop[slot + 2].index;
+            if (t + 2 < M) {
+                // lane c < 54 also stages stencil combo c of that move: its cell, the cell's
+                // count and first two members (the look-ahead's combo and state order)
+                const Proposal& Q2 = s_prop[slot + 2];
+                const int in2 = Q2.index;
                 const double4 p2 = pos[in2];
-                s_la_pos[0] = p2.x;
-                s_la_pos[1] = p2.y;
-                s_la_pos[2] = p2.z;
-                s_la_pos[3] = p2.w;
-                s_la_ti = static_cast<int>(meta[in2] & 0xffu);
-                s_la_co = cell_of[in2];
+                const int t2 = static_cast<int>(meta[in2] & 0xffu);
+                if (lane == 1) {
+                    s_la_pos[0] = p2.x;
+                    s_la_pos[1] = p2.y;
+                    s_la_pos[2] = p2.z;
+                    s_la_pos[3] = p2.w;
+                    s_la_ti = t2;
+                    s_la_co = cell_of[in2];
+                }
+                if (lane < 54) {
+                    const double sc2 = s_dp[t2] * Q2.length_u;
+                    const int st = lane / 27, c = lane % 27;
+                    double px = p2.x, py = p2.y, pz = p2.z;
+                    if (st == 0) {
+                        px = wrap_s(p2.x + Q2.dir[0] * sc2, L, half, inv);
+                        py = wrap_s(p2.y + Q2.dir[1] * sc2, L, half, inv);
+                        pz = wrap_s(p2.z + Q2.dir[2] * sc2, L, half, inv);
+                    }
+                    const int cx = (cell_coord_s(px, half, inv_cell, nc0) + (c % 3) - 1 + nc0) % nc0;
+                    const int cy = (cell_coord_s(py, half, inv_cell, nc1) + ((c / 3) % 3) - 1 + nc1) % nc1;
+                    const int cz = (cell_coord_s(pz, half, inv_cell, nc2) + (c / 9) - 1 + nc2) % nc2;
+                    const int cell = (cz * nc1 + cy) * nc0 + cx;
+                    s_la_cell[lane] = cell;
+                    s_la_cnt[lane] = cell_count[cell];
+                    s_la_m0[lane] = ccap > 0 ? cell_members[cell * ccap] : -1;
+                    s_la_m1[lane] = ccap > 1 ? cell_members[cell * ccap + 1] : -1;
+                }
             }
             int nn = 0, nold = 0, kk = -1;
             if (cn != co) {
