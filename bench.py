@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (STEP_MOVES moves), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs scaled by the copy-probe calibration
 # (profiles/mc_split_kernel__nacl20k_kmax20.txt)
-HEADLINE_TRAFFIC_BYTES = 378.0e6
+HEADLINE_TRAFFIC_BYTES = 378.3e6
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 STEP_MOVES = 20000
